@@ -1,0 +1,119 @@
+/* Additive C entry points of the MI355X build.
+ *
+ * Nothing here changes or replaces a reference symbol (include/ocean/ocean.h
+ * is the drop-in ABI). These are the entry points the GPU design adds, as
+ * additive symbols in the sense of the reference's own versioning rules
+ * (reference include/ocean/ocean.h:68-74,207-215): batched queries, device
+ * pointers, and C access to the reference's C++-only FoamField and
+ * CascadeStack (reference include/ocean/foam.hpp:84-142,
+ * include/ocean/cascade.hpp:45-92). Each declaration names the reference
+ * interface it extends.
+ *
+ * Status returns: 0 on success, otherwise an ocean_status (ocean.h) or, for
+ * a device failure, OCEAN_ERROR_UNKNOWN with the HIP error available from
+ * ocean_amd_last_error().
+ */
+#ifndef OCEANLIB_OCEAN_AMD_H
+#define OCEANLIB_OCEAN_AMD_H
+
+#include "ocean/ocean.h"
+
+#if defined(__cplusplus)
+extern "C" {
+#endif
+
+/* ---- device-resident outputs -------------------------------------------
+ * Extends ocean_get_buffers (reference include/ocean/ocean.h:140-171,
+ * src/c_api.cpp:180-192): the same N*N RGBA32F layout, but device pointers,
+ * valid until ocean_destroy. `stream` is the hipStream_t the ocean enqueues
+ * its frames on; work queued on it after ocean_update sees that frame. */
+typedef struct ocean_amd_device_buffers {
+    const float* displacement;
+    const float* normal;
+    const float* velocity;     /* NULL unless compute_velocity */
+    uint32_t     size;
+    float        patch_length;
+    void*        stream;
+} ocean_amd_device_buffers;
+
+int  ocean_amd_get_device_buffers(const ocean_sim* sim, ocean_amd_device_buffers* out);
+/* Blocks until every frame and query enqueued on the ocean has finished. */
+void ocean_amd_synchronize(const ocean_sim* sim);
+/* First HIP error latched since creation (hipError_t value), 0 if none.
+ * update() is noexcept in the reference (include/ocean/ocean.hpp:198-202), so
+ * device failures are latched here instead of thrown. */
+int  ocean_amd_last_error(const ocean_sim* sim);
+
+/* ---- batched queries ----------------------------------------------------
+ * Batched form of ocean_sample_ex / ocean_height_at (reference
+ * src/c_api.cpp:199-202,330-350; src/ocean.cpp:375-452): result q is exactly
+ * what ocean_sample_ex(sim, xz[2q], xz[2q+1]) returns.
+ *   _batch        : host arrays; blocks until the results are back.
+ *   _batch_device : device arrays; enqueued on the ocean's stream, returns
+ *                   at once. `out` and/or `heights` may be NULL.          */
+int ocean_amd_sample_batch(const ocean_sim* sim, const float* xz, uint32_t count,
+                           ocean_surface_ex* out);
+int ocean_amd_sample_batch_device(const ocean_sim* sim, const float* xz_device,
+                                  uint32_t count, ocean_surface_ex* out_device,
+                                  float* heights_device);
+
+/* ---- host-side tables (no device needed) -------------------------------
+ * The constant tables the frame kernels read, as the library builds them:
+ * h0 (2*N*N floats, re/im interleaved) and omega (N*N), reference
+ * src/core/spectrum.cpp:264-339; twiddles (N/2 pairs cos, +sin), reference
+ * src/core/fft.cpp:24-33. */
+int ocean_amd_build_spectrum(const ocean_desc* desc, float* h0, float* omega);
+int ocean_amd_build_twiddles(uint32_t n, float* tw);
+
+/* ---- persistent advected foam ------------------------------------------
+ * C access to the reference's C++ FoamField (include/ocean/foam.hpp:38-142,
+ * src/foam.cpp). Same descriptor fields and defaults (FoamDesc), same
+ * fixed-step accumulator; the field lives in device memory. */
+typedef struct ocean_amd_foam ocean_amd_foam;
+
+typedef struct ocean_amd_foam_desc {
+    size_t   struct_size;
+    float    decay;          /* 1/s                                        */
+    float    source_gain;
+    float    advect_scale;
+    float    wind_drift_x;   /* m/s                                        */
+    float    wind_drift_z;
+    float    fixed_dt;       /* s                                          */
+    uint32_t max_substeps;
+} ocean_amd_foam_desc;
+
+void            ocean_amd_foam_desc_init(ocean_amd_foam_desc* desc);
+ocean_amd_foam* ocean_amd_foam_create(const ocean_sim* sim, const ocean_amd_foam_desc* desc,
+                                      ocean_status* out_status);
+void            ocean_amd_foam_destroy(ocean_amd_foam* foam);
+void            ocean_amd_foam_update(ocean_amd_foam* foam, float dt);
+void            ocean_amd_foam_clear(ocean_amd_foam* foam);
+/* N*N floats: host mirror (copied on request) / device pointer. */
+const float*    ocean_amd_foam_data(const ocean_amd_foam* foam);
+const float*    ocean_amd_foam_device_data(const ocean_amd_foam* foam);
+uint32_t        ocean_amd_foam_last_substeps(const ocean_amd_foam* foam);
+float           ocean_amd_foam_at(const ocean_amd_foam* foam, float world_x, float world_z);
+double          ocean_amd_foam_coverage(const ocean_amd_foam* foam);
+
+/* ---- cascades ------------------------------------------------------------
+ * C access to the reference's C++ CascadeStack (include/ocean/cascade.hpp:
+ * 45-92, src/cascade.cpp). update() enqueues every level on one stream;
+ * queries sum levels in the reference's order (src/cascade.cpp:82-153). */
+typedef struct ocean_amd_cascade ocean_amd_cascade;
+
+ocean_amd_cascade* ocean_amd_cascade_create(const ocean_desc* levels, uint32_t count,
+                                            ocean_status* out_status);
+void          ocean_amd_cascade_destroy(ocean_amd_cascade* stack);
+void          ocean_amd_cascade_update(ocean_amd_cascade* stack, double time);
+uint32_t      ocean_amd_cascade_level_count(const ocean_amd_cascade* stack);
+ocean_buffers ocean_amd_cascade_get_buffers(const ocean_amd_cascade* stack, uint32_t level);
+void          ocean_amd_cascade_sample(const ocean_amd_cascade* stack, float world_x,
+                                       float world_z, ocean_surface_ex* out);
+int           ocean_amd_cascade_sample_batch(const ocean_amd_cascade* stack, const float* xz,
+                                             uint32_t count, ocean_surface_ex* out);
+
+#if defined(__cplusplus)
+}
+#endif
+
+#endif /* OCEANLIB_OCEAN_AMD_H */

@@ -1,0 +1,456 @@
+// C ABI over the C++ classes: include/ocean/ocean.h (the reference's ABI) and
+// include/ocean/ocean_amd.h (additions).
+//
+// Same firewall rules as the reference's src/c_api.cpp: the handle is the C++
+// object itself, no exception crosses, NULL is tolerated everywhere, and
+// struct_size decides how much of a caller's descriptor is read.
+#include "ocean/ocean.h"
+#include "ocean/ocean_amd.h"
+#include "ocean/ocean.hpp"
+
+#include "ocean_device.hpp"
+#include "spectrum_host.hpp"
+
+#include <cstring>
+#include <new>
+#include <stdexcept>
+
+namespace {
+
+ocean::Ocean* to_cpp(ocean_sim* h) noexcept { return reinterpret_cast<ocean::Ocean*>(h); }
+const ocean::Ocean* to_cpp(const ocean_sim* h) noexcept
+{
+    return reinterpret_cast<const ocean::Ocean*>(h);
+}
+
+// reference src/c_api.cpp:46-60
+bool copy_desc(const ocean_desc* src, ocean_desc& dst) noexcept
+{
+    if (src == nullptr) return false;
+    if (src->struct_size == 0 || src->struct_size > sizeof(ocean_desc)) return false;
+    ocean_desc_init(&dst);
+    std::memcpy(&dst, src, src->struct_size);
+    dst.struct_size = sizeof(ocean_desc);
+    return true;
+}
+
+// reference src/c_api.cpp:62-95
+ocean::OceanDesc translate(const ocean_desc& c) noexcept
+{
+    ocean::OceanDesc d;
+    d.size = c.size;
+    d.patch_length = c.patch_length;
+    d.spectrum.wind_speed = c.spectrum.wind_speed;
+    d.spectrum.fetch = c.spectrum.fetch;
+    d.spectrum.wind_direction = c.spectrum.wind_direction;
+    d.spectrum.peak_enhancement = c.spectrum.peak_enhancement;
+    d.spectrum.swell = c.spectrum.swell;
+    d.spectrum.small_wave_cutoff = c.spectrum.small_wave_cutoff;
+    d.spectrum.gravity = c.spectrum.gravity;
+    d.spectrum.depth = c.water_depth;
+    d.choppiness = c.choppiness;
+    d.foam_threshold = c.foam_threshold;
+    d.seed = c.seed;
+    d.parallel_for = reinterpret_cast<ocean::ParallelForFn>(c.parallel_for);
+    d.parallel_for_user = c.parallel_for_user;
+    d.thread_count = c.thread_count;
+    d.compute_velocity = (c.compute_velocity != 0);
+    return d;
+}
+
+void to_surface_ex(const ocean::Surface& s, ocean_surface_ex* out) noexcept
+{
+    out->height = s.height;
+    out->offset_x = s.offset_x;
+    out->offset_z = s.offset_z;
+    out->normal_x = s.normal_x;
+    out->normal_y = s.normal_y;
+    out->normal_z = s.normal_z;
+    out->foam = s.foam;
+    out->velocity_x = s.velocity_x;
+    out->velocity_y = s.velocity_y;
+    out->velocity_z = s.velocity_z;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ocean_status_string(ocean_status status)
+{
+    switch (status) {
+        case OCEAN_OK: return "ok";
+        case OCEAN_ERROR_INVALID_ARG: return "invalid argument";
+        case OCEAN_ERROR_OUT_OF_MEMORY: return "out of memory";
+        case OCEAN_ERROR_UNKNOWN: return "unknown error";
+    }
+    return "unrecognised status";
+}
+
+void ocean_desc_init(ocean_desc* desc)
+{
+    if (desc == nullptr) return;
+    const ocean::OceanDesc d{};
+    std::memset(desc, 0, sizeof(*desc));
+    desc->struct_size = sizeof(ocean_desc);
+    desc->size = d.size;
+    desc->patch_length = d.patch_length;
+    desc->spectrum.wind_speed = d.spectrum.wind_speed;
+    desc->spectrum.fetch = d.spectrum.fetch;
+    desc->spectrum.wind_direction = d.spectrum.wind_direction;
+    desc->spectrum.peak_enhancement = d.spectrum.peak_enhancement;
+    desc->spectrum.swell = d.spectrum.swell;
+    desc->spectrum.small_wave_cutoff = d.spectrum.small_wave_cutoff;
+    desc->spectrum.gravity = d.spectrum.gravity;
+    desc->choppiness = d.choppiness;
+    desc->foam_threshold = d.foam_threshold;
+    desc->seed = d.seed;
+    desc->thread_count = d.thread_count;
+    desc->water_depth = d.spectrum.depth;
+    desc->compute_velocity = d.compute_velocity ? 1 : 0;
+}
+
+ocean_sim* ocean_create(const ocean_desc* desc, ocean_status* out_status)
+{
+    auto fail = [&](ocean_status s) -> ocean_sim* {
+        if (out_status != nullptr) *out_status = s;
+        return nullptr;
+    };
+    ocean_desc resolved;
+    if (!copy_desc(desc, resolved)) return fail(OCEAN_ERROR_INVALID_ARG);
+    try {
+        auto* sim = new ocean::Ocean(translate(resolved));
+        if (out_status != nullptr) *out_status = OCEAN_OK;
+        return reinterpret_cast<ocean_sim*>(sim);
+    } catch (const std::invalid_argument&) {
+        return fail(OCEAN_ERROR_INVALID_ARG);
+    } catch (const std::bad_alloc&) {
+        return fail(OCEAN_ERROR_OUT_OF_MEMORY);
+    } catch (...) {
+        return fail(OCEAN_ERROR_UNKNOWN);
+    }
+}
+
+void ocean_destroy(ocean_sim* sim) { delete to_cpp(sim); }
+
+void ocean_update(ocean_sim* sim, double time)
+{
+    if (sim == nullptr) return;
+    to_cpp(sim)->update(time);
+}
+
+ocean_buffers ocean_get_buffers(const ocean_sim* sim)
+{
+    ocean_buffers out;
+    std::memset(&out, 0, sizeof(out));
+    if (sim == nullptr) return out;
+    const ocean::Buffers b = to_cpp(sim)->buffers();
+    out.displacement = b.displacement;
+    out.normal = b.normal;
+    out.size = b.size;
+    out.patch_length = b.patch_length;
+    return out;
+}
+
+double ocean_get_time(const ocean_sim* sim) { return sim == nullptr ? 0.0 : to_cpp(sim)->time(); }
+
+float ocean_height_at(const ocean_sim* sim, float world_x, float world_z)
+{
+    return sim == nullptr ? 0.0f : to_cpp(sim)->height_at(world_x, world_z);
+}
+
+ocean_surface ocean_sample_at(const ocean_sim* sim, float world_x, float world_z)
+{
+    ocean_surface out;
+    std::memset(&out, 0, sizeof(out));
+    out.normal_y = 1.0f;
+    if (sim == nullptr) return out;
+    const ocean::Surface s = to_cpp(sim)->sample_at(world_x, world_z);
+    out.height = s.height;
+    out.offset_x = s.offset_x;
+    out.offset_z = s.offset_z;
+    out.normal_x = s.normal_x;
+    out.normal_y = s.normal_y;
+    out.normal_z = s.normal_z;
+    out.foam = s.foam;
+    return out;
+}
+
+void ocean_sample_ex(const ocean_sim* sim, float world_x, float world_z, ocean_surface_ex* out)
+{
+    if (out == nullptr) return;
+    std::memset(out, 0, sizeof(*out));
+    out->normal_y = 1.0f;
+    if (sim == nullptr) return;
+    to_surface_ex(to_cpp(sim)->sample_at(world_x, world_z), out);
+}
+
+const float* ocean_get_velocity(const ocean_sim* sim)
+{
+    if (sim == nullptr) return nullptr;
+    return to_cpp(sim)->buffers().velocity;
+}
+
+const char* ocean_simd_level(void) { return "gfx950"; }
+
+const char* ocean_force_simd_level(const char* name)
+{
+    (void)name;
+    return ocean_simd_level();
+}
+
+void ocean_version(uint32_t* major, uint32_t* minor, uint32_t* patch)
+{
+    if (major != nullptr) *major = 1;
+    if (minor != nullptr) *minor = 0;
+    if (patch != nullptr) *patch = 0;
+}
+
+// ---- ocean_amd.h: device buffers, batched queries, host tables ------------
+
+int ocean_amd_get_device_buffers(const ocean_sim* sim, ocean_amd_device_buffers* out)
+{
+    if (sim == nullptr || out == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    const ocean::DeviceBuffers b = to_cpp(sim)->device_buffers();
+    out->displacement = b.displacement;
+    out->normal = b.normal;
+    out->velocity = b.velocity;
+    out->size = b.size;
+    out->patch_length = b.patch_length;
+    out->stream = b.stream;
+    return OCEAN_OK;
+}
+
+void ocean_amd_synchronize(const ocean_sim* sim)
+{
+    if (sim != nullptr) to_cpp(sim)->synchronize();
+}
+
+int ocean_amd_last_error(const ocean_sim* sim) { return sim == nullptr ? 0 : to_cpp(sim)->last_error(); }
+
+int ocean_amd_sample_batch(const ocean_sim* sim, const float* xz, uint32_t count,
+                           ocean_surface_ex* out)
+{
+    static_assert(sizeof(ocean_surface_ex) == sizeof(ocean::Surface));
+    if (sim == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    if (count == 0) return OCEAN_OK;
+    if (xz == nullptr || out == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    return to_cpp(sim)->sample_batch(xz, count, reinterpret_cast<ocean::Surface*>(out))
+               ? OCEAN_OK
+               : OCEAN_ERROR_UNKNOWN;
+}
+
+int ocean_amd_sample_batch_device(const ocean_sim* sim, const float* xz_device, uint32_t count,
+                                  ocean_surface_ex* out_device, float* heights_device)
+{
+    if (sim == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    if (count == 0) return OCEAN_OK;
+    if (xz_device == nullptr || (out_device == nullptr && heights_device == nullptr))
+        return OCEAN_ERROR_INVALID_ARG;
+    return to_cpp(sim)->sample_batch_device(xz_device, count,
+                                            reinterpret_cast<ocean::Surface*>(out_device),
+                                            heights_device)
+               ? OCEAN_OK
+               : OCEAN_ERROR_UNKNOWN;
+}
+
+int ocean_amd_build_spectrum(const ocean_desc* desc, float* h0, float* omega)
+{
+    ocean_desc resolved;
+    if (!copy_desc(desc, resolved) || h0 == nullptr || omega == nullptr)
+        return OCEAN_ERROR_INVALID_ARG;
+    const ocean::OceanDesc d = translate(resolved);
+    if (d.size < 16 || d.size > 4096 || (d.size & (d.size - 1)) != 0)
+        return OCEAN_ERROR_INVALID_ARG;
+    oceanamd::build_spectrum_tables(d, h0, omega, 0, d.size);
+    return OCEAN_OK;
+}
+
+int ocean_amd_build_twiddles(uint32_t n, float* tw)
+{
+    if (tw == nullptr || n < 2 || (n & (n - 1)) != 0) return OCEAN_ERROR_INVALID_ARG;
+    oceanamd::build_twiddles(n, tw);
+    return OCEAN_OK;
+}
+
+}  // extern "C"
+
+// ---- ocean_amd.h: foam and cascades ---------------------------------------
+
+#include "ocean/cascade.hpp"
+#include "ocean/foam.hpp"
+
+#include <vector>
+
+namespace {
+
+ocean::FoamField* to_foam(ocean_amd_foam* f) noexcept
+{
+    return reinterpret_cast<ocean::FoamField*>(f);
+}
+const ocean::FoamField* to_foam(const ocean_amd_foam* f) noexcept
+{
+    return reinterpret_cast<const ocean::FoamField*>(f);
+}
+ocean::CascadeStack* to_stack(ocean_amd_cascade* s) noexcept
+{
+    return reinterpret_cast<ocean::CascadeStack*>(s);
+}
+const ocean::CascadeStack* to_stack(const ocean_amd_cascade* s) noexcept
+{
+    return reinterpret_cast<const ocean::CascadeStack*>(s);
+}
+
+}  // namespace
+
+extern "C" {
+
+void ocean_amd_foam_desc_init(ocean_amd_foam_desc* desc)
+{
+    if (desc == nullptr) return;
+    const ocean::FoamDesc d{};
+    std::memset(desc, 0, sizeof(*desc));
+    desc->struct_size = sizeof(*desc);
+    desc->decay = d.decay;
+    desc->source_gain = d.source_gain;
+    desc->advect_scale = d.advect_scale;
+    desc->wind_drift_x = d.wind_drift_x;
+    desc->wind_drift_z = d.wind_drift_z;
+    desc->fixed_dt = d.fixed_dt;
+    desc->max_substeps = d.max_substeps;
+}
+
+ocean_amd_foam* ocean_amd_foam_create(const ocean_sim* sim, const ocean_amd_foam_desc* desc,
+                                      ocean_status* out_status)
+{
+    auto fail = [&](ocean_status s) -> ocean_amd_foam* {
+        if (out_status != nullptr) *out_status = s;
+        return nullptr;
+    };
+    if (sim == nullptr || desc == nullptr || desc->struct_size == 0 ||
+        desc->struct_size > sizeof(ocean_amd_foam_desc))
+        return fail(OCEAN_ERROR_INVALID_ARG);
+    ocean_amd_foam_desc c;
+    ocean_amd_foam_desc_init(&c);
+    std::memcpy(&c, desc, desc->struct_size);
+    ocean::FoamDesc d;
+    d.decay = c.decay;
+    d.source_gain = c.source_gain;
+    d.advect_scale = c.advect_scale;
+    d.wind_drift_x = c.wind_drift_x;
+    d.wind_drift_z = c.wind_drift_z;
+    d.fixed_dt = c.fixed_dt;
+    d.max_substeps = c.max_substeps;
+    try {
+        auto* f = new ocean::FoamField(*to_cpp(sim), d);
+        if (out_status != nullptr) *out_status = OCEAN_OK;
+        return reinterpret_cast<ocean_amd_foam*>(f);
+    } catch (const std::invalid_argument&) {
+        return fail(OCEAN_ERROR_INVALID_ARG);
+    } catch (const std::bad_alloc&) {
+        return fail(OCEAN_ERROR_OUT_OF_MEMORY);
+    } catch (...) {
+        return fail(OCEAN_ERROR_UNKNOWN);
+    }
+}
+
+void ocean_amd_foam_destroy(ocean_amd_foam* foam) { delete to_foam(foam); }
+void ocean_amd_foam_update(ocean_amd_foam* foam, float dt)
+{
+    if (foam != nullptr) to_foam(foam)->update(dt);
+}
+void ocean_amd_foam_clear(ocean_amd_foam* foam)
+{
+    if (foam != nullptr) to_foam(foam)->clear();
+}
+const float* ocean_amd_foam_data(const ocean_amd_foam* foam)
+{
+    return foam == nullptr ? nullptr : to_foam(foam)->data();
+}
+const float* ocean_amd_foam_device_data(const ocean_amd_foam* foam)
+{
+    return foam == nullptr ? nullptr : to_foam(foam)->device_data();
+}
+uint32_t ocean_amd_foam_last_substeps(const ocean_amd_foam* foam)
+{
+    return foam == nullptr ? 0u : to_foam(foam)->last_substeps();
+}
+float ocean_amd_foam_at(const ocean_amd_foam* foam, float world_x, float world_z)
+{
+    return foam == nullptr ? 0.0f : to_foam(foam)->foam_at(world_x, world_z);
+}
+double ocean_amd_foam_coverage(const ocean_amd_foam* foam)
+{
+    return foam == nullptr ? 0.0 : to_foam(foam)->coverage();
+}
+
+ocean_amd_cascade* ocean_amd_cascade_create(const ocean_desc* levels, uint32_t count,
+                                            ocean_status* out_status)
+{
+    auto fail = [&](ocean_status s) -> ocean_amd_cascade* {
+        if (out_status != nullptr) *out_status = s;
+        return nullptr;
+    };
+    if (levels == nullptr || count == 0) return fail(OCEAN_ERROR_INVALID_ARG);
+    std::vector<ocean::CascadeLevel> descs;
+    descs.reserve(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        ocean_desc r;
+        if (!copy_desc(&levels[i], r)) return fail(OCEAN_ERROR_INVALID_ARG);
+        descs.push_back(translate(r));
+    }
+    try {
+        auto* s = new ocean::CascadeStack(std::span<const ocean::CascadeLevel>(descs));
+        if (out_status != nullptr) *out_status = OCEAN_OK;
+        return reinterpret_cast<ocean_amd_cascade*>(s);
+    } catch (const std::invalid_argument&) {
+        return fail(OCEAN_ERROR_INVALID_ARG);
+    } catch (const std::bad_alloc&) {
+        return fail(OCEAN_ERROR_OUT_OF_MEMORY);
+    } catch (...) {
+        return fail(OCEAN_ERROR_UNKNOWN);
+    }
+}
+
+void ocean_amd_cascade_destroy(ocean_amd_cascade* stack) { delete to_stack(stack); }
+void ocean_amd_cascade_update(ocean_amd_cascade* stack, double time)
+{
+    if (stack != nullptr) to_stack(stack)->update(time);
+}
+uint32_t ocean_amd_cascade_level_count(const ocean_amd_cascade* stack)
+{
+    return stack == nullptr ? 0u : static_cast<uint32_t>(to_stack(stack)->level_count());
+}
+ocean_buffers ocean_amd_cascade_get_buffers(const ocean_amd_cascade* stack, uint32_t level)
+{
+    ocean_buffers out;
+    std::memset(&out, 0, sizeof(out));
+    if (stack == nullptr || level >= to_stack(stack)->level_count()) return out;
+    const ocean::Buffers b = to_stack(stack)->buffers(level);
+    out.displacement = b.displacement;
+    out.normal = b.normal;
+    out.size = b.size;
+    out.patch_length = b.patch_length;
+    return out;
+}
+void ocean_amd_cascade_sample(const ocean_amd_cascade* stack, float world_x, float world_z,
+                              ocean_surface_ex* out)
+{
+    if (out == nullptr) return;
+    std::memset(out, 0, sizeof(*out));
+    out->normal_y = 1.0f;
+    if (stack == nullptr) return;
+    to_surface_ex(to_stack(stack)->sample_at(world_x, world_z), out);
+}
+int ocean_amd_cascade_sample_batch(const ocean_amd_cascade* stack, const float* xz,
+                                   uint32_t count, ocean_surface_ex* out)
+{
+    if (stack == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    if (count == 0) return OCEAN_OK;
+    if (xz == nullptr || out == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    return to_stack(stack)->sample_batch(xz, count, reinterpret_cast<ocean::Surface*>(out))
+               ? OCEAN_OK
+               : OCEAN_ERROR_UNKNOWN;
+}
+
+}  // extern "C"

@@ -1,0 +1,215 @@
+// FoamField on the GPU: reference src/foam.cpp, with each fixed step one
+// kernel (ocean_hip_foam_step) on the ocean's stream and the field resident in
+// device memory.
+#include "ocean/foam.hpp"
+
+#include "ocean_device.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <utility>
+
+namespace ocean {
+namespace {
+
+// reference src/foam.cpp:144-165
+void validate(const FoamDesc& d)
+{
+    if (!(d.decay >= 0.0f)) throw std::invalid_argument("ocean: foam decay must be non-negative");
+    if (!(d.source_gain >= 0.0f))
+        throw std::invalid_argument("ocean: foam source_gain must be non-negative");
+    if (!(d.fixed_dt > 0.0f)) throw std::invalid_argument("ocean: foam fixed_dt must be positive");
+    if (d.max_substeps < 1) throw std::invalid_argument("ocean: foam max_substeps must be >= 1");
+    if (!std::isfinite(d.advect_scale) || !std::isfinite(d.wind_drift_x) ||
+        !std::isfinite(d.wind_drift_z))
+        throw std::invalid_argument("ocean: foam drift parameters must be finite");
+}
+
+}  // namespace
+
+struct FoamField::Impl {
+    const Ocean* ocean = nullptr;
+    FoamDesc desc;
+    std::uint32_t n = 0;
+    std::size_t cells = 0;
+    float cell = 0.0f;
+
+    float* dev = nullptr;       // 2*cells floats: the ping-pong pair
+    float* cur = nullptr;
+    float* next = nullptr;
+    double* scratch = nullptr;  // coverage partials + result
+    float* q_xz = nullptr;      // one-point query staging
+    float* q_out = nullptr;
+    float* host = nullptr;      // pinned mirror of `cur`
+    double* host_cov = nullptr;
+    mutable bool stale = true;
+
+    double accum = 0.0;
+    std::uint32_t substeps_last = 0;
+
+    ~Impl()
+    {
+        if (ocean) hipStreamSynchronize(ocean->device().stream);
+        if (dev) hipFree(dev);
+        if (host) hipHostFree(host);
+        if (host_cov) hipHostFree(host_cov);
+    }
+};
+
+FoamField::FoamField(const Ocean& ocean, const FoamDesc& desc)
+{
+    validate(desc);
+    const DeviceBuffers b = ocean.device_buffers();
+    if (b.velocity == nullptr)
+        throw std::invalid_argument(
+            "ocean: FoamField requires OceanDesc::compute_velocity to be set");
+    impl_ = std::make_unique<Impl>();
+    Impl& m = *impl_;
+    m.ocean = &ocean;
+    m.desc = desc;
+    m.n = b.size;
+    m.cells = static_cast<std::size_t>(b.size) * b.size;
+    m.cell = b.patch_length / static_cast<float>(b.size);
+
+    auto& od = ocean.device();
+    od.make_current();
+    const std::uint32_t blocks = ocean_hip_coverage_blocks(static_cast<std::uint32_t>(m.cells));
+    const std::size_t bytes = m.cells * 2 * sizeof(float) + (blocks + 2) * sizeof(double) + 64;
+    detail::hip_check(hipMalloc(reinterpret_cast<void**>(&m.dev), bytes), "hipMalloc foam");
+    m.cur = m.dev;
+    m.next = m.dev + m.cells;
+    m.scratch = reinterpret_cast<double*>(m.dev + 2 * m.cells);
+    m.q_xz = reinterpret_cast<float*>(m.scratch + blocks + 2);
+    m.q_out = m.q_xz + 2;
+    detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m.host), m.cells * sizeof(float),
+                                    hipHostMallocDefault),
+                      "hipHostMalloc foam");
+    detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m.host_cov), 4 * sizeof(double),
+                                    hipHostMallocDefault),
+                      "hipHostMalloc foam");
+    detail::hip_check(hipMemsetAsync(m.dev, 0, m.cells * 2 * sizeof(float), od.stream), "clear");
+    detail::hip_check(hipStreamSynchronize(od.stream), "clear");
+}
+
+FoamField::~FoamField() = default;
+FoamField::FoamField(FoamField&&) noexcept = default;
+FoamField& FoamField::operator=(FoamField&&) noexcept = default;
+
+// reference src/foam.cpp:218-252 (same accumulator arithmetic in double)
+void FoamField::update(float dt) noexcept
+{
+    Impl& m = *impl_;
+    if (!(dt > 0.0f)) dt = 0.0f;
+    m.accum += static_cast<double>(dt);
+    const double step = static_cast<double>(m.desc.fixed_dt);
+    const double eps = step * 1e-6;
+    std::uint32_t steps = 0;
+    if (m.accum + eps >= step) steps = static_cast<std::uint32_t>((m.accum + eps) / step);
+    if (steps > m.desc.max_substeps) {
+        steps = m.desc.max_substeps;
+        m.accum = 0.0;
+    } else {
+        m.accum -= static_cast<double>(steps) * step;
+        if (m.accum < 0.0) m.accum = 0.0;
+    }
+    m.substeps_last = steps;
+    if (steps == 0) return;
+
+    auto& od = m.ocean->device();
+    if (od.level.vel == nullptr) return;
+    od.make_current();
+    // Per-step constants, computed on the host exactly as foam.cpp:94-100.
+    const float fdt = m.desc.fixed_dt;
+    ocean_hip_foam_params p;
+    p.dt = fdt;
+    p.inv_cell = 1.0f / m.cell;
+    p.decay = std::exp(-m.desc.decay * fdt);
+    p.gain = m.desc.source_gain * fdt;
+    p.scale = m.desc.advect_scale;
+    p.drift_x = m.desc.wind_drift_x;
+    p.drift_z = m.desc.wind_drift_z;
+    for (std::uint32_t s = 0; s < steps; ++s) {
+        od.note(ocean_hip_foam_step(&od.level, m.cur, m.next, &p, od.stream));
+        std::swap(m.cur, m.next);
+    }
+    m.stale = true;
+}
+
+void FoamField::clear() noexcept
+{
+    Impl& m = *impl_;
+    auto& od = m.ocean->device();
+    od.make_current();
+    od.note(hipMemsetAsync(m.dev, 0, m.cells * 2 * sizeof(float), od.stream));
+    m.accum = 0.0;
+    m.stale = true;
+}
+
+const float* FoamField::data() const noexcept
+{
+    Impl& m = *impl_;
+    if (m.stale) {
+        auto& od = m.ocean->device();
+        od.make_current();
+        od.note(hipMemcpyAsync(m.host, m.cur, m.cells * sizeof(float), hipMemcpyDeviceToHost,
+                               od.stream));
+        od.note(hipStreamSynchronize(od.stream));
+        m.stale = false;
+    }
+    return m.host;
+}
+
+const float* FoamField::device_data() const noexcept { return impl_->cur; }
+std::uint32_t FoamField::size() const noexcept { return impl_->n; }
+const FoamDesc& FoamField::desc() const noexcept { return impl_->desc; }
+std::uint32_t FoamField::last_substeps() const noexcept { return impl_->substeps_last; }
+
+// reference src/foam.cpp:262-285: timestep, substep clamp and scheduler are
+// construction-time choices and survive a retune.
+void FoamField::set_desc(const FoamDesc& d)
+{
+    validate(d);
+    Impl& m = *impl_;
+    const FoamDesc keep = m.desc;
+    m.desc = d;
+    m.desc.fixed_dt = keep.fixed_dt;
+    m.desc.max_substeps = keep.max_substeps;
+    m.desc.parallel_for = keep.parallel_for;
+    m.desc.parallel_for_user = keep.parallel_for_user;
+    m.desc.thread_count = keep.thread_count;
+}
+
+float FoamField::foam_at(float world_x, float world_z) const noexcept
+{
+    const Impl& m = *impl_;
+    auto& od = m.ocean->device();
+    od.make_current();
+    float* hx = od.h_q_xz;
+    hx[0] = world_x;
+    hx[1] = world_z;
+    od.note(hipMemcpyAsync(m.q_xz, hx, 8, hipMemcpyHostToDevice, od.stream));
+    od.note(ocean_hip_foam_query(&od.level, m.cur, m.cell, m.q_xz, 1, m.q_out, od.stream));
+    float r = 0.0f;
+    od.note(hipMemcpyAsync(m.host_cov, m.q_out, 4, hipMemcpyDeviceToHost, od.stream));
+    od.note(hipStreamSynchronize(od.stream));
+    std::memcpy(&r, m.host_cov, 4);
+    return r;
+}
+
+double FoamField::coverage() const noexcept
+{
+    const Impl& m = *impl_;
+    auto& od = m.ocean->device();
+    od.make_current();
+    double* out = m.scratch + ocean_hip_coverage_blocks(static_cast<std::uint32_t>(m.cells));
+    od.note(ocean_hip_coverage(m.cur, static_cast<std::uint32_t>(m.cells), m.scratch, out,
+                               od.stream));
+    od.note(hipMemcpyAsync(m.host_cov, out, sizeof(double), hipMemcpyDeviceToHost, od.stream));
+    od.note(hipStreamSynchronize(od.stream));
+    return m.host_cov[0];
+}
+
+}  // namespace ocean

@@ -1,0 +1,300 @@
+// ocean::Ocean on one MI355X.
+//
+// Mirrors the reference's Ocean (src/ocean.cpp:65-289): validate, allocate
+// everything once, evaluate t = 0 before returning; update() is absolute-time
+// and allocation-free. The difference is that the frame is two HIP kernels
+// enqueued on the ocean's stream (ocean_hip_frame) and the outputs live in
+// device memory, copied to a pinned host mirror only when buffers() is asked.
+#include "ocean/ocean.hpp"
+
+#include "ocean_device.hpp"
+#include "spectrum_host.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace ocean {
+namespace {
+
+constexpr double kInvTwoPiD = 0.159154943091895335769;
+
+constexpr bool is_power_of_two(std::uint32_t v) noexcept { return v != 0 && (v & (v - 1)) == 0; }
+
+// reference src/ocean.cpp:28-59 (same checks, same order, same exception type)
+void validate(const OceanDesc& d)
+{
+    if (!is_power_of_two(d.size) || d.size < 16 || d.size > 4096) {
+        throw std::invalid_argument("ocean: size must be a power of two in [16, 4096], got " +
+                                    std::to_string(d.size));
+    }
+    if (!(d.patch_length > 0.0f)) throw std::invalid_argument("ocean: patch_length must be positive");
+    if (!(d.spectrum.gravity > 0.0f)) throw std::invalid_argument("ocean: gravity must be positive");
+    if (!(d.spectrum.fetch > 0.0f)) throw std::invalid_argument("ocean: fetch must be positive");
+    if (!(d.spectrum.wind_speed >= 0.0f))
+        throw std::invalid_argument("ocean: wind_speed must be non-negative");
+    if (!(d.spectrum.peak_enhancement >= 1.0f))
+        throw std::invalid_argument("ocean: peak_enhancement must be >= 1 (1 = Pierson-Moskowitz)");
+    if (!(d.choppiness >= 0.0f)) throw std::invalid_argument("ocean: choppiness must be non-negative");
+}
+
+std::size_t align_up(std::size_t v, std::size_t a) { return (v + a - 1) / a * a; }
+
+// Spectrum rows are independent (one PCG32 stream per row), so the host build
+// splits them across threads with bit-identical results.
+void build_tables_threaded(const OceanDesc& d, float* h0, float* omega)
+{
+    const std::uint32_t n = d.size;
+    unsigned workers = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (static_cast<std::size_t>(n) * n < 16384) workers = 1;
+    std::vector<std::thread> pool;
+    const std::uint32_t chunk = (n + workers - 1) / workers;
+    for (unsigned w = 0; w < workers; ++w) {
+        const std::uint32_t b = w * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        pool.emplace_back([&, b, e] { oceanamd::build_spectrum_tables(d, h0, omega, b, e); });
+    }
+    for (auto& t : pool) t.join();
+}
+
+}  // namespace
+
+namespace detail {
+
+OceanDevice::~OceanDevice()
+{
+    if (stream) hipStreamSynchronize(stream);
+    if (arena) hipFree(arena);
+    if (h_disp) hipHostFree(h_disp);
+    if (h_normal) hipHostFree(h_normal);
+    if (h_vel) hipHostFree(h_vel);
+    if (h_q_xz) hipHostFree(h_q_xz);
+    if (h_q_out) hipHostFree(h_q_out);
+    if (own_stream && stream) hipStreamDestroy(stream);
+}
+
+void OceanDevice::make_current() const noexcept
+{
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != device) hipSetDevice(device);
+}
+
+void OceanDevice::create(const OceanDesc& desc, hipStream_t shared_stream)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        throw std::runtime_error("ocean: no HIP device available");
+    hip_check(hipGetDevice(&device), "hipGetDevice");
+
+    const std::uint32_t n = desc.size;
+    const std::size_t cells = static_cast<std::size_t>(n) * n;
+    fields = desc.compute_velocity ? 6 : 4;
+
+    if (shared_stream) {
+        stream = shared_stream;
+    } else {
+        hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        own_stream = true;
+    }
+
+    // Device arena layout (256-byte aligned pieces).
+    const std::size_t a = 256;
+    std::size_t off = 0;
+    const std::size_t o_h0 = off;     off = align_up(off + cells * 8, a);
+    const std::size_t o_omega = off;  off = align_up(off + cells * 4, a);
+    const std::size_t o_tw = off;     off = align_up(off + std::max<std::size_t>(n / 2, 1) * 8, a);
+    const std::size_t o_spec = off;   off = align_up(off + cells * 8 * fields, a);
+    const std::size_t o_disp = off;   off = align_up(off + cells * 16, a);
+    const std::size_t o_norm = off;   off = align_up(off + cells * 16, a);
+    std::size_t o_vel = 0;
+    if (desc.compute_velocity) { o_vel = off; off = align_up(off + cells * 16, a); }
+    const std::size_t o_qxz = off;    off = align_up(off + std::size_t(kQueryStage) * 8, a);
+    const std::size_t o_qout = off;   off = align_up(off + std::size_t(kQueryStage) * sizeof(ocean_hip_surface), a);
+
+    hip_check(hipMalloc(&arena, off), "hipMalloc");
+    char* base = static_cast<char*>(arena);
+
+    level.n = n;
+    level.patch_length = desc.patch_length;
+    level.choppiness = desc.choppiness;
+    level.foam_threshold = desc.foam_threshold;
+    // reference src/core/evolve.cpp:247-248
+    level.inv_foam_threshold = (desc.foam_threshold > 1e-6f) ? (1.0f / desc.foam_threshold) : 0.0f;
+    level.h0 = reinterpret_cast<const float*>(base + o_h0);
+    level.omega = reinterpret_cast<const float*>(base + o_omega);
+    level.twiddles = reinterpret_cast<const float*>(base + o_tw);
+    level.spec = reinterpret_cast<float*>(base + o_spec);
+    level.disp = reinterpret_cast<float*>(base + o_disp);
+    level.normal = reinterpret_cast<float*>(base + o_norm);
+    level.vel = desc.compute_velocity ? reinterpret_cast<float*>(base + o_vel) : nullptr;
+    q_xz = reinterpret_cast<float*>(base + o_qxz);
+    q_out = reinterpret_cast<ocean_hip_surface*>(base + o_qout);
+
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_disp), cells * 16, hipHostMallocDefault), "hipHostMalloc");
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_normal), cells * 16, hipHostMallocDefault), "hipHostMalloc");
+    if (desc.compute_velocity)
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_vel), cells * 16, hipHostMallocDefault), "hipHostMalloc");
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_q_xz), std::size_t(kQueryStage) * 8, hipHostMallocDefault), "hipHostMalloc");
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_q_out), std::size_t(kQueryStage) * sizeof(ocean_hip_surface), hipHostMallocDefault), "hipHostMalloc");
+
+    // Constant tables: built on the host with the reference's libm chain,
+    // staged through the (not yet used) pinned mirrors, uploaded once.
+    float* h0_host = h_disp;            // cells*8 bytes fit in cells*16
+    float* omega_host = h_normal;
+    build_tables_threaded(desc, h0_host, omega_host);
+    std::vector<float> tw(std::max<std::size_t>(n, 2));
+    oceanamd::build_twiddles(n, tw.data());
+    hip_check(hipMemcpyAsync(base + o_h0, h0_host, cells * 8, hipMemcpyHostToDevice, stream), "upload h0");
+    hip_check(hipMemcpyAsync(base + o_omega, omega_host, cells * 4, hipMemcpyHostToDevice, stream), "upload omega");
+    hip_check(hipMemcpyAsync(base + o_tw, tw.data(), (n / 2) * 8, hipMemcpyHostToDevice, stream), "upload twiddles");
+    hip_check(hipStreamSynchronize(stream), "upload");
+}
+
+void OceanDevice::enqueue_frame(double time) noexcept
+{
+    make_current();
+    // Hoisted exactly as the reference hoists it (src/core/evolve.cpp:33).
+    const double t2 = time * kInvTwoPiD;
+    note(ocean_hip_frame(&level, t2, stream));
+    mirror_stale = true;
+}
+
+void OceanDevice::refresh_mirror() noexcept
+{
+    if (!mirror_stale) return;
+    make_current();
+    const std::size_t bytes = static_cast<std::size_t>(level.n) * level.n * 16;
+    note(hipMemcpyAsync(h_disp, level.disp, bytes, hipMemcpyDeviceToHost, stream));
+    note(hipMemcpyAsync(h_normal, level.normal, bytes, hipMemcpyDeviceToHost, stream));
+    if (level.vel) note(hipMemcpyAsync(h_vel, level.vel, bytes, hipMemcpyDeviceToHost, stream));
+    note(hipStreamSynchronize(stream));
+    mirror_stale = false;
+}
+
+bool OceanDevice::sample_host(const float* xz, std::uint32_t count, Surface* out) noexcept
+{
+    static_assert(sizeof(Surface) == sizeof(ocean_hip_surface));
+    make_current();
+    for (std::uint32_t done = 0; done < count;) {
+        const std::uint32_t m = std::min(kQueryStage, count - done);
+        std::memcpy(h_q_xz, xz + 2 * std::size_t(done), std::size_t(m) * 8);
+        note(hipMemcpyAsync(q_xz, h_q_xz, std::size_t(m) * 8, hipMemcpyHostToDevice, stream));
+        note(ocean_hip_sample(&level, q_xz, m, q_out, nullptr, stream));
+        note(hipMemcpyAsync(h_q_out, q_out, std::size_t(m) * sizeof(ocean_hip_surface),
+                            hipMemcpyDeviceToHost, stream));
+        note(hipStreamSynchronize(stream));
+        std::memcpy(out + done, h_q_out, std::size_t(m) * sizeof(ocean_hip_surface));
+        done += m;
+    }
+    return status == 0;
+}
+
+}  // namespace detail
+
+struct Ocean::Impl {
+    OceanDesc desc;
+    double time = 0.0;
+    detail::OceanDevice dev;
+};
+
+Ocean::Ocean(const OceanDesc& desc) : Ocean(desc, nullptr) {}
+
+Ocean::Ocean(const OceanDesc& desc, void* shared_stream)
+{
+    validate(desc);
+    impl_ = std::make_unique<Impl>();
+    impl_->desc = desc;
+    impl_->dev.create(desc, static_cast<hipStream_t>(shared_stream));
+    // Valid, fully evaluated state at t = 0 before returning (reference
+    // src/ocean.cpp:271-278).
+    impl_->dev.enqueue_frame(0.0);
+    impl_->time = 0.0;
+    detail::hip_check(hipStreamSynchronize(impl_->dev.stream), "initial frame");
+    if (impl_->dev.status != 0)
+        throw std::runtime_error("ocean: initial frame failed: " +
+                                 std::string(hipGetErrorString(hipError_t(impl_->dev.status))));
+}
+
+Ocean::~Ocean() = default;
+Ocean::Ocean(Ocean&&) noexcept = default;
+Ocean& Ocean::operator=(Ocean&&) noexcept = default;
+
+void Ocean::update(double time) noexcept
+{
+    impl_->dev.enqueue_frame(time);
+    impl_->time = time;
+}
+
+Buffers Ocean::buffers() const noexcept
+{
+    impl_->dev.refresh_mirror();
+    Buffers b;
+    b.displacement = impl_->dev.h_disp;
+    b.normal = impl_->dev.h_normal;
+    b.velocity = impl_->dev.h_vel;
+    b.size = impl_->desc.size;
+    b.patch_length = impl_->desc.patch_length;
+    return b;
+}
+
+DeviceBuffers Ocean::device_buffers() const noexcept
+{
+    DeviceBuffers b;
+    b.displacement = impl_->dev.level.disp;
+    b.normal = impl_->dev.level.normal;
+    b.velocity = impl_->dev.level.vel;
+    b.size = impl_->desc.size;
+    b.patch_length = impl_->desc.patch_length;
+    b.stream = impl_->dev.stream;
+    return b;
+}
+
+float Ocean::height_at(float world_x, float world_z) const noexcept
+{
+    return sample_at(world_x, world_z).height;
+}
+
+Surface Ocean::sample_at(float world_x, float world_z) const noexcept
+{
+    const float xz[2] = {world_x, world_z};
+    Surface s;
+    if (!impl_->dev.sample_host(xz, 1, &s)) return Surface{};
+    return s;
+}
+
+bool Ocean::sample_batch(const float* xz, std::uint32_t count, Surface* out) const noexcept
+{
+    if (count == 0) return true;
+    if (xz == nullptr || out == nullptr) return false;
+    return impl_->dev.sample_host(xz, count, out);
+}
+
+bool Ocean::sample_batch_device(const float* xz_dev, std::uint32_t count, Surface* out_dev,
+                                float* heights_dev) const noexcept
+{
+    if (count == 0) return true;
+    if (xz_dev == nullptr || (out_dev == nullptr && heights_dev == nullptr)) return false;
+    impl_->dev.make_current();
+    impl_->dev.note(ocean_hip_sample(&impl_->dev.level, xz_dev, count,
+                                     reinterpret_cast<ocean_hip_surface*>(out_dev), heights_dev,
+                                     impl_->dev.stream));
+    return impl_->dev.status == 0;
+}
+
+void Ocean::synchronize() const noexcept
+{
+    impl_->dev.make_current();
+    impl_->dev.note(hipStreamSynchronize(impl_->dev.stream));
+}
+
+int Ocean::last_error() const noexcept { return impl_->dev.status; }
+const OceanDesc& Ocean::desc() const noexcept { return impl_->desc; }
+double Ocean::time() const noexcept { return impl_->time; }
+detail::OceanDevice& Ocean::device() const noexcept { return impl_->dev; }
+
+}  // namespace ocean

@@ -1,0 +1,66 @@
+// Device state of one ocean level, shared by Ocean, FoamField, CascadeStack
+// and the batched paths. Internal header.
+#pragma once
+
+#include "hip_launch.h"
+#include "ocean/ocean.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace ocean::detail {
+
+// Raised inside constructors; mapped to OCEAN_ERROR_* at the C boundary.
+inline void hip_check(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return;
+    if (e == hipErrorOutOfMemory) throw std::bad_alloc();
+    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Largest batch one host-array query call stages at a time.
+inline constexpr std::uint32_t kQueryStage = 65536;
+
+struct OceanDevice {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    ocean_hip_level level{};
+    int fields = 4;            // complex fields per frame: 4, or 6 with velocity
+
+    void* arena = nullptr;     // one hipMalloc holds every device buffer
+    float* q_xz = nullptr;     // device query staging
+    ocean_hip_surface* q_out = nullptr;
+
+    float* h_disp = nullptr;   // pinned host mirrors (hipHostMalloc)
+    float* h_normal = nullptr;
+    float* h_vel = nullptr;
+    float* h_q_xz = nullptr;
+    ocean_hip_surface* h_q_out = nullptr;
+
+    bool mirror_stale = true;
+    int status = 0;            // first HIP error seen (sticky)
+
+    OceanDevice() = default;
+    OceanDevice(const OceanDevice&) = delete;
+    OceanDevice& operator=(const OceanDevice&) = delete;
+    ~OceanDevice();
+
+    // Allocates and uploads; throws on failure (state is released by the dtor).
+    void create(const OceanDesc& desc, hipStream_t shared_stream);
+
+    void note(hipError_t e) noexcept
+    {
+        if (e != hipSuccess && status == 0) status = static_cast<int>(e);
+    }
+    void make_current() const noexcept;
+    void enqueue_frame(double time) noexcept;
+    // Copies the device outputs into the pinned mirrors if they are stale.
+    void refresh_mirror() noexcept;
+    bool sample_host(const float* xz, std::uint32_t count, Surface* out) noexcept;
+};
+
+}  // namespace ocean::detail

@@ -1,0 +1,63 @@
+// CPU emulation of the GPU team FFT (oceanlib_amd/csrc/fft_team.hpp): the
+// same fft_team<N> code runs with one host thread per lane and a
+// std::barrier standing in for the block barrier. Used by
+// tests/test_fft_team_emulation.py to compare the kernel's butterfly graph
+// with the oracle's FFT bit for bit, without a GPU.
+#include "../../oceanlib_amd/csrc/fft_team.hpp"
+
+#include <barrier>
+#include <cstdint>
+#include <thread>
+#include <vector>
+
+using namespace oceanamd;
+
+template <int N>
+static void run(float* re, float* im, const float* twr, const float* twi)
+{
+    constexpr int T = FftShape<N>::T;
+    std::vector<float2> tw(N / 2);
+    for (int j = 0; j < N / 2; ++j) tw[j] = make_float2(twr[j], twi[j]);
+    std::vector<float2> xch(lds_span<N>());
+    std::barrier bar(T);
+    std::vector<std::thread> lanes;
+    std::vector<float> ore(N), oim(N);
+    for (int J = 0; J < T; ++J) {
+        lanes.emplace_back([&, J] {
+            float r[8], i[8];
+            uint32_t idx[8];
+            for (int a = 0; a < 8; ++a) {
+                r[a] = re[J + a * T];
+                i[a] = im[J + a * T];
+            }
+            fft_team<N>(r, i, idx, uint32_t(J), xch.data(), tw.data(),
+                        [&] { bar.arrive_and_wait(); });
+            for (int a = 0; a < 8; ++a) {
+                ore[idx[a]] = r[a];
+                oim[idx[a]] = i[a];
+            }
+        });
+    }
+    for (auto& t : lanes) t.join();
+    for (int j = 0; j < N; ++j) {
+        re[j] = ore[j];
+        im[j] = oim[j];
+    }
+}
+
+extern "C" int emul_fft_team(uint32_t n, float* re, float* im, const float* twr,
+                             const float* twi)
+{
+    switch (n) {
+        case 16: run<16>(re, im, twr, twi); return 0;
+        case 32: run<32>(re, im, twr, twi); return 0;
+        case 64: run<64>(re, im, twr, twi); return 0;
+        case 128: run<128>(re, im, twr, twi); return 0;
+        case 256: run<256>(re, im, twr, twi); return 0;
+        case 512: run<512>(re, im, twr, twi); return 0;
+        case 1024: run<1024>(re, im, twr, twi); return 0;
+        case 2048: run<2048>(re, im, twr, twi); return 0;
+        case 4096: run<4096>(re, im, twr, twi); return 0;
+    }
+    return -1;
+}

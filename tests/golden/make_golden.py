@@ -1,0 +1,166 @@
+"""Writes the golden fixtures in tests/golden/ from the reference library.
+
+The reference is compiled from its own, unmodified sources by oracle/Makefile
+(`make -C oracle ref`, needs the reference tree at /root/reference) into
+oracle/_ref/liboceanref.so; this script drives it through ctypes and stores
+only data: raw small buffers (.npz, no pickles) and 64-bit digests of larger
+ones (JSON). The fixtures pin the oracle (tests/test_oracle_*.py) and, on a
+GPU, the HIP path (tests/test_gpu_*.py) to the reference's own output.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import json
+import os
+import platform
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oceanlib_amd.capi import OceanSurfaceEx, default_desc  # noqa: E402
+from oracle_ffi import RefOcean, FoamParams, default_foam_params, fptr, ref  # noqa: E402
+from oceanlib_amd.workloads import (CONFIG1, CONFIG2, CONFIG3_LEVELS, SMALL_CASES,  # noqa: E402
+                                    desc_from, query_points)
+
+
+def digest(a) -> str:
+    """First 16 hex digits of SHA-256 over the raw little-endian bytes."""
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()[:16]
+
+
+def glibc() -> str:
+    return " ".join(platform.libc_ver())
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def small_buffers(r):
+    out = {}
+    for name, kw, times in SMALL_CASES:
+        o = RefOcean(desc_from(kw))
+        for t in times:
+            o.update(t)
+            key = f"{name}_t{t:g}"
+            out[key + "_disp"] = o.displacement()
+            out[key + "_normal"] = o.normal()
+            v = o.velocity()
+            if v is not None:
+                out[key + "_vel"] = v
+        o.close()
+    # spectrum tables of the first small case
+    d = desc_from(SMALL_CASES[0][1])
+    n = d.size
+    names = ("h0_re", "h0_im", "h0c_re", "h0c_im", "omega", "kx", "kz", "k_inv")
+    arrs = {k: np.zeros(n * n, np.float32) for k in names}
+    r.oref_spectrum(C.byref(d), *[fptr(arrs[k]) for k in names])
+    for k, v in arrs.items():
+        out[f"{SMALL_CASES[0][0]}_spectrum_{k}"] = v
+    return out
+
+
+def run_config1():
+    o = RefOcean(desc_from(CONFIG1))
+    frames = []
+    for k in range(0, 601):
+        o.update(k / 60.0)
+        frames.append([digest(o.displacement()), digest(o.normal())])
+    o.close()
+    return frames
+
+
+def run_config2(r, frames=600):
+    """Single heavy-chop patch with velocity + persistent foam + 4096 queries/frame."""
+    desc = desc_from(CONFIG2)
+    o = RefOcean(desc)
+    fp = default_foam_params()
+    foam = r.oref_foam_create(o.h, C.byref(fp))
+    assert foam
+    n = desc.size
+    xz = query_points(frames, 4096, desc.patch_length)
+    out = []
+    surf = OceanSurfaceEx()
+    for k in range(frames):
+        t = k / 60.0
+        o.update(t)
+        r.oref_foam_update(foam, np.float32(1.0 / 60.0))
+        fbuf = np.ctypeslib.as_array(r.oref_foam_data(foam), shape=(n * n,)).copy()
+        q = np.zeros((4096, 10), np.float32)
+        for i in range(4096):
+            r.oref_sample_ex(o.h, float(xz[k, i, 0]), float(xz[k, i, 1]), C.byref(surf))
+            q[i] = np.frombuffer(bytes(surf), np.float32)
+        out.append([digest(o.displacement()), digest(o.normal()), digest(o.velocity()),
+                    digest(fbuf), digest(q)])
+    r.oref_foam_destroy(foam)
+    o.close()
+    return out
+
+
+def run_config3_cascades(r, frames=(0, 1, 2, 60, 600, 1199)):
+    levels = [desc_from(kw) for kw in CONFIG3_LEVELS]
+    arr = (type(levels[0]) * len(levels))(*levels)
+    h = r.oref_cascade_create(arr, len(levels))
+    assert h
+    out = {}
+    surf = OceanSurfaceEx()
+    pts = query_points(1, 512, 800.0)[0]
+    for k in frames:
+        t = k / 60.0
+        r.oref_cascade_update(h, t)
+        rows = []
+        for li, d in enumerate(levels):
+            n = d.size
+            disp = np.ctypeslib.as_array(r.oref_cascade_displacement(h, li), shape=(4 * n * n,))
+            nrm = np.ctypeslib.as_array(r.oref_cascade_normal(h, li), shape=(4 * n * n,))
+            rows.append([digest(disp), digest(nrm)])
+        q = np.zeros((len(pts), 10), np.float32)
+        for i, (x, z) in enumerate(pts):
+            r.oref_cascade_sample(h, float(x), float(z), C.byref(surf))
+            q[i] = np.frombuffer(bytes(surf), np.float32)
+        out[str(k)] = {"levels": rows, "queries": digest(q)}
+    r.oref_cascade_destroy(h)
+    return out
+
+
+def main():
+    r = ref()
+    if r is None:
+        raise SystemExit("oracle/_ref/liboceanref.so missing: run `make -C oracle ref` first")
+    small = small_buffers(r)
+    np.savez_compressed(os.path.join(HERE, "small_buffers.npz"), **small)
+    meta = {
+        "generator": "tests/golden/make_golden.py",
+        "reference_build": "oracle/Makefile target ref (g++ -O3 -ffp-contract=off)",
+        "glibc": glibc(),
+        "cpu": cpu_model(),
+        "digest": "sha256 of raw float32 bytes, first 16 hex digits",
+    }
+    print("config1 ...", flush=True)
+    c1 = run_config1()
+    print("config2 ...", flush=True)
+    c2 = run_config2(r)
+    print("config3 cascades ...", flush=True)
+    c3 = run_config3_cascades(r)
+    with open(os.path.join(HERE, "digests.json"), "w") as f:
+        json.dump({"meta": meta, "config1": c1, "config2": c2, "config3_cascade": c3}, f,
+                  indent=0)
+    print("wrote", os.path.join(HERE, "small_buffers.npz"), os.path.join(HERE, "digests.json"))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,177 @@
+"""GPU parity: the HIP frame path through the C ABI vs the CPU oracle, bit for bit.
+
+Mirrors the reference's determinism contract (every SIMD level memcmp-equal to
+scalar, reference tests/test_simd.cpp:113-166; C == C++ bit for bit,
+tests/test_c_api.cpp:51-95): here the HIP kernels must equal the oracle's
+restatement of the reference's scalar path, word for word.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oceanlib_amd import capi
+from oracle_ffi import OracleOcean, assert_bitexact, fptr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return capi.load()
+
+
+class GpuOcean:
+    def __init__(self, lib, desc):
+        self.lib = lib
+        st = C.c_int(-1)
+        self.h = lib.ocean_create(C.byref(desc), C.byref(st))
+        assert self.h, f"ocean_create status {st.value}"
+        self.n = desc.size
+        self.desc = desc
+
+    def update(self, t):
+        self.lib.ocean_update(self.h, t)
+
+    def buffers(self):
+        b = self.lib.ocean_get_buffers(self.h)
+        n4 = 4 * self.n * self.n
+        d = np.ctypeslib.as_array(b.displacement, shape=(n4,)).copy()
+        nr = np.ctypeslib.as_array(b.normal, shape=(n4,)).copy()
+        vp = self.lib.ocean_get_velocity(self.h)
+        v = np.ctypeslib.as_array(vp, shape=(n4,)).copy() if vp else None
+        return d, nr, v
+
+    def sample_batch(self, xz):
+        xz = np.ascontiguousarray(xz, dtype=np.float32)
+        out = (capi.OceanSurfaceEx * len(xz))()
+        rc = self.lib.ocean_amd_sample_batch(self.h, fptr(xz.reshape(-1)), len(xz), out)
+        assert rc == 0
+        return out
+
+    def close(self):
+        if self.h:
+            assert self.lib.ocean_amd_last_error(self.h) == 0
+            self.lib.ocean_destroy(self.h)
+            self.h = None
+
+
+def check_frame(g, o, t, what):
+    g.update(t)
+    o.update(t)
+    d, nr, v = g.buffers()
+    assert_bitexact(d, o.displacement(), f"{what} t={t} displacement")
+    assert_bitexact(nr, o.normal(), f"{what} t={t} normal")
+    ov = o.velocity()
+    if ov is not None:
+        assert v is not None
+        assert_bitexact(v, ov, f"{what} t={t} velocity")
+
+
+C1 = dict(size=256, patch_length=200.0, wind_speed=12.0, seed=1337, choppiness=1.0)
+C2 = dict(size=512, patch_length=250.0, wind_speed=22.0, choppiness=1.3, compute_velocity=1,
+          water_depth=50.0, seed=7)
+
+
+def test_config1_plain_patch(lib):
+    """N=256, 200 m, 12 m/s, seed 1337: t = 0..10 s at 1/60 s (sampled) plus far seeks."""
+    d = capi.default_desc(lib, **C1)
+    g, o = GpuOcean(lib, d), OracleOcean(d)
+    try:
+        d0, n0, _ = g.buffers()   # the state at creation is t = 0
+        assert_bitexact(d0, o.displacement(), "creation displacement")
+        assert_bitexact(n0, o.normal(), "creation normal")
+        for k in list(range(0, 601, 37)) + [600]:
+            check_frame(g, o, k / 60.0, "C1")
+        for t in (3600.0, 36000.0, 1e-7, 123.456):
+            check_frame(g, o, t, "C1 seek")
+    finally:
+        g.close()
+
+
+def test_config2_heavy_chop_velocity_depth(lib):
+    """N=512, 250 m, 22 m/s, chop 1.3, velocity on, depth 50 m, seed 7."""
+    d = capi.default_desc(lib, **C2)
+    g, o = GpuOcean(lib, d), OracleOcean(d)
+    try:
+        for k in (0, 1, 2, 59, 300, 599):
+            check_frame(g, o, k / 60.0, "C2")
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 128, 1024])
+def test_every_grid_size(lib, n):
+    d = capi.default_desc(lib, size=n, patch_length=float(n) * 0.7, wind_speed=9.0,
+                          seed=1000 + n, compute_velocity=int(n % 64 == 0))
+    g, o = GpuOcean(lib, d), OracleOcean(d)
+    try:
+        for t in (0.5, 17.25):
+            check_frame(g, o, t, f"N={n}")
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("n", [2048, 4096])
+def test_largest_grid_sizes(lib, n):
+    """The reference's upper bound (src/ocean.cpp:33-37)."""
+    d = capi.default_desc(lib, size=n, patch_length=1000.0, wind_speed=14.0, seed=5)
+    g, o = GpuOcean(lib, d), OracleOcean(d)
+    try:
+        check_frame(g, o, 2.0, f"N={n}")
+    finally:
+        g.close()
+
+
+def test_special_cases(lib):
+    """Zero wind -> bit-flat sea; chop 0 -> J = 1, zero horizontal displacement
+    (reference tests/test_api.cpp:88-106, tests/test_simulation.cpp:275-296)."""
+    d = capi.default_desc(lib, size=64, wind_speed=0.0)
+    g = GpuOcean(lib, d)
+    try:
+        g.update(5.0)
+        disp, nrm, _ = g.buffers()
+        assert not np.any(disp.reshape(-1, 4)[:, :3])
+        assert np.all(nrm.reshape(-1, 4)[:, 1] == 1.0)
+    finally:
+        g.close()
+    d = capi.default_desc(lib, size=64, choppiness=0.0, wind_speed=15.0)
+    g, o = GpuOcean(lib, d), OracleOcean(d)
+    try:
+        check_frame(g, o, 3.0, "chop0")
+        disp, nrm, _ = g.buffers()
+        assert np.all(disp.reshape(-1, 4)[:, 0] == 0.0)
+        assert np.all(disp.reshape(-1, 4)[:, 2] == 0.0)
+        assert np.all(nrm.reshape(-1, 4)[:, 3] == 1.0)
+        assert np.all(disp.reshape(-1, 4)[:, 3] == 0.0)
+    finally:
+        g.close()
+
+
+def test_queries_bitexact(lib):
+    """Batched and single queries vs the oracle's sample_at (src/ocean.cpp:375-452)."""
+    d = capi.default_desc(lib, **C2)
+    g, o = GpuOcean(lib, d), OracleOcean(d)
+    try:
+        t = 7.5
+        g.update(t)
+        o.update(t)
+        rng = np.random.default_rng(3)
+        xz = rng.uniform(0.0, 250.0, size=(4096, 2)).astype(np.float32)
+        xz[:8] = [[0, 0], [-0.0, 250.0], [-1e3, 3e3], [249.999, 0.001], [125, 125],
+                  [-250, -250], [1e5, -1e5], [0.5, -0.5]]
+        out = g.sample_batch(xz)
+        for q in range(len(xz)):
+            want = o.sample(float(xz[q, 0]), float(xz[q, 1]))
+            assert bytes(out[q]) == bytes(want), (q, xz[q])
+        # single-query C entry points
+        for q in range(0, 64, 7):
+            x, z = float(xz[q, 0]), float(xz[q, 1])
+            want = o.sample(x, z)
+            h = lib.ocean_height_at(g.h, x, z)
+            assert np.float32(h).tobytes() == np.float32(want.height).tobytes()
+            s = capi.OceanSurfaceEx()
+            lib.ocean_sample_ex(g.h, x, z, C.byref(s))
+            assert bytes(s) == bytes(want)
+    finally:
+        g.close()

@@ -1,0 +1,262 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-frame wave synthesis on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (one "step" = one frame of it): the single heavy-chop patch -
+N = 512, 250 m, 22 m/s, choppiness 1.3, orbital velocity on, finite depth
+50 m, seed 7, t advancing 1/60 s per frame; each frame is
+    ocean_update(t) -> FoamField step (dt = 1/60) -> 4096 height_at queries,
+all enqueued through the C ABI on the ocean's stream with every input already
+resident in device memory. The timed region is K frames bracketed by a
+device-wide synchronize on both sides.
+
+Prints one JSON line: value = frames (updates) per second, plus
+  roofline     : the dominant kernel's algorithmic bytes per launch over its
+                 mean device time (HIP events on the ocean's stream);
+  cpu_baseline : the reference library itself (oracle/_ref, compiled from the
+                 reference's sources, AVX2 kernels + its thread pool) running
+                 a bounded sample of the same frames on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "updates/s at N=512 (1 and batched) vs CPU; height_at queries/s @0.01mm"
+HBM_PEAK_GBS = 8000.0
+QUERIES_PER_FRAME = 4096
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(n: int, fields: int, velocity: bool):
+    """Per-launch algorithmic HBM bytes of the two frame kernels.
+
+    row pass : reads h0 (8 B) + omega (4 B) per cell once, writes the F
+               complex fields (8F B per cell);
+    col pass : reads the F fields (8F B), writes displacement + normal
+               (+ velocity) RGBA32F (16 B each per cell).
+    """
+    cells = n * n
+    row = cells * (12 + 8 * fields)
+    col = cells * (8 * fields + 16 * (3 if velocity else 2))
+    return row, col
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE), scaled by the copy-kernel calibration, if recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+        k = rec["kernels"][kernel]
+        return (k["fetch_kib"] + k["write_kib"]) * 1024.0 * rec["copy_factor"]
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
+
+
+def cpu_baseline(desc, frames_budget_s: float = 12.0):
+    """The reference library (oracle/_ref) on this host, same frames."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    cores = max(1, min(16, os.cpu_count() or 1))
+    kind = "reference"
+    try:
+        from oracle_ffi import ref, default_foam_params
+        r = ref()
+    except OSError:
+        r = None
+    if r is None:
+        return None
+    r.oref_sample_batch.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_void_p]
+    r.oref_height_batch.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32,
+                                    C.POINTER(C.c_float)]
+    d = type(desc).from_buffer_copy(desc)
+    d.thread_count = cores
+    st = C.c_int(-1)
+    h = r.oref_create(C.byref(d), C.byref(st))
+    fp = default_foam_params()
+    foam = r.oref_foam_create(h, C.byref(fp))
+    from oceanlib_amd.workloads import query_points
+    xz = query_points(64, QUERIES_PER_FRAME, d.patch_length)
+    heights = np.zeros(QUERIES_PER_FRAME, np.float32)
+    fptr = C.POINTER(C.c_float)
+    for k in range(5):  # warmup
+        r.oref_update(h, k / 60.0)
+        r.oref_foam_update(foam, np.float32(1 / 60))
+        r.oref_height_batch(h, xz[k % 64].ctypes.data_as(fptr), QUERIES_PER_FRAME,
+                            heights.ctypes.data_as(fptr))
+    frames = 0
+    t0 = time.perf_counter()
+    while True:
+        k = frames + 5
+        r.oref_update(h, k / 60.0)
+        r.oref_foam_update(foam, np.float32(1 / 60))
+        r.oref_height_batch(h, xz[k % 64].ctypes.data_as(fptr), QUERIES_PER_FRAME,
+                            heights.ctypes.data_as(fptr))
+        frames += 1
+        el = time.perf_counter() - t0
+        if el >= frames_budget_s or frames >= 5000:
+            break
+    r.oref_foam_destroy(foam)
+    r.oref_destroy(h)
+    return {"value": frames / el, "unit": "updates/s", "cores": cores, "kind": kind,
+            "sample": f"{frames} frames of the same workload ({el:.1f} s): reference "
+                      f"Ocean::update with velocity, FoamField::update and "
+                      f"{QUERIES_PER_FRAME} height_at calls per frame, thread_count={cores}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="run only warmup + timed frames (for rocprofv3 runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not "
+              "supported (independent oceans would shard by ocean across GPUs with no "
+              "communication; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from oceanlib_amd import capi
+    from oceanlib_amd.workloads import CONFIG2, desc_from, query_points
+
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a HIP device")
+    lib = capi.load()
+    desc = desc_from(CONFIG2)
+    n = desc.size
+    st = C.c_int(-1)
+    sim = lib.ocean_create(C.byref(desc), C.byref(st))
+    if not sim:
+        raise SystemExit(f"ocean_create failed: {st.value}")
+    fd = capi.FoamDesc()
+    lib.ocean_amd_foam_desc_init(C.byref(fd))
+    foam = lib.ocean_amd_foam_create(sim, C.byref(fd), C.byref(st))
+    if not foam:
+        raise SystemExit(f"foam create failed: {st.value}")
+
+    total = args.warmup + args.steps
+    frames_xz = 64
+    xz_host = query_points(frames_xz, QUERIES_PER_FRAME, desc.patch_length)
+    xz_dev = torch.from_numpy(xz_host.reshape(frames_xz, -1)).cuda()
+    heights = torch.empty((frames_xz, QUERIES_PER_FRAME), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+
+    def frame(k):
+        lib.ocean_update(sim, k / 60.0)
+        lib.ocean_amd_foam_update(foam, np.float32(1.0 / 60.0))
+        j = k % frames_xz
+        rc = lib.ocean_amd_sample_batch_device(sim, C.c_void_p(xz_dev[j].data_ptr()),
+                                               QUERIES_PER_FRAME, None,
+                                               C.c_void_p(heights[j].data_ptr()))
+        if rc != 0:
+            raise SystemExit(f"query failed rc={rc}")
+
+    log(f"bench: warmup {args.warmup} frames")
+    for k in range(args.warmup):
+        frame(k)
+    lib.ocean_amd_synchronize(sim)
+    torch.cuda.synchronize()
+    log(f"bench: timing {args.steps} frames")
+    t0 = time.perf_counter()
+    for k in range(args.warmup, total):
+        frame(k)
+    lib.ocean_amd_synchronize(sim)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    if lib.ocean_amd_last_error(sim) != 0:
+        raise SystemExit(f"device error {lib.ocean_amd_last_error(sim)}")
+    ms_per_step = el * 1000.0 / args.steps
+    value = args.steps / el
+    if args.profile_only:
+        print(json.dumps({"profile_only": True, "ms_per_step": ms_per_step}), flush=True)
+        return
+
+    # Per-kernel device time (HIP events on the ocean's stream).
+    row_ms, col_ms = C.c_float(), C.c_float()
+    assert lib.ocean_amd_profile_frame(sim, 100.0, 1.0 / 60.0, 200, C.byref(row_ms),
+                                       C.byref(col_ms)) == 0
+    fields = 6
+    row_b, col_b = algorithmic_bytes(n, fields, True)
+    kernels = {
+        "row_pass": {"ms": row_ms.value, "alg_bytes": row_b,
+                     "gbs": row_b / (row_ms.value * 1e-3) / 1e9},
+        "col_pass": {"ms": col_ms.value, "alg_bytes": col_b,
+                     "gbs": col_b / (col_ms.value * 1e-3) / 1e9},
+    }
+    dom = max(kernels, key=lambda k: kernels[k]["ms"])
+    achieved = kernels[dom]["gbs"]
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": traffic_from_profiles(dom), "kernel": dom,
+                "alg_bytes_per_launch": kernels[dom]["alg_bytes"]}
+
+    # height_at throughput: 1M queries per launch on the same sea.
+    nq = 1 << 20
+    rng = np.random.default_rng(11)
+    big = torch.from_numpy(rng.uniform(0, desc.patch_length, (nq, 2)).astype(np.float32)).cuda()
+    hq = torch.empty(nq, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(3):
+        lib.ocean_amd_sample_batch_device(sim, C.c_void_p(big.data_ptr()), nq, None,
+                                          C.c_void_p(hq.data_ptr()))
+    lib.ocean_amd_synchronize(sim)
+    reps = 20
+    tq = time.perf_counter()
+    for _ in range(reps):
+        lib.ocean_amd_sample_batch_device(sim, C.c_void_p(big.data_ptr()), nq, None,
+                                          C.c_void_p(hq.data_ptr()))
+    lib.ocean_amd_synchronize(sim)
+    qps = reps * nq / (time.perf_counter() - tq)
+
+    cpu = None if args.no_cpu else cpu_baseline(desc)
+
+    lib.ocean_amd_foam_destroy(foam)
+    lib.ocean_destroy(sim)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "single heavy-chop patch: N=512, 250 m, 22 m/s, chop 1.3, "
+                               "velocity + persistent advected foam, depth 50 m, seed 7, "
+                               "dt 1/60 s, 4096 height_at queries per frame",
+                   "size": n, "patch_length": 250.0, "queries_per_frame": QUERIES_PER_FRAME,
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+        "update_device_ms": round(row_ms.value + col_ms.value, 5),
+        "height_at_queries_per_s": round(qps, 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

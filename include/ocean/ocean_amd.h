@@ -44,6 +44,14 @@ void ocean_amd_synchronize(const ocean_sim* sim);
  * device failures are latched here instead of thrown. */
 int  ocean_amd_last_error(const ocean_sim* sim);
 
+/* Device time of one frame's two kernels, measured with hipEvents on the
+ * ocean's own stream over `iters` frames at times t0 + i*dt: mean ms of the
+ * row pass (evolve + row FFT) and of the column pass (column FFT + pack).
+ * Leaves the ocean at the last frame timed. Diagnostic; no reference
+ * counterpart (the reference's stage timing is bench/bench_update.cpp:211-252). */
+int ocean_amd_profile_frame(ocean_sim* sim, double t0, double dt, uint32_t iters,
+                            float* row_ms, float* col_ms);
+
 /* ---- batched queries ----------------------------------------------------
  * Batched form of ocean_sample_ex / ocean_height_at (reference
  * src/c_api.cpp:199-202,330-350; src/ocean.cpp:375-452): result q is exactly

@@ -168,6 +168,7 @@ AMD_SIGNATURES = {
     "ocean_amd_get_device_buffers": (C.c_int, [_P, C.c_void_p]),
     "ocean_amd_synchronize": (None, [_P]),
     "ocean_amd_last_error": (C.c_int, [_P]),
+    "ocean_amd_profile_frame": (C.c_int, [_P, C.c_double, C.c_double, C.c_uint32, _F, _F]),
     "ocean_amd_sample_batch": (C.c_int, [_P, _F, C.c_uint32, C.POINTER(OceanSurfaceEx)]),
     "ocean_amd_sample_batch_device": (C.c_int, [_P, C.c_void_p, C.c_uint32, C.c_void_p,
                                                 C.c_void_p]),

@@ -11,6 +11,8 @@
 #include "ocean_device.hpp"
 #include "spectrum_host.hpp"
 
+#include <hip/hip_runtime_api.h>
+
 #include <cstring>
 #include <new>
 #include <stdexcept>
@@ -219,6 +221,41 @@ int ocean_amd_get_device_buffers(const ocean_sim* sim, ocean_amd_device_buffers*
     out->patch_length = b.patch_length;
     out->stream = b.stream;
     return OCEAN_OK;
+}
+
+int ocean_amd_profile_frame(ocean_sim* sim, double t0, double dt, uint32_t iters, float* row_ms,
+                            float* col_ms)
+{
+    if (sim == nullptr || iters == 0) return OCEAN_ERROR_INVALID_ARG;
+    ocean::Ocean& o = *to_cpp(sim);
+    auto& dev = o.device();
+    dev.make_current();
+    hipEvent_t ev[3];
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) return OCEAN_ERROR_UNKNOWN;
+    double row = 0.0, col = 0.0;
+    int rc = OCEAN_OK;
+    for (uint32_t i = 0; i < iters && rc == OCEAN_OK; ++i) {
+        const double t = t0 + dt * i;
+        const double t2 = t * 0.159154943091895335769;
+        hipEventRecord(ev[0], dev.stream);
+        dev.note(ocean_hip_frame_passes(&dev.level, t2, dev.stream, 1u));
+        hipEventRecord(ev[1], dev.stream);
+        dev.note(ocean_hip_frame_passes(&dev.level, t2, dev.stream, 2u));
+        hipEventRecord(ev[2], dev.stream);
+        if (hipEventSynchronize(ev[2]) != hipSuccess) rc = OCEAN_ERROR_UNKNOWN;
+        float a = 0.0f, b = 0.0f;
+        hipEventElapsedTime(&a, ev[0], ev[1]);
+        hipEventElapsedTime(&b, ev[1], ev[2]);
+        row += a;
+        col += b;
+        o.update(t);  // keeps time() and the host-mirror state consistent
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    o.synchronize();
+    if (row_ms) *row_ms = static_cast<float>(row / iters);
+    if (col_ms) *col_ms = static_cast<float>(col / iters);
+    return dev.status == 0 ? rc : OCEAN_ERROR_UNKNOWN;
 }
 
 void ocean_amd_synchronize(const ocean_sim* sim)

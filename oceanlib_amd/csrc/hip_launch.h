@@ -41,6 +41,10 @@ typedef struct ocean_hip_foam_params {
 /* Enqueue one frame (row pass + column pass) of one level on stream s. */
 hipError_t ocean_hip_frame(const ocean_hip_level* level, double time_over_two_pi,
                            hipStream_t s);
+/* The same with a pass mask (1 = row pass, 2 = column pass), for timing the
+ * two kernels separately. */
+hipError_t ocean_hip_frame_passes(const ocean_hip_level* level, double time_over_two_pi,
+                                  hipStream_t s, unsigned passes);
 
 /* Batched sample_at over `count` (x, z) pairs (device float2). `out` and/or
  * `heights` may be NULL. */

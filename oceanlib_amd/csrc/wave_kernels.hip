@@ -474,7 +474,8 @@ __global__ void coverage_final_kernel(const double* __restrict__ partial, uint32
 }
 
 template <int N>
-hipError_t launch_frame_n(const ocean_hip_level* L, double time_over_two_pi, hipStream_t s)
+hipError_t launch_frame_n(const ocean_hip_level* L, double time_over_two_pi, hipStream_t s,
+                          unsigned passes)
 {
     const bool v = L->vel != nullptr;
     const dim3 rg(N / RowShape<N>::TEAMS), rb(RowShape<N>::BLOCK);
@@ -486,15 +487,19 @@ hipError_t launch_frame_n(const ocean_hip_level* L, double time_over_two_pi, hip
     float4* nrm = reinterpret_cast<float4*>(L->normal);
     float4* vel = reinterpret_cast<float4*>(L->vel);
     if (v) {
-        hipLaunchKernelGGL((row_pass_kernel<N, 6>), rg, rb, 0, s, h0, L->omega, tw, spec,
-                           time_over_two_pi, L->patch_length);
-        hipLaunchKernelGGL((col_pass_kernel<N, 6>), cg, cb, 0, s, spec, tw, disp, nrm, vel,
-                           L->choppiness, L->foam_threshold, L->inv_foam_threshold);
+        if (passes & 1u)
+            hipLaunchKernelGGL((row_pass_kernel<N, 6>), rg, rb, 0, s, h0, L->omega, tw, spec,
+                               time_over_two_pi, L->patch_length);
+        if (passes & 2u)
+            hipLaunchKernelGGL((col_pass_kernel<N, 6>), cg, cb, 0, s, spec, tw, disp, nrm, vel,
+                               L->choppiness, L->foam_threshold, L->inv_foam_threshold);
     } else {
-        hipLaunchKernelGGL((row_pass_kernel<N, 4>), rg, rb, 0, s, h0, L->omega, tw, spec,
-                           time_over_two_pi, L->patch_length);
-        hipLaunchKernelGGL((col_pass_kernel<N, 4>), cg, cb, 0, s, spec, tw, disp, nrm, vel,
-                           L->choppiness, L->foam_threshold, L->inv_foam_threshold);
+        if (passes & 1u)
+            hipLaunchKernelGGL((row_pass_kernel<N, 4>), rg, rb, 0, s, h0, L->omega, tw, spec,
+                               time_over_two_pi, L->patch_length);
+        if (passes & 2u)
+            hipLaunchKernelGGL((col_pass_kernel<N, 4>), cg, cb, 0, s, spec, tw, disp, nrm, vel,
+                               L->choppiness, L->foam_threshold, L->inv_foam_threshold);
     }
     return hipGetLastError();
 }
@@ -504,21 +509,27 @@ hipError_t launch_frame_n(const ocean_hip_level* L, double time_over_two_pi, hip
 
 using namespace oceanamd;
 
+extern "C" hipError_t ocean_hip_frame_passes(const ocean_hip_level* L, double time_over_two_pi,
+                                             hipStream_t s, unsigned passes)
+{
+    switch (L->n) {
+        case 16: return launch_frame_n<16>(L, time_over_two_pi, s, passes);
+        case 32: return launch_frame_n<32>(L, time_over_two_pi, s, passes);
+        case 64: return launch_frame_n<64>(L, time_over_two_pi, s, passes);
+        case 128: return launch_frame_n<128>(L, time_over_two_pi, s, passes);
+        case 256: return launch_frame_n<256>(L, time_over_two_pi, s, passes);
+        case 512: return launch_frame_n<512>(L, time_over_two_pi, s, passes);
+        case 1024: return launch_frame_n<1024>(L, time_over_two_pi, s, passes);
+        case 2048: return launch_frame_n<2048>(L, time_over_two_pi, s, passes);
+        case 4096: return launch_frame_n<4096>(L, time_over_two_pi, s, passes);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 extern "C" hipError_t ocean_hip_frame(const ocean_hip_level* L, double time_over_two_pi,
                                       hipStream_t s)
 {
-    switch (L->n) {
-        case 16: return launch_frame_n<16>(L, time_over_two_pi, s);
-        case 32: return launch_frame_n<32>(L, time_over_two_pi, s);
-        case 64: return launch_frame_n<64>(L, time_over_two_pi, s);
-        case 128: return launch_frame_n<128>(L, time_over_two_pi, s);
-        case 256: return launch_frame_n<256>(L, time_over_two_pi, s);
-        case 512: return launch_frame_n<512>(L, time_over_two_pi, s);
-        case 1024: return launch_frame_n<1024>(L, time_over_two_pi, s);
-        case 2048: return launch_frame_n<2048>(L, time_over_two_pi, s);
-        case 4096: return launch_frame_n<4096>(L, time_over_two_pi, s);
-        default: return hipErrorInvalidValue;
-    }
+    return ocean_hip_frame_passes(L, time_over_two_pi, s, 3u);
 }
 
 extern "C" hipError_t ocean_hip_sample(const ocean_hip_level* L, const float* xz, uint32_t count,

@@ -257,3 +257,26 @@ OREF int oref_iwave_taps(std::uint32_t radius, double cell, double depth, float*
         return -1;
     }
 }
+
+// A caller's loop of sample_at over an array of points, kept in C++ so the
+// CPU baseline times the reference's query, not the FFI call.
+OREF void oref_sample_batch(const ocean_sim* s, const float* xz, std::uint32_t count,
+                            ocean_surface_ex* out)
+{
+    const Ocean& o = *reinterpret_cast<const Ocean*>(s);
+    for (std::uint32_t i = 0; i < count; ++i) {
+        const Surface r = o.sample_at(xz[2 * i], xz[2 * i + 1]);
+        ocean_surface_ex& e = out[i];
+        e.height = r.height; e.offset_x = r.offset_x; e.offset_z = r.offset_z;
+        e.normal_x = r.normal_x; e.normal_y = r.normal_y; e.normal_z = r.normal_z;
+        e.foam = r.foam; e.velocity_x = r.velocity_x; e.velocity_y = r.velocity_y;
+        e.velocity_z = r.velocity_z;
+    }
+}
+
+OREF void oref_height_batch(const ocean_sim* s, const float* xz, std::uint32_t count,
+                            float* heights)
+{
+    const Ocean& o = *reinterpret_cast<const Ocean*>(s);
+    for (std::uint32_t i = 0; i < count; ++i) heights[i] = o.height_at(xz[2 * i], xz[2 * i + 1]);
+}

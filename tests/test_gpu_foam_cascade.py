@@ -1,0 +1,207 @@
+"""GPU parity for the persistent foam field and the cascade stack.
+
+FoamField (reference src/foam.cpp:89-139,218-252,291-305) and CascadeStack
+(reference src/cascade.cpp:42-153) through the additive C API, against the
+oracle bit for bit and against the golden digests of the reference run.
+"""
+import ctypes as C
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oceanlib_amd import capi
+from oceanlib_amd.workloads import CONFIG2, CONFIG3_LEVELS, desc_from, query_points
+from oracle_ffi import OracleOcean, as_array, assert_bitexact, default_foam_params, fptr, oracle
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def digest(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()[:16]
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return capi.load()
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with open(os.path.join(GOLDEN, "digests.json")) as f:
+        return json.load(f)
+
+
+def test_config2_foam_and_queries_vs_golden_and_oracle(lib, golden):
+    """120 frames of the heavy-chop patch: surface, velocity, advected foam and
+    4096 queries per frame against the reference's digests; foam_at and
+    coverage against the oracle."""
+    desc = desc_from(CONFIG2)
+    n = desc.size
+    st = C.c_int(-1)
+    sim = lib.ocean_create(C.byref(desc), C.byref(st))
+    assert sim, st.value
+    fd = capi.FoamDesc()
+    lib.ocean_amd_foam_desc_init(C.byref(fd))
+    foam = lib.ocean_amd_foam_create(sim, C.byref(fd), C.byref(st))
+    assert foam, st.value
+    o = OracleOcean(desc)
+    olib = oracle()
+    fp = default_foam_params()
+    ofoam = olib.orc_foam_create(o.h, C.byref(fp))
+    xz = query_points(600, 4096, desc.patch_length)
+    try:
+        for k in range(120):
+            t = k / 60.0
+            lib.ocean_update(sim, t)
+            lib.ocean_amd_foam_update(foam, np.float32(1.0 / 60.0))
+            o.update(t)
+            olib.orc_foam_update(ofoam, np.float32(1.0 / 60.0))
+            assert lib.ocean_amd_foam_last_substeps(foam) == olib.orc_foam_last_substeps(ofoam)
+            want = golden["config2"][k]
+            b = lib.ocean_get_buffers(sim)
+            disp = np.ctypeslib.as_array(b.displacement, shape=(4 * n * n,))
+            assert digest(disp) == want[0], k
+            if k % 10 == 0 or k == 119:
+                nrm = np.ctypeslib.as_array(b.normal, shape=(4 * n * n,))
+                vel = np.ctypeslib.as_array(lib.ocean_get_velocity(sim), shape=(4 * n * n,))
+                fb = np.ctypeslib.as_array(lib.ocean_amd_foam_data(foam), shape=(n * n,))
+                assert digest(nrm) == want[1], k
+                assert digest(vel) == want[2], k
+                assert digest(fb) == want[3], (k, "foam")
+                assert_bitexact(fb.copy(), as_array(olib.orc_foam_data(ofoam), n * n), "foam")
+                pts = np.ascontiguousarray(xz[k].reshape(-1))
+                out = (capi.OceanSurfaceEx * 4096)()
+                assert lib.ocean_amd_sample_batch(sim, fptr(pts), 4096, out) == 0
+                q = np.frombuffer(bytes(out), np.float32).reshape(4096, 10)
+                assert digest(q) == want[4], (k, "queries")
+        rng = np.random.default_rng(2)
+        for x, z in rng.uniform(-100, 400, size=(64, 2)).astype(np.float32):
+            got = lib.ocean_amd_foam_at(foam, float(x), float(z))
+            ref = olib.orc_foam_at(ofoam, float(x), float(z))
+            assert np.float32(got).tobytes() == np.float32(ref).tobytes(), (x, z)
+        cov = lib.ocean_amd_foam_coverage(foam)
+        fb = as_array(olib.orc_foam_data(ofoam), n * n).astype(np.float64)
+        seq = 0.0
+        for v in fb:  # the reference's sequential double sum (src/foam.cpp:307-313)
+            seq += v
+        assert abs(cov - seq / fb.size) <= 1e-12 * max(1.0, abs(seq / fb.size))
+        assert cov > 0.0
+        # clear -> zero field, accumulator reset (reference src/foam.cpp:254-260)
+        lib.ocean_amd_foam_clear(foam)
+        assert lib.ocean_amd_foam_coverage(foam) == 0.0
+        assert lib.ocean_amd_last_error(sim) == 0
+    finally:
+        olib.orc_foam_destroy(ofoam)
+        lib.ocean_amd_foam_destroy(foam)
+        lib.ocean_destroy(sim)
+
+
+def test_foam_requires_velocity_and_valid_desc(lib):
+    d = desc_from(dict(size=32))
+    st = C.c_int(-1)
+    sim = lib.ocean_create(C.byref(d), C.byref(st))
+    assert sim
+    try:
+        fd = capi.FoamDesc()
+        lib.ocean_amd_foam_desc_init(C.byref(fd))
+        assert not lib.ocean_amd_foam_create(sim, C.byref(fd), C.byref(st))
+        assert st.value == capi.OCEAN_ERROR_INVALID_ARG
+    finally:
+        lib.ocean_destroy(sim)
+    d = desc_from(dict(size=32, compute_velocity=1))
+    sim = lib.ocean_create(C.byref(d), C.byref(st))
+    try:
+        for field, bad in (("decay", -1.0), ("fixed_dt", 0.0), ("max_substeps", 0)):
+            fd = capi.FoamDesc()
+            lib.ocean_amd_foam_desc_init(C.byref(fd))
+            setattr(fd, field, bad)
+            assert not lib.ocean_amd_foam_create(sim, C.byref(fd), C.byref(st)), field
+            assert st.value == capi.OCEAN_ERROR_INVALID_ARG
+    finally:
+        lib.ocean_destroy(sim)
+
+
+def test_foam_substep_clamp_and_whole_cell_shift(lib):
+    """A huge dt runs max_substeps and discards the backlog (reference
+    src/foam.cpp:234-240); the accumulator matches the oracle's step counts."""
+    d = desc_from(dict(size=64, compute_velocity=1, wind_speed=20.0, choppiness=1.5))
+    st = C.c_int(-1)
+    sim = lib.ocean_create(C.byref(d), C.byref(st))
+    fd = capi.FoamDesc()
+    lib.ocean_amd_foam_desc_init(C.byref(fd))
+    fd.wind_drift_x = 3.0
+    foam = lib.ocean_amd_foam_create(sim, C.byref(fd), C.byref(st))
+    o = OracleOcean(d)
+    olib = oracle()
+    fp = default_foam_params()
+    fp.wind_drift_x = 3.0
+    of = olib.orc_foam_create(o.h, C.byref(fp))
+    try:
+        for k, dt in enumerate((1 / 60, 1.0, 0.004, 0.013, 1 / 60, 1 / 30, 0.0, -1.0, 0.5)):
+            t = 0.1 * k
+            lib.ocean_update(sim, t)
+            o.update(t)
+            lib.ocean_amd_foam_update(foam, np.float32(dt))
+            olib.orc_foam_update(of, np.float32(dt))
+            assert lib.ocean_amd_foam_last_substeps(foam) == olib.orc_foam_last_substeps(of)
+            fb = np.ctypeslib.as_array(lib.ocean_amd_foam_data(foam), shape=(64 * 64,)).copy()
+            assert_bitexact(fb, as_array(olib.orc_foam_data(of), 64 * 64), f"foam k={k}")
+        assert olib.orc_foam_last_substeps(of) == 4  # dt 0.5 s clamps at max_substeps
+    finally:
+        olib.orc_foam_destroy(of)
+        lib.ocean_amd_foam_destroy(foam)
+        lib.ocean_destroy(sim)
+
+
+def test_config3_cascade_vs_golden_and_oracle(lib, golden):
+    levels = [desc_from(kw) for kw in CONFIG3_LEVELS]
+    arr = (capi.OceanDesc * 3)(*levels)
+    st = C.c_int(-1)
+    h = lib.ocean_amd_cascade_create(arr, 3, C.byref(st))
+    assert h, st.value
+    assert lib.ocean_amd_cascade_level_count(h) == 3
+    orcs = [OracleOcean(d) for d in levels]
+    olib = oracle()
+    harr = (C.c_void_p * 3)(*[o.h for o in orcs])
+    pts = query_points(1, 512, 800.0)[0]
+    try:
+        for k in ("0", "1", "2", "60", "600", "1199"):
+            t = int(k) / 60.0
+            lib.ocean_amd_cascade_update(h, t)
+            g = golden["config3_cascade"][k]
+            for li, d in enumerate(levels):
+                b = lib.ocean_amd_cascade_get_buffers(h, li)
+                nn = 4 * d.size * d.size
+                disp = np.ctypeslib.as_array(b.displacement, shape=(nn,))
+                nrm = np.ctypeslib.as_array(b.normal, shape=(nn,))
+                assert [digest(disp), digest(nrm)] == g["levels"][li], (k, li)
+            out = (capi.OceanSurfaceEx * len(pts))()
+            flat = np.ascontiguousarray(pts.reshape(-1))
+            assert lib.ocean_amd_cascade_sample_batch(h, fptr(flat), len(pts), out) == 0
+            q = np.frombuffer(bytes(out), np.float32).reshape(len(pts), 10)
+            assert digest(q) == g["queries"], k
+            # single-point form and the oracle
+            for o in orcs:
+                o.update(t)
+            for i in range(0, len(pts), 97):
+                s = capi.OceanSurfaceEx()
+                lib.ocean_amd_cascade_sample(h, float(pts[i, 0]), float(pts[i, 1]), C.byref(s))
+                w = capi.OceanSurfaceEx()
+                olib.orc_cascade_sample(harr, 3, float(pts[i, 0]), float(pts[i, 1]), C.byref(w))
+                assert bytes(s) == bytes(w)
+    finally:
+        lib.ocean_amd_cascade_destroy(h)
+
+
+def test_cascade_rejects_empty_and_bad_levels(lib):
+    st = C.c_int(-1)
+    assert not lib.ocean_amd_cascade_create(None, 0, C.byref(st))
+    assert st.value == capi.OCEAN_ERROR_INVALID_ARG
+    levels = [desc_from(dict(size=64)), desc_from(dict(size=100))]
+    arr = (capi.OceanDesc * 2)(*levels)
+    assert not lib.ocean_amd_cascade_create(arr, 2, C.byref(st))
+    assert st.value == capi.OCEAN_ERROR_INVALID_ARG

@@ -42,6 +42,8 @@ public:
     bool sample_batch_device(const float* xz_dev, std::uint32_t count, Surface* out_dev,
                              float* heights_dev) const noexcept;
     [[nodiscard]] void* stream() const noexcept;
+    // Device array of the levels' views (used by combined queries).
+    [[nodiscard]] const void* device_levels() const noexcept;
 
 private:
     struct Impl;

@@ -72,6 +72,11 @@ int ocean_amd_sample_batch_device(const ocean_sim* sim, const float* xz_device,
  * src/core/fft.cpp:24-33. */
 int ocean_amd_build_spectrum(const ocean_desc* desc, float* h0, float* omega);
 int ocean_amd_build_twiddles(uint32_t n, float* tw);
+/* The interaction field's (2P+1)^2 convolution taps as the library fits them
+ * (reference src/core/iwave_kernel.cpp:177-437, least-squares method) and the
+ * largest realised symbol; returns the tap count or -1. */
+int ocean_amd_build_iwave_taps(uint32_t radius, double cell, double depth, float* taps,
+                               double* max_symbol);
 
 /* ---- persistent advected foam ------------------------------------------
  * C access to the reference's C++ FoamField (include/ocean/foam.hpp:38-142,

@@ -174,6 +174,8 @@ AMD_SIGNATURES = {
                                                 C.c_void_p]),
     "ocean_amd_build_spectrum": (C.c_int, [C.POINTER(OceanDesc), _F, _F]),
     "ocean_amd_build_twiddles": (C.c_int, [C.c_uint32, _F]),
+    "ocean_amd_build_iwave_taps": (C.c_int, [C.c_uint32, C.c_double, C.c_double, _F,
+                                             C.POINTER(C.c_double)]),
     "ocean_amd_foam_desc_init": (None, [C.POINTER(FoamDesc)]),
     "ocean_amd_foam_create": (_P, [_P, C.POINTER(FoamDesc), C.POINTER(C.c_int)]),
     "ocean_amd_foam_destroy": (None, [_P]),

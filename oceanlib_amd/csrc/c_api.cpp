@@ -233,28 +233,26 @@ int ocean_amd_profile_frame(ocean_sim* sim, double t0, double dt, uint32_t iters
     hipEvent_t ev[3];
     for (auto& e : ev)
         if (hipEventCreate(&e) != hipSuccess) return OCEAN_ERROR_UNKNOWN;
-    double row = 0.0, col = 0.0;
-    int rc = OCEAN_OK;
-    for (uint32_t i = 0; i < iters && rc == OCEAN_OK; ++i) {
-        const double t = t0 + dt * i;
-        const double t2 = t * 0.159154943091895335769;
-        hipEventRecord(ev[0], dev.stream);
-        dev.note(ocean_hip_frame_passes(&dev.level, t2, dev.stream, 1u));
-        hipEventRecord(ev[1], dev.stream);
-        dev.note(ocean_hip_frame_passes(&dev.level, t2, dev.stream, 2u));
-        hipEventRecord(ev[2], dev.stream);
-        if (hipEventSynchronize(ev[2]) != hipSuccess) rc = OCEAN_ERROR_UNKNOWN;
-        float a = 0.0f, b = 0.0f;
-        hipEventElapsedTime(&a, ev[0], ev[1]);
-        hipEventElapsedTime(&b, ev[1], ev[2]);
-        row += a;
-        col += b;
-        o.update(t);  // keeps time() and the host-mirror state consistent
-    }
+    // Back-to-back launches of each pass between two events, so the mean
+    // excludes per-launch event overhead (comparable to a profiler's kernel
+    // duration plus the dispatch gap).
+    hipEventRecord(ev[0], dev.stream);
+    for (uint32_t i = 0; i < iters; ++i)
+        dev.note(ocean_hip_frame_passes(&dev.level, (t0 + dt * i) * 0.159154943091895335769,
+                                        dev.stream, 1u));
+    hipEventRecord(ev[1], dev.stream);
+    for (uint32_t i = 0; i < iters; ++i)
+        dev.note(ocean_hip_frame_passes(&dev.level, 0.0, dev.stream, 2u));
+    hipEventRecord(ev[2], dev.stream);
+    int rc = hipEventSynchronize(ev[2]) == hipSuccess ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
+    float a = 0.0f, b = 0.0f;
+    hipEventElapsedTime(&a, ev[0], ev[1]);
+    hipEventElapsedTime(&b, ev[1], ev[2]);
     for (auto& e : ev) hipEventDestroy(e);
+    o.update(t0 + dt * (iters - 1));  // a complete, consistent frame again
     o.synchronize();
-    if (row_ms) *row_ms = static_cast<float>(row / iters);
-    if (col_ms) *col_ms = static_cast<float>(col / iters);
+    if (row_ms) *row_ms = a / static_cast<float>(iters);
+    if (col_ms) *col_ms = b / static_cast<float>(iters);
     return dev.status == 0 ? rc : OCEAN_ERROR_UNKNOWN;
 }
 
@@ -491,3 +489,221 @@ int ocean_amd_cascade_sample_batch(const ocean_amd_cascade* stack, const float* 
 }
 
 }  // extern "C"
+
+// ---- ocean.h: local interaction (reference src/c_api.cpp:256-518) ----------
+
+#include "ocean/interaction.hpp"
+
+namespace {
+
+ocean::InteractionField* to_field(ocean_interaction* f) noexcept
+{
+    return reinterpret_cast<ocean::InteractionField*>(f);
+}
+const ocean::InteractionField* to_cfield(const ocean_interaction* f) noexcept
+{
+    return reinterpret_cast<const ocean::InteractionField*>(f);
+}
+
+// reference src/c_api.cpp:289-301
+bool copy_interaction_desc(const ocean_interaction_desc* src, ocean_interaction_desc& dst) noexcept
+{
+    if (src == nullptr) return false;
+    if (src->struct_size == 0 || src->struct_size > sizeof(ocean_interaction_desc)) return false;
+    ocean_interaction_desc_init(&dst);
+    std::memcpy(&dst, src, src->struct_size);
+    dst.struct_size = sizeof(ocean_interaction_desc);
+    return true;
+}
+
+// reference src/c_api.cpp:303-325
+ocean::InteractionDesc translate(const ocean_interaction_desc& c) noexcept
+{
+    ocean::InteractionDesc d;
+    d.size = c.size;
+    d.extent = c.extent;
+    d.kernel_radius = c.kernel_radius;
+    d.damping = c.damping;
+    d.absorb_cells = c.absorb_cells;
+    d.fixed_dt = c.fixed_dt;
+    d.max_substeps = c.max_substeps;
+    d.max_sources = c.max_sources;
+    d.gravity = c.gravity;
+    d.depth = c.water_depth;
+    d.obstruction = (c.obstruction == OCEAN_OBSTRUCTION_DIRICHLET) ? ocean::Obstruction::Dirichlet
+                                                                  : ocean::Obstruction::Neumann;
+    d.parallel_for = reinterpret_cast<ocean::ParallelForFn>(c.parallel_for);
+    d.parallel_for_user = c.parallel_for_user;
+    d.thread_count = c.thread_count;
+    return d;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ocean_interaction_desc_init(ocean_interaction_desc* desc)
+{
+    if (desc == nullptr) return;
+    const ocean::InteractionDesc d{};
+    std::memset(desc, 0, sizeof(*desc));
+    desc->struct_size = sizeof(*desc);
+    desc->size = d.size;
+    desc->extent = d.extent;
+    desc->kernel_radius = d.kernel_radius;
+    desc->damping = d.damping;
+    desc->absorb_cells = d.absorb_cells;
+    desc->fixed_dt = d.fixed_dt;
+    desc->max_substeps = d.max_substeps;
+    desc->max_sources = d.max_sources;
+    desc->gravity = d.gravity;
+    desc->water_depth = d.depth;
+    desc->obstruction = (d.obstruction == ocean::Obstruction::Dirichlet) ? OCEAN_OBSTRUCTION_DIRICHLET
+                                                                        : OCEAN_OBSTRUCTION_NEUMANN;
+    desc->thread_count = d.thread_count;
+}
+
+ocean_interaction* ocean_interaction_create(const ocean_interaction_desc* desc,
+                                            ocean_status* out_status)
+{
+    auto fail = [&](ocean_status st) -> ocean_interaction* {
+        if (out_status != nullptr) *out_status = st;
+        return nullptr;
+    };
+    ocean_interaction_desc resolved;
+    if (!copy_interaction_desc(desc, resolved)) return fail(OCEAN_ERROR_INVALID_ARG);
+    try {
+        auto* f = new ocean::InteractionField(translate(resolved));
+        if (out_status != nullptr) *out_status = OCEAN_OK;
+        return reinterpret_cast<ocean_interaction*>(f);
+    } catch (const std::invalid_argument&) {
+        return fail(OCEAN_ERROR_INVALID_ARG);
+    } catch (const std::bad_alloc&) {
+        return fail(OCEAN_ERROR_OUT_OF_MEMORY);
+    } catch (...) {
+        return fail(OCEAN_ERROR_UNKNOWN);
+    }
+}
+
+void ocean_interaction_destroy(ocean_interaction* field) { delete to_field(field); }
+
+void ocean_interaction_add(ocean_interaction* field, const ocean_disturbance* d)
+{
+    if (field == nullptr || d == nullptr) return;
+    ocean::Disturbance s;
+    s.world_x = d->world_x;
+    s.world_z = d->world_z;
+    s.radius = d->radius;
+    s.strength = d->strength;
+    s.velocity_y = d->velocity_y;
+    s.velocity_x = d->velocity_x;
+    s.velocity_z = d->velocity_z;
+    s.kind = (d->kind == OCEAN_SOURCE_CONTINUOUS) ? ocean::SourceKind::Continuous
+                                                  : ocean::SourceKind::Impulse;
+    to_field(field)->add(s);
+}
+
+uint64_t ocean_interaction_dropped(const ocean_interaction* field)
+{
+    return field == nullptr ? 0u : to_cfield(field)->dropped_sources();
+}
+
+void ocean_interaction_update(ocean_interaction* field, float dt)
+{
+    if (field != nullptr) to_field(field)->update(dt);
+}
+
+int32_t ocean_interaction_recenter(ocean_interaction* field, float world_x, float world_z)
+{
+    if (field == nullptr) return 0;
+    to_field(field)->recenter(world_x, world_z);
+    return to_field(field)->recentered() ? 1 : 0;
+}
+
+void ocean_interaction_set_obstruction(ocean_interaction* field, const uint8_t* mask)
+{
+    if (field != nullptr) to_field(field)->set_obstruction(mask);
+}
+
+ocean_interaction_buffers ocean_interaction_get_buffers(const ocean_interaction* field)
+{
+    ocean_interaction_buffers b;
+    std::memset(&b, 0, sizeof(b));
+    if (field == nullptr) return b;
+    const ocean::InteractionBuffers ib = to_cfield(field)->buffers();
+    b.field = ib.field;
+    b.size = ib.size;
+    b.extent = ib.extent;
+    b.origin_x = ib.origin_x;
+    b.origin_z = ib.origin_z;
+    return b;
+}
+
+void ocean_interaction_sample_at(const ocean_interaction* field, float world_x, float world_z,
+                                 ocean_interaction_sample* out)
+{
+    if (out == nullptr) return;
+    std::memset(out, 0, sizeof(*out));
+    if (field == nullptr) return;
+    const ocean::InteractionSample s = to_cfield(field)->sample_at(world_x, world_z);
+    out->height = s.height;
+    out->slope_x = s.slope_x;
+    out->slope_z = s.slope_z;
+    out->velocity_y = s.velocity_y;
+}
+
+float ocean_interaction_height_at(const ocean_interaction* field, float world_x, float world_z)
+{
+    return field == nullptr ? 0.0f : to_cfield(field)->height_at(world_x, world_z);
+}
+
+float ocean_interaction_fixed_dt(const ocean_interaction* field)
+{
+    return field == nullptr ? 0.0f : to_cfield(field)->fixed_dt();
+}
+
+float ocean_interaction_dt_limit(const ocean_interaction* field)
+{
+    return field == nullptr ? 0.0f : to_cfield(field)->stable_dt_limit();
+}
+
+float ocean_interaction_dispersion_error(const ocean_interaction* field)
+{
+    return field == nullptr ? 0.0f : to_cfield(field)->kernel_dispersion_error();
+}
+
+// reference src/c_api.cpp:494-516
+void ocean_sample_combined(const ocean_sim* sim, const ocean_interaction* field, float world_x,
+                           float world_z, ocean_surface_ex* out)
+{
+    if (out == nullptr) return;
+    if (field == nullptr) {
+        ocean_sample_ex(sim, world_x, world_z, out);
+        return;
+    }
+    if (sim == nullptr) {
+        std::memset(out, 0, sizeof(*out));
+        out->normal_y = 1.0f;
+        const ocean::InteractionSample e = to_cfield(field)->sample_at(world_x, world_z);
+        out->height = e.height;
+        out->velocity_y = e.velocity_y;
+        return;
+    }
+    const ocean::WaterSurface water{*to_cpp(sim), *to_cfield(field)};
+    to_surface_ex(water.sample_at(world_x, world_z), out);
+}
+
+}  // extern "C"
+
+#include "iwave_kernel_host.hpp"
+
+extern "C" int ocean_amd_build_iwave_taps(uint32_t radius, double cell, double depth, float* taps,
+                                          double* max_symbol)
+{
+    if (taps == nullptr || radius < 1 || radius > 12 || !(cell > 0.0)) return -1;
+    oceanamd::IWaveStencil k;
+    if (!oceanamd::build_iwave_stencil(radius, cell, depth, k)) return -1;
+    std::memcpy(taps, k.taps.data(), k.taps.size() * sizeof(float));
+    if (max_symbol != nullptr) *max_symbol = k.max_symbol;
+    return static_cast<int>(k.taps.size());
+}

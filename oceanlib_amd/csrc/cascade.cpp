@@ -99,6 +99,7 @@ const OceanDesc& CascadeStack::desc(std::size_t level) const noexcept
 const Ocean& CascadeStack::level(std::size_t level) const noexcept { return impl_->levels[level]; }
 double CascadeStack::time() const noexcept { return impl_->time; }
 void* CascadeStack::stream() const noexcept { return impl_->stream.s; }
+const void* CascadeStack::device_levels() const noexcept { return impl_->dev_levels; }
 
 float CascadeStack::height_at(float world_x, float world_z) const noexcept
 {

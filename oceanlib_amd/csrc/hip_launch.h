@@ -72,6 +72,72 @@ uint32_t ocean_hip_coverage_blocks(uint32_t n);
 hipError_t ocean_hip_coverage(const float* v, uint32_t n, double* partial, double* out,
                               hipStream_t s);
 
+/* ---- iWave interaction field -------------------------------------------- */
+
+/* Device view of one interaction field. Planes have a P-cell zero halo;
+ * `cur`/`old` point at interior cell (0,0) and rows are `stride` floats. */
+typedef struct ocean_hip_iwave {
+    uint32_t n;
+    uint32_t p;
+    uint32_t stride;
+    uint32_t pad_;
+    float* cur;            /* h^n                                         */
+    float* old;            /* h^{n-1}; overwritten with h^{n+1} by a step */
+    const float* taps;     /* (2P+1)^2, row-major j-outer                 */
+    const float* c1;       /* N*N: 1/(1 + alpha dt)                       */
+    const float* c2;       /* N*N: 1 - alpha dt                           */
+    float* out;            /* N*N float4: eta, dEta/dx, dEta/dz, dEta/dt  */
+    float gdt2;            /* g dt^2                                      */
+    float dt;
+    float inv_two_cell;    /* 0.5 / cell                                  */
+    float inv_dt;          /* 1 / dt                                      */
+    float cell;
+    int32_t origin_x;      /* grid origin in whole cells                  */
+    int32_t origin_z;
+    int32_t pad2_;
+} ocean_hip_iwave;
+
+/* One source stamp of one substep, with the host-side scalars of the
+ * reference's inject() (src/interaction.cpp:322-348) already evaluated. */
+typedef struct ocean_hip_stamp {
+    float gx, gz;          /* source position, cells                      */
+    float inv2s2;          /* 1 / (2 sigma^2)                             */
+    float disp;            /* -strength * scale                           */
+    float vel;             /* velocity_y * scale                          */
+    int32_t x0, x1, z0, z1;/* clipped stamp box (inclusive)               */
+    int32_t pad_;
+} ocean_hip_stamp;
+
+/* Stamps `count` sources, in order, into cur/old (mean removal included).
+ * `means` is device scratch of `count` floats. */
+hipError_t ocean_hip_iwave_inject(const ocean_hip_iwave* f, const ocean_hip_stamp* stamps,
+                                  uint32_t count, float* means, hipStream_t s);
+/* Solid cells: Dirichlet zero (dirichlet != 0) or copy from the nearest
+ * fluid cell. `solid`, `nearest` are device arrays of cell indices. */
+hipError_t ocean_hip_iwave_obstruct(const ocean_hip_iwave* f, const uint32_t* solid,
+                                    const uint32_t* nearest, uint32_t count, int dirichlet,
+                                    hipStream_t s);
+/* One leapfrog step: old <- c1 (2 cur - c2 old - g dt^2 (G * cur)). */
+hipError_t ocean_hip_iwave_step(const ocean_hip_iwave* f, hipStream_t s);
+/* energy() of the reference: `scratch` holds 2*N*N doubles; result in *out. */
+hipError_t ocean_hip_iwave_energy(const ocean_hip_iwave* f, double gravity, double* scratch,
+                                  double* out, hipStream_t s);
+/* RGBA output from cur/old. */
+hipError_t ocean_hip_iwave_finalize(const ocean_hip_iwave* f, hipStream_t s);
+/* dst interior <- src interior shifted by whole cells (0 where uncovered). */
+hipError_t ocean_hip_iwave_shift(const float* src, float* dst, uint32_t n, uint32_t stride,
+                                 int32_t dx, int32_t dz, hipStream_t s);
+/* Interaction-only queries (src/interaction.cpp:729-773). out: 4 floats each. */
+hipError_t ocean_hip_iwave_sample(const ocean_hip_iwave* f, const float* xz, uint32_t count,
+                                  float* out, hipStream_t s);
+/* WaterSurface::sample_at (src/interaction.cpp:863-901): the FFT surface of
+ * `level_count` levels (one ocean = 1 level; a cascade = its levels, combined
+ * as a cascade) plus the field (may be NULL). */
+hipError_t ocean_hip_combined_sample(const ocean_hip_level* levels, uint32_t level_count,
+                                     int as_cascade, const ocean_hip_iwave* field,
+                                     const float* xz, uint32_t count, ocean_hip_surface* out,
+                                     hipStream_t s);
+
 #if defined(__cplusplus)
 }
 #endif

@@ -115,6 +115,7 @@ void OceanDevice::create(const OceanDesc& desc, hipStream_t shared_stream)
     if (desc.compute_velocity) { o_vel = off; off = align_up(off + cells * 16, a); }
     const std::size_t o_qxz = off;    off = align_up(off + std::size_t(kQueryStage) * 8, a);
     const std::size_t o_qout = off;   off = align_up(off + std::size_t(kQueryStage) * sizeof(ocean_hip_surface), a);
+    const std::size_t o_lvl = off;    off = align_up(off + sizeof(ocean_hip_level), a);
 
     hip_check(hipMalloc(&arena, off), "hipMalloc");
     char* base = static_cast<char*>(arena);
@@ -134,6 +135,7 @@ void OceanDevice::create(const OceanDesc& desc, hipStream_t shared_stream)
     level.vel = desc.compute_velocity ? reinterpret_cast<float*>(base + o_vel) : nullptr;
     q_xz = reinterpret_cast<float*>(base + o_qxz);
     q_out = reinterpret_cast<ocean_hip_surface*>(base + o_qout);
+    d_level = reinterpret_cast<ocean_hip_level*>(base + o_lvl);
 
     hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_disp), cells * 16, hipHostMallocDefault), "hipHostMalloc");
     hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_normal), cells * 16, hipHostMallocDefault), "hipHostMalloc");
@@ -152,6 +154,7 @@ void OceanDevice::create(const OceanDesc& desc, hipStream_t shared_stream)
     hip_check(hipMemcpyAsync(base + o_h0, h0_host, cells * 8, hipMemcpyHostToDevice, stream), "upload h0");
     hip_check(hipMemcpyAsync(base + o_omega, omega_host, cells * 4, hipMemcpyHostToDevice, stream), "upload omega");
     hip_check(hipMemcpyAsync(base + o_tw, tw.data(), (n / 2) * 8, hipMemcpyHostToDevice, stream), "upload twiddles");
+    hip_check(hipMemcpyAsync(d_level, &level, sizeof(level), hipMemcpyHostToDevice, stream), "upload level");
     hip_check(hipStreamSynchronize(stream), "upload");
 }
 

@@ -29,6 +29,7 @@ struct OceanDevice {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     ocean_hip_level level{};
+    ocean_hip_level* d_level = nullptr;  // device copy of `level` (for multi-level kernels)
     int fields = 4;            // complex fields per frame: 4, or 6 with velocity
 
     void* arena = nullptr;     // one hipMalloc holds every device buffer

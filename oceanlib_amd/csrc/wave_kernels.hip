@@ -22,6 +22,7 @@
 #include "device_math.hpp"
 #include "fft_team.hpp"
 #include "hip_launch.h"
+#include "sample_device.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -215,110 +216,6 @@ __global__ __launch_bounds__(ColShape<N>::BLOCK) void col_pass_kernel(
     }
 }
 
-// ---------------------------------------------------------------------------
-// Queries: reference src/ocean.cpp:309-452 (kInversionSteps = 4).
-// ---------------------------------------------------------------------------
-
-struct Tap {
-    uint32_t i00, i10, i01, i11;
-    float w00, w10, w01, w11;
-};
-
-__device__ __forceinline__ uint32_t wrap_index(int i, int n)
-{
-    i %= n;
-    if (i < 0) i += n;
-    return static_cast<uint32_t>(i);
-}
-
-__device__ __forceinline__ Tap make_tap(float u, float v, uint32_t n, float inv_cell)
-{
-    const float gu = u * inv_cell;
-    const float gv = v * inv_cell;
-    const float fu = floorf(gu);
-    const float fv = floorf(gv);
-    const int ni = static_cast<int>(n);
-    const uint32_t x0 = wrap_index(static_cast<int>(fu), ni);
-    const uint32_t z0 = wrap_index(static_cast<int>(fv), ni);
-    const uint32_t x1 = (x0 + 1u) % n;
-    const uint32_t z1 = (z0 + 1u) % n;
-    const float tu = gu - fu;
-    const float tv = gv - fv;
-    Tap t;
-    t.i00 = z0 * n + x0;
-    t.i10 = z0 * n + x1;
-    t.i01 = z1 * n + x0;
-    t.i11 = z1 * n + x1;
-    t.w00 = (1.0f - tu) * (1.0f - tv);
-    t.w10 = tu * (1.0f - tv);
-    t.w01 = (1.0f - tu) * tv;
-    t.w11 = tu * tv;
-    return t;
-}
-
-__device__ __forceinline__ float4 fetch4(const float4* __restrict__ b, const Tap& t)
-{
-    const float4 p = b[t.i00], q = b[t.i10], r = b[t.i01], s = b[t.i11];
-    float4 o;
-    o.x = p.x * t.w00 + q.x * t.w10 + r.x * t.w01 + s.x * t.w11;
-    o.y = p.y * t.w00 + q.y * t.w10 + r.y * t.w01 + s.y * t.w11;
-    o.z = p.z * t.w00 + q.z * t.w10 + r.z * t.w01 + s.z * t.w11;
-    o.w = p.w * t.w00 + q.w * t.w10 + r.w * t.w01 + s.w * t.w11;
-    return o;
-}
-
-// One point of Ocean::sample_at on one level (reference src/ocean.cpp:375-452).
-// Returns the final tap so callers can reuse the solved parameter position.
-__device__ __forceinline__ ocean_hip_surface sample_level(const float4* __restrict__ disp,
-                                                          const float4* __restrict__ normal,
-                                                          const float4* __restrict__ vel,
-                                                          uint32_t n, float inv_cell, float wx,
-                                                          float wz)
-{
-    float u = wx, v = wz;
-#pragma unroll
-    for (int it = 0; it < 4; ++it) {
-        const Tap t = make_tap(u, v, n, inv_cell);
-        const float4 d = fetch4(disp, t);
-        u = wx - d.x;
-        v = wz - d.z;
-    }
-    const Tap t = make_tap(u, v, n, inv_cell);
-    const float4 d = fetch4(disp, t);
-    const float4 nr = fetch4(normal, t);
-    float nx = nr.x, ny = nr.y, nz = nr.z;
-    const float len2 = nx * nx + ny * ny + nz * nz;
-    if (len2 > 0.0f) {
-        const float inv = 1.0f / sqrtf(len2);
-        nx *= inv;
-        ny *= inv;
-        nz *= inv;
-    } else {
-        nx = 0.0f;
-        ny = 1.0f;
-        nz = 0.0f;
-    }
-    ocean_hip_surface s;
-    s.height = d.y;
-    s.offset_x = d.x;
-    s.offset_z = d.z;
-    s.normal_x = nx;
-    s.normal_y = ny;
-    s.normal_z = nz;
-    s.foam = d.w;
-    if (vel != nullptr) {
-        const float4 w = fetch4(vel, t);
-        s.velocity_x = w.x;
-        s.velocity_y = w.y;
-        s.velocity_z = w.z;
-    } else {
-        s.velocity_x = 0.0f;
-        s.velocity_y = 0.0f;
-        s.velocity_z = 0.0f;
-    }
-    return s;
-}
-
 __global__ __launch_bounds__(kBlock) void sample_kernel(
     const float4* __restrict__ disp, const float4* __restrict__ normal,
     const float4* __restrict__ vel, uint32_t n, float inv_cell,
@@ -333,8 +230,6 @@ __global__ __launch_bounds__(kBlock) void sample_kernel(
     if (out != nullptr) out[q] = s;
 }
 
-// Cascade query: reference src/cascade.cpp:82-153 (levels summed in order,
-// normals summed then renormalised, foam screen-blended).
 __global__ __launch_bounds__(kBlock) void cascade_sample_kernel(
     const ocean_hip_level* __restrict__ levels, uint32_t level_count,
     const float2* __restrict__ xz, uint32_t count, ocean_hip_surface* __restrict__ out,
@@ -343,44 +238,7 @@ __global__ __launch_bounds__(kBlock) void cascade_sample_kernel(
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= count) return;
     const float2 p = xz[q];
-    float height = 0.0f, ox = 0.0f, oz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
-    float transp = 1.0f, vx = 0.0f, vy = 0.0f, vz = 0.0f;
-    for (uint32_t l = 0; l < level_count; ++l) {
-        const ocean_hip_level L = levels[l];
-        const float inv_cell = static_cast<float>(L.n) / L.patch_length;
-        const ocean_hip_surface s = sample_level(
-            reinterpret_cast<const float4*>(L.disp), reinterpret_cast<const float4*>(L.normal),
-            reinterpret_cast<const float4*>(L.vel), L.n, inv_cell, p.x, p.y);
-        height += s.height;
-        ox += s.offset_x;
-        oz += s.offset_z;
-        nx += s.normal_x;
-        ny += s.normal_y;
-        nz += s.normal_z;
-        vx += s.velocity_x;
-        vy += s.velocity_y;
-        vz += s.velocity_z;
-        transp *= (1.0f - s.foam);
-    }
-    ocean_hip_surface o;
-    o.height = height;
-    o.offset_x = ox;
-    o.offset_z = oz;
-    o.foam = 1.0f - transp;
-    o.velocity_x = vx;
-    o.velocity_y = vy;
-    o.velocity_z = vz;
-    const float len2 = nx * nx + ny * ny + nz * nz;
-    if (len2 > 0.0f) {
-        const float inv = 1.0f / sqrtf(len2);
-        o.normal_x = nx * inv;
-        o.normal_y = ny * inv;
-        o.normal_z = nz * inv;
-    } else {
-        o.normal_x = 0.0f;
-        o.normal_y = 1.0f;
-        o.normal_z = 0.0f;
-    }
+    const ocean_hip_surface o = sample_cascade(levels, level_count, p.x, p.y);
     if (heights != nullptr) heights[q] = o.height;
     if (out != nullptr) out[q] = o;
 }
@@ -388,26 +246,6 @@ __global__ __launch_bounds__(kBlock) void cascade_sample_kernel(
 // ---------------------------------------------------------------------------
 // Foam advection step: reference src/foam.cpp:22-41 (sample_wrap) and 89-139.
 // ---------------------------------------------------------------------------
-
-__device__ __forceinline__ float sample_wrap(const float* __restrict__ f, float u, float v, int n)
-{
-    const float fu = floorf(u);
-    const float fv = floorf(v);
-    const float tu = u - fu;
-    const float tv = v - fv;
-    int x0 = static_cast<int>(fu) % n;
-    if (x0 < 0) x0 += n;
-    int z0 = static_cast<int>(fv) % n;
-    if (z0 < 0) z0 += n;
-    const int x1 = (x0 + 1) % n;
-    const int z1 = (z0 + 1) % n;
-    const float a = f[z0 * n + x0];
-    const float b = f[z0 * n + x1];
-    const float c = f[z1 * n + x0];
-    const float d = f[z1 * n + x1];
-    return a * (1.0f - tu) * (1.0f - tv) + b * tu * (1.0f - tv) + c * (1.0f - tu) * tv +
-           d * tu * tv;
-}
 
 __global__ __launch_bounds__(kBlock) void foam_step_kernel(
     const float4* __restrict__ disp, const float4* __restrict__ vel,

@@ -44,3 +44,46 @@ def query_points(frames: int, per_frame: int, extent: float, seed: int = 0xC2) -
     rng = np.random.default_rng(seed)
     u = rng.integers(0, 1 << 24, size=(frames, per_frame, 2)).astype(np.float32)
     return (u / np.float32(1 << 24)) * np.float32(extent)
+
+
+# Interaction field of the three-cascade scene: 256^2 cells of 0.1 m, kernel
+# radius 6 (13x13 taps), other fields at the reference defaults
+# (include/ocean/interaction.hpp:64-139).
+CONFIG3_INTERACTION = dict(size=256, extent=25.6, kernel_radius=6)
+
+
+def config3_sources(count: int = 8, seed: int = 0x1A):
+    """Eight continuous sources on circles around the grid centre: radius
+    0.5-2 m, circle radius 5-20 m, speed 1-6 m/s (fixed numpy PCG64 draws)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        out.append(dict(radius=float(rng.uniform(0.5, 2.0)),
+                        circle=float(rng.uniform(5.0, 20.0)),
+                        speed=float(rng.uniform(1.0, 6.0)),
+                        phase=float(rng.uniform(0.0, 2 * np.pi)),
+                        strength=float(rng.uniform(0.02, 0.08))))
+    return out
+
+
+def config3_disturbances(sources, t: float, centre=(12.8, 12.8)):
+    """World-space disturbance records for time t: position on the circle and
+    the tangential velocity (so a substep sweep draws a continuous wake)."""
+    recs = []
+    for s in sources:
+        w = s["speed"] / s["circle"]
+        a = s["phase"] + w * t
+        recs.append(dict(world_x=np.float32(centre[0] + s["circle"] * np.cos(a)),
+                         world_z=np.float32(centre[1] + s["circle"] * np.sin(a)),
+                         radius=np.float32(s["radius"]), strength=np.float32(s["strength"]),
+                         velocity_y=np.float32(0.0),
+                         velocity_x=np.float32(-s["speed"] * np.sin(a)),
+                         velocity_z=np.float32(s["speed"] * np.cos(a)), kind=1))
+    return recs
+
+
+def config3_hull_mask(n: int = 256) -> np.ndarray:
+    """One static rectangular hull: cells x in [100, 156), z in [120, 136)."""
+    m = np.zeros((n, n), np.uint8)
+    m[120 * n // 256:136 * n // 256, 100 * n // 256:156 * n // 256] = 1
+    return m

@@ -137,6 +137,26 @@ def run_config3_cascades(r, frames=(0, 1, 2, 60, 600, 1199)):
     return out
 
 
+def run_config3_interaction(r, frames=1200, every=50):
+    """The three-cascade scene's interaction field: 256^2 cells of 0.1 m,
+    radius 6, eight moving continuous sources, one Neumann hull."""
+    from iwave_helpers import Field, idesc
+    from oceanlib_amd.workloads import (CONFIG3_INTERACTION, config3_disturbances,
+                                        config3_hull_mask, config3_sources)
+    f = Field("oref", r, idesc(r.oref_interaction_desc_init, **CONFIG3_INTERACTION))
+    f.set_obstruction(config3_hull_mask())
+    srcs = config3_sources()
+    out = {}
+    for k in range(frames):
+        for rec in config3_disturbances(srcs, k / 60.0):
+            f.add(rec)
+        f.update(1.0 / 60.0)
+        if k % every == 0 or k == frames - 1:
+            out[str(k)] = digest(f.field())
+    f.close()
+    return out
+
+
 def main():
     r = ref()
     if r is None:
@@ -156,9 +176,11 @@ def main():
     c2 = run_config2(r)
     print("config3 cascades ...", flush=True)
     c3 = run_config3_cascades(r)
+    print("config3 interaction ...", flush=True)
+    c3i = run_config3_interaction(r)
     with open(os.path.join(HERE, "digests.json"), "w") as f:
-        json.dump({"meta": meta, "config1": c1, "config2": c2, "config3_cascade": c3}, f,
-                  indent=0)
+        json.dump({"meta": meta, "config1": c1, "config2": c2, "config3_cascade": c3,
+                   "config3_interaction": c3i}, f, indent=0)
     print("wrote", os.path.join(HERE, "small_buffers.npz"), os.path.join(HERE, "digests.json"))
 
 

@@ -67,6 +67,23 @@ _ORACLE_ONLY = {
     "orc_foam_data": (_F, [_P]),
     "orc_foam_last_substeps": (C.c_uint32, [_P]),
     "orc_foam_at": (C.c_float, [_P, C.c_float, C.c_float]),
+    "orc_iwave_taps": (C.c_int, [C.c_uint32, C.c_double, C.c_double, _F,
+                                 C.POINTER(C.c_double)]),
+    "orc_iw_create": (_P, [C.POINTER(InteractionDesc)]),
+    "orc_iw_destroy": (None, [_P]),
+    "orc_iw_add": (None, [_P, C.POINTER(Disturbance)]),
+    "orc_iw_dropped": (C.c_uint64, [_P]),
+    "orc_iw_update": (None, [_P, C.c_float]),
+    "orc_iw_last_substeps": (C.c_uint32, [_P]),
+    "orc_iw_recenter": (C.c_int32, [_P, C.c_float, C.c_float]),
+    "orc_iw_set_obstruction": (None, [_P, C.POINTER(C.c_uint8)]),
+    "orc_iw_sample_at": (None, [_P, C.c_float, C.c_float, C.POINTER(InteractionSample)]),
+    "orc_iw_field": (_F, [_P]),
+    "orc_iw_fixed_dt": (C.c_float, [_P]),
+    "orc_iw_dt_limit": (C.c_float, [_P]),
+    "orc_iw_origin_x": (C.c_float, [_P]),
+    "orc_iw_origin_z": (C.c_float, [_P]),
+    "orc_compose": (None, [C.POINTER(OceanSurfaceEx), C.POINTER(InteractionSample)]),
 }
 
 _REF_ONLY = {

@@ -1,0 +1,125 @@
+"""Pins the iWave part of the oracle (oracle/iwave_oracle.c) to the reference.
+
+Bit-for-bit against the reference's own InteractionField compiled into
+oracle/_ref (reference src/interaction.cpp, src/core/iwave_*.cpp): kernel
+taps from the least-squares fit, fields after every checked frame of the
+moving-source scene with a Neumann or Dirichlet hull, recentring, impulses
+kept over zero-length frames, the substep clamp and queue overflow. The
+expf the Ricker stamp depends on is the host libm's in both.
+"""
+import numpy as np
+import pytest
+
+from oceanlib_amd.workloads import (CONFIG3_INTERACTION, config3_disturbances, config3_hull_mask,
+                                    config3_sources)
+from iwave_helpers import Field, idesc
+from oracle_ffi import assert_bitexact, fptr, oracle, ref
+
+needs_ref = pytest.mark.skipif(ref() is None, reason="reference build oracle/_ref not present")
+
+
+@needs_ref
+@pytest.mark.parametrize("radius,cell,depth", [(6, 0.1, 0.0), (1, 0.25, 0.0), (4, 0.25, 2.0),
+                                               (12, 0.05, 0.0), (8, 0.5, 0.3)])
+def test_kernel_taps_vs_reference(radius, cell, depth):
+    w = 2 * radius + 1
+    a = np.zeros(w * w, np.float32)
+    b = np.zeros(w * w, np.float32)
+    assert oracle().orc_iwave_taps(radius, cell, depth, fptr(a), None) == w * w
+    assert ref().oref_iwave_taps(radius, cell, depth, fptr(b)) == w * w
+    assert_bitexact(a, b, "taps")
+    t = a.reshape(w, w)
+    # D4 symmetry and zero sum (reference tests/test_iwave_kernel.cpp)
+    assert np.array_equal(t, t.T) and np.array_equal(t, t[::-1]) and np.array_equal(t, t[:, ::-1])
+    assert abs(float(t.astype(np.float64).sum())) < 1e-5
+
+
+def run_pair(kind_a, lib_a, kind_b, lib_b, desc_kw, frames, mask=None, recenter_at=None,
+             impulses=None, check_every=10, dts=None):
+    init = ref().oref_interaction_desc_init
+    d = idesc(init, **desc_kw)
+    fa, fb = Field(kind_a, lib_a, d), Field(kind_b, lib_b, d)
+    try:
+        assert fa.fixed_dt() == fb.fixed_dt() and fa.dt_limit() == fb.dt_limit()
+        if mask is not None:
+            fa.set_obstruction(mask)
+            fb.set_obstruction(mask)
+        srcs = config3_sources()
+        for k in range(frames):
+            t = k / 60.0
+            for rec in config3_disturbances(srcs, t):
+                fa.add(rec)
+                fb.add(rec)
+            if impulses and k in impulses:
+                for rec in impulses[k]:
+                    fa.add(rec)
+                    fb.add(rec)
+            if recenter_at and k in recenter_at:
+                x, z = recenter_at[k]
+                assert fa.recenter(x, z) == fb.recenter(x, z)
+            dt = dts[k % len(dts)] if dts else 1.0 / 60.0
+            fa.update(dt)
+            fb.update(dt)
+            if k % check_every == 0 or k == frames - 1:
+                assert_bitexact(fa.field(), fb.field(), f"frame {k}")
+        rng = np.random.default_rng(4)
+        for x, z in rng.uniform(-2, 28, size=(200, 2)).astype(np.float32):
+            assert bytes(fa.sample(x, z)) == bytes(fb.sample(x, z))
+        assert fa.dropped() == fb.dropped()
+    finally:
+        fa.close()
+        fb.close()
+
+
+@needs_ref
+def test_moving_sources_with_neumann_hull_vs_reference():
+    run_pair("orc", oracle(), "oref", ref(), CONFIG3_INTERACTION, 240,
+             mask=config3_hull_mask())
+
+
+@needs_ref
+def test_dirichlet_recentre_impulses_clamp_vs_reference():
+    imp = {5: [dict(world_x=np.float32(12.0), world_z=np.float32(9.0), radius=np.float32(0.3),
+                    strength=np.float32(0.2), velocity_y=np.float32(-1.5), velocity_x=0.0,
+                    velocity_z=0.0, kind=0)],
+           31: [dict(world_x=np.float32(3.0), world_z=np.float32(20.0), radius=np.float32(0.01),
+                     strength=np.float32(0.1), velocity_y=0.0, velocity_x=0.0, velocity_z=0.0,
+                     kind=0)]}
+    kw = dict(CONFIG3_INTERACTION, size=128, extent=16.0, kernel_radius=4, obstruction=1,
+              max_sources=6, absorb_cells=8)
+    # dt pattern: normal, zero (impulses must survive), a hitch that clamps, short
+    run_pair("orc", oracle(), "oref", ref(), kw, 90, mask=config3_hull_mask(128),
+             recenter_at={20: (9.0, 7.5), 50: (9.05, 7.5), 70: (200.0, 0.0)}, impulses=imp,
+             dts=[1 / 60, 0.0, 1 / 60, 0.5, 0.004, 1 / 30], check_every=3)
+
+
+@needs_ref
+def test_desc_validation_matches_reference():
+    init = ref().oref_interaction_desc_init
+    bad = [dict(size=48), dict(size=16), dict(size=2048), dict(extent=0.0),
+           dict(kernel_radius=0), dict(kernel_radius=13), dict(size=32, kernel_radius=12),
+           dict(damping=-1.0), dict(absorb_cells=200), dict(max_substeps=0),
+           dict(max_sources=0), dict(gravity=0.0), dict(fixed_dt=-0.1), dict(fixed_dt=10.0)]
+    import ctypes as C
+    for kw in bad:
+        d = idesc(init, **kw)
+        st = C.c_int(-1)
+        assert not ref().oref_interaction_create(C.byref(d), C.byref(st)), kw
+        assert not oracle().orc_iw_create(C.byref(d)), kw
+
+
+@pytest.mark.parametrize("radius,cell,depth", [(6, 0.1, 0.0), (6, 0.25, 0.0), (3, 0.1, 1.5),
+                                               (12, 0.05, 0.0)])
+def test_library_stencil_equals_oracle(radius, cell, depth):
+    """The taps the library uploads (host C++ fit) equal the oracle's."""
+    from oceanlib_amd import capi
+    import ctypes as C
+    lib = capi.load()
+    w = 2 * radius + 1
+    a = np.zeros(w * w, np.float32)
+    b = np.zeros(w * w, np.float32)
+    ma, mb = C.c_double(), C.c_double()
+    assert lib.ocean_amd_build_iwave_taps(radius, cell, depth, fptr(a), C.byref(ma)) == w * w
+    assert oracle().orc_iwave_taps(radius, cell, depth, fptr(b), C.byref(mb)) == w * w
+    assert_bitexact(a, b, "taps")
+    assert ma.value == mb.value

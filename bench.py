@@ -53,6 +53,15 @@ def algorithmic_bytes(n: int, fields: int, velocity: bool):
     return row, col
 
 
+def frame_bytes(n: int, fields: int, velocity: bool) -> int:
+    """Algorithmic HBM bytes of one level's frame: h0 + omega read once
+    (12 B/cell), RGBA32F outputs written once (32 B/cell, +16 with velocity),
+    and, for N >= 256 only, one write + read of the F complex fields between
+    the passes (16F B/cell; a smaller level's fields could stay on chip)."""
+    inter = 16 * fields if n >= 256 else 0
+    return n * n * (12 + 32 + (16 if velocity else 0) + inter)
+
+
 def traffic_from_profiles(kernel: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
     WRITE_SIZE), scaled by the copy-kernel calibration, if recorded."""
@@ -117,12 +126,109 @@ def cpu_baseline(desc, frames_budget_s: float = 12.0):
                       f"{QUERIES_PER_FRAME} height_at calls per frame, thread_count={cores}"}
 
 
+def batched_leg(lib, frames: int, warmup: int):
+    """The batched uneven seas (256 seas, 1-3 levels of N 64..512 each) as one
+    OceanBatch: autotuned, then `frames` frames timed between two
+    synchronizes. Reports 512^2-equivalent patch updates per second (cells
+    per second / 512^2) and the batch's HBM roofline fraction."""
+    import numpy as np
+    from oceanlib_amd.workloads import config4_levels, desc_from
+    flat, _ = config4_levels()
+    descs = [desc_from(kw) for kw in flat]
+    arr = (capi_mod().OceanDesc * len(descs))(*descs)
+    st = C.c_int(-1)
+    t0 = time.perf_counter()
+    b = lib.ocean_amd_batch_create(arr, len(descs), C.byref(st))
+    if not b:
+        raise SystemExit(f"ocean_amd_batch_create failed: {st.value}")
+    build_s = time.perf_counter() - t0
+    log(f"bench: batch of {len(descs)} levels built in {build_s:.1f} s; autotuning")
+    lib.ocean_amd_batch_autotune(b, 5)
+    groups = []
+    for g in range(lib.ocean_amd_batch_group_count(b)):
+        info = capi_mod().BatchGroup()
+        lib.ocean_amd_batch_group_info(b, g, C.byref(info))
+        groups.append({k: getattr(info, k) for k, _ in info._fields_})
+    for k in range(warmup):
+        lib.ocean_amd_batch_update(b, k / 60.0)
+    if lib.ocean_amd_batch_synchronize(b) != 0:
+        raise SystemExit("batch warmup failed")
+    log(f"bench: timing {frames} batched frames")
+    t0 = time.perf_counter()
+    for k in range(warmup, warmup + frames):
+        lib.ocean_amd_batch_update(b, k / 60.0)
+    if lib.ocean_amd_batch_synchronize(b) != 0:
+        raise SystemExit(f"batch error {lib.ocean_amd_batch_last_error(b)}")
+    el = time.perf_counter() - t0
+    lib.ocean_amd_batch_destroy(b)
+    cells = sum(kw["size"] ** 2 for kw in flat)
+    alg = sum(frame_bytes(kw["size"], 4, False) for kw in flat)
+    fps = frames / el
+    gbs = alg * fps / 1e9
+    return {
+        "workload": "256 batched uneven seas, 1-3 levels each, N in {64,128,256,512}, "
+                    "300-frame fixture workload (PCG32 seed 0xC4)",
+        "levels": len(flat), "cells_per_frame": cells,
+        "frames_per_s": round(fps, 2), "ms_per_frame": round(1000.0 / fps, 4),
+        "patch_updates_512_per_s": round(cells * fps / (512 * 512), 1),
+        "alg_bytes_per_frame": alg,
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5)},
+        "groups": groups, "build_s": round(build_s, 2), "frames": frames,
+    }
+
+
+def batched_cpu_baseline(budget_s: float = 8.0):
+    """The reference library on a bounded sample of the batched seas' levels
+    (the reference steps levels one after another, src/cascade.cpp:42-55):
+    cells per second over this host's cores, as 512^2-equivalent updates/s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    try:
+        from oracle_ffi import ref
+        r = ref()
+    except OSError:
+        r = None
+    if r is None:
+        return None
+    from oceanlib_amd.workloads import config4_levels, desc_from
+    cores = max(1, min(16, os.cpu_count() or 1))
+    flat, _ = config4_levels(24)
+    handles = []
+    for kw in flat:
+        d = desc_from(dict(kw, thread_count=cores))
+        st = C.c_int(-1)
+        handles.append((r.oref_create(C.byref(d), C.byref(st)), kw["size"] ** 2))
+    cells = 0
+    frames = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        for h, c in handles:
+            r.oref_update(h, frames / 60.0)
+            cells += c
+        frames += 1
+    el = time.perf_counter() - t0
+    for h, _ in handles:
+        r.oref_destroy(h)
+    return {"value": round(cells / el / (512 * 512), 1), "unit": "512^2 patch-updates/s",
+            "cores": cores, "kind": "reference",
+            "sample": f"{frames} frames of the first 24 seas ({len(flat)} levels, "
+                      f"{el:.1f} s): reference Ocean::update level by level, "
+                      f"thread_count={cores}"}
+
+
+def capi_mod():
+    from oceanlib_amd import capi
+    return capi
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=600)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--batch-frames", type=int, default=120,
+                    help="timed frames of the batched-seas leg (0 skips it)")
     ap.add_argument("--profile-only", action="store_true",
                     help="run only warmup + timed frames (for rocprofv3 runs)")
     args = ap.parse_args()
@@ -226,7 +332,10 @@ def main():
     lib.ocean_amd_synchronize(sim)
     qps = reps * nq / (time.perf_counter() - tq)
 
+    batched = batched_leg(lib, args.batch_frames, 5) if args.batch_frames > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(desc)
+    if batched is not None and not args.no_cpu:
+        batched["cpu_baseline"] = batched_cpu_baseline()
 
     lib.ocean_amd_foam_destroy(foam)
     lib.ocean_destroy(sim)
@@ -254,6 +363,7 @@ def main():
         "kernels": kernels,
         "update_device_ms": round(row_ms.value + col_ms.value, 5),
         "height_at_queries_per_s": round(qps, 1),
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

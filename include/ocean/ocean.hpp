@@ -85,6 +85,7 @@ namespace detail {
 struct OceanDevice;
 }
 class CascadeStack;
+class OceanBatch;
 
 class Ocean {
 public:
@@ -126,8 +127,12 @@ public:
 
 private:
     friend class CascadeStack;
-    // A level of a CascadeStack: enqueues on the stack's stream.
-    Ocean(const OceanDesc& desc, void* shared_stream);
+    friend class OceanBatch;
+    // A level of a CascadeStack or OceanBatch: enqueues on the owner's
+    // stream; lazy_host defers the host mirrors until first requested.
+    Ocean(const OceanDesc& desc, void* shared_stream, bool lazy_host);
+    // The owner enqueued this level's frame for `time` itself.
+    void note_frame(double time) noexcept;
 
     struct Impl;
     std::unique_ptr<Impl> impl_;

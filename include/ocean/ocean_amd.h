@@ -125,6 +125,62 @@ void          ocean_amd_cascade_sample(const ocean_amd_cascade* stack, float wor
 int           ocean_amd_cascade_sample_batch(const ocean_amd_cascade* stack, const float* xz,
                                              uint32_t count, ocean_surface_ex* out);
 
+/* ---- batches of seas ------------------------------------------------------
+ * No reference counterpart: the reference steps each ocean (or cascade level)
+ * on its own, one after another (src/cascade.cpp:42-55; src/c_api.cpp:148-152
+ * per ocean_update call). A batch steps many independent levels of uneven N
+ * with one persistent launch per (N, velocity) group and pass, replayed as a
+ * hipGraph. Every level's buffers and queries equal those of an ocean_sim
+ * created from the same desc and updated to the same time. */
+typedef struct ocean_amd_batch ocean_amd_batch;
+
+typedef struct ocean_amd_batch_group {
+    uint32_t size;        /* N of the group's levels                      */
+    uint32_t velocity;    /* 1 when the group carries velocity            */
+    uint32_t levels;
+    uint32_t items;       /* work items per pass                          */
+    uint32_t grid_row;    /* persistent grid sizes in use                 */
+    uint32_t grid_col;
+    float    row_ms;      /* device time of each pass at the last autotune */
+    float    col_ms;
+} ocean_amd_batch_group;
+
+ocean_amd_batch* ocean_amd_batch_create(const ocean_desc* levels, uint32_t count,
+                                        ocean_status* out_status);
+void          ocean_amd_batch_destroy(ocean_amd_batch* batch);
+/* Every level to the same absolute time; enqueues and returns. */
+void          ocean_amd_batch_update(ocean_amd_batch* batch, double time);
+uint32_t      ocean_amd_batch_level_count(const ocean_amd_batch* batch);
+/* One level as a read-only ocean_sim owned by the batch: every function of
+ * ocean.h and of this header that takes a const ocean_sim* works on it
+ * (buffers, velocity, queries, device buffers). Never pass it to
+ * ocean_update or ocean_destroy. */
+const ocean_sim* ocean_amd_batch_level(const ocean_amd_batch* batch, uint32_t level);
+/* Queries over levels [first, first + level_count): one level answers as
+ * ocean_sample_ex, several as a cascade of those levels. */
+int           ocean_amd_batch_sample(const ocean_amd_batch* batch, uint32_t first,
+                                     uint32_t level_count, const float* xz, uint32_t count,
+                                     ocean_surface_ex* out);
+int           ocean_amd_batch_sample_device(const ocean_amd_batch* batch, uint32_t first,
+                                            uint32_t level_count, const float* xz_dev,
+                                            uint32_t count, ocean_surface_ex* out_dev,
+                                            float* heights_dev);
+/* Times candidate grid sizes (reps launches each) and keeps the fastest. */
+void          ocean_amd_batch_autotune(ocean_amd_batch* batch, uint32_t reps);
+uint32_t      ocean_amd_batch_group_count(const ocean_amd_batch* batch);
+int           ocean_amd_batch_group_info(const ocean_amd_batch* batch, uint32_t group,
+                                         ocean_amd_batch_group* out);
+/* Checksums of levels [first, first + count) after the work queued so far
+ * (blocks until done): per level its buffers (displacement, normal, then
+ * velocity when on) as uint32 words w_i, i counted across the buffers, summed
+ * as sum(w_i * K_i) mod 2^64 with K_i = (i * 0x9E3779B97F4A7C15 +
+ * 0x632BE59BD9B4E019) | 1. For replay and lockstep checks between machines:
+ * equal buffers give equal sums, and any one changed word changes the sum. */
+int           ocean_amd_batch_checksums(const ocean_amd_batch* batch, uint32_t first,
+                                        uint32_t count, uint64_t* out);
+int           ocean_amd_batch_synchronize(const ocean_amd_batch* batch);
+int           ocean_amd_batch_last_error(const ocean_amd_batch* batch);
+
 #if defined(__cplusplus)
 }
 #endif

@@ -125,6 +125,19 @@ class FoamDesc(C.Structure):
     ]
 
 
+class DeviceBuffers(C.Structure):
+    """ocean_amd_device_buffers (include/ocean/ocean_amd.h)."""
+    _fields_ = [("displacement", C.c_void_p), ("normal", C.c_void_p), ("velocity", C.c_void_p),
+                ("size", C.c_uint32), ("patch_length", C.c_float), ("stream", C.c_void_p)]
+
+
+class BatchGroup(C.Structure):
+    """ocean_amd_batch_group (include/ocean/ocean_amd.h)."""
+    _fields_ = [(n, C.c_uint32) for n in
+                ("size", "velocity", "levels", "items", "grid_row", "grid_col")] + \
+               [("row_ms", C.c_float), ("col_ms", C.c_float)]
+
+
 # (name, restype, argtypes) for every symbol the headers declare.
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
@@ -194,6 +207,22 @@ AMD_SIGNATURES = {
     "ocean_amd_cascade_sample": (None, [_P, C.c_float, C.c_float, C.POINTER(OceanSurfaceEx)]),
     "ocean_amd_cascade_sample_batch": (C.c_int, [_P, _F, C.c_uint32,
                                                  C.POINTER(OceanSurfaceEx)]),
+    "ocean_amd_batch_create": (_P, [C.POINTER(OceanDesc), C.c_uint32, C.POINTER(C.c_int)]),
+    "ocean_amd_batch_destroy": (None, [_P]),
+    "ocean_amd_batch_update": (None, [_P, C.c_double]),
+    "ocean_amd_batch_level_count": (C.c_uint32, [_P]),
+    "ocean_amd_batch_level": (_P, [_P, C.c_uint32]),
+    "ocean_amd_batch_sample": (C.c_int, [_P, C.c_uint32, C.c_uint32, _F, C.c_uint32,
+                                         C.POINTER(OceanSurfaceEx)]),
+    "ocean_amd_batch_sample_device": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_void_p,
+                                                C.c_uint32, C.c_void_p, C.c_void_p]),
+    "ocean_amd_batch_autotune": (None, [_P, C.c_uint32]),
+    "ocean_amd_batch_group_count": (C.c_uint32, [_P]),
+    "ocean_amd_batch_group_info": (C.c_int, [_P, C.c_uint32, C.POINTER(BatchGroup)]),
+    "ocean_amd_batch_checksums": (C.c_int, [_P, C.c_uint32, C.c_uint32,
+                                            C.POINTER(C.c_uint64)]),
+    "ocean_amd_batch_synchronize": (C.c_int, [_P]),
+    "ocean_amd_batch_last_error": (C.c_int, [_P]),
 }
 
 

@@ -1,0 +1,324 @@
+// OceanBatch: many ocean levels stepped by one hipGraph per frame.
+//
+// Levels are ordinary Oceans built on the batch's stream with deferred host
+// mirrors. Levels whose N the batched kernels support are grouped by
+// (N, velocity); a group's work items are (level, block) pairs, ordered level
+// by level so consecutive items share a level's tables in cache. The graph
+// holds one row -> column chain per group; update() sets the frame time and
+// replays it.
+#include "ocean/batch.hpp"
+
+#include "ocean_device.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <vector>
+
+namespace ocean {
+
+namespace {
+constexpr double kInvTwoPiD = 0.159154943091895335769;
+}
+
+struct OceanBatch::Impl {
+    struct Stream {
+        hipStream_t s = nullptr;
+        ~Stream()
+        {
+            if (s) hipStreamDestroy(s);
+        }
+    } stream;  // declared first: destroyed after the levels, which sync on it
+    std::vector<Ocean> levels;
+    std::vector<std::size_t> loose;  // levels stepped by their own kernels
+    std::vector<ocean_hip_batch_group> groups;
+    std::vector<BatchGroupInfo> info;
+    ocean_hip_level* d_levels = nullptr;
+    std::uint32_t* d_items = nullptr;
+    ocean_hip_batch_graph* graph = nullptr;
+    float* q_xz = nullptr;
+    ocean_hip_surface* q_out = nullptr;
+    float* h_xz = nullptr;
+    ocean_hip_surface* h_out = nullptr;
+    unsigned long long* d_sums = nullptr;  // one checksum per level
+    unsigned long long* h_sums = nullptr;
+    int cus = 1;
+    double time = 0.0;
+    int status = 0;
+
+    void note(hipError_t e) noexcept
+    {
+        if (e != hipSuccess && status == 0) status = static_cast<int>(e);
+    }
+
+    ~Impl()
+    {
+        if (stream.s) hipStreamSynchronize(stream.s);
+        ocean_hip_batch_graph_destroy(graph);
+        if (d_levels) hipFree(d_levels);
+        if (h_xz) hipHostFree(h_xz);
+        if (h_out) hipHostFree(h_out);
+        if (h_sums) hipHostFree(h_sums);
+    }
+
+    void rebuild_graph()
+    {
+        ocean_hip_batch_graph_destroy(graph);
+        graph = nullptr;
+        if (groups.empty()) return;
+        detail::hip_check(ocean_hip_batch_graph_create(groups.data(),
+                                                       static_cast<std::uint32_t>(groups.size()),
+                                                       &graph),
+                          "batch graph");
+    }
+};
+
+OceanBatch::OceanBatch(std::span<const OceanDesc> descs)
+{
+    if (descs.empty()) throw std::invalid_argument("OceanBatch: at least one level is required");
+    if (descs.size() >= (1u << 20)) throw std::invalid_argument("OceanBatch: too many levels");
+    for (const OceanDesc& d : descs) detail::validate_desc(d);
+    impl_ = std::make_unique<Impl>();
+    Impl& m = *impl_;
+    detail::hip_check(hipStreamCreateWithFlags(&m.stream.s, hipStreamNonBlocking), "hipStreamCreate");
+    int dev = 0;
+    detail::hip_check(hipGetDevice(&dev), "hipGetDevice");
+    detail::hip_check(hipDeviceGetAttribute(&m.cus, hipDeviceAttributeMultiprocessorCount, dev),
+                      "hipDeviceGetAttribute");
+    m.levels.reserve(descs.size());
+    for (const OceanDesc& d : descs) m.levels.push_back(Ocean(d, m.stream.s, true));
+    const std::uint32_t count = static_cast<std::uint32_t>(m.levels.size());
+
+    // Group by (N, velocity); everything else is stepped level by level.
+    std::map<std::pair<std::uint32_t, int>, std::vector<std::uint32_t>> by;
+    for (std::uint32_t i = 0; i < count; ++i) {
+        const auto& L = m.levels[i].device().level;
+        if (ocean_hip_batchable(L.n))
+            by[{L.n, L.vel ? 6 : 4}].push_back(i);
+        else
+            m.loose.push_back(i);
+    }
+    std::vector<std::uint32_t> items;
+    std::vector<std::size_t> first;
+    for (auto& [key, lv] : by) {
+        first.push_back(items.size());
+        for (std::uint32_t li : lv)
+            for (std::uint32_t b = 0; b < ocean_hip_batch_items(key.first); ++b)
+                items.push_back(ocean_hip_batch_item(li, b));
+    }
+    const std::size_t lv_bytes = (count * sizeof(ocean_hip_level) + 255) / 256 * 256;
+    const std::size_t it_bytes = (std::max<std::size_t>(items.size(), 1) * 4 + 255) / 256 * 256;
+    const std::size_t sum_bytes = (std::size_t(count) * 8 + 255) / 256 * 256;
+    const std::size_t q = detail::kQueryStage;
+    detail::hip_check(hipMalloc(reinterpret_cast<void**>(&m.d_levels),
+                                lv_bytes + it_bytes + sum_bytes + q * 8 +
+                                    q * sizeof(ocean_hip_surface)),
+                      "hipMalloc batch");
+    char* base = reinterpret_cast<char*>(m.d_levels);
+    m.d_items = reinterpret_cast<std::uint32_t*>(base + lv_bytes);
+    m.d_sums = reinterpret_cast<unsigned long long*>(base + lv_bytes + it_bytes);
+    m.q_xz = reinterpret_cast<float*>(base + lv_bytes + it_bytes + sum_bytes);
+    m.q_out = reinterpret_cast<ocean_hip_surface*>(m.q_xz + 2 * q);
+    detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m.h_sums), std::size_t(count) * 8,
+                                    hipHostMallocDefault), "hipHostMalloc");
+    detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m.h_xz), q * 8, hipHostMallocDefault), "hipHostMalloc");
+    detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m.h_out), q * sizeof(ocean_hip_surface), hipHostMallocDefault), "hipHostMalloc");
+
+    std::vector<ocean_hip_level> views;
+    views.reserve(count);
+    for (const Ocean& o : m.levels) views.push_back(o.device().level);
+    detail::hip_check(hipMemcpy(m.d_levels, views.data(), count * sizeof(ocean_hip_level),
+                                hipMemcpyHostToDevice), "upload batch levels");
+    if (!items.empty())
+        detail::hip_check(hipMemcpy(m.d_items, items.data(), items.size() * 4, hipMemcpyHostToDevice),
+                          "upload batch items");
+
+    std::size_t g = 0;
+    for (auto& [key, lv] : by) {
+        const std::size_t end = (g + 1 < first.size()) ? first[g + 1] : items.size();
+        ocean_hip_batch_group gr{};
+        gr.n = key.first;
+        gr.fields = static_cast<std::uint32_t>(key.second);
+        gr.levels = m.d_levels;
+        gr.items = m.d_items + first[g];
+        gr.n_items = static_cast<std::uint32_t>(end - first[g]);
+        // Until autotune() runs: as many blocks as stay resident.
+        const int occ_r = std::max(1, ocean_hip_batch_occupancy(gr.n, gr.fields, 1));
+        const int occ_c = std::max(1, ocean_hip_batch_occupancy(gr.n, gr.fields, 0));
+        gr.grid_row = std::min<std::uint32_t>(gr.n_items, std::uint32_t(m.cus * occ_r));
+        gr.grid_col = std::min<std::uint32_t>(gr.n_items, std::uint32_t(m.cus * occ_c));
+        m.groups.push_back(gr);
+        BatchGroupInfo bi;
+        bi.size = gr.n;
+        bi.velocity = gr.fields == 6;
+        bi.levels = static_cast<std::uint32_t>(lv.size());
+        bi.items = gr.n_items;
+        bi.grid_row = gr.grid_row;
+        bi.grid_col = gr.grid_col;
+        m.info.push_back(bi);
+        ++g;
+    }
+    m.rebuild_graph();
+    detail::hip_check(hipStreamSynchronize(m.stream.s), "batch init");
+}
+
+OceanBatch::~OceanBatch() = default;
+OceanBatch::OceanBatch(OceanBatch&&) noexcept = default;
+OceanBatch& OceanBatch::operator=(OceanBatch&&) noexcept = default;
+
+void OceanBatch::update(double time) noexcept
+{
+    Impl& m = *impl_;
+    m.levels.front().device().make_current();
+    const double t2 = time * kInvTwoPiD;  // hoisted as the reference hoists it (evolve.cpp:33)
+    if (m.graph) m.note(ocean_hip_batch_graph_launch(m.graph, t2, m.stream.s));
+    for (std::size_t i : m.loose) m.levels[i].update(time);
+    for (Ocean& o : m.levels) o.note_frame(time);
+    m.time = time;
+}
+
+double OceanBatch::time() const noexcept { return impl_->time; }
+std::size_t OceanBatch::level_count() const noexcept { return impl_->levels.size(); }
+const Ocean& OceanBatch::level(std::size_t i) const noexcept { return impl_->levels[i]; }
+std::size_t OceanBatch::group_count() const noexcept { return impl_->groups.size(); }
+BatchGroupInfo OceanBatch::group(std::size_t g) const noexcept { return impl_->info[g]; }
+void* OceanBatch::stream() const noexcept { return impl_->stream.s; }
+
+bool OceanBatch::checksums(std::size_t first, std::size_t count, std::uint64_t* out) const noexcept
+{
+    static_assert(sizeof(unsigned long long) == sizeof(std::uint64_t));
+    Impl& m = *impl_;
+    if (out == nullptr || first > m.levels.size() || count > m.levels.size() - first) return false;
+    if (count == 0) return true;
+    m.levels.front().device().make_current();
+    m.note(ocean_hip_level_checksums(m.d_levels + first, static_cast<std::uint32_t>(count),
+                                     m.d_sums, m.stream.s));
+    m.note(hipMemcpyAsync(m.h_sums, m.d_sums, count * 8, hipMemcpyDeviceToHost, m.stream.s));
+    m.note(hipStreamSynchronize(m.stream.s));
+    if (m.status != 0) return false;
+    std::memcpy(out, m.h_sums, count * 8);
+    return true;
+}
+
+void OceanBatch::synchronize() const noexcept
+{
+    impl_->note(hipStreamSynchronize(impl_->stream.s));
+}
+
+int OceanBatch::last_error() const noexcept
+{
+    if (impl_->status) return impl_->status;
+    for (const Ocean& o : impl_->levels)
+        if (o.last_error()) return o.last_error();
+    return 0;
+}
+
+bool OceanBatch::sample_device(std::size_t first, std::size_t count, const float* xz_dev,
+                               std::uint32_t n, Surface* out_dev, float* heights_dev) const noexcept
+{
+    Impl& m = *impl_;
+    if (count == 0 || first + count > m.levels.size()) return false;
+    if (n == 0) return true;
+    if (xz_dev == nullptr || (out_dev == nullptr && heights_dev == nullptr)) return false;
+    auto* out = reinterpret_cast<ocean_hip_surface*>(out_dev);
+    if (count == 1) {
+        const auto& L = m.levels[first].device().level;
+        m.note(ocean_hip_sample(&L, xz_dev, n, out, heights_dev, m.stream.s));
+    } else {
+        m.note(ocean_hip_cascade_sample(m.d_levels + first, static_cast<std::uint32_t>(count),
+                                        xz_dev, n, out, heights_dev, m.stream.s));
+    }
+    return m.status == 0;
+}
+
+bool OceanBatch::sample(std::size_t first, std::size_t count, const float* xz, std::uint32_t n,
+                        Surface* out) const noexcept
+{
+    static_assert(sizeof(Surface) == sizeof(ocean_hip_surface));
+    Impl& m = *impl_;
+    if (count == 0 || first + count > m.levels.size()) return false;
+    if (n == 0) return true;
+    if (xz == nullptr || out == nullptr) return false;
+    for (std::uint32_t done = 0; done < n;) {
+        const std::uint32_t k = std::min(detail::kQueryStage, n - done);
+        std::memcpy(m.h_xz, xz + 2 * std::size_t(done), std::size_t(k) * 8);
+        m.note(hipMemcpyAsync(m.q_xz, m.h_xz, std::size_t(k) * 8, hipMemcpyHostToDevice, m.stream.s));
+        if (!sample_device(first, count, m.q_xz, k, reinterpret_cast<Surface*>(m.q_out), nullptr))
+            return false;
+        m.note(hipMemcpyAsync(m.h_out, m.q_out, std::size_t(k) * sizeof(ocean_hip_surface),
+                              hipMemcpyDeviceToHost, m.stream.s));
+        m.note(hipStreamSynchronize(m.stream.s));
+        std::memcpy(out + done, m.h_out, std::size_t(k) * sizeof(ocean_hip_surface));
+        done += k;
+    }
+    return m.status == 0;
+}
+
+void OceanBatch::autotune(std::uint32_t reps) noexcept
+{
+    Impl& m = *impl_;
+    if (reps == 0) reps = 1;
+    m.levels.front().device().make_current();
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return;
+    }
+    const double t2 = m.time * kInvTwoPiD;
+    for (std::size_t g = 0; g < m.groups.size(); ++g) {
+        ocean_hip_batch_group& gr = m.groups[g];
+        for (int pass = 0; pass < 2; ++pass) {
+            std::vector<std::uint32_t> cand;
+            for (int k : {1, 2, 3, 4, 6, 8, 12, 16}) {
+                const std::uint32_t c = std::uint32_t(m.cus) * std::uint32_t(k);
+                if (c < gr.n_items) cand.push_back(c);
+            }
+            cand.push_back(gr.n_items);  // one block per item
+            std::uint32_t best = pass == 0 ? gr.grid_row : gr.grid_col;
+            float best_ms = 1e30f;
+            for (std::uint32_t c : cand) {
+                ocean_hip_batch_group t = gr;
+                t.grid_row = c;
+                t.grid_col = c;
+                m.note(ocean_hip_batch_group_frame(&t, t2, 1u << pass, m.stream.s));  // warm
+                hipEventRecord(e0, m.stream.s);
+                for (std::uint32_t r = 0; r < reps; ++r)
+                    m.note(ocean_hip_batch_group_frame(&t, t2, 1u << pass, m.stream.s));
+                hipEventRecord(e1, m.stream.s);
+                hipEventSynchronize(e1);
+                float ms = 0.0f;
+                hipEventElapsedTime(&ms, e0, e1);
+                ms /= static_cast<float>(reps);
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = c;
+                }
+            }
+            if (pass == 0) {
+                gr.grid_row = best;
+                m.info[g].grid_row = best;
+                m.info[g].row_ms = best_ms;
+            } else {
+                gr.grid_col = best;
+                m.info[g].grid_col = best;
+                m.info[g].col_ms = best_ms;
+            }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    // The timing launches rewrote every level at the batch's current time
+    // (the column passes from the last row pass): the buffers are unchanged.
+    try {
+        m.rebuild_graph();
+    } catch (...) {
+        m.status = m.status ? m.status : static_cast<int>(hipErrorUnknown);
+    }
+    m.note(hipStreamSynchronize(m.stream.s));
+}
+
+}  // namespace ocean

@@ -490,6 +490,141 @@ int ocean_amd_cascade_sample_batch(const ocean_amd_cascade* stack, const float* 
 
 }  // extern "C"
 
+// ---- ocean_amd.h: batches of seas ------------------------------------------
+
+#include "ocean/batch.hpp"
+
+namespace {
+
+ocean::OceanBatch* to_batch(ocean_amd_batch* b) noexcept
+{
+    return reinterpret_cast<ocean::OceanBatch*>(b);
+}
+const ocean::OceanBatch* to_batch(const ocean_amd_batch* b) noexcept
+{
+    return reinterpret_cast<const ocean::OceanBatch*>(b);
+}
+
+bool batch_range(const ocean_amd_batch* b, uint32_t first, uint32_t count) noexcept
+{
+    if (b == nullptr || count == 0) return false;
+    const std::size_t n = to_batch(b)->level_count();
+    return first < n && count <= n - first;
+}
+
+}  // namespace
+
+extern "C" {
+
+ocean_amd_batch* ocean_amd_batch_create(const ocean_desc* levels, uint32_t count,
+                                        ocean_status* out_status)
+{
+    auto fail = [&](ocean_status s) -> ocean_amd_batch* {
+        if (out_status != nullptr) *out_status = s;
+        return nullptr;
+    };
+    if (levels == nullptr || count == 0) return fail(OCEAN_ERROR_INVALID_ARG);
+    std::vector<ocean::OceanDesc> descs;
+    descs.reserve(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        ocean_desc r;
+        if (!copy_desc(&levels[i], r)) return fail(OCEAN_ERROR_INVALID_ARG);
+        descs.push_back(translate(r));
+    }
+    try {
+        auto* b = new ocean::OceanBatch(std::span<const ocean::OceanDesc>(descs));
+        if (out_status != nullptr) *out_status = OCEAN_OK;
+        return reinterpret_cast<ocean_amd_batch*>(b);
+    } catch (const std::invalid_argument&) {
+        return fail(OCEAN_ERROR_INVALID_ARG);
+    } catch (const std::bad_alloc&) {
+        return fail(OCEAN_ERROR_OUT_OF_MEMORY);
+    } catch (...) {
+        return fail(OCEAN_ERROR_UNKNOWN);
+    }
+}
+
+void ocean_amd_batch_destroy(ocean_amd_batch* batch) { delete to_batch(batch); }
+void ocean_amd_batch_update(ocean_amd_batch* batch, double time)
+{
+    if (batch != nullptr) to_batch(batch)->update(time);
+}
+uint32_t ocean_amd_batch_level_count(const ocean_amd_batch* batch)
+{
+    return batch == nullptr ? 0u : static_cast<uint32_t>(to_batch(batch)->level_count());
+}
+const ocean_sim* ocean_amd_batch_level(const ocean_amd_batch* batch, uint32_t level)
+{
+    if (!batch_range(batch, level, 1)) return nullptr;
+    return reinterpret_cast<const ocean_sim*>(&to_batch(batch)->level(level));
+}
+int ocean_amd_batch_sample(const ocean_amd_batch* batch, uint32_t first, uint32_t level_count,
+                           const float* xz, uint32_t count, ocean_surface_ex* out)
+{
+    if (!batch_range(batch, first, level_count)) return OCEAN_ERROR_INVALID_ARG;
+    if (count == 0) return OCEAN_OK;
+    if (xz == nullptr || out == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    return to_batch(batch)->sample(first, level_count, xz, count,
+                                   reinterpret_cast<ocean::Surface*>(out))
+               ? OCEAN_OK
+               : OCEAN_ERROR_UNKNOWN;
+}
+int ocean_amd_batch_sample_device(const ocean_amd_batch* batch, uint32_t first,
+                                  uint32_t level_count, const float* xz_dev, uint32_t count,
+                                  ocean_surface_ex* out_dev, float* heights_dev)
+{
+    if (!batch_range(batch, first, level_count)) return OCEAN_ERROR_INVALID_ARG;
+    if (count == 0) return OCEAN_OK;
+    if (xz_dev == nullptr || (out_dev == nullptr && heights_dev == nullptr))
+        return OCEAN_ERROR_INVALID_ARG;
+    return to_batch(batch)->sample_device(first, level_count, xz_dev, count,
+                                          reinterpret_cast<ocean::Surface*>(out_dev), heights_dev)
+               ? OCEAN_OK
+               : OCEAN_ERROR_UNKNOWN;
+}
+void ocean_amd_batch_autotune(ocean_amd_batch* batch, uint32_t reps)
+{
+    if (batch != nullptr) to_batch(batch)->autotune(reps);
+}
+uint32_t ocean_amd_batch_group_count(const ocean_amd_batch* batch)
+{
+    return batch == nullptr ? 0u : static_cast<uint32_t>(to_batch(batch)->group_count());
+}
+int ocean_amd_batch_group_info(const ocean_amd_batch* batch, uint32_t group,
+                               ocean_amd_batch_group* out)
+{
+    if (batch == nullptr || out == nullptr || group >= to_batch(batch)->group_count())
+        return OCEAN_ERROR_INVALID_ARG;
+    const ocean::BatchGroupInfo g = to_batch(batch)->group(group);
+    out->size = g.size;
+    out->velocity = g.velocity ? 1u : 0u;
+    out->levels = g.levels;
+    out->items = g.items;
+    out->grid_row = g.grid_row;
+    out->grid_col = g.grid_col;
+    out->row_ms = g.row_ms;
+    out->col_ms = g.col_ms;
+    return OCEAN_OK;
+}
+int ocean_amd_batch_checksums(const ocean_amd_batch* batch, uint32_t first, uint32_t count,
+                              uint64_t* out)
+{
+    if (!batch_range(batch, first, count) || out == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    return to_batch(batch)->checksums(first, count, out) ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
+}
+int ocean_amd_batch_synchronize(const ocean_amd_batch* batch)
+{
+    if (batch == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    to_batch(batch)->synchronize();
+    return to_batch(batch)->last_error() == 0 ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
+}
+int ocean_amd_batch_last_error(const ocean_amd_batch* batch)
+{
+    return batch == nullptr ? 0 : to_batch(batch)->last_error();
+}
+
+}  // extern "C"
+
 // ---- ocean.h: local interaction (reference src/c_api.cpp:256-518) ----------
 
 #include "ocean/interaction.hpp"

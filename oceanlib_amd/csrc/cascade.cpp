@@ -50,7 +50,7 @@ CascadeStack::CascadeStack(std::span<const CascadeLevel> levels)
     detail::hip_check(hipStreamCreateWithFlags(&m.stream.s, hipStreamNonBlocking),
                       "hipStreamCreate");
     m.levels.reserve(levels.size());
-    for (const CascadeLevel& d : levels) m.levels.push_back(Ocean(d, m.stream.s));
+    for (const CascadeLevel& d : levels) m.levels.push_back(Ocean(d, m.stream.s, false));
 
     const std::size_t lv_bytes = (levels.size() * sizeof(ocean_hip_level) + 255) / 256 * 256;
     const std::size_t q = detail::kQueryStage;

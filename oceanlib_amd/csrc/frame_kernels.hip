@@ -1,0 +1,346 @@
+// Frame kernels: the row pass and the column pass of one frame
+// (frame_device.hpp), launched either per ocean level or batched.
+//
+// Per level: row_pass_kernel / col_pass_kernel with block shapes chosen per N
+// (mirror-paired row teams; C columns per block, lanes column-fastest so each
+// global access is C consecutive float2).
+//
+// Batched: the levels of a batch are grouped by (N, F); each group is stepped
+// by one persistent launch per pass. A 256-lane block processes one work item
+// at a time - PER = 2048/N rows, or PER columns, of one level - and strides
+// through the group's item list by gridDim.x, so the grid size is a tuning
+// knob (the autotuner's) that never changes arithmetic. In the batch's
+// hipGraph each group is a row -> column chain and the chains of different
+// groups are independent, so they overlap on the device.
+#include "frame_device.hpp"
+#include "hip_launch.h"
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace oceanamd {
+namespace {
+
+constexpr int kBlock = 256;
+
+template <int N>
+struct RowShape {
+    static constexpr int T = FftShape<N>::T;
+    static constexpr int B0 = T > kBlock ? T : kBlock;
+    static constexpr int BLOCK = B0 < N * T ? B0 : N * T;  // never more teams than rows
+    static constexpr int TEAMS = BLOCK / T;
+};
+
+template <int N>
+struct ColShape {
+    static constexpr int T = FftShape<N>::T;
+    static constexpr int C0 = (kBlock / T) > 8 ? (kBlock / T) : 8;
+    static constexpr int C1 = C0 < N ? C0 : N;
+    static constexpr int C2 = (C1 * T) > 512 ? (512 / T) : C1;  // <= 512 lanes: no spills at F=6
+    static constexpr int C = (C2 * N * 8) > 65536 ? (65536 / (N * 8)) : C2;
+    static constexpr int BLOCK = C * T;
+};
+
+template <int N, int F>
+__global__ __launch_bounds__(RowShape<N>::BLOCK) void row_pass_kernel(ocean_hip_level L,
+                                                                      double time_over_two_pi)
+{
+    __shared__ float2 tw[N / 2];
+    __shared__ float2 xch[RowShape<N>::TEAMS * lds_span<N>()];
+    row_block<N, F, RowShape<N>::TEAMS>(L, blockIdx.x, time_over_two_pi, tw, xch);
+}
+
+template <int N, int F>
+__global__ __launch_bounds__(ColShape<N>::BLOCK) void col_pass_kernel(ocean_hip_level L)
+{
+    __shared__ float2 tw[N / 2];
+    __shared__ float2 xch[ColShape<N>::C * lds_span<N>()];
+    col_block<N, F, ColShape<N>::C>(L, blockIdx.x, tw, xch);
+}
+
+template <int N>
+hipError_t launch_frame_n(const ocean_hip_level* L, double time_over_two_pi, hipStream_t s,
+                          unsigned passes)
+{
+    const dim3 rg(N / RowShape<N>::TEAMS), rb(RowShape<N>::BLOCK);
+    const dim3 cg(N / ColShape<N>::C), cb(ColShape<N>::BLOCK);
+    if (L->vel != nullptr) {
+        if (passes & 1u) hipLaunchKernelGGL((row_pass_kernel<N, 6>), rg, rb, 0, s, *L, time_over_two_pi);
+        if (passes & 2u) hipLaunchKernelGGL((col_pass_kernel<N, 6>), cg, cb, 0, s, *L);
+    } else {
+        if (passes & 1u) hipLaunchKernelGGL((row_pass_kernel<N, 4>), rg, rb, 0, s, *L, time_over_two_pi);
+        if (passes & 2u) hipLaunchKernelGGL((col_pass_kernel<N, 4>), cg, cb, 0, s, *L);
+    }
+    return hipGetLastError();
+}
+
+// ---- batched persistent passes (N in [64, 2048], 256-lane blocks) --------
+
+constexpr uint32_t kItemBlockBits = 12;
+constexpr uint32_t kItemBlockMask = (1u << kItemBlockBits) - 1u;
+
+template <int N>
+struct BatchShape {
+    static constexpr int PER = 2048 / N;  // rows (row pass) or columns (column pass) per item
+    static_assert(PER * FftShape<N>::T == kBlock, "batched blocks are 256 lanes");
+};
+
+template <int N, int F>
+__global__ __launch_bounds__(kBlock) void batch_row_kernel(const ocean_hip_level* __restrict__ levels,
+                                                           const uint32_t* __restrict__ items,
+                                                           uint32_t n_items, double t2)
+{
+    __shared__ float2 tw[N / 2];
+    __shared__ float2 xch[BatchShape<N>::PER * lds_span<N>()];
+    for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
+        const uint32_t it = items[i];
+        row_block<N, F, BatchShape<N>::PER>(levels[it >> kItemBlockBits], it & kItemBlockMask, t2,
+                                            tw, xch);
+    }
+}
+
+template <int N, int F>
+__global__ __launch_bounds__(kBlock) void batch_col_kernel(const ocean_hip_level* __restrict__ levels,
+                                                           const uint32_t* __restrict__ items,
+                                                           uint32_t n_items)
+{
+    __shared__ float2 tw[N / 2];
+    __shared__ float2 xch[BatchShape<N>::PER * lds_span<N>()];
+    for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
+        const uint32_t it = items[i];
+        col_block<N, F, BatchShape<N>::PER>(levels[it >> kItemBlockBits], it & kItemBlockMask,
+                                            tw, xch);
+    }
+}
+
+template <int N>
+void* batch_fn(int fields, bool row)
+{
+    if (fields == 6)
+        return row ? reinterpret_cast<void*>(batch_row_kernel<N, 6>)
+                   : reinterpret_cast<void*>(batch_col_kernel<N, 6>);
+    return row ? reinterpret_cast<void*>(batch_row_kernel<N, 4>)
+               : reinterpret_cast<void*>(batch_col_kernel<N, 4>);
+}
+
+void* batch_kernel(uint32_t n, int fields, bool row)
+{
+    switch (n) {
+        case 64: return batch_fn<64>(fields, row);
+        case 128: return batch_fn<128>(fields, row);
+        case 256: return batch_fn<256>(fields, row);
+        case 512: return batch_fn<512>(fields, row);
+        case 1024: return batch_fn<1024>(fields, row);
+        case 2048: return batch_fn<2048>(fields, row);
+        default: return nullptr;
+    }
+}
+
+// Node parameters of one group's pass; `args` must outlive the call using them.
+hipKernelNodeParams group_params(ocean_hip_batch_group& g, bool row, double* t2, void** args)
+{
+    args[0] = &g.levels;
+    args[1] = &g.items;
+    args[2] = &g.n_items;
+    args[3] = t2;
+    hipKernelNodeParams p{};
+    p.func = batch_kernel(g.n, static_cast<int>(g.fields), row);
+    p.gridDim = dim3(row ? g.grid_row : g.grid_col);
+    p.blockDim = dim3(kBlock);
+    p.sharedMemBytes = 0;
+    p.kernelParams = args;
+    p.extra = nullptr;
+    return p;
+}
+
+}  // namespace
+}  // namespace oceanamd
+
+using namespace oceanamd;
+
+extern "C" hipError_t ocean_hip_frame_passes(const ocean_hip_level* L, double time_over_two_pi,
+                                             hipStream_t s, unsigned passes)
+{
+    switch (L->n) {
+        case 16: return launch_frame_n<16>(L, time_over_two_pi, s, passes);
+        case 32: return launch_frame_n<32>(L, time_over_two_pi, s, passes);
+        case 64: return launch_frame_n<64>(L, time_over_two_pi, s, passes);
+        case 128: return launch_frame_n<128>(L, time_over_two_pi, s, passes);
+        case 256: return launch_frame_n<256>(L, time_over_two_pi, s, passes);
+        case 512: return launch_frame_n<512>(L, time_over_two_pi, s, passes);
+        case 1024: return launch_frame_n<1024>(L, time_over_two_pi, s, passes);
+        case 2048: return launch_frame_n<2048>(L, time_over_two_pi, s, passes);
+        case 4096: return launch_frame_n<4096>(L, time_over_two_pi, s, passes);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+extern "C" hipError_t ocean_hip_frame(const ocean_hip_level* L, double time_over_two_pi,
+                                      hipStream_t s)
+{
+    return ocean_hip_frame_passes(L, time_over_two_pi, s, 3u);
+}
+
+extern "C" int ocean_hip_batchable(uint32_t n) { return batch_kernel(n, 4, true) != nullptr; }
+
+extern "C" uint32_t ocean_hip_batch_items(uint32_t n) { return n * n / 2048u; }
+
+extern "C" uint32_t ocean_hip_batch_item(uint32_t level, uint32_t block)
+{
+    return (level << kItemBlockBits) | block;
+}
+
+extern "C" hipError_t ocean_hip_batch_group_frame(const ocean_hip_batch_group* group,
+                                                  double time_over_two_pi, unsigned passes,
+                                                  hipStream_t s)
+{
+    ocean_hip_batch_group g = *group;
+    if (g.n_items == 0) return hipSuccess;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (!(passes & (1u << pass))) continue;
+        double t2 = time_over_two_pi;
+        void* args[4];
+        const hipKernelNodeParams p = group_params(g, pass == 0, &t2, args);
+        if (p.func == nullptr) return hipErrorInvalidValue;
+        const hipError_t e = hipLaunchKernel(p.func, p.gridDim, p.blockDim, p.kernelParams, 0, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// The batch's frame as one instantiated graph: per group a row node and a
+// column node depending on it; the only per-frame input, the time, is a
+// kernel argument of the row nodes, replaced before each launch.
+struct ocean_hip_batch_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<ocean_hip_batch_group> groups;
+    std::vector<hipGraphNode_t> row_nodes;
+};
+
+extern "C" hipError_t ocean_hip_batch_graph_create(const ocean_hip_batch_group* groups,
+                                                   uint32_t count, ocean_hip_batch_graph** out)
+{
+    auto* g = new ocean_hip_batch_graph();
+    g->groups.assign(groups, groups + count);
+    hipError_t e = hipGraphCreate(&g->graph, 0);
+    for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
+        ocean_hip_batch_group& gr = g->groups[i];
+        if (gr.n_items == 0) continue;
+        double t2 = 0.0;
+        void* rargs[4];
+        void* cargs[4];
+        const hipKernelNodeParams rp = group_params(gr, true, &t2, rargs);
+        const hipKernelNodeParams cp = group_params(gr, false, &t2, cargs);
+        if (rp.func == nullptr || cp.func == nullptr) {
+            e = hipErrorInvalidValue;
+            break;
+        }
+        hipGraphNode_t rn = nullptr, cn = nullptr;
+        e = hipGraphAddKernelNode(&rn, g->graph, nullptr, 0, &rp);
+        if (e == hipSuccess) e = hipGraphAddKernelNode(&cn, g->graph, &rn, 1, &cp);
+        g->row_nodes.push_back(rn);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        ocean_hip_batch_graph_destroy(g);
+        return e;
+    }
+    *out = g;
+    return hipSuccess;
+}
+
+extern "C" hipError_t ocean_hip_batch_graph_launch(ocean_hip_batch_graph* g,
+                                                   double time_over_two_pi, hipStream_t s)
+{
+    std::size_t k = 0;
+    for (ocean_hip_batch_group& gr : g->groups) {
+        if (gr.n_items == 0) continue;
+        double t2 = time_over_two_pi;
+        void* args[4];
+        const hipKernelNodeParams rp = group_params(gr, true, &t2, args);
+        const hipError_t e = hipGraphExecKernelNodeSetParams(g->exec, g->row_nodes[k++], &rp);
+        if (e != hipSuccess) return e;
+    }
+    return hipGraphLaunch(g->exec, s);
+}
+
+extern "C" void ocean_hip_batch_graph_destroy(ocean_hip_batch_graph* g)
+{
+    if (g == nullptr) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+}
+
+extern "C" int ocean_hip_batch_occupancy(uint32_t n, uint32_t fields, int row)
+{
+    void* fn = batch_kernel(n, static_cast<int>(fields), row != 0);
+    if (fn == nullptr) return 0;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, kBlock, 0) != hipSuccess) return 0;
+    return blocks;
+}
+
+// ---- buffer checksums -----------------------------------------------------
+// Per level: the output buffers (displacement, normal, then velocity when
+// present) as uint32 words w_i, i counted across the buffers in that order,
+// summed as sum(w_i * K_i) mod 2^64 with K_i = (i * STEP + BASE) | 1. Addition
+// mod 2^64 is commutative, so the block-level split and the atomics give the
+// same value as a sequential sum.
+
+namespace oceanamd {
+namespace {
+
+constexpr uint64_t kChecksumStep = 0x9E3779B97F4A7C15ull;
+constexpr uint64_t kChecksumBase = 0x632BE59BD9B4E019ull;
+constexpr uint32_t kChecksumBlocksPerLevel = 32;
+
+__global__ __launch_bounds__(256) void checksum_kernel(const ocean_hip_level* __restrict__ levels,
+                                                       unsigned long long* __restrict__ out)
+{
+    __shared__ unsigned long long part[256];
+    const uint32_t lv = blockIdx.x / kChecksumBlocksPerLevel;
+    const uint32_t blk = blockIdx.x % kChecksumBlocksPerLevel;
+    const ocean_hip_level L = levels[lv];
+    const uint64_t per = 4ull * L.n * L.n;  // words per buffer
+    const uint32_t nbuf = L.vel != nullptr ? 3u : 2u;
+    const uint64_t total = per * nbuf;
+    uint64_t sum = 0;
+    for (uint64_t i = uint64_t(blk) * 256u + threadIdx.x; i < total;
+         i += uint64_t(kChecksumBlocksPerLevel) * 256u) {
+        const uint32_t b = uint32_t(i / per);
+        const uint64_t j = i - uint64_t(b) * per;
+        const float* buf = b == 0 ? L.disp : (b == 1 ? L.normal : L.vel);
+        const uint32_t w = reinterpret_cast<const uint32_t*>(buf)[j];
+        sum += uint64_t(w) * ((i * kChecksumStep + kChecksumBase) | 1ull);
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out + lv, part[0]);
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+}  // namespace
+}  // namespace oceanamd
+
+extern "C" hipError_t ocean_hip_level_checksums(const ocean_hip_level* levels, uint32_t count,
+                                                unsigned long long* out, hipStream_t s)
+{
+    if (count == 0) return hipSuccess;
+    if (uint64_t(count) * kChecksumBlocksPerLevel > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3((count + 255) / 256), dim3(256), 0, s, out, count);
+    hipLaunchKernelGGL(checksum_kernel, dim3(count * kChecksumBlocksPerLevel), dim3(256), 0, s,
+                       levels, out);
+    return hipGetLastError();
+}

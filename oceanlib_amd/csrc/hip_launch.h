@@ -72,6 +72,44 @@ uint32_t ocean_hip_coverage_blocks(uint32_t n);
 hipError_t ocean_hip_coverage(const float* v, uint32_t n, double* partial, double* out,
                               hipStream_t s);
 
+/* ---- batched frames ------------------------------------------------------
+ * Levels with N in [64, 2048] can be stepped together: every level is cut
+ * into N*N/2048 work items (rows for the row pass, columns for the column
+ * pass) of 256 lanes, and one persistent launch per pass strides a block
+ * through the item list. Items are packed as (level << 12) | block. */
+int ocean_hip_batchable(uint32_t n);
+uint32_t ocean_hip_batch_items(uint32_t n);
+uint32_t ocean_hip_batch_item(uint32_t level, uint32_t block);
+/* One (N, F) group of a batch: its work items over the batch's level array. */
+typedef struct ocean_hip_batch_group {
+    uint32_t n;                    /* N of every level in the group            */
+    uint32_t fields;               /* 4, or 6 with velocity                    */
+    const ocean_hip_level* levels; /* device array of the whole batch's levels */
+    const uint32_t* items;         /* device array of this group's items       */
+    uint32_t n_items;
+    uint32_t grid_row;             /* persistent grid sizes (autotuned)        */
+    uint32_t grid_col;
+    uint32_t pad_;
+} ocean_hip_batch_group;
+/* Direct launches of one group (passes: 1 row, 2 column, 3 both), for tuning. */
+hipError_t ocean_hip_batch_group_frame(const ocean_hip_batch_group* group,
+                                       double time_over_two_pi, unsigned passes, hipStream_t s);
+/* Every group's frame as one instantiated hipGraph; launch sets the time. */
+typedef struct ocean_hip_batch_graph ocean_hip_batch_graph;
+hipError_t ocean_hip_batch_graph_create(const ocean_hip_batch_group* groups, uint32_t count,
+                                        ocean_hip_batch_graph** out);
+hipError_t ocean_hip_batch_graph_launch(ocean_hip_batch_graph* g, double time_over_two_pi,
+                                        hipStream_t s);
+void ocean_hip_batch_graph_destroy(ocean_hip_batch_graph* g);
+/* Resident blocks per CU of a group's batched kernel (occupancy). */
+int ocean_hip_batch_occupancy(uint32_t n, uint32_t fields, int row);
+
+/* Per level of a device array of `count` levels: the checksum of its output
+ * buffers (displacement, normal, velocity words w_i; sum of w_i * K_i mod
+ * 2^64, K_i = (i * 0x9E3779B97F4A7C15 + 0x632BE59BD9B4E019) | 1) into out[]. */
+hipError_t ocean_hip_level_checksums(const ocean_hip_level* levels, uint32_t count,
+                                     unsigned long long* out, hipStream_t s);
+
 /* ---- iWave interaction field -------------------------------------------- */
 
 /* Device view of one interaction field. Planes have a P-cell zero halo;

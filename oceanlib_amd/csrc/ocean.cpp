@@ -70,6 +70,7 @@ OceanDevice::~OceanDevice()
 {
     if (stream) hipStreamSynchronize(stream);
     if (arena) hipFree(arena);
+    if (q_stage) hipFree(q_stage);
     if (h_disp) hipHostFree(h_disp);
     if (h_normal) hipHostFree(h_normal);
     if (h_vel) hipHostFree(h_vel);
@@ -84,7 +85,7 @@ void OceanDevice::make_current() const noexcept
     if (hipGetDevice(&cur) == hipSuccess && cur != device) hipSetDevice(device);
 }
 
-void OceanDevice::create(const OceanDesc& desc, hipStream_t shared_stream)
+void OceanDevice::create(const OceanDesc& desc, hipStream_t shared_stream, bool lazy_host)
 {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
@@ -113,8 +114,11 @@ void OceanDevice::create(const OceanDesc& desc, hipStream_t shared_stream)
     const std::size_t o_norm = off;   off = align_up(off + cells * 16, a);
     std::size_t o_vel = 0;
     if (desc.compute_velocity) { o_vel = off; off = align_up(off + cells * 16, a); }
-    const std::size_t o_qxz = off;    off = align_up(off + std::size_t(kQueryStage) * 8, a);
-    const std::size_t o_qout = off;   off = align_up(off + std::size_t(kQueryStage) * sizeof(ocean_hip_surface), a);
+    // Query staging only for oceans that answer host queries themselves; the
+    // levels of a batch get it on first use.
+    const std::size_t q = lazy_host ? 0 : std::size_t(kQueryStage);
+    const std::size_t o_qxz = off;    off = align_up(off + q * 8, a);
+    const std::size_t o_qout = off;   off = align_up(off + q * sizeof(ocean_hip_surface), a);
     const std::size_t o_lvl = off;    off = align_up(off + sizeof(ocean_hip_level), a);
 
     hip_check(hipMalloc(&arena, off), "hipMalloc");
@@ -133,21 +137,31 @@ void OceanDevice::create(const OceanDesc& desc, hipStream_t shared_stream)
     level.disp = reinterpret_cast<float*>(base + o_disp);
     level.normal = reinterpret_cast<float*>(base + o_norm);
     level.vel = desc.compute_velocity ? reinterpret_cast<float*>(base + o_vel) : nullptr;
-    q_xz = reinterpret_cast<float*>(base + o_qxz);
-    q_out = reinterpret_cast<ocean_hip_surface*>(base + o_qout);
+    if (!lazy_host) {
+        q_xz = reinterpret_cast<float*>(base + o_qxz);
+        q_out = reinterpret_cast<ocean_hip_surface*>(base + o_qout);
+    }
     d_level = reinterpret_cast<ocean_hip_level*>(base + o_lvl);
 
-    hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_disp), cells * 16, hipHostMallocDefault), "hipHostMalloc");
-    hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_normal), cells * 16, hipHostMallocDefault), "hipHostMalloc");
-    if (desc.compute_velocity)
-        hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_vel), cells * 16, hipHostMallocDefault), "hipHostMalloc");
-    hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_q_xz), std::size_t(kQueryStage) * 8, hipHostMallocDefault), "hipHostMalloc");
-    hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_q_out), std::size_t(kQueryStage) * sizeof(ocean_hip_surface), hipHostMallocDefault), "hipHostMalloc");
+    std::vector<float> staging;
+    float* h0_host;
+    float* omega_host;
+    if (!lazy_host) {
+        alloc_host_mirrors();
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_q_xz), std::size_t(kQueryStage) * 8, hipHostMallocDefault), "hipHostMalloc");
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_q_out), std::size_t(kQueryStage) * sizeof(ocean_hip_surface), hipHostMallocDefault), "hipHostMalloc");
+        if (status != 0) throw std::bad_alloc();
+        // staged through the (not yet used) pinned mirrors
+        h0_host = h_disp;               // cells*8 bytes fit in cells*16
+        omega_host = h_normal;
+    } else {
+        staging.resize(cells * 3);
+        h0_host = staging.data();
+        omega_host = staging.data() + cells * 2;
+    }
 
-    // Constant tables: built on the host with the reference's libm chain,
-    // staged through the (not yet used) pinned mirrors, uploaded once.
-    float* h0_host = h_disp;            // cells*8 bytes fit in cells*16
-    float* omega_host = h_normal;
+    // Constant tables: built on the host with the reference's libm chain and
+    // uploaded once.
     build_tables_threaded(desc, h0_host, omega_host);
     std::vector<float> tw(std::max<std::size_t>(n, 2));
     oceanamd::build_twiddles(n, tw.data());
@@ -167,10 +181,20 @@ void OceanDevice::enqueue_frame(double time) noexcept
     mirror_stale = true;
 }
 
+void OceanDevice::alloc_host_mirrors() noexcept
+{
+    const std::size_t bytes = static_cast<std::size_t>(level.n) * level.n * 16;
+    if (!h_disp) note(hipHostMalloc(reinterpret_cast<void**>(&h_disp), bytes, hipHostMallocDefault));
+    if (!h_normal) note(hipHostMalloc(reinterpret_cast<void**>(&h_normal), bytes, hipHostMallocDefault));
+    if (level.vel && !h_vel) note(hipHostMalloc(reinterpret_cast<void**>(&h_vel), bytes, hipHostMallocDefault));
+}
+
 void OceanDevice::refresh_mirror() noexcept
 {
     if (!mirror_stale) return;
     make_current();
+    alloc_host_mirrors();
+    if (status != 0) return;
     const std::size_t bytes = static_cast<std::size_t>(level.n) * level.n * 16;
     note(hipMemcpyAsync(h_disp, level.disp, bytes, hipMemcpyDeviceToHost, stream));
     note(hipMemcpyAsync(h_normal, level.normal, bytes, hipMemcpyDeviceToHost, stream));
@@ -183,6 +207,16 @@ bool OceanDevice::sample_host(const float* xz, std::uint32_t count, Surface* out
 {
     static_assert(sizeof(Surface) == sizeof(ocean_hip_surface));
     make_current();
+    if (q_xz == nullptr) {  // a batch level's first host query: its staging, once
+        void* d = nullptr;
+        note(hipMalloc(&d, std::size_t(kQueryStage) * (8 + sizeof(ocean_hip_surface))));
+        note(hipHostMalloc(reinterpret_cast<void**>(&h_q_xz), std::size_t(kQueryStage) * 8, hipHostMallocDefault));
+        note(hipHostMalloc(reinterpret_cast<void**>(&h_q_out), std::size_t(kQueryStage) * sizeof(ocean_hip_surface), hipHostMallocDefault));
+        if (status != 0) return false;
+        q_stage = d;
+        q_xz = static_cast<float*>(d);
+        q_out = reinterpret_cast<ocean_hip_surface*>(q_xz + 2 * std::size_t(kQueryStage));
+    }
     for (std::uint32_t done = 0; done < count;) {
         const std::uint32_t m = std::min(kQueryStage, count - done);
         std::memcpy(h_q_xz, xz + 2 * std::size_t(done), std::size_t(m) * 8);
@@ -205,14 +239,14 @@ struct Ocean::Impl {
     detail::OceanDevice dev;
 };
 
-Ocean::Ocean(const OceanDesc& desc) : Ocean(desc, nullptr) {}
+Ocean::Ocean(const OceanDesc& desc) : Ocean(desc, nullptr, false) {}
 
-Ocean::Ocean(const OceanDesc& desc, void* shared_stream)
+Ocean::Ocean(const OceanDesc& desc, void* shared_stream, bool lazy_host)
 {
     validate(desc);
     impl_ = std::make_unique<Impl>();
     impl_->desc = desc;
-    impl_->dev.create(desc, static_cast<hipStream_t>(shared_stream));
+    impl_->dev.create(desc, static_cast<hipStream_t>(shared_stream), lazy_host);
     // Valid, fully evaluated state at t = 0 before returning (reference
     // src/ocean.cpp:271-278).
     impl_->dev.enqueue_frame(0.0);
@@ -299,5 +333,13 @@ int Ocean::last_error() const noexcept { return impl_->dev.status; }
 const OceanDesc& Ocean::desc() const noexcept { return impl_->desc; }
 double Ocean::time() const noexcept { return impl_->time; }
 detail::OceanDevice& Ocean::device() const noexcept { return impl_->dev; }
+
+void Ocean::note_frame(double time) noexcept
+{
+    impl_->time = time;
+    impl_->dev.mirror_stale = true;
+}
+
+void detail::validate_desc(const OceanDesc& desc) { validate(desc); }
 
 }  // namespace ocean

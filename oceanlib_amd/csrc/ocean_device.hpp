@@ -21,6 +21,11 @@ inline void hip_check(hipError_t e, const char* what)
     throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// The reference's descriptor checks (src/ocean.cpp:28-59); throws
+// std::invalid_argument. Owners of several levels run it on every level
+// before touching the GPU.
+void validate_desc(const OceanDesc& desc);
+
 // Largest batch one host-array query call stages at a time.
 inline constexpr std::uint32_t kQueryStage = 65536;
 
@@ -33,6 +38,7 @@ struct OceanDevice {
     int fields = 4;            // complex fields per frame: 4, or 6 with velocity
 
     void* arena = nullptr;     // one hipMalloc holds every device buffer
+    void* q_stage = nullptr;   // lazily allocated query staging (batch levels)
     float* q_xz = nullptr;     // device query staging
     ocean_hip_surface* q_out = nullptr;
 
@@ -51,7 +57,10 @@ struct OceanDevice {
     ~OceanDevice();
 
     // Allocates and uploads; throws on failure (state is released by the dtor).
-    void create(const OceanDesc& desc, hipStream_t shared_stream);
+    // lazy_host: no pinned mirrors or query staging until first asked for
+    // (the levels of a large batch).
+    void create(const OceanDesc& desc, hipStream_t shared_stream, bool lazy_host = false);
+    void alloc_host_mirrors() noexcept;
 
     void note(hipError_t e) noexcept
     {

@@ -87,3 +87,98 @@ def config3_hull_mask(n: int = 256) -> np.ndarray:
     m = np.zeros((n, n), np.uint8)
     m[120 * n // 256:136 * n // 256, 100 * n // 256:156 * n // 256] = 1
     return m
+
+
+class Pcg32:
+    """PCG32 (XSH-RR 64/32, O'Neill 2014), the generator the library itself
+    uses for h0 (src/core/rng.hpp:35-62): seeding as pcg32_srandom."""
+
+    _MULT = 6364136223846793005
+    _MASK = (1 << 64) - 1
+
+    def __init__(self, seed: int, stream: int):
+        self.state = 0
+        self.inc = ((stream << 1) | 1) & self._MASK
+        self.next()
+        self.state = (self.state + seed) & self._MASK
+        self.next()
+
+    def next(self) -> int:
+        old = self.state
+        self.state = (old * self._MULT + self.inc) & self._MASK
+        xs = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return ((xs >> rot) | (xs << ((32 - rot) & 31))) & 0xFFFFFFFF
+
+    def uniform(self) -> float:
+        """[0, 1) with 32 random bits."""
+        return self.next() / 4294967296.0
+
+    def below(self, n: int) -> int:
+        """Integer in [0, n), rejection-sampled (no modulo bias)."""
+        threshold = (1 << 32) % n
+        while True:
+            r = self.next()
+            if r >= threshold:
+                return r % n
+
+
+# Batched uneven seas: 256 seas from PCG32(seed 0xC4, stream 1), drawn per sea
+# in this order: N uniform over {64, 128, 256, 512}; patch length
+# log-uniform over [50, 1000] m; wind uniform over [3, 25] m/s; depth infinite
+# or uniform over [10, 200] m (50/50); 1-3 cascade levels. Level k of sea i
+# has patch length L * CONFIG4_LEVEL_RATIO^k and seed i. 300 frames at 1/60 s.
+CONFIG4_SEAS = 256
+CONFIG4_FRAMES = 300
+CONFIG4_LEVEL_RATIO = 0.19
+
+
+def config4_seas(count: int = CONFIG4_SEAS, seed: int = 0xC4):
+    """List of seas; each sea is a list of level kwargs for desc_from."""
+    rng = Pcg32(seed, 1)
+    seas = []
+    for i in range(count):
+        n = (64, 128, 256, 512)[rng.below(4)]
+        length = 50.0 * 20.0 ** rng.uniform()
+        wind = 3.0 + 22.0 * rng.uniform()
+        depth = 0.0 if rng.uniform() < 0.5 else 10.0 + 190.0 * rng.uniform()
+        levels = 1 + rng.below(3)
+        seas.append([dict(size=n, patch_length=float(np.float32(length * CONFIG4_LEVEL_RATIO ** k)),
+                          wind_speed=float(np.float32(wind)), water_depth=float(np.float32(depth)),
+                          seed=i, thread_count=1)
+                     for k in range(levels)])
+    return seas
+
+
+def config4_levels(count: int = CONFIG4_SEAS):
+    """The seas' levels flattened in sea order, with (sea, level) labels."""
+    flat, labels = [], []
+    for i, sea in enumerate(config4_seas(count)):
+        for k, kw in enumerate(sea):
+            flat.append(kw)
+            labels.append((i, k))
+    return flat, labels
+
+
+def config4_cells(count: int = CONFIG4_SEAS) -> int:
+    return sum(kw["size"] ** 2 for kw in config4_levels(count)[0])
+
+
+# Buffer checksum used for the batched-seas fixtures: the level's output
+# buffers (displacement, then normal, then velocity when present) read as
+# little-endian uint32 words w_i, summed as sum(w_i * K_i) mod 2^64 with odd
+# K_i = (i * 0x9E3779B97F4A7C15 + 0x632BE59BD9B4E019) | 1. Any single changed
+# word changes the sum (K_i is odd, so w * K_i == 0 mod 2^64 forces w == 0).
+CHECKSUM_STEP = 0x9E3779B97F4A7C15
+CHECKSUM_BASE = 0x632BE59BD9B4E019
+
+
+def checksum_weights(words: int) -> np.ndarray:
+    i = np.arange(words, dtype=np.uint64)
+    return (i * np.uint64(CHECKSUM_STEP) + np.uint64(CHECKSUM_BASE)) | np.uint64(1)
+
+
+def checksum_words(words: np.ndarray, weights: np.ndarray | None = None) -> int:
+    w = np.ascontiguousarray(words).view(np.uint32).astype(np.uint64).ravel()
+    k = checksum_weights(w.size) if weights is None else weights
+    return int(np.sum(w * k, dtype=np.uint64))

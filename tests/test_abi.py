@@ -186,3 +186,43 @@ def test_library_host_tables_equal_oracle(kw):
     oracle().orc_twiddles(n, fptr(a), fptr(b))
     assert_bitexact(tw[0::2], a, "tw cos")
     assert_bitexact(tw[1::2], b, "tw sin")
+
+
+def test_batch_create_rejects_bad_input_without_device():
+    """A batch validates every level like ocean_create before any device work."""
+    lib = capi.load()
+    st = C.c_int(-1)
+    assert not lib.ocean_amd_batch_create(None, 3, C.byref(st))
+    assert st.value == capi.OCEAN_ERROR_INVALID_ARG
+    good = capi.default_desc(lib, size=64)
+    arr = (capi.OceanDesc * 2)(good, good)
+    st = C.c_int(-1)
+    assert not lib.ocean_amd_batch_create(arr, 0, C.byref(st))
+    assert st.value == capi.OCEAN_ERROR_INVALID_ARG
+    for kw in (dict(size=100), dict(patch_length=-1.0), dict(choppiness=-0.1)):
+        arr = (capi.OceanDesc * 2)(good, capi.default_desc(lib, **kw))
+        st = C.c_int(-1)
+        assert not lib.ocean_amd_batch_create(arr, 2, C.byref(st)), kw
+        assert st.value == capi.OCEAN_ERROR_INVALID_ARG, kw
+    bad = capi.default_desc(lib)
+    bad.struct_size = 0
+    arr = (capi.OceanDesc * 2)(good, bad)
+    assert not lib.ocean_amd_batch_create(arr, 2, C.byref(st))
+    assert st.value == capi.OCEAN_ERROR_INVALID_ARG
+
+
+def test_batch_null_handles_are_tolerated():
+    lib = capi.load()
+    lib.ocean_amd_batch_destroy(None)
+    lib.ocean_amd_batch_update(None, 1.0)
+    lib.ocean_amd_batch_autotune(None, 3)
+    assert lib.ocean_amd_batch_level_count(None) == 0
+    assert not lib.ocean_amd_batch_level(None, 0)
+    assert lib.ocean_amd_batch_group_count(None) == 0
+    g = capi.BatchGroup()
+    assert lib.ocean_amd_batch_group_info(None, 0, C.byref(g)) == capi.OCEAN_ERROR_INVALID_ARG
+    assert lib.ocean_amd_batch_synchronize(None) == capi.OCEAN_ERROR_INVALID_ARG
+    assert lib.ocean_amd_batch_last_error(None) == 0
+    out = (capi.OceanSurfaceEx * 1)()
+    xz = np.zeros(2, np.float32)
+    assert lib.ocean_amd_batch_sample(None, 0, 1, fptr(xz), 1, out) == capi.OCEAN_ERROR_INVALID_ARG

@@ -62,15 +62,15 @@ def frame_bytes(n: int, fields: int, velocity: bool) -> int:
     return n * n * (12 + 32 + (16 if velocity else 0) + inter)
 
 
-def traffic_from_profiles(kernel: str):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
-    WRITE_SIZE), scaled by the copy-kernel calibration, if recorded."""
+def traffic_from_profiles(kernels):
+    """HBM bytes per launch (summed over `kernels`) measured with rocprofv3
+    --pmc FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for
+    a 256 MiB copy (profiles/traffic_pmc.md); None if not recorded."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            rec = json.load(f)
-        k = rec["kernels"][kernel]
-        return (k["fetch_kib"] + k["write_kib"]) * 1024.0 * rec["copy_factor"]
+            rec = json.load(f)["kernels"]
+        return float(sum(rec[k]["traffic_bytes"] for k in kernels))
     except (OSError, KeyError, ValueError, TypeError):
         return None
 
@@ -126,7 +126,7 @@ def cpu_baseline(desc, frames_budget_s: float = 12.0):
                       f"{QUERIES_PER_FRAME} height_at calls per frame, thread_count={cores}"}
 
 
-def batched_leg(lib, frames: int, warmup: int):
+def batched_leg(lib, frames: int, warmup: int, tune: bool = True):
     """The batched uneven seas (256 seas, 1-3 levels of N 64..512 each) as one
     OceanBatch: autotuned, then `frames` frames timed between two
     synchronizes. Reports 512^2-equivalent patch updates per second (cells
@@ -142,8 +142,9 @@ def batched_leg(lib, frames: int, warmup: int):
     if not b:
         raise SystemExit(f"ocean_amd_batch_create failed: {st.value}")
     build_s = time.perf_counter() - t0
-    log(f"bench: batch of {len(descs)} levels built in {build_s:.1f} s; autotuning")
-    lib.ocean_amd_batch_autotune(b, 5)
+    log(f"bench: batch of {len(descs)} levels built in {build_s:.1f} s")
+    if tune:
+        lib.ocean_amd_batch_autotune(b, 5)
     groups = []
     for g in range(lib.ocean_amd_batch_group_count(b)):
         info = capi_mod().BatchGroup()
@@ -173,7 +174,10 @@ def batched_leg(lib, frames: int, warmup: int):
         "patch_updates_512_per_s": round(cells * fps / (512 * 512), 1),
         "alg_bytes_per_frame": alg,
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5)},
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5),
+                     "traffic": traffic_from_profiles(
+                         [f"batch_{p}_kernel<{g['size']}, {6 if g['velocity'] else 4}>"
+                          for g in groups for p in ("row", "col")])},
         "groups": groups, "build_s": round(build_s, 2), "frames": frames,
     }
 
@@ -229,6 +233,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--batch-frames", type=int, default=120,
                     help="timed frames of the batched-seas leg (0 skips it)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched leg: keep the default grid sizes")
     ap.add_argument("--profile-only", action="store_true",
                     help="run only warmup + timed frames (for rocprofv3 runs)")
     args = ap.parse_args()
@@ -292,7 +298,17 @@ def main():
     ms_per_step = el * 1000.0 / args.steps
     value = args.steps / el
     if args.profile_only:
-        print(json.dumps({"profile_only": True, "ms_per_step": ms_per_step}), flush=True)
+        # A plain copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE
+        # counters (256 MiB read + 256 MiB written per launch).
+        src = torch.empty(1 << 26, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        rec = {"profile_only": True, "ms_per_step": ms_per_step, "copy_bytes": 4 << 26}
+        if args.batch_frames > 0:
+            rec["batched"] = batched_leg(lib, args.batch_frames, 2, tune=not args.no_autotune)
+        print(json.dumps(rec), flush=True)
         return
 
     # Per-kernel device time (HIP events on the ocean's stream).
@@ -311,7 +327,7 @@ def main():
     achieved = kernels[dom]["gbs"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": traffic_from_profiles(dom), "kernel": dom,
+                "traffic": traffic_from_profiles([dom]), "kernel": dom,
                 "alg_bytes_per_launch": kernels[dom]["alg_bytes"]}
 
     # height_at throughput: 1M queries per launch on the same sea.

@@ -22,7 +22,31 @@ namespace ocean {
 
 namespace {
 constexpr double kInvTwoPiD = 0.159154943091895335769;
+constexpr std::size_t kXcds = 8;
+
+// Workgroups go to the 8 XCDs round-robin by block id and each XCD has its
+// own L2; the column items of one level share 128-byte lines with their
+// neighbours. A persistent block b of a grid G (G a multiple of 8, or one
+// block per item) takes list positions b, b + G, ..., all on XCD b % 8, so
+// position p is given logical item start(p % 8) + p / 8: each XCD works
+// through one contiguous range of the logical (level-major) order.
+std::vector<std::uint32_t> xcd_order(const std::vector<std::uint32_t>& logical)
+{
+    const std::size_t n = logical.size(), q = n / kXcds, r = n % kXcds;
+    std::vector<std::uint32_t> out(n);
+    for (std::size_t p = 0; p < n; ++p) {
+        const std::size_t x = p % kXcds;
+        out[p] = logical[x * q + std::min(x, r) + p / kXcds];
+    }
+    return out;
 }
+
+// Persistent grids are kept a multiple of the XCD count (see xcd_order).
+std::uint32_t whole_xcds(std::uint32_t g)
+{
+    return g >= kXcds ? g - g % std::uint32_t(kXcds) : g;
+}
+}  // namespace
 
 struct OceanBatch::Impl {
     struct Stream {
@@ -105,9 +129,12 @@ OceanBatch::OceanBatch(std::span<const OceanDesc> descs)
     std::vector<std::size_t> first;
     for (auto& [key, lv] : by) {
         first.push_back(items.size());
+        std::vector<std::uint32_t> logical;
         for (std::uint32_t li : lv)
             for (std::uint32_t b = 0; b < ocean_hip_batch_items(key.first); ++b)
-                items.push_back(ocean_hip_batch_item(li, b));
+                logical.push_back(ocean_hip_batch_item(li, b));
+        const std::vector<std::uint32_t> placed = xcd_order(logical);
+        items.insert(items.end(), placed.begin(), placed.end());
     }
     const std::size_t lv_bytes = (count * sizeof(ocean_hip_level) + 255) / 256 * 256;
     const std::size_t it_bytes = (std::max<std::size_t>(items.size(), 1) * 4 + 255) / 256 * 256;
@@ -148,8 +175,8 @@ OceanBatch::OceanBatch(std::span<const OceanDesc> descs)
         // Until autotune() runs: as many blocks as stay resident.
         const int occ_r = std::max(1, ocean_hip_batch_occupancy(gr.n, gr.fields, 1));
         const int occ_c = std::max(1, ocean_hip_batch_occupancy(gr.n, gr.fields, 0));
-        gr.grid_row = std::min<std::uint32_t>(gr.n_items, std::uint32_t(m.cus * occ_r));
-        gr.grid_col = std::min<std::uint32_t>(gr.n_items, std::uint32_t(m.cus * occ_c));
+        gr.grid_row = std::min<std::uint32_t>(gr.n_items, whole_xcds(std::uint32_t(m.cus * occ_r)));
+        gr.grid_col = std::min<std::uint32_t>(gr.n_items, whole_xcds(std::uint32_t(m.cus * occ_c)));
         m.groups.push_back(gr);
         BatchGroupInfo bi;
         bi.size = gr.n;
@@ -274,7 +301,7 @@ void OceanBatch::autotune(std::uint32_t reps) noexcept
         for (int pass = 0; pass < 2; ++pass) {
             std::vector<std::uint32_t> cand;
             for (int k : {1, 2, 3, 4, 6, 8, 12, 16}) {
-                const std::uint32_t c = std::uint32_t(m.cus) * std::uint32_t(k);
+                const std::uint32_t c = whole_xcds(std::uint32_t(m.cus) * std::uint32_t(k));
                 if (c < gr.n_items) cand.push_back(c);
             }
             cand.push_back(gr.n_items);  // one block per item

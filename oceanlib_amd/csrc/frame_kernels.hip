@@ -51,12 +51,23 @@ __global__ __launch_bounds__(RowShape<N>::BLOCK) void row_pass_kernel(ocean_hip_
     row_block<N, F, RowShape<N>::TEAMS>(L, blockIdx.x, time_over_two_pi, tw, xch);
 }
 
+// Workgroups go to the 8 XCDs round-robin by block id, and each XCD has its
+// own L2. Column blocks read C*8 bytes of every row, so blocks of adjacent
+// column groups share 128-byte lines; mapping block b to group
+// (b % 8) * (G / 8) + b / 8 keeps each XCD on a contiguous range of groups, so
+// a shared line is fetched from HBM once instead of once per XCD.
+__device__ __forceinline__ uint32_t xcd_group(uint32_t b, uint32_t g)
+{
+    if (g % 8u != 0u) return b;
+    return (b % 8u) * (g / 8u) + b / 8u;
+}
+
 template <int N, int F>
 __global__ __launch_bounds__(ColShape<N>::BLOCK) void col_pass_kernel(ocean_hip_level L)
 {
     __shared__ float2 tw[N / 2];
     __shared__ float2 xch[ColShape<N>::C * lds_span<N>()];
-    col_block<N, F, ColShape<N>::C>(L, blockIdx.x, tw, xch);
+    col_block<N, F, ColShape<N>::C>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch);
 }
 
 template <int N>

@@ -73,6 +73,9 @@ public:
     void synchronize() const noexcept;
     [[nodiscard]] int last_error() const noexcept;
     [[nodiscard]] void* stream() const noexcept;
+    // Device array of the levels' views (internal layout), for the query
+    // kernels of this library.
+    [[nodiscard]] const void* device_levels() const noexcept;
 
 private:
     struct Impl;

@@ -181,6 +181,62 @@ int           ocean_amd_batch_checksums(const ocean_amd_batch* batch, uint32_t f
 int           ocean_amd_batch_synchronize(const ocean_amd_batch* batch);
 int           ocean_amd_batch_last_error(const ocean_amd_batch* batch);
 
+/* ---- converged height queries and the learned start point ----------------
+ * No reference counterpart. ocean_height_at takes the reference's four
+ * fixed-point steps (src/ocean.cpp:375-400); these iterate the same map on
+ * every level until the step is <= tol metres (at most 64 taps a level) and
+ * return the sum of the levels' heights at their solved points, as a cascade
+ * sums them (src/cascade.cpp:118-130). All arrays are device memory; the
+ * work is enqueued on the batch's / cascade's stream. With a proposal, each
+ * of the first three levels starts from the network's proposed point; the
+ * plain iteration refines it to the same tolerance and restarts from the
+ * query point when its step does not fall. iters (count * level_count bytes:
+ * taps per level) and flags (count bytes: 1 = a level hit 64 taps, 2 = a
+ * proposal was abandoned) may be NULL. Calls sharing a proposal are
+ * serialised on its scratch. */
+#define OCEAN_AMD_PROPOSAL_PARAMS 9360u
+#define OCEAN_AMD_PROPOSAL_INPUTS 64u   /* 60 used */
+#define OCEAN_AMD_PROPOSAL_OUTPUTS 16u  /* 6 used  */
+#define OCEAN_AMD_PROPOSAL_MAX_BATCH 8192u
+
+typedef struct ocean_amd_proposal ocean_amd_proposal;
+
+/* params: OCEAN_AMD_PROPOSAL_PARAMS floats (W1 [64][64], b1 [64], W2 [64][64],
+ * b2 [64], W3 [64][16], b3 [16]; rows are inputs), or NULL for zeros. */
+ocean_amd_proposal* ocean_amd_proposal_create(const float* params, uint32_t max_queries,
+                                              ocean_status* out_status);
+void ocean_amd_proposal_destroy(ocean_amd_proposal* model);
+int  ocean_amd_proposal_set_params(ocean_amd_proposal* model, const float* params);
+int  ocean_amd_proposal_get_params(const ocean_amd_proposal* model, float* out);
+/* rows x 64 features -> rows x 16 outputs (device). */
+int  ocean_amd_proposal_forward(const ocean_amd_proposal* model, const float* feats_dev,
+                                uint32_t rows, float* out_dev);
+/* One Adam step on rows_dev[batch] (batch a multiple of 128, <= MAX_BATCH);
+ * *loss (may be NULL) receives the minibatch's mean squared error. */
+int  ocean_amd_proposal_train_step(ocean_amd_proposal* model, const float* feats_dev,
+                                   const float* targets_dev, const uint32_t* rows_dev,
+                                   uint32_t batch, float lr, float* loss);
+/* The minibatch gradient (PARAMS floats, host) and loss, without a step. */
+int  ocean_amd_proposal_gradient(ocean_amd_proposal* model, const float* feats_dev,
+                                 const float* targets_dev, const uint32_t* rows_dev,
+                                 uint32_t batch, float* grad, float* loss);
+int  ocean_amd_proposal_synchronize(const ocean_amd_proposal* model);
+
+int  ocean_amd_batch_height_converged(const ocean_amd_batch* batch, uint32_t first,
+                                      uint32_t level_count, const float* xz_dev, uint32_t count,
+                                      float tol, const ocean_amd_proposal* model,
+                                      float* heights_dev, uint8_t* iters_dev, uint8_t* flags_dev);
+int  ocean_amd_cascade_height_converged(const ocean_amd_cascade* stack, const float* xz_dev,
+                                        uint32_t count, float tol, const ocean_amd_proposal* model,
+                                        float* heights_dev, uint8_t* iters_dev,
+                                        uint8_t* flags_dev);
+/* Training pairs from batch levels [first, first + 3): features (count x 64),
+ * targets (count x 16: each level's solved start offset in stencil units,
+ * plain iteration to tol), ok (count bytes), spacing (3 int32), all device. */
+int  ocean_amd_proposal_pairs(const ocean_amd_batch* batch, uint32_t first, const float* xz_dev,
+                              uint32_t count, float tol, float* feats_dev, float* targets_dev,
+                              uint8_t* ok_dev, int32_t* spacing_dev);
+
 #if defined(__cplusplus)
 }
 #endif

@@ -138,6 +138,11 @@ class BatchGroup(C.Structure):
                [("row_ms", C.c_float), ("col_ms", C.c_float)]
 
 
+PROPOSAL_PARAMS = 9360
+PROPOSAL_INPUTS = 64
+PROPOSAL_OUTPUTS = 16
+
+
 # (name, restype, argtypes) for every symbol the headers declare.
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
@@ -222,6 +227,23 @@ AMD_SIGNATURES = {
     "ocean_amd_batch_checksums": (C.c_int, [_P, C.c_uint32, C.c_uint32,
                                             C.POINTER(C.c_uint64)]),
     "ocean_amd_batch_synchronize": (C.c_int, [_P]),
+    "ocean_amd_proposal_create": (_P, [_F, C.c_uint32, C.POINTER(C.c_int)]),
+    "ocean_amd_proposal_destroy": (None, [_P]),
+    "ocean_amd_proposal_set_params": (C.c_int, [_P, _F]),
+    "ocean_amd_proposal_get_params": (C.c_int, [_P, _F]),
+    "ocean_amd_proposal_forward": (C.c_int, [_P, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "ocean_amd_proposal_train_step": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                                C.c_float, _F]),
+    "ocean_amd_proposal_gradient": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                              _F, _F]),
+    "ocean_amd_proposal_synchronize": (C.c_int, [_P]),
+    "ocean_amd_batch_height_converged": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_void_p,
+                                                   C.c_uint32, C.c_float, _P, C.c_void_p,
+                                                   C.c_void_p, C.c_void_p]),
+    "ocean_amd_cascade_height_converged": (C.c_int, [_P, C.c_void_p, C.c_uint32, C.c_float, _P,
+                                                     C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ocean_amd_proposal_pairs": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_uint32, C.c_float,
+                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ocean_amd_batch_last_error": (C.c_int, [_P]),
 }
 

@@ -213,6 +213,7 @@ const Ocean& OceanBatch::level(std::size_t i) const noexcept { return impl_->lev
 std::size_t OceanBatch::group_count() const noexcept { return impl_->groups.size(); }
 BatchGroupInfo OceanBatch::group(std::size_t g) const noexcept { return impl_->info[g]; }
 void* OceanBatch::stream() const noexcept { return impl_->stream.s; }
+const void* OceanBatch::device_levels() const noexcept { return impl_->d_levels; }
 
 bool OceanBatch::checksums(std::size_t first, std::size_t count, std::uint64_t* out) const noexcept
 {

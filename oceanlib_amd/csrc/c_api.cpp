@@ -842,3 +842,153 @@ extern "C" int ocean_amd_build_iwave_taps(uint32_t radius, double cell, double d
     if (max_symbol != nullptr) *max_symbol = k.max_symbol;
     return static_cast<int>(k.taps.size());
 }
+
+// ---- ocean_amd.h: converged queries and the learned proposal ---------------
+
+#include "ocean/proposal.hpp"
+
+namespace {
+
+ocean::HeightProposal* to_model(ocean_amd_proposal* p) noexcept
+{
+    return reinterpret_cast<ocean::HeightProposal*>(p);
+}
+const ocean::HeightProposal* to_model(const ocean_amd_proposal* p) noexcept
+{
+    return reinterpret_cast<const ocean::HeightProposal*>(p);
+}
+
+}  // namespace
+
+extern "C" {
+
+ocean_amd_proposal* ocean_amd_proposal_create(const float* params, uint32_t max_queries,
+                                              ocean_status* out_status)
+{
+    auto fail = [&](ocean_status s) -> ocean_amd_proposal* {
+        if (out_status != nullptr) *out_status = s;
+        return nullptr;
+    };
+    if (max_queries == 0) return fail(OCEAN_ERROR_INVALID_ARG);
+    try {
+        auto* p = new ocean::HeightProposal(params, max_queries);
+        if (out_status != nullptr) *out_status = OCEAN_OK;
+        return reinterpret_cast<ocean_amd_proposal*>(p);
+    } catch (const std::invalid_argument&) {
+        return fail(OCEAN_ERROR_INVALID_ARG);
+    } catch (const std::bad_alloc&) {
+        return fail(OCEAN_ERROR_OUT_OF_MEMORY);
+    } catch (...) {
+        return fail(OCEAN_ERROR_UNKNOWN);
+    }
+}
+
+void ocean_amd_proposal_destroy(ocean_amd_proposal* model) { delete to_model(model); }
+
+int ocean_amd_proposal_set_params(ocean_amd_proposal* model, const float* params)
+{
+    if (model == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    return to_model(model)->set_params(params) ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
+}
+
+int ocean_amd_proposal_get_params(const ocean_amd_proposal* model, float* out)
+{
+    if (model == nullptr || out == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    return to_model(model)->get_params(out) ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
+}
+
+int ocean_amd_proposal_forward(const ocean_amd_proposal* model, const float* feats_dev,
+                               uint32_t rows, float* out_dev)
+{
+    if (model == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    if (rows == 0) return OCEAN_OK;
+    if (feats_dev == nullptr || out_dev == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    return to_model(model)->forward(feats_dev, rows, out_dev) ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
+}
+
+namespace {
+bool batch_ok(uint32_t batch) noexcept
+{
+    return batch != 0 && batch % 128u == 0 && batch <= OCEAN_AMD_PROPOSAL_MAX_BATCH;
+}
+}  // namespace
+
+int ocean_amd_proposal_train_step(ocean_amd_proposal* model, const float* feats_dev,
+                                  const float* targets_dev, const uint32_t* rows_dev, uint32_t batch,
+                                  float lr, float* loss)
+{
+    if (model == nullptr || feats_dev == nullptr || targets_dev == nullptr || rows_dev == nullptr ||
+        !batch_ok(batch) || !(lr > 0.0f))
+        return OCEAN_ERROR_INVALID_ARG;
+    return to_model(model)->train_step(feats_dev, targets_dev, rows_dev, batch, lr, loss)
+               ? OCEAN_OK
+               : OCEAN_ERROR_UNKNOWN;
+}
+
+int ocean_amd_proposal_gradient(ocean_amd_proposal* model, const float* feats_dev,
+                                const float* targets_dev, const uint32_t* rows_dev, uint32_t batch,
+                                float* grad, float* loss)
+{
+    if (model == nullptr || feats_dev == nullptr || targets_dev == nullptr || rows_dev == nullptr ||
+        grad == nullptr || !batch_ok(batch))
+        return OCEAN_ERROR_INVALID_ARG;
+    return to_model(model)->gradient(feats_dev, targets_dev, rows_dev, batch, grad, loss)
+               ? OCEAN_OK
+               : OCEAN_ERROR_UNKNOWN;
+}
+
+int ocean_amd_proposal_synchronize(const ocean_amd_proposal* model)
+{
+    if (model == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    to_model(model)->synchronize();
+    return to_model(model)->last_error() == 0 ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
+}
+
+int ocean_amd_batch_height_converged(const ocean_amd_batch* batch, uint32_t first,
+                                     uint32_t level_count, const float* xz_dev, uint32_t count,
+                                     float tol, const ocean_amd_proposal* model, float* heights_dev,
+                                     uint8_t* iters_dev, uint8_t* flags_dev)
+{
+    if (!batch_range(batch, first, level_count) || !(tol > 0.0f)) return OCEAN_ERROR_INVALID_ARG;
+    if (count == 0) return OCEAN_OK;
+    if (xz_dev == nullptr || heights_dev == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    if (model != nullptr && (level_count > 3u || count > to_model(model)->max_queries()))
+        return OCEAN_ERROR_INVALID_ARG;
+    return ocean::height_converged(*to_batch(batch), first, level_count, xz_dev, count, tol,
+                                   to_model(model), heights_dev, iters_dev, flags_dev)
+               ? OCEAN_OK
+               : OCEAN_ERROR_UNKNOWN;
+}
+
+int ocean_amd_cascade_height_converged(const ocean_amd_cascade* stack, const float* xz_dev,
+                                       uint32_t count, float tol, const ocean_amd_proposal* model,
+                                       float* heights_dev, uint8_t* iters_dev, uint8_t* flags_dev)
+{
+    if (stack == nullptr || !(tol > 0.0f)) return OCEAN_ERROR_INVALID_ARG;
+    if (count == 0) return OCEAN_OK;
+    if (xz_dev == nullptr || heights_dev == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    if (model != nullptr &&
+        (to_stack(stack)->level_count() > 3u || count > to_model(model)->max_queries()))
+        return OCEAN_ERROR_INVALID_ARG;
+    return ocean::height_converged(*to_stack(stack), xz_dev, count, tol, to_model(model), heights_dev,
+                                   iters_dev, flags_dev)
+               ? OCEAN_OK
+               : OCEAN_ERROR_UNKNOWN;
+}
+
+int ocean_amd_proposal_pairs(const ocean_amd_batch* batch, uint32_t first, const float* xz_dev,
+                             uint32_t count, float tol, float* feats_dev, float* targets_dev,
+                             uint8_t* ok_dev, int32_t* spacing_dev)
+{
+    if (!batch_range(batch, first, 3u) || !(tol > 0.0f)) return OCEAN_ERROR_INVALID_ARG;
+    if (count == 0) return OCEAN_OK;
+    if (xz_dev == nullptr || feats_dev == nullptr || targets_dev == nullptr || ok_dev == nullptr ||
+        spacing_dev == nullptr)
+        return OCEAN_ERROR_INVALID_ARG;
+    return ocean::proposal_pairs(*to_batch(batch), first, xz_dev, count, tol, feats_dev, targets_dev,
+                                 ok_dev, spacing_dev)
+               ? OCEAN_OK
+               : OCEAN_ERROR_UNKNOWN;
+}
+
+}  // extern "C"

@@ -110,6 +110,59 @@ int ocean_hip_batch_occupancy(uint32_t n, uint32_t fields, int row);
 hipError_t ocean_hip_level_checksums(const ocean_hip_level* levels, uint32_t count,
                                      unsigned long long* out, hipStream_t s);
 
+/* ---- converged queries and the learned start-point proposal --------------
+ * Parameters of the proposal MLP, fp32, in this order: W1 [IN][HIDDEN]
+ * (row = input), b1 [HIDDEN], W2 [HIDDEN][HIDDEN], b2, W3 [HIDDEN][OUT], b3
+ * [OUT]. Inputs 60..63 and outputs 6..15 are padding (zero). */
+#define OCEAN_HIP_MLP_LEVELS 3
+#define OCEAN_HIP_MLP_IN 64
+#define OCEAN_HIP_MLP_HIDDEN 64
+#define OCEAN_HIP_MLP_OUT 16
+#define OCEAN_HIP_MLP_PARAMS 9360
+typedef struct ocean_hip_mlp {
+    uint16_t* bf16;   /* the parameters rounded to bf16 (what MFMA reads)   */
+    float* master;    /* fp32 parameters (biases are read from here)        */
+    float* m1;        /* Adam first / second moments                        */
+    float* m2;
+    float* grad;      /* the last summed gradient                           */
+} ocean_hip_mlp;
+
+/* Stencil spacing per level from its displacement RMS (device int32[count]). */
+hipError_t ocean_hip_mlp_spacing(const ocean_hip_level* levels, uint32_t level_count,
+                                 int32_t* spacing, hipStream_t s);
+/* Heights at `count` points (device float2), each level inverted until the
+ * fixed-point step is <= tol metres (at most 64 taps), summed over levels.
+ * mlp != NULL: start from the proposal (needs `spacing` from
+ * ocean_hip_mlp_spacing and `prop` scratch of count * OUT floats).
+ * iters (count * level_count bytes) and flags (count bytes: 1 = a level did
+ * not converge, 2 = a proposal was abandoned) may be NULL. */
+hipError_t ocean_hip_height_converged(const ocean_hip_level* levels, uint32_t level_count,
+                                      const float* xz, uint32_t count, float tol,
+                                      const ocean_hip_mlp* mlp, const int32_t* spacing, float* prop,
+                                      float* heights, uint8_t* iters, uint8_t* flags,
+                                      hipStream_t s);
+/* Training pairs for the first MLP_LEVELS levels: features (count * IN),
+ * targets (count * OUT) from a plain solve to `tol`, ok[q] = 1 if every
+ * level converged. */
+hipError_t ocean_hip_mlp_pairs(const ocean_hip_level* levels, const float* xz, uint32_t count,
+                               float tol, const int32_t* spacing, float* feats, float* targets,
+                               uint8_t* ok, hipStream_t s);
+hipError_t ocean_hip_mlp_forward(const ocean_hip_mlp* mlp, const float* feats, uint32_t rows,
+                                 float* out, hipStream_t s);
+/* bf16 <- master (after the master parameters were written). */
+hipError_t ocean_hip_mlp_sync_bf16(ocean_hip_mlp* mlp, hipStream_t s);
+/* One minibatch (rows[batch] index feats/targets; batch a multiple of 128):
+ * gradient of the mean squared error over the 6 outputs into mlp->grad and,
+ * if lr > 0, one Adam step (c1 = 1 - 0.9^t, c2 = 1 - 0.999^t). partial holds
+ * (batch / 128) * PARAMS floats, loss_partial batch / 128 floats; *loss (may
+ * be NULL) gets the sum of squared errors. */
+uint32_t ocean_hip_mlp_train_blocks(uint32_t batch);
+hipError_t ocean_hip_mlp_step(ocean_hip_mlp* mlp, const float* feats, const float* targets,
+                              const uint32_t* rows, uint32_t batch, float lr, float c1, float c2,
+                              float* partial, float* loss_partial, float* loss, hipStream_t s);
+/* Raises the kernels' dynamic LDS limits once per process; 0 on success. */
+int ocean_hip_mlp_configure(void);
+
 /* ---- iWave interaction field -------------------------------------------- */
 
 /* Device view of one interaction field. Planes have a P-cell zero halo;

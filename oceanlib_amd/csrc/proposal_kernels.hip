@@ -1,0 +1,763 @@
+// Converged height queries and the learned start-point proposal.
+//
+// The reference's height_at takes four fixed-point steps from the query point
+// (src/ocean.cpp:375-400). The converged form iterates the same map,
+// u <- X - Dx(u,v), v <- Z - Dz(u,v), on each level until the step falls to a
+// tolerance, in (integer base cell + float offset) coordinates so that float
+// resolution at kilometre coordinates does not limit it. The learned form
+// starts each level from a small MLP's proposal instead of the query point;
+// the plain iteration refines it to the same tolerance and restarts from the
+// query point if the step does not fall.
+//
+// The MLP (per query, 3 levels): 60 features -> 64 -> 64 -> 6, ReLU, bf16
+// weights and activations on MFMA (v_mfma_f32_16x16x16_bf16), fp32
+// accumulate, fp32 biases. Features of level l: the (dx, dz) of the 3x3
+// nodes at spacing s_l cells around the node nearest the query, divided by
+// s_l * cell_l, and the query's offset from that node divided by s_l. Outputs:
+// the start offset of each level, (u - X, v - Z) / (s_l * cell_l). The spacing
+// s_l = clamp(round(2 * rms_l / cell_l), 1, 16) follows each level's current
+// displacement RMS, so the stencil spans the region the solution lies in.
+#include "hip_launch.h"
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+namespace oceanamd {
+namespace {
+
+constexpr int kLevels = OCEAN_HIP_MLP_LEVELS;
+constexpr int kIn = OCEAN_HIP_MLP_IN;    // 60 used + 4 zero
+constexpr int kHid = OCEAN_HIP_MLP_HIDDEN;
+constexpr int kOut = OCEAN_HIP_MLP_OUT;  // 6 used + 10 zero
+constexpr int kOutUsed = 2 * kLevels;
+constexpr int kFeatPerLevel = 20;
+constexpr int oW1 = 0;
+constexpr int ob1 = oW1 + kIn * kHid;
+constexpr int oW2 = ob1 + kHid;
+constexpr int ob2 = oW2 + kHid * kHid;
+constexpr int oW3 = ob2 + kHid;
+constexpr int ob3 = oW3 + kHid * kOut;
+constexpr int kParams = ob3 + kOut;
+static_assert(kParams == OCEAN_HIP_MLP_PARAMS, "parameter layout");
+constexpr int kMaxIter = 64;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// fp32 -> bf16, round to nearest even (NaN stays NaN).
+__device__ __forceinline__ uint16_t to_bf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// 4 consecutive bf16 from LDS (8-byte aligned).
+__device__ __forceinline__ bf16x4 ld4(const uint16_t* p)
+{
+    return *reinterpret_cast<const bf16x4*>(p);
+}
+// 4 bf16 at stride `s`.
+__device__ __forceinline__ bf16x4 ld4s(const uint16_t* p, int s)
+{
+    bf16x4 r;
+    r[0] = short(p[0]);
+    r[1] = short(p[s]);
+    r[2] = short(p[2 * s]);
+    r[3] = short(p[3 * s]);
+    return r;
+}
+
+// ---- geometry ---------------------------------------------------------------
+
+struct Base {
+    int32_t ix, iz;  // integer cell of the query point (wrapped to [0, N))
+    float fx, fz;    // offset inside that cell, [0, 1)
+};
+
+__device__ __forceinline__ Base base_cell(float X, float Z, const ocean_hip_level& L)
+{
+    const double g = double(L.n) / double(L.patch_length);
+    const double gx = double(X) * g, gz = double(Z) * g;
+    const double flx = floor(gx), flz = floor(gz);
+    const int64_t m = int64_t(L.n) - 1;
+    Base b;
+    b.ix = int32_t(int64_t(flx) & m);
+    b.iz = int32_t(int64_t(flz) & m);
+    b.fx = float(gx - flx);
+    b.fz = float(gz - flz);
+    return b;
+}
+
+// Bilinear (dx, dz[, h]) at cell position (ix + px, iz + pz), periodic.
+template <bool HEIGHT>
+__device__ __forceinline__ void tap(const float4* __restrict__ disp, uint32_t n, const Base& b,
+                                    float px, float pz, float& dx, float& dz, float& h)
+{
+    const float flx = floorf(px), flz = floorf(pz);
+    const float tu = px - flx, tv = pz - flz;
+    const uint32_t mask = n - 1u;
+    const uint32_t x0 = uint32_t(b.ix + int32_t(flx)) & mask;
+    const uint32_t z0 = uint32_t(b.iz + int32_t(flz)) & mask;
+    const uint32_t x1 = (x0 + 1u) & mask, z1 = (z0 + 1u) & mask;
+    const float4 a = disp[z0 * n + x0], c = disp[z0 * n + x1];
+    const float4 d = disp[z1 * n + x0], e = disp[z1 * n + x1];
+    const float w00 = (1.0f - tu) * (1.0f - tv), w10 = tu * (1.0f - tv);
+    const float w01 = (1.0f - tu) * tv, w11 = tu * tv;
+    dx = a.x * w00 + c.x * w10 + d.x * w01 + e.x * w11;
+    dz = a.z * w00 + c.z * w10 + d.z * w01 + e.z * w11;
+    if (HEIGHT) h = a.y * w00 + c.y * w10 + d.y * w01 + e.y * w11;
+}
+
+struct Solve {
+    float du, dv;  // solved offset from the query point, cells
+    int iters;     // taps taken by the inversion
+    bool converged;
+    bool fell_back;
+};
+
+// Fixed-point inversion of one level from the start offset (du0, dv0) cells.
+__device__ __forceinline__ Solve solve_level(const ocean_hip_level& L, const Base& b, float du0,
+                                             float dv0, float tol_cells, int max_iter, bool learned)
+{
+    const float4* __restrict__ disp = reinterpret_cast<const float4*>(L.disp);
+    const float inv_cell = float(L.n) / L.patch_length;
+    Solve s{du0, dv0, 0, false, false};
+    float prev = 3.0e38f;
+    float h;
+    while (s.iters < max_iter) {
+        float dx, dz;
+        tap<false>(disp, L.n, b, b.fx + s.du, b.fz + s.dv, dx, dz, h);
+        ++s.iters;
+        const float nu = -dx * inv_cell, nv = -dz * inv_cell;
+        const float step = fmaxf(fabsf(nu - s.du), fabsf(nv - s.dv));
+        s.du = nu;
+        s.dv = nv;
+        if (step <= tol_cells) {
+            s.converged = true;
+            break;
+        }
+        if (learned && !s.fell_back && s.iters >= 2 && !(step < prev)) {
+            // The proposal's residual does not fall: restart from the query point.
+            s.fell_back = true;
+            s.du = 0.0f;
+            s.dv = 0.0f;
+            prev = 3.0e38f;
+            continue;
+        }
+        prev = step;
+    }
+    return s;
+}
+
+// Features of one level (20 floats) at spacing s.
+__device__ __forceinline__ void level_features(const ocean_hip_level& L, const Base& b, int s,
+                                               float* f)
+{
+    const float4* __restrict__ disp = reinterpret_cast<const float4*>(L.disp);
+    const uint32_t n = L.n, mask = n - 1u;
+    const int up_x = b.fx >= 0.5f ? 1 : 0, up_z = b.fz >= 0.5f ? 1 : 0;
+    const int32_t cx = b.ix + up_x, cz = b.iz + up_z;
+    const float inv_h = float(L.n) / (L.patch_length * float(s));
+    int k = 0;
+#pragma unroll
+    for (int bz = -1; bz <= 1; ++bz) {
+#pragma unroll
+        for (int ax = -1; ax <= 1; ++ax) {
+            const uint32_t x = uint32_t(cx + ax * s) & mask;
+            const uint32_t z = uint32_t(cz + bz * s) & mask;
+            const float4 v = disp[z * n + x];
+            f[k++] = v.x * inv_h;
+            f[k++] = v.z * inv_h;
+        }
+    }
+    f[k++] = (b.fx - float(up_x)) / float(s);
+    f[k++] = (b.fz - float(up_z)) / float(s);
+}
+
+// ---- the MLP on one wavefront (64 rows) ---------------------------------------
+//
+// v_mfma_f32_16x16x16_bf16 lane layout: A[i][k] from lane i + 16 * (k / 4),
+// element k % 4; B[k][j] from lane j + 16 * (k / 4), element k % 4; D[i][j]
+// in lane j + 16 * (i / 4), element i % 4.
+
+struct InferWeights {  // LDS copies, transposed (B operand: 4 consecutive k)
+    uint16_t w1t[kHid * kIn];    // [j][k] = W1[k][j]
+    uint16_t w2t[kHid * kHid];
+    uint16_t w3t[kOut * kHid];
+    float b1[kHid], b2[kHid], b3[kOut];
+};
+struct TrainWeights : InferWeights {  // plus row-major copies (backward: B = W^T)
+    uint16_t w1[kIn * kHid];
+    uint16_t w2[kHid * kHid];
+    uint16_t w3[kHid * kOut];
+};
+
+__device__ void load_weights(InferWeights& w, const uint16_t* __restrict__ pb,
+                             const float* __restrict__ pf, TrainWeights* tw)
+{
+    for (int i = threadIdx.x; i < kIn * kHid; i += blockDim.x) {
+        const int k = i / kHid, j = i % kHid;
+        const uint16_t v = pb[oW1 + i];
+        w.w1t[j * kIn + k] = v;
+        if (tw) tw->w1[i] = v;
+    }
+    for (int i = threadIdx.x; i < kHid * kHid; i += blockDim.x) {
+        const int k = i / kHid, j = i % kHid;
+        const uint16_t v = pb[oW2 + i];
+        w.w2t[j * kHid + k] = v;
+        if (tw) tw->w2[i] = v;
+    }
+    for (int i = threadIdx.x; i < kHid * kOut; i += blockDim.x) {
+        const int k = i / kOut, j = i % kOut;
+        const uint16_t v = pb[oW3 + i];
+        w.w3t[j * kHid + k] = v;
+        if (tw) tw->w3[i] = v;
+    }
+    for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
+        w.b1[i] = pf[ob1 + i];
+        w.b2[i] = pf[ob2 + i];
+    }
+    for (int i = threadIdx.x; i < kOut; i += blockDim.x) w.b3[i] = pf[ob3 + i];
+}
+
+// The forward pass of one wavefront's 64 rows. X holds the bf16 inputs
+// (64 x kIn) and is reused for layer 2's output; H (64 x kHid) holds layer 1's
+// output and then the fp32 outputs Y (64 x kOut), returned in `y` per lane.
+__device__ void forward_wave(const InferWeights& w, uint16_t* X, uint16_t* H, float* y);
+
+// out[64 x NO] = act(in[64 x NI] * W + b); in/out are bf16 row-major tiles
+// (pitch NI / NO); wt is W transposed [NO][NI]. RELU: hidden layer (bf16
+// out); otherwise the fp32 result goes to `outf` (pitch NO).
+template <int NI, int NO, bool RELU>
+__device__ __forceinline__ void dense(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wt,
+                                      const float* __restrict__ bias, uint16_t* __restrict__ out,
+                                      float* __restrict__ outf)
+{
+    const int l = threadIdx.x & 63;
+    const int li = l & 15, lk = (l >> 4) * 4;
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+#pragma unroll
+        for (int ct = 0; ct < NO / 16; ++ct) {
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kb = 0; kb < NI / 16; ++kb) {
+                const bf16x4 a = ld4(in + (rt * 16 + li) * NI + kb * 16 + lk);
+                const bf16x4 bb = ld4(wt + (ct * 16 + li) * NI + kb * 16 + lk);
+                acc = mfma16(a, bb, acc);
+            }
+            const int j = ct * 16 + li;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int i = rt * 16 + lk + e;
+                const float z = acc[e] + bias[j];
+                if (RELU)
+                    out[i * NO + j] = to_bf16(z > 0.0f ? z : 0.0f);
+                else
+                    outf[i * NO + j] = z;
+            }
+        }
+    }
+}
+
+__device__ void forward_wave(const InferWeights& w, uint16_t* X, uint16_t* H, float* y)
+{
+    const int l = threadIdx.x & 63;
+    dense<kIn, kHid, true>(X, w.w1t, w.b1, H, nullptr);
+    __syncthreads();
+    dense<kHid, kHid, true>(H, w.w2t, w.b2, X, nullptr);
+    __syncthreads();
+    float* Y = reinterpret_cast<float*>(H);
+    dense<kHid, kOut, false>(X, w.w3t, w.b3, nullptr, Y);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kOut; ++k) y[k] = Y[l * kOut + k];
+}
+
+// ---- kernels ------------------------------------------------------------------
+
+// Displacement RMS of each level -> stencil spacing (one block per level).
+__global__ __launch_bounds__(256) void spacing_kernel(const ocean_hip_level* __restrict__ levels,
+                                                      int32_t* __restrict__ spacing)
+{
+    __shared__ double part[256];
+    const ocean_hip_level L = levels[blockIdx.x];
+    const float4* __restrict__ disp = reinterpret_cast<const float4*>(L.disp);
+    const uint32_t cells = L.n * L.n;
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < cells; i += 256u) {
+        const float4 v = disp[i];
+        acc += double(v.x) * double(v.x) + double(v.z) * double(v.z);
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double rms = sqrt(part[0] / (2.0 * double(cells)));
+        const double cell = double(L.patch_length) / double(L.n);
+        int s = int(rint(2.0 * rms / cell));
+        spacing[blockIdx.x] = s < 1 ? 1 : (s > 16 ? 16 : s);
+    }
+}
+
+// Proposals: 256 lanes = 4 waves of 64 queries; features -> MLP -> the
+// start offsets of each level in cells, prop[q * kOut + 2l (+1)].
+__global__ __launch_bounds__(256) void proposal_kernel(const ocean_hip_level* __restrict__ levels,
+                                                       uint32_t level_count,
+                                                       const float2* __restrict__ xz, uint32_t count,
+                                                       const uint16_t* __restrict__ pb,
+                                                       const float* __restrict__ pf,
+                                                       const int32_t* __restrict__ spacing,
+                                                       float* __restrict__ prop)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    InferWeights& w = *reinterpret_cast<InferWeights*>(smem);
+    uint16_t* tiles = reinterpret_cast<uint16_t*>(smem + sizeof(InferWeights));
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    uint16_t* X = tiles + wave * (2 * 64 * 64);
+    uint16_t* H = X + 64 * 64;
+    load_weights(w, pb, pf, nullptr);
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    const bool live = q < count;
+    const float2 p = live ? xz[q] : make_float2(0.0f, 0.0f);
+    float f[kIn];
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) f[k] = 0.0f;
+#pragma unroll
+    for (int lv = 0; lv < kLevels; ++lv) {  // constant indices keep f in registers
+        if (live && uint32_t(lv) < level_count) {
+            const ocean_hip_level L = levels[lv];
+            level_features(L, base_cell(p.x, p.y, L), spacing[lv], f + lv * kFeatPerLevel);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) X[l * kIn + k] = to_bf16(f[k]);
+    __syncthreads();
+    float y[kOut];
+    forward_wave(w, X, H, y);
+    if (!live) return;
+    for (uint32_t lv = 0; lv < level_count && lv < uint32_t(kLevels); ++lv) {
+        const float sp = float(spacing[lv]);
+        prop[size_t(q) * kOut + 2 * lv] = y[2 * lv] * sp;
+        prop[size_t(q) * kOut + 2 * lv + 1] = y[2 * lv + 1] * sp;
+    }
+}
+
+// Converged queries, one lane per query; LEARNED starts each of the first
+// kLevels levels from prop (cells) and falls back to the query point.
+template <bool LEARNED>
+__global__ __launch_bounds__(256) void converged_kernel(
+    const ocean_hip_level* __restrict__ levels, uint32_t level_count, const float2* __restrict__ xz,
+    uint32_t count, float tol, const float* __restrict__ prop, float* __restrict__ heights,
+    uint8_t* __restrict__ iters, uint8_t* __restrict__ flags)
+{
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q >= count) return;
+    const float2 p = xz[q];
+    float height = 0.0f;
+    uint8_t fl = 0;
+    for (uint32_t lv = 0; lv < level_count; ++lv) {
+        const ocean_hip_level L = levels[lv];
+        const Base b = base_cell(p.x, p.y, L);
+        const float inv_cell = float(L.n) / L.patch_length;
+        const bool use = LEARNED && lv < uint32_t(kLevels);
+        float du0 = 0.0f, dv0 = 0.0f;
+        if (use) {
+            du0 = prop[size_t(q) * kOut + 2 * lv];
+            dv0 = prop[size_t(q) * kOut + 2 * lv + 1];
+            if (!(fabsf(du0) < 1.0e4f) || !(fabsf(dv0) < 1.0e4f)) {
+                du0 = dv0 = 0.0f;  // not finite or far off the patch: abandoned at once
+                fl |= 2u;
+            }
+        }
+        const Solve s = solve_level(L, b, du0, dv0, tol * inv_cell, kMaxIter, use);
+        float dx, dz, h;
+        tap<true>(reinterpret_cast<const float4*>(L.disp), L.n, b, b.fx + s.du, b.fz + s.dv, dx, dz, h);
+        height += h;
+        if (iters) iters[size_t(q) * level_count + lv] = uint8_t(s.iters > 255 ? 255 : s.iters);
+        if (!s.converged) fl |= 1u;
+        if (s.fell_back) fl |= 2u;
+    }
+    heights[q] = height;
+    if (flags) flags[q] = fl;
+}
+
+// Training pairs: features and the solved start offsets (tight tolerance).
+__global__ __launch_bounds__(256) void pairs_kernel(const ocean_hip_level* __restrict__ levels,
+                                                    const float2* __restrict__ xz, uint32_t count,
+                                                    float tol, const int32_t* __restrict__ spacing,
+                                                    float* __restrict__ feats, float* __restrict__ targets,
+                                                    uint8_t* __restrict__ ok)
+{
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q >= count) return;
+    const float2 p = xz[q];
+    float f[kIn];
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) f[k] = 0.0f;
+    float t[kOut];
+#pragma unroll
+    for (int k = 0; k < kOut; ++k) t[k] = 0.0f;
+    bool good = true;
+    for (int lv = 0; lv < kLevels; ++lv) {
+        const ocean_hip_level L = levels[lv];
+        const Base b = base_cell(p.x, p.y, L);
+        const int s = spacing[lv];
+        level_features(L, b, s, f + lv * kFeatPerLevel);
+        const float inv_cell = float(L.n) / L.patch_length;
+        const Solve r = solve_level(L, b, 0.0f, 0.0f, tol * inv_cell, 255, false);
+        good = good && r.converged;
+        t[2 * lv] = r.du / float(s);
+        t[2 * lv + 1] = r.dv / float(s);
+    }
+    for (int k = 0; k < kIn; ++k) feats[size_t(q) * kIn + k] = f[k];
+    for (int k = 0; k < kOut; ++k) targets[size_t(q) * kOut + k] = t[k];
+    ok[q] = good ? 1u : 0u;
+}
+
+// Forward only (rows of fp32 features -> fp32 outputs), 64 rows per wave.
+__global__ __launch_bounds__(256) void forward_kernel(const float* __restrict__ feats, uint32_t rows,
+                                                      const uint16_t* __restrict__ pb,
+                                                      const float* __restrict__ pf,
+                                                      float* __restrict__ out)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    InferWeights& w = *reinterpret_cast<InferWeights*>(smem);
+    uint16_t* tiles = reinterpret_cast<uint16_t*>(smem + sizeof(InferWeights));
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    uint16_t* X = tiles + wave * (2 * 64 * 64);
+    uint16_t* H = X + 64 * 64;
+    load_weights(w, pb, pf, nullptr);
+    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    for (int k = 0; k < kIn; ++k) X[l * kIn + k] = to_bf16(r < rows ? feats[size_t(r) * kIn + k] : 0.0f);
+    __syncthreads();
+    float y[kOut];
+    forward_wave(w, X, H, y);
+    if (r < rows)
+        for (int k = 0; k < kOut; ++k) out[size_t(r) * kOut + k] = y[k];
+}
+
+// ---- training: one block = 2 waves = 128 rows of the minibatch -----------------
+
+constexpr int kTrainRows = 128;
+
+struct TrainTiles {
+    uint16_t X[kTrainRows * kIn];
+    uint16_t H1[kTrainRows * kHid];
+    uint16_t H2[kTrainRows * kHid];
+    uint16_t dY[kTrainRows * kOut];
+    uint16_t dZ2[kTrainRows * kHid];
+    uint16_t dZ1[kTrainRows * kHid];
+    float sq[kTrainRows];
+};
+
+// dst[a x b tile grid] = sum over the block's rows of A^T[.][r] * B[r][.]:
+// dW tile (kt, jt) of A (pitch NA) and B (pitch NB); ONES: A is all ones (bias
+// gradient, row 0 of the result is used).
+template <int NA, int NB, bool ONES>
+__device__ __forceinline__ f32x4 grad_tile(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B,
+                                           int kt, int jt)
+{
+    const int l = threadIdx.x & 63;
+    const int li = l & 15, lk = (l >> 4) * 4;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const short one = short(0x3f80);
+#pragma unroll
+    for (int rb = 0; rb < kTrainRows / 16; ++rb) {
+        const int r = rb * 16 + lk;
+        bf16x4 a;
+        if (ONES) {
+            a[0] = a[1] = a[2] = a[3] = one;
+        } else {
+            a = ld4s(A + r * NA + kt * 16 + li, NA);
+        }
+        const bf16x4 b = ld4s(B + r * NB + jt * 16 + li, NB);
+        acc = mfma16(a, b, acc);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ void store_grad_tile(float* __restrict__ dst, int pitch, int kt, int jt,
+                                                f32x4 acc, bool row0_only)
+{
+    const int l = threadIdx.x & 63;
+    const int j = jt * 16 + (l & 15);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int i = (l >> 4) * 4 + e;
+        if (row0_only) {
+            if (i == 0) dst[j] = acc[e];
+        } else {
+            dst[(kt * 16 + i) * pitch + j] = acc[e];
+        }
+    }
+}
+
+__global__ __launch_bounds__(128) void train_kernel(const float* __restrict__ feats,
+                                                    const float* __restrict__ targets,
+                                                    const uint32_t* __restrict__ rows, uint32_t batch,
+                                                    const uint16_t* __restrict__ pb,
+                                                    const float* __restrict__ pf,
+                                                    float* __restrict__ partial,
+                                                    float* __restrict__ loss_partial)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    TrainWeights& w = *reinterpret_cast<TrainWeights*>(smem);
+    TrainTiles& t = *reinterpret_cast<TrainTiles*>(smem + sizeof(TrainWeights));
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int lr = wave * 64 + l;  // row inside the block
+    const uint32_t row = rows[blockIdx.x * kTrainRows + lr];
+    load_weights(w, pb, pf, &w);
+    for (int k = 0; k < kIn; ++k) t.X[lr * kIn + k] = to_bf16(feats[size_t(row) * kIn + k]);
+    __syncthreads();
+    // forward, each wave on its 64 rows
+    float Yf[kOut];
+    {
+        float* Y = reinterpret_cast<float*>(t.dZ2) + wave * 64 * kOut;  // scratch until dZ2 is written
+        dense<kIn, kHid, true>(t.X + wave * 64 * kIn, w.w1t, w.b1, t.H1 + wave * 64 * kHid, nullptr);
+        __syncthreads();
+        dense<kHid, kHid, true>(t.H1 + wave * 64 * kHid, w.w2t, w.b2, t.H2 + wave * 64 * kHid, nullptr);
+        __syncthreads();
+        dense<kHid, kOut, false>(t.H2 + wave * 64 * kHid, w.w3t, w.b3, nullptr, Y);
+        __syncthreads();
+        for (int k = 0; k < kOut; ++k) Yf[k] = Y[l * kOut + k];
+        __syncthreads();
+    }
+    // loss 1/(B*6) sum (y - t)^2 and dY = 2 (y - t) / (B*6)
+    const float scale = 2.0f / (float(batch) * float(kOutUsed));
+    float sq = 0.0f;
+    for (int k = 0; k < kOut; ++k) {
+        float g = 0.0f;
+        if (k < kOutUsed) {
+            const float d = Yf[k] - targets[size_t(row) * kOut + k];
+            sq += d * d;
+            g = d * scale;
+        }
+        t.dY[lr * kOut + k] = to_bf16(g);
+    }
+    t.sq[lr] = sq;
+    __syncthreads();
+    // dH2 = dY W3^T, dZ2 = dH2 * (H2 > 0)
+    {
+        const int li = l & 15, lk = (l >> 4) * 4;
+        const uint16_t* dY = t.dY + wave * 64 * kOut;
+        const uint16_t* H2 = t.H2 + wave * 64 * kHid;
+        uint16_t* dZ2 = t.dZ2 + wave * 64 * kHid;
+        for (int rt = 0; rt < 4; ++rt)
+            for (int ct = 0; ct < kHid / 16; ++ct) {
+                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                const bf16x4 a = ld4(dY + (rt * 16 + li) * kOut + lk);
+                const bf16x4 b = ld4(w.w3 + (ct * 16 + li) * kOut + lk);
+                acc = mfma16(a, b, acc);
+                const int j = ct * 16 + li;
+                for (int e = 0; e < 4; ++e) {
+                    const int i = rt * 16 + lk + e;
+                    dZ2[i * kHid + j] = (H2[i * kHid + j] & 0x7fffu) != 0u && (H2[i * kHid + j] & 0x8000u) == 0u
+                                            ? to_bf16(acc[e])
+                                            : uint16_t(0);
+                }
+            }
+    }
+    __syncthreads();
+    // dH1 = dZ2 W2^T, dZ1 = dH1 * (H1 > 0)
+    {
+        const int li = l & 15, lk = (l >> 4) * 4;
+        const uint16_t* dZ2 = t.dZ2 + wave * 64 * kHid;
+        const uint16_t* H1 = t.H1 + wave * 64 * kHid;
+        uint16_t* dZ1 = t.dZ1 + wave * 64 * kHid;
+        for (int rt = 0; rt < 4; ++rt)
+            for (int ct = 0; ct < kHid / 16; ++ct) {
+                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                for (int kb = 0; kb < kHid / 16; ++kb) {
+                    const bf16x4 a = ld4(dZ2 + (rt * 16 + li) * kHid + kb * 16 + lk);
+                    const bf16x4 b = ld4(w.w2 + (ct * 16 + li) * kHid + kb * 16 + lk);
+                    acc = mfma16(a, b, acc);
+                }
+                const int j = ct * 16 + li;
+                for (int e = 0; e < 4; ++e) {
+                    const int i = rt * 16 + lk + e;
+                    dZ1[i * kHid + j] = (H1[i * kHid + j] & 0x7fffu) != 0u && (H1[i * kHid + j] & 0x8000u) == 0u
+                                            ? to_bf16(acc[e])
+                                            : uint16_t(0);
+                }
+            }
+    }
+    __syncthreads();
+    // weight and bias gradients over the block's 128 rows: 45 tiles over 2 waves
+    float* P = partial + size_t(blockIdx.x) * kParams;
+    for (int tile = wave; tile < 45; tile += 2) {
+        if (tile < 16) {  // dW1 = X^T dZ1
+            const int kt = tile / 4, jt = tile % 4;
+            store_grad_tile(P + oW1, kHid, kt, jt, grad_tile<kIn, kHid, false>(t.X, t.dZ1, kt, jt), false);
+        } else if (tile < 32) {  // dW2 = H1^T dZ2
+            const int kt = (tile - 16) / 4, jt = (tile - 16) % 4;
+            store_grad_tile(P + oW2, kHid, kt, jt, grad_tile<kHid, kHid, false>(t.H1, t.dZ2, kt, jt), false);
+        } else if (tile < 36) {  // dW3 = H2^T dY
+            const int kt = tile - 32;
+            store_grad_tile(P + oW3, kOut, kt, 0, grad_tile<kHid, kOut, false>(t.H2, t.dY, kt, 0), false);
+        } else if (tile < 40) {  // db1 = 1^T dZ1
+            const int jt = tile - 36;
+            store_grad_tile(P + ob1, 0, 0, jt, grad_tile<kIn, kHid, true>(nullptr, t.dZ1, 0, jt), true);
+        } else if (tile < 44) {  // db2
+            const int jt = tile - 40;
+            store_grad_tile(P + ob2, 0, 0, jt, grad_tile<kHid, kHid, true>(nullptr, t.dZ2, 0, jt), true);
+        } else {  // db3
+            store_grad_tile(P + ob3, 0, 0, 0, grad_tile<kHid, kOut, true>(nullptr, t.dY, 0, 0), true);
+        }
+    }
+    if (threadIdx.x == 0) {
+        float s = 0.0f;
+        for (int r = 0; r < kTrainRows; ++r) s += t.sq[r];
+        loss_partial[blockIdx.x] = s;
+    }
+}
+
+// Sum of the per-block partial gradients (block order) and, if lr > 0, one
+// Adam step on the fp32 master parameters, refreshing their bf16 copies.
+__global__ __launch_bounds__(256) void adam_kernel(const float* __restrict__ partial, uint32_t blocks,
+                                                   float* __restrict__ grad, float* __restrict__ master,
+                                                   float* __restrict__ m1, float* __restrict__ m2,
+                                                   uint16_t* __restrict__ pb, float lr, float c1,
+                                                   float c2)
+{
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p >= uint32_t(kParams)) return;
+    float g = 0.0f;
+    for (uint32_t b = 0; b < blocks; ++b) g += partial[size_t(b) * kParams + p];
+    grad[p] = g;
+    if (!(lr > 0.0f)) return;
+    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1.0e-8f;
+    const float a = beta1 * m1[p] + (1.0f - beta1) * g;
+    const float v = beta2 * m2[p] + (1.0f - beta2) * g * g;
+    m1[p] = a;
+    m2[p] = v;
+    const float mhat = a / c1, vhat = v / c2;
+    const float w = master[p] - lr * mhat / (sqrtf(vhat) + eps);
+    master[p] = w;
+    pb[p] = to_bf16(w);
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ src, uint16_t* __restrict__ dst, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = to_bf16(src[i]);
+}
+
+__global__ void loss_sum_kernel(const float* __restrict__ part, uint32_t n, float* __restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float s = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) s += part[i];
+    *out = s;
+}
+
+constexpr size_t kInferSmem = sizeof(InferWeights) + 4 * 2 * 64 * 64 * sizeof(uint16_t);
+constexpr size_t kTrainSmem = sizeof(TrainWeights) + sizeof(TrainTiles);
+static_assert(kInferSmem <= 160 * 1024 && kTrainSmem <= 160 * 1024, "LDS");
+
+}  // namespace
+}  // namespace oceanamd
+
+using namespace oceanamd;
+
+extern "C" uint32_t ocean_hip_mlp_train_blocks(uint32_t batch) { return batch / uint32_t(kTrainRows); }
+
+extern "C" hipError_t ocean_hip_mlp_spacing(const ocean_hip_level* levels, uint32_t level_count,
+                                            int32_t* spacing, hipStream_t s)
+{
+    if (level_count == 0) return hipSuccess;
+    hipLaunchKernelGGL(spacing_kernel, dim3(level_count), dim3(256), 0, s, levels, spacing);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ocean_hip_height_converged(const ocean_hip_level* levels, uint32_t level_count,
+                                                 const float* xz, uint32_t count, float tol,
+                                                 const ocean_hip_mlp* mlp, const int32_t* spacing,
+                                                 float* prop, float* heights, uint8_t* iters,
+                                                 uint8_t* flags, hipStream_t s)
+{
+    if (count == 0) return hipSuccess;
+    const dim3 grid((count + 255u) / 256u);
+    const float2* p = reinterpret_cast<const float2*>(xz);
+    if (mlp != nullptr) {
+        if (spacing == nullptr || prop == nullptr) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(proposal_kernel, grid, dim3(256), kInferSmem, s, levels, level_count, p,
+                           count, mlp->bf16, mlp->master, spacing, prop);
+        hipLaunchKernelGGL(converged_kernel<true>, grid, dim3(256), 0, s, levels, level_count, p, count,
+                           tol, prop, heights, iters, flags);
+    } else {
+        hipLaunchKernelGGL(converged_kernel<false>, grid, dim3(256), 0, s, levels, level_count, p, count,
+                           tol, nullptr, heights, iters, flags);
+    }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ocean_hip_mlp_pairs(const ocean_hip_level* levels, const float* xz,
+                                          uint32_t count, float tol, const int32_t* spacing,
+                                          float* feats, float* targets, uint8_t* ok, hipStream_t s)
+{
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(pairs_kernel, dim3((count + 255u) / 256u), dim3(256), 0, s, levels,
+                       reinterpret_cast<const float2*>(xz), count, tol, spacing, feats, targets, ok);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ocean_hip_mlp_forward(const ocean_hip_mlp* mlp, const float* feats,
+                                            uint32_t rows, float* out, hipStream_t s)
+{
+    if (rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(forward_kernel, dim3((rows + 255u) / 256u), dim3(256), kInferSmem, s, feats,
+                       rows, mlp->bf16, mlp->master, out);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ocean_hip_mlp_sync_bf16(ocean_hip_mlp* mlp, hipStream_t s)
+{
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, mlp->master,
+                       mlp->bf16, uint32_t(kParams));
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ocean_hip_mlp_step(ocean_hip_mlp* mlp, const float* feats, const float* targets,
+                                         const uint32_t* rows, uint32_t batch, float lr, float c1,
+                                         float c2, float* partial, float* loss_partial, float* loss,
+                                         hipStream_t s)
+{
+    const uint32_t blocks = batch / uint32_t(kTrainRows);
+    if (blocks == 0 || blocks * uint32_t(kTrainRows) != batch) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(train_kernel, dim3(blocks), dim3(128), kTrainSmem, s, feats, targets, rows, batch,
+                       mlp->bf16, mlp->master, partial, loss_partial);
+    hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, partial, blocks,
+                       mlp->grad, mlp->master, mlp->m1, mlp->m2, mlp->bf16, lr, c1, c2);
+    if (loss != nullptr) hipLaunchKernelGGL(loss_sum_kernel, dim3(1), dim3(64), 0, s, loss_partial, blocks, loss);
+    return hipGetLastError();
+}
+
+extern "C" int ocean_hip_mlp_configure()
+{
+    static int done = 0;
+    if (done) return 0;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&proposal_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, int(kInferSmem)) != hipSuccess)
+        return -1;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&forward_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, int(kInferSmem)) != hipSuccess)
+        return -1;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&train_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, int(kTrainSmem)) != hipSuccess)
+        return -1;
+    done = 1;
+    return 0;
+}

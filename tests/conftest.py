@@ -7,6 +7,9 @@ if ROOT not in sys.path:
 TESTS = os.path.dirname(os.path.abspath(__file__))
 if TESTS not in sys.path:
     sys.path.insert(0, TESTS)
+ORACLE = os.path.join(ROOT, "oracle")  # the numpy oracles (test infrastructure)
+if ORACLE not in sys.path:
+    sys.path.append(ORACLE)
 
 
 def pytest_configure(config):

@@ -220,6 +220,84 @@ def batched_cpu_baseline(budget_s: float = 8.0):
                       f"thread_count={cores}"}
 
 
+def learned_leg(lib, reps: int = 10):
+    """Converged height queries, plain vs learned start point, on the learned
+    proposal's sea: three 512^2 levels (800 / 150 / 25 m, 22 m/s, choppiness
+    1.3, seeds 1337-1339) queried as a cascade at t = 5 s, 1M points uniform
+    over a 2 km square, each level inverted until the step is <= 2.5e-6 m.
+    Reports queries/s of both forms (HIP events around `reps` back-to-back
+    calls on the levels' stream), taps per level (mean and per-wavefront
+    maximum) and the largest height error against the same points solved
+    plain to 1e-7 m."""
+    import numpy as np
+    import torch
+    from oceanlib_amd import capi
+    from oceanlib_amd.workloads import desc_from
+    params_path = os.path.join(ROOT, "oceanlib_amd", "data", "height_proposal.npy")
+    if not os.path.exists(params_path):
+        return None
+    params = np.load(params_path, allow_pickle=False)
+    levels = [dict(size=512, patch_length=L, wind_speed=22.0, choppiness=1.3, seed=sd)
+              for L, sd in ((800.0, 1337), (150.0, 1338), (25.0, 1339))]
+    descs = (capi.OceanDesc * 3)(*[desc_from(kw) for kw in levels])
+    st = C.c_int(-1)
+    b = lib.ocean_amd_batch_create(descs, 3, C.byref(st))
+    if not b:
+        raise SystemExit(f"batch create failed: {st.value}")
+    lib.ocean_amd_batch_update(b, 5.0)
+    db = capi.DeviceBuffers()
+    assert lib.ocean_amd_get_device_buffers(lib.ocean_amd_batch_level(b, 0), C.byref(db)) == 0
+    stream = torch.cuda.ExternalStream(db.stream)
+    nq = 1 << 20
+    tol = 2.5e-6
+    rng = np.random.default_rng(0xC5)
+    xz = torch.from_numpy(rng.uniform(0.0, 2000.0, (nq, 2)).astype(np.float32)).cuda()
+    model = lib.ocean_amd_proposal_create(params.ctypes.data_as(capi._F), nq, C.byref(st))
+    if not model:
+        raise SystemExit(f"proposal create failed: {st.value}")
+    ref = torch.empty(nq, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    assert lib.ocean_amd_batch_height_converged(b, 0, 3, C.c_void_p(xz.data_ptr()), nq, 1e-7, None,
+                                                C.c_void_p(ref.data_ptr()), None, None) == 0
+    out = {}
+    for name, m in (("plain", None), ("learned", model)):
+        h = torch.empty(nq, dtype=torch.float32, device="cuda")
+        it = torch.empty((nq, 3), dtype=torch.uint8, device="cuda")
+        fl = torch.empty(nq, dtype=torch.uint8, device="cuda")
+
+        def call(stats: bool):
+            r = lib.ocean_amd_batch_height_converged(
+                b, 0, 3, C.c_void_p(xz.data_ptr()), nq, tol, m, C.c_void_p(h.data_ptr()),
+                C.c_void_p(it.data_ptr()) if stats else None,
+                C.c_void_p(fl.data_ptr()) if stats else None)
+            assert r == 0
+        call(True)
+        assert lib.ocean_amd_batch_synchronize(b) == 0
+        itn = it.cpu().numpy().astype(np.int64)
+        fln = fl.cpu().numpy()
+        err = (h - ref).abs().max().item()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            call(False)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"queries_per_s": round(nq / (ms * 1e-3), 1), "ms_per_1M": round(ms, 4),
+                     "taps_mean_per_level": [round(float(v), 3) for v in itn.mean(0)],
+                     "taps_wave_max_per_level": [round(float(v), 3)
+                                                 for v in itn.reshape(-1, 64, 3).max(1).mean(0)],
+                     "max_err_mm": round(err * 1e3, 5),
+                     "not_converged": int(np.count_nonzero(fln & 1)),
+                     "fallbacks": int(np.count_nonzero(fln & 2))}
+    lib.ocean_amd_proposal_destroy(model)
+    lib.ocean_amd_batch_destroy(b)
+    out["speedup"] = round(out["learned"]["queries_per_s"] / out["plain"]["queries_per_s"], 4)
+    out["workload"] = ("3 x 512^2 levels as a cascade (800/150/25 m, 22 m/s, chop 1.3, seeds "
+                       "1337-1339), 1M points over a 2 km square, step tolerance 2.5e-6 m")
+    return out
+
+
 def capi_mod():
     from oceanlib_amd import capi
     return capi
@@ -233,6 +311,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--batch-frames", type=int, default=120,
                     help="timed frames of the batched-seas leg (0 skips it)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the converged / learned height query leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched leg: keep the default grid sizes")
     ap.add_argument("--profile-only", action="store_true",
@@ -349,6 +429,7 @@ def main():
     qps = reps * nq / (time.perf_counter() - tq)
 
     batched = batched_leg(lib, args.batch_frames, 5) if args.batch_frames > 0 else None
+    learned = learned_leg(lib) if not args.no_learned else None
     cpu = None if args.no_cpu else cpu_baseline(desc)
     if batched is not None and not args.no_cpu:
         batched["cpu_baseline"] = batched_cpu_baseline()
@@ -380,6 +461,7 @@ def main():
         "update_device_ms": round(row_ms.value + col_ms.value, 5),
         "height_at_queries_per_s": round(qps, 1),
         "batched": batched,
+        "converged_height_at": learned,
     }
     print(json.dumps(out), flush=True)
 

@@ -119,21 +119,26 @@ hipError_t ocean_hip_level_checksums(const ocean_hip_level* levels, uint32_t cou
 #define OCEAN_HIP_MLP_HIDDEN 64
 #define OCEAN_HIP_MLP_OUT 16
 #define OCEAN_HIP_MLP_PARAMS 9360
+#define OCEAN_HIP_MLP_BF16 18432   /* W1^T, W2^T, W3^T, W1, W2, W3 in bf16 */
+#define OCEAN_HIP_MLP_PROP 8       /* floats of proposal scratch per query */
+#define OCEAN_HIP_MLP_SPACING_PARTIALS 64  /* doubles of scratch per level */
 typedef struct ocean_hip_mlp {
-    uint16_t* bf16;   /* the parameters rounded to bf16 (what MFMA reads)   */
+    uint16_t* bf16;   /* the weights rounded to bf16, in the kernels' layouts */
     float* master;    /* fp32 parameters (biases are read from here)        */
     float* m1;        /* Adam first / second moments                        */
     float* m2;
     float* grad;      /* the last summed gradient                           */
 } ocean_hip_mlp;
 
-/* Stencil spacing per level from its displacement RMS (device int32[count]). */
+/* Stencil spacing per level from its displacement RMS (device int32[count]);
+ * with `partial` (count * SPACING_PARTIALS doubles, count <= 256) the sum is
+ * spread over 64 blocks per level, else one block per level. */
 hipError_t ocean_hip_mlp_spacing(const ocean_hip_level* levels, uint32_t level_count,
-                                 int32_t* spacing, hipStream_t s);
+                                 int32_t* spacing, double* partial, hipStream_t s);
 /* Heights at `count` points (device float2), each level inverted until the
  * fixed-point step is <= tol metres (at most 64 taps), summed over levels.
  * mlp != NULL: start from the proposal (needs `spacing` from
- * ocean_hip_mlp_spacing and `prop` scratch of count * OUT floats).
+ * ocean_hip_mlp_spacing and `prop` scratch of count * MLP_PROP floats).
  * iters (count * level_count bytes) and flags (count bytes: 1 = a level did
  * not converge, 2 = a proposal was abandoned) may be NULL. */
 hipError_t ocean_hip_height_converged(const ocean_hip_level* levels, uint32_t level_count,

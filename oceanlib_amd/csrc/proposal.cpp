@@ -28,7 +28,8 @@ struct HeightProposal::Impl {
     void* arena = nullptr;
     ocean_hip_mlp mlp{};
     std::int32_t* spacing = nullptr;   // 3 per call (queries)
-    float* prop = nullptr;             // max_queries * outputs
+    double* spacing_partial = nullptr; // 3 * 64
+    float* prop = nullptr;             // max_queries * OCEAN_HIP_MLP_PROP
     float* partial = nullptr;          // (max_batch / 128) * params
     float* loss_partial = nullptr;
     float* loss = nullptr;
@@ -90,8 +91,8 @@ bool converged(const ocean_hip_level* levels, std::size_t count_levels, hipStrea
         HeightProposal::Impl& m = model->impl();
         if (count_levels > ProposalShape::levels || count > m.max_queries) return false;
         if (!wait_for(model, s)) return false;
-        if (ocean_hip_mlp_spacing(levels, static_cast<std::uint32_t>(count_levels), m.spacing, s) !=
-            hipSuccess)
+        if (ocean_hip_mlp_spacing(levels, static_cast<std::uint32_t>(count_levels), m.spacing,
+                                  m.spacing_partial, s) != hipSuccess)
             return false;
         const bool ok = ocean_hip_height_converged(levels, static_cast<std::uint32_t>(count_levels),
                                                    xz_dev, count, tol, &m.mlp, m.spacing, m.prop,
@@ -120,10 +121,11 @@ HeightProposal::HeightProposal(const float* params, std::uint32_t max_queries)
     const std::size_t P = ProposalShape::params;
     const std::size_t blocks = ProposalShape::max_batch / 128;
     const std::size_t sizes[] = {
-        up256(P * 2),                                   // bf16
+        up256(std::size_t(OCEAN_HIP_MLP_BF16) * 2),     // bf16
         up256(P * 4), up256(P * 4), up256(P * 4), up256(P * 4),  // master, m1, m2, grad
         256,                                            // spacing
-        up256(std::size_t(max_queries) * ProposalShape::outputs * 4),  // prop
+        up256(ProposalShape::levels * OCEAN_HIP_MLP_SPACING_PARTIALS * 8),
+        up256(std::size_t(max_queries) * OCEAN_HIP_MLP_PROP * 4),  // prop
         up256(blocks * P * 4),                          // partial
         up256(blocks * 4),                              // loss partial
         256,                                            // loss
@@ -143,6 +145,7 @@ HeightProposal::HeightProposal(const float* params, std::uint32_t max_queries)
     take(m.mlp.m2);
     take(m.mlp.grad);
     take(m.spacing);
+    take(m.spacing_partial);
     take(m.prop);
     take(m.partial);
     take(m.loss_partial);
@@ -275,7 +278,8 @@ bool proposal_pairs(const OceanBatch& batch, std::size_t first, const float* xz_
     batch.level(0).device().make_current();
     const auto* levels = static_cast<const ocean_hip_level*>(batch.device_levels()) + first;
     const auto s = static_cast<hipStream_t>(batch.stream());
-    if (ocean_hip_mlp_spacing(levels, ProposalShape::levels, spacing_dev, s) != hipSuccess) return false;
+    if (ocean_hip_mlp_spacing(levels, ProposalShape::levels, spacing_dev, nullptr, s) != hipSuccess)
+        return false;
     return ocean_hip_mlp_pairs(levels, xz_dev, count, tol, spacing_dev, feats_dev, targets_dev, ok_dev,
                                s) == hipSuccess;
 }

@@ -189,38 +189,49 @@ __device__ __forceinline__ void level_features(const ocean_hip_level& L, const B
 // element k % 4; B[k][j] from lane j + 16 * (k / 4), element k % 4; D[i][j]
 // in lane j + 16 * (i / 4), element i % 4.
 
-struct InferWeights {  // LDS copies, transposed (B operand: 4 consecutive k)
-    uint16_t w1t[kHid * kIn];    // [j][k] = W1[k][j]
-    uint16_t w2t[kHid * kHid];
-    uint16_t w3t[kOut * kHid];
+// The bf16 parameters in device memory, in the layouts the kernels read:
+// W1^T [64][64], W2^T [64][64], W3^T [16][64] (rows are output units: the
+// B operand takes 4 consecutive inputs), then W1, W2, W3 row-major (the
+// backward pass's B = W^T operand). Biases are read in fp32 from the master.
+constexpr int bT1 = 0, bT2 = bT1 + kHid * kIn, bT3 = bT2 + kHid * kHid;
+constexpr int bR1 = bT3 + kOut * kHid, bR2 = bR1 + kIn * kHid, bR3 = bR2 + kHid * kHid;
+constexpr int kBf16 = bR3 + kHid * kOut;
+static_assert(kBf16 == OCEAN_HIP_MLP_BF16, "bf16 layout");
+
+// bf16 position(s) of master parameter p; -1 for biases.
+__device__ __forceinline__ void bf16_slots(int p, int& t, int& r)
+{
+    t = r = -1;
+    if (p < ob1) {
+        const int i = p - oW1, k = i / kHid, j = i % kHid;
+        t = bT1 + j * kIn + k;
+        r = bR1 + i;
+    } else if (p >= oW2 && p < ob2) {
+        const int i = p - oW2, k = i / kHid, j = i % kHid;
+        t = bT2 + j * kHid + k;
+        r = bR2 + i;
+    } else if (p >= oW3 && p < ob3) {
+        const int i = p - oW3, k = i / kOut, j = i % kOut;
+        t = bT3 + j * kHid + k;
+        r = bR3 + i;
+    }
+}
+
+// Inference LDS: the transposed weights at a padded row pitch, so the
+// 16 rows an MFMA operand load touches fall in distinct banks.
+constexpr int kPW = 68;
+struct InferLds {
+    uint16_t wt[(kHid + kHid + kOut) * kPW];  // W1^T rows, then W2^T, then W3^T
     float b1[kHid], b2[kHid], b3[kOut];
 };
-struct TrainWeights : InferWeights {  // plus row-major copies (backward: B = W^T)
-    uint16_t w1[kIn * kHid];
-    uint16_t w2[kHid * kHid];
-    uint16_t w3[kHid * kOut];
-};
+constexpr int kWaveTile = 64 * kPW + 2 * 16 * kPW;  // X (64 rows) + two 16-row buffers
 
-__device__ void load_weights(InferWeights& w, const uint16_t* __restrict__ pb,
-                             const float* __restrict__ pf, TrainWeights* tw)
+__device__ void load_infer(InferLds& w, const uint16_t* __restrict__ pb, const float* __restrict__ pf)
 {
-    for (int i = threadIdx.x; i < kIn * kHid; i += blockDim.x) {
-        const int k = i / kHid, j = i % kHid;
-        const uint16_t v = pb[oW1 + i];
-        w.w1t[j * kIn + k] = v;
-        if (tw) tw->w1[i] = v;
-    }
-    for (int i = threadIdx.x; i < kHid * kHid; i += blockDim.x) {
-        const int k = i / kHid, j = i % kHid;
-        const uint16_t v = pb[oW2 + i];
-        w.w2t[j * kHid + k] = v;
-        if (tw) tw->w2[i] = v;
-    }
-    for (int i = threadIdx.x; i < kHid * kOut; i += blockDim.x) {
-        const int k = i / kOut, j = i % kOut;
-        const uint16_t v = pb[oW3 + i];
-        w.w3t[j * kHid + k] = v;
-        if (tw) tw->w3[i] = v;
+    const uint2* src = reinterpret_cast<const uint2*>(pb + bT1);
+    for (int c = threadIdx.x; c < (kHid + kHid + kOut) * 16; c += blockDim.x) {
+        const int row = c >> 4, part = c & 15;
+        *reinterpret_cast<uint2*>(w.wt + row * kPW + part * 4) = src[c];
     }
     for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
         w.b1[i] = pf[ob1 + i];
@@ -229,67 +240,144 @@ __device__ void load_weights(InferWeights& w, const uint16_t* __restrict__ pb,
     for (int i = threadIdx.x; i < kOut; i += blockDim.x) w.b3[i] = pf[ob3 + i];
 }
 
-// The forward pass of one wavefront's 64 rows. X holds the bf16 inputs
-// (64 x kIn) and is reused for layer 2's output; H (64 x kHid) holds layer 1's
-// output and then the fp32 outputs Y (64 x kOut), returned in `y` per lane.
-__device__ void forward_wave(const InferWeights& w, uint16_t* X, uint16_t* H, float* y);
+// One 16-row tile: out = act(in * W + b). in: 16 rows at pitch PI; wt: W^T
+// [NO][PWT]; RELU: bf16 out at pitch PO; else fp32 outf at pitch PO.
+template <int NI, int NO, int PI, int PWT, int PO, bool RELU>
+__device__ __forceinline__ void dense_tile(const uint16_t* __restrict__ in,
+                                           const uint16_t* __restrict__ wt,
+                                           const float* __restrict__ bias, uint16_t* __restrict__ out,
+                                           float* __restrict__ outf)
+{
+    const int l = threadIdx.x & 63;
+    const int li = l & 15, lk = (l >> 4) * 4;
+#pragma unroll
+    for (int ct = 0; ct < NO / 16; ++ct) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kb = 0; kb < NI / 16; ++kb) {
+            const bf16x4 a = ld4(in + li * PI + kb * 16 + lk);
+            const bf16x4 bb = ld4(wt + (ct * 16 + li) * PWT + kb * 16 + lk);
+            acc = mfma16(a, bb, acc);
+        }
+        const int j = ct * 16 + li;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int i = lk + e;
+            const float z = acc[e] + bias[j];
+            if (RELU)
+                out[i * PO + j] = to_bf16(z > 0.0f ? z : 0.0f);
+            else
+                outf[i * PO + j] = z;
+        }
+    }
+}
 
-// out[64 x NO] = act(in[64 x NI] * W + b); in/out are bf16 row-major tiles
-// (pitch NI / NO); wt is W transposed [NO][NI]. RELU: hidden layer (bf16
-// out); otherwise the fp32 result goes to `outf` (pitch NO).
+// Writes lane l's 64 features as row l of X (bf16, 8-byte stores).
+__device__ __forceinline__ void store_row(uint16_t* X, const float* f)
+{
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int c = 0; c < kIn / 4; ++c) {
+        const uint32_t lo = uint32_t(to_bf16(f[4 * c])) | (uint32_t(to_bf16(f[4 * c + 1])) << 16);
+        const uint32_t hi = uint32_t(to_bf16(f[4 * c + 2])) | (uint32_t(to_bf16(f[4 * c + 3])) << 16);
+        *reinterpret_cast<uint2*>(X + l * kPW + 4 * c) = make_uint2(lo, hi);
+    }
+}
+
+// The forward pass of one wavefront's 64 rows (X), 16 rows at a time through
+// the two small buffers; lane l receives row l's outputs in y. Every wave of
+// the block calls it (it contains block barriers).
+__device__ void forward_wave(const InferLds& w, uint16_t* tile, float* y)
+{
+    const int l = threadIdx.x & 63;
+    uint16_t* X = tile;
+    uint16_t* Ha = X + 64 * kPW;
+    uint16_t* Hb = Ha + 16 * kPW;
+    float* Yf = reinterpret_cast<float*>(Ha);  // 16 x 16 fp32 fits in Ha
+#pragma unroll 1
+    for (int rt = 0; rt < 4; ++rt) {
+        dense_tile<kIn, kHid, kPW, kPW, kPW, true>(X + rt * 16 * kPW, w.wt, w.b1, Ha, nullptr);
+        __syncthreads();
+        dense_tile<kHid, kHid, kPW, kPW, kPW, true>(Ha, w.wt + kHid * kPW, w.b2, Hb, nullptr);
+        __syncthreads();
+        dense_tile<kHid, kOut, kPW, kPW, kOut, false>(Hb, w.wt + 2 * kHid * kPW, w.b3, nullptr, Yf);
+        __syncthreads();
+        if ((l >> 4) == rt) {
+#pragma unroll
+            for (int k = 0; k < kOut; ++k) y[k] = Yf[(l & 15) * kOut + k];
+        }
+        __syncthreads();
+    }
+}
+
+// ---- training layouts (unpadded; the trainer is not the hot path) --------------
+
+struct TrainWeights {
+    uint16_t w1t[kHid * kIn];    // W1^T [j][k]
+    uint16_t w2t[kHid * kHid];
+    uint16_t w3t[kOut * kHid];
+    uint16_t w1[kIn * kHid];     // W1 [k][j]
+    uint16_t w2[kHid * kHid];
+    uint16_t w3[kHid * kOut];
+    float b1[kHid], b2[kHid], b3[kOut];
+};
+
+__device__ void load_train(TrainWeights& w, const uint16_t* __restrict__ pb, const float* __restrict__ pf)
+{
+    const uint2* src = reinterpret_cast<const uint2*>(pb);
+    uint2* dst = reinterpret_cast<uint2*>(w.w1t);  // the six arrays in the global order
+    for (int c = threadIdx.x; c < kBf16 / 4; c += blockDim.x) dst[c] = src[c];
+    for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
+        w.b1[i] = pf[ob1 + i];
+        w.b2[i] = pf[ob2 + i];
+    }
+    for (int i = threadIdx.x; i < kOut; i += blockDim.x) w.b3[i] = pf[ob3 + i];
+}
+
+// out[64 x NO] = act(in[64 x NI] * W + b) over a whole wave's rows, unpadded.
 template <int NI, int NO, bool RELU>
 __device__ __forceinline__ void dense(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wt,
                                       const float* __restrict__ bias, uint16_t* __restrict__ out,
                                       float* __restrict__ outf)
 {
-    const int l = threadIdx.x & 63;
-    const int li = l & 15, lk = (l >> 4) * 4;
 #pragma unroll
-    for (int rt = 0; rt < 4; ++rt) {
-#pragma unroll
-        for (int ct = 0; ct < NO / 16; ++ct) {
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int kb = 0; kb < NI / 16; ++kb) {
-                const bf16x4 a = ld4(in + (rt * 16 + li) * NI + kb * 16 + lk);
-                const bf16x4 bb = ld4(wt + (ct * 16 + li) * NI + kb * 16 + lk);
-                acc = mfma16(a, bb, acc);
-            }
-            const int j = ct * 16 + li;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int i = rt * 16 + lk + e;
-                const float z = acc[e] + bias[j];
-                if (RELU)
-                    out[i * NO + j] = to_bf16(z > 0.0f ? z : 0.0f);
-                else
-                    outf[i * NO + j] = z;
-            }
-        }
-    }
-}
-
-__device__ void forward_wave(const InferWeights& w, uint16_t* X, uint16_t* H, float* y)
-{
-    const int l = threadIdx.x & 63;
-    dense<kIn, kHid, true>(X, w.w1t, w.b1, H, nullptr);
-    __syncthreads();
-    dense<kHid, kHid, true>(H, w.w2t, w.b2, X, nullptr);
-    __syncthreads();
-    float* Y = reinterpret_cast<float*>(H);
-    dense<kHid, kOut, false>(X, w.w3t, w.b3, nullptr, Y);
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kOut; ++k) y[k] = Y[l * kOut + k];
+    for (int rt = 0; rt < 4; ++rt)
+        dense_tile<NI, NO, NI, NI, NO, RELU>(in + rt * 16 * NI, wt, bias,
+                                             RELU ? out + rt * 16 * NO : nullptr,
+                                             RELU ? nullptr : outf + rt * 16 * NO);
 }
 
 // ---- kernels ------------------------------------------------------------------
 
-// Displacement RMS of each level -> stencil spacing (one block per level).
+// Displacement RMS of each level -> stencil spacing. One block per level
+// (spacing_kernel), or kSpacingBlocks blocks per level writing partial sums
+// that spacing_final_kernel adds in block order (spacing_partial_kernel).
+constexpr int kSpacingBlocks = 64;
+
+__device__ __forceinline__ int32_t spacing_of(double sum, const ocean_hip_level& L)
+{
+    const double cells = double(L.n) * double(L.n);
+    const double rms = sqrt(sum / (2.0 * cells));
+    const double cell = double(L.patch_length) / double(L.n);
+    const int s = int(rint(2.0 * rms / cell));
+    return s < 1 ? 1 : (s > 16 ? 16 : s);
+}
+
+__device__ __forceinline__ double block_sum(double acc)
+{
+    __shared__ double part[256];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    return part[0];
+}
+
 __global__ __launch_bounds__(256) void spacing_kernel(const ocean_hip_level* __restrict__ levels,
                                                       int32_t* __restrict__ spacing)
 {
-    __shared__ double part[256];
     const ocean_hip_level L = levels[blockIdx.x];
     const float4* __restrict__ disp = reinterpret_cast<const float4*>(L.disp);
     const uint32_t cells = L.n * L.n;
@@ -298,22 +386,40 @@ __global__ __launch_bounds__(256) void spacing_kernel(const ocean_hip_level* __r
         const float4 v = disp[i];
         acc += double(v.x) * double(v.x) + double(v.z) * double(v.z);
     }
-    part[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
-        __syncthreads();
+    const double sum = block_sum(acc);
+    if (threadIdx.x == 0) spacing[blockIdx.x] = spacing_of(sum, L);
+}
+
+__global__ __launch_bounds__(256) void spacing_partial_kernel(const ocean_hip_level* __restrict__ levels,
+                                                              double* __restrict__ partial)
+{
+    const ocean_hip_level L = levels[blockIdx.y];
+    const float4* __restrict__ disp = reinterpret_cast<const float4*>(L.disp);
+    const uint32_t cells = L.n * L.n;
+    double acc = 0.0;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < cells; i += kSpacingBlocks * 256u) {
+        const float4 v = disp[i];
+        acc += double(v.x) * double(v.x) + double(v.z) * double(v.z);
     }
-    if (threadIdx.x == 0) {
-        const double rms = sqrt(part[0] / (2.0 * double(cells)));
-        const double cell = double(L.patch_length) / double(L.n);
-        int s = int(rint(2.0 * rms / cell));
-        spacing[blockIdx.x] = s < 1 ? 1 : (s > 16 ? 16 : s);
-    }
+    const double sum = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.y * kSpacingBlocks + blockIdx.x] = sum;
+}
+
+__global__ void spacing_final_kernel(const ocean_hip_level* __restrict__ levels,
+                                     const double* __restrict__ partial, uint32_t level_count,
+                                     int32_t* __restrict__ spacing)
+{
+    const uint32_t lv = threadIdx.x;
+    if (blockIdx.x != 0 || lv >= level_count) return;
+    double sum = 0.0;
+    for (int b = 0; b < kSpacingBlocks; ++b) sum += partial[lv * kSpacingBlocks + b];
+    spacing[lv] = spacing_of(sum, levels[lv]);
 }
 
 // Proposals: 256 lanes = 4 waves of 64 queries; features -> MLP -> the
-// start offsets of each level in cells, prop[q * kOut + 2l (+1)].
+// start offsets of each level in cells, prop[q * kPropStride + 2l (+1)].
+constexpr int kPropStride = 8;
+
 __global__ __launch_bounds__(256) void proposal_kernel(const ocean_hip_level* __restrict__ levels,
                                                        uint32_t level_count,
                                                        const float2* __restrict__ xz, uint32_t count,
@@ -323,12 +429,9 @@ __global__ __launch_bounds__(256) void proposal_kernel(const ocean_hip_level* __
                                                        float* __restrict__ prop)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    InferWeights& w = *reinterpret_cast<InferWeights*>(smem);
-    uint16_t* tiles = reinterpret_cast<uint16_t*>(smem + sizeof(InferWeights));
-    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-    uint16_t* X = tiles + wave * (2 * 64 * 64);
-    uint16_t* H = X + 64 * 64;
-    load_weights(w, pb, pf, nullptr);
+    InferLds& w = *reinterpret_cast<InferLds*>(smem);
+    uint16_t* tile = reinterpret_cast<uint16_t*>(smem + sizeof(InferLds)) + (threadIdx.x >> 6) * kWaveTile;
+    load_infer(w, pb, pf);
     const uint32_t q = blockIdx.x * 256u + threadIdx.x;
     const bool live = q < count;
     const float2 p = live ? xz[q] : make_float2(0.0f, 0.0f);
@@ -342,16 +445,18 @@ __global__ __launch_bounds__(256) void proposal_kernel(const ocean_hip_level* __
             level_features(L, base_cell(p.x, p.y, L), spacing[lv], f + lv * kFeatPerLevel);
         }
     }
-#pragma unroll
-    for (int k = 0; k < kIn; ++k) X[l * kIn + k] = to_bf16(f[k]);
+    store_row(tile, f);
     __syncthreads();
     float y[kOut];
-    forward_wave(w, X, H, y);
+    forward_wave(w, tile, y);
     if (!live) return;
-    for (uint32_t lv = 0; lv < level_count && lv < uint32_t(kLevels); ++lv) {
-        const float sp = float(spacing[lv]);
-        prop[size_t(q) * kOut + 2 * lv] = y[2 * lv] * sp;
-        prop[size_t(q) * kOut + 2 * lv + 1] = y[2 * lv + 1] * sp;
+#pragma unroll
+    for (int lv = 0; lv < kLevels; ++lv) {
+        if (uint32_t(lv) < level_count) {
+            const float sp = float(spacing[lv]);
+            prop[size_t(q) * kPropStride + 2 * lv] = y[2 * lv] * sp;
+            prop[size_t(q) * kPropStride + 2 * lv + 1] = y[2 * lv + 1] * sp;
+        }
     }
 }
 
@@ -375,8 +480,8 @@ __global__ __launch_bounds__(256) void converged_kernel(
         const bool use = LEARNED && lv < uint32_t(kLevels);
         float du0 = 0.0f, dv0 = 0.0f;
         if (use) {
-            du0 = prop[size_t(q) * kOut + 2 * lv];
-            dv0 = prop[size_t(q) * kOut + 2 * lv + 1];
+            du0 = prop[size_t(q) * kPropStride + 2 * lv];
+            dv0 = prop[size_t(q) * kPropStride + 2 * lv + 1];
             if (!(fabsf(du0) < 1.0e4f) || !(fabsf(dv0) < 1.0e4f)) {
                 du0 = dv0 = 0.0f;  // not finite or far off the patch: abandoned at once
                 fl |= 2u;
@@ -427,24 +532,24 @@ __global__ __launch_bounds__(256) void pairs_kernel(const ocean_hip_level* __res
     ok[q] = good ? 1u : 0u;
 }
 
-// Forward only (rows of fp32 features -> fp32 outputs), 64 rows per wave.
+// Forward only (rows of fp32 features -> fp32 outputs), the proposal
+// kernel's network code, 64 rows per wave.
 __global__ __launch_bounds__(256) void forward_kernel(const float* __restrict__ feats, uint32_t rows,
                                                       const uint16_t* __restrict__ pb,
                                                       const float* __restrict__ pf,
                                                       float* __restrict__ out)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    InferWeights& w = *reinterpret_cast<InferWeights*>(smem);
-    uint16_t* tiles = reinterpret_cast<uint16_t*>(smem + sizeof(InferWeights));
-    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-    uint16_t* X = tiles + wave * (2 * 64 * 64);
-    uint16_t* H = X + 64 * 64;
-    load_weights(w, pb, pf, nullptr);
+    InferLds& w = *reinterpret_cast<InferLds*>(smem);
+    uint16_t* tile = reinterpret_cast<uint16_t*>(smem + sizeof(InferLds)) + (threadIdx.x >> 6) * kWaveTile;
+    load_infer(w, pb, pf);
     const uint32_t r = blockIdx.x * 256u + threadIdx.x;
-    for (int k = 0; k < kIn; ++k) X[l * kIn + k] = to_bf16(r < rows ? feats[size_t(r) * kIn + k] : 0.0f);
+    float f[kIn];
+    for (int k = 0; k < kIn; ++k) f[k] = r < rows ? feats[size_t(r) * kIn + k] : 0.0f;
+    store_row(tile, f);
     __syncthreads();
     float y[kOut];
-    forward_wave(w, X, H, y);
+    forward_wave(w, tile, y);
     if (r < rows)
         for (int k = 0; k < kOut; ++k) out[size_t(r) * kOut + k] = y[k];
 }
@@ -519,7 +624,7 @@ __global__ __launch_bounds__(128) void train_kernel(const float* __restrict__ fe
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int lr = wave * 64 + l;  // row inside the block
     const uint32_t row = rows[blockIdx.x * kTrainRows + lr];
-    load_weights(w, pb, pf, &w);
+    load_train(w, pb, pf);
     for (int k = 0; k < kIn; ++k) t.X[lr * kIn + k] = to_bf16(feats[size_t(row) * kIn + k]);
     __syncthreads();
     // forward, each wave on its 64 rows
@@ -646,13 +751,24 @@ __global__ __launch_bounds__(256) void adam_kernel(const float* __restrict__ par
     const float mhat = a / c1, vhat = v / c2;
     const float w = master[p] - lr * mhat / (sqrtf(vhat) + eps);
     master[p] = w;
-    pb[p] = to_bf16(w);
+    int t, r;
+    bf16_slots(int(p), t, r);
+    if (t >= 0) {
+        pb[t] = to_bf16(w);
+        pb[r] = to_bf16(w);
+    }
 }
 
-__global__ void to_bf16_kernel(const float* __restrict__ src, uint16_t* __restrict__ dst, uint32_t n)
+__global__ void to_bf16_kernel(const float* __restrict__ master, uint16_t* __restrict__ pb)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = to_bf16(src[i]);
+    const int p = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p >= kParams) return;
+    int t, r;
+    bf16_slots(p, t, r);
+    if (t >= 0) {
+        pb[t] = to_bf16(master[p]);
+        pb[r] = to_bf16(master[p]);
+    }
 }
 
 __global__ void loss_sum_kernel(const float* __restrict__ part, uint32_t n, float* __restrict__ out)
@@ -663,7 +779,7 @@ __global__ void loss_sum_kernel(const float* __restrict__ part, uint32_t n, floa
     *out = s;
 }
 
-constexpr size_t kInferSmem = sizeof(InferWeights) + 4 * 2 * 64 * 64 * sizeof(uint16_t);
+constexpr size_t kInferSmem = sizeof(InferLds) + 4 * kWaveTile * sizeof(uint16_t);
 constexpr size_t kTrainSmem = sizeof(TrainWeights) + sizeof(TrainTiles);
 static_assert(kInferSmem <= 160 * 1024 && kTrainSmem <= 160 * 1024, "LDS");
 
@@ -675,10 +791,18 @@ using namespace oceanamd;
 extern "C" uint32_t ocean_hip_mlp_train_blocks(uint32_t batch) { return batch / uint32_t(kTrainRows); }
 
 extern "C" hipError_t ocean_hip_mlp_spacing(const ocean_hip_level* levels, uint32_t level_count,
-                                            int32_t* spacing, hipStream_t s)
+                                            int32_t* spacing, double* partial, hipStream_t s)
 {
     if (level_count == 0) return hipSuccess;
-    hipLaunchKernelGGL(spacing_kernel, dim3(level_count), dim3(256), 0, s, levels, spacing);
+    if (partial == nullptr) {
+        hipLaunchKernelGGL(spacing_kernel, dim3(level_count), dim3(256), 0, s, levels, spacing);
+    } else {
+        if (level_count > 256u) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(spacing_partial_kernel, dim3(kSpacingBlocks, level_count), dim3(256), 0, s,
+                           levels, partial);
+        hipLaunchKernelGGL(spacing_final_kernel, dim3(1), dim3(256), 0, s, levels, partial, level_count,
+                           spacing);
+    }
     return hipGetLastError();
 }
 
@@ -726,7 +850,7 @@ extern "C" hipError_t ocean_hip_mlp_forward(const ocean_hip_mlp* mlp, const floa
 extern "C" hipError_t ocean_hip_mlp_sync_bf16(ocean_hip_mlp* mlp, hipStream_t s)
 {
     hipLaunchKernelGGL(to_bf16_kernel, dim3((kParams + 255) / 256), dim3(256), 0, s, mlp->master,
-                       mlp->bf16, uint32_t(kParams));
+                       mlp->bf16);
     return hipGetLastError();
 }
 

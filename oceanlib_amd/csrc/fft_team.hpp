@@ -37,6 +37,11 @@ struct FftShape {
 __host__ __device__ __forceinline__ uint32_t lds_pad(uint32_t i) { return i + (i >> 4); }
 template <int N>
 constexpr int lds_span() { return N + N / 16; }
+// Distance between consecutive teams' exchange areas: lds_span rounded up to
+// 2 mod 16 float2 (4 mod 32 banks), so the teams a wavefront's 16-lane group
+// spans (column teams are interleaved lane by lane) hit different banks.
+template <int N>
+constexpr int team_stride() { return lds_span<N>() + ((2 - lds_span<N>() % 16) + 16) % 16; }
 
 // r butterfly stages (r <= 3) starting at stage t0, on a lane's 8 registers.
 // idx[a] tracks the reference-array index register a currently represents.
@@ -78,7 +83,8 @@ struct PassPlan {
 
 // Full inverse transform of one complex sequence held by a team.
 //   re/im   : in: register a = element J + a*T; out: element idx[a]
-//   xch     : this team's LDS exchange area (lds_span<N>() float2)
+//   xch     : this team's LDS exchange area (lds_span<N>() float2; teams of
+//             a block are team_stride<N>() apart)
 //   tw      : LDS twiddles, N/2 entries (cos, +sin) of 2*pi*j/N
 //   sync    : barrier callable (a block barrier in practice)
 template <int N, typename Sync>

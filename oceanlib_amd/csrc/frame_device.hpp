@@ -114,7 +114,7 @@ __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk
         }
     }
 
-    float2* x_team = xch + team * lds_span<N>();
+    float2* x_team = xch + team * team_stride<N>();
 #pragma unroll
     for (int f = 0; f < F; ++f) {
         uint32_t idx[8];
@@ -163,7 +163,7 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
     uint32_t idx[8];
 #pragma unroll
     for (int f = 0; f < F; ++f)
-        fft_team<N>(fr[f], fi[f], idx, J, xch + c * lds_span<N>(), tw, BlockSync{});
+        fft_team<N>(fr[f], fi[f], idx, J, xch + c * team_stride<N>(), tw, BlockSync{});
 
     // reference src/core/evolve.cpp:256-325
 #pragma unroll

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(RowShape<N>::BLOCK) void row_pass_kernel(ocean_hip_
                                                                       double time_over_two_pi)
 {
     __shared__ float2 tw[N / 2];
-    __shared__ float2 xch[RowShape<N>::TEAMS * lds_span<N>()];
+    __shared__ float2 xch[RowShape<N>::TEAMS * team_stride<N>()];
     row_block<N, F, RowShape<N>::TEAMS>(L, blockIdx.x, time_over_two_pi, tw, xch);
 }
 
@@ -66,7 +66,7 @@ template <int N, int F>
 __global__ __launch_bounds__(ColShape<N>::BLOCK) void col_pass_kernel(ocean_hip_level L)
 {
     __shared__ float2 tw[N / 2];
-    __shared__ float2 xch[ColShape<N>::C * lds_span<N>()];
+    __shared__ float2 xch[ColShape<N>::C * team_stride<N>()];
     col_block<N, F, ColShape<N>::C>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch);
 }
 
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void batch_row_kernel(const ocean_hip_level
                                                            uint32_t n_items, double t2)
 {
     __shared__ float2 tw[N / 2];
-    __shared__ float2 xch[BatchShape<N>::PER * lds_span<N>()];
+    __shared__ float2 xch[BatchShape<N>::PER * team_stride<N>()];
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const uint32_t it = items[i];
         row_block<N, F, BatchShape<N>::PER>(levels[it >> kItemBlockBits], it & kItemBlockMask, t2,
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void batch_col_kernel(const ocean_hip_level
                                                            uint32_t n_items)
 {
     __shared__ float2 tw[N / 2];
-    __shared__ float2 xch[BatchShape<N>::PER * lds_span<N>()];
+    __shared__ float2 xch[BatchShape<N>::PER * team_stride<N>()];
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const uint32_t it = items[i];
         col_block<N, F, BatchShape<N>::PER>(levels[it >> kItemBlockBits], it & kItemBlockMask,

@@ -344,6 +344,16 @@ def main():
     if not foam:
         raise SystemExit(f"foam create failed: {st.value}")
 
+    # Launch shapes of the two frame kernels, timed on the device (the
+    # autotuner; every shape gives the same bits).
+    shape_r, shape_c = C.c_uint32(), C.c_uint32()
+    tune_r, tune_c = C.c_float(), C.c_float()
+    if lib.ocean_amd_autotune_frame(sim, 20, C.byref(shape_r), C.byref(shape_c), C.byref(tune_r),
+                                    C.byref(tune_c)) != 0:
+        raise SystemExit("frame autotune failed")
+    log(f"bench: frame shapes row {shape_r.value} col {shape_c.value} "
+        f"({tune_r.value * 1e3:.1f} / {tune_c.value * 1e3:.1f} us)")
+
     total = args.warmup + args.steps
     frames_xz = 64
     xz_host = query_points(frames_xz, QUERIES_PER_FRAME, desc.patch_length)
@@ -459,6 +469,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
         "update_device_ms": round(row_ms.value + col_ms.value, 5),
+        "frame_shape": {"row_variant": shape_r.value, "col_variant": shape_c.value},
         "height_at_queries_per_s": round(qps, 1),
         "batched": batched,
         "converged_height_at": learned,

@@ -256,6 +256,82 @@ int ocean_amd_profile_frame(ocean_sim* sim, double t0, double dt, uint32_t iters
     return dev.status == 0 ? rc : OCEAN_ERROR_UNKNOWN;
 }
 
+int ocean_amd_frame_shapes(uint32_t n, uint32_t* row_variants, uint32_t* col_variants)
+{
+    int r = 0, c = 0;
+    if (ocean_hip_frame_shapes(n, &r, &c) != 0) return OCEAN_ERROR_INVALID_ARG;
+    if (row_variants) *row_variants = static_cast<uint32_t>(r);
+    if (col_variants) *col_variants = static_cast<uint32_t>(c);
+    return OCEAN_OK;
+}
+
+int ocean_amd_set_frame_shape(ocean_sim* sim, uint32_t row_variant, uint32_t col_variant)
+{
+    if (sim == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    ocean::Ocean& o = *to_cpp(sim);
+    auto& dev = o.device();
+    int r = 0, c = 0;
+    if (ocean_hip_frame_shapes(dev.level.n, &r, &c) != 0 || row_variant >= uint32_t(r) ||
+        col_variant >= uint32_t(c))
+        return OCEAN_ERROR_INVALID_ARG;
+    dev.make_current();
+    dev.level.shape = row_variant | (col_variant << 8);
+    dev.note(hipMemcpyAsync(dev.d_level, &dev.level, sizeof(dev.level), hipMemcpyHostToDevice, dev.stream));
+    dev.note(hipStreamSynchronize(dev.stream));
+    return dev.status == 0 ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
+}
+
+int ocean_amd_autotune_frame(ocean_sim* sim, uint32_t reps, uint32_t* row_variant,
+                             uint32_t* col_variant, float* row_ms, float* col_ms)
+{
+    if (sim == nullptr || reps == 0) return OCEAN_ERROR_INVALID_ARG;
+    ocean::Ocean& o = *to_cpp(sim);
+    auto& dev = o.device();
+    int nr = 0, nc = 0;
+    if (ocean_hip_frame_shapes(dev.level.n, &nr, &nc) != 0) return OCEAN_ERROR_INVALID_ARG;
+    dev.make_current();
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return OCEAN_ERROR_UNKNOWN;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return OCEAN_ERROR_UNKNOWN;
+    }
+    const double t2 = o.time() * 0.159154943091895335769;
+    const uint32_t saved = dev.level.shape;
+    float best[2] = {1e30f, 1e30f};
+    uint32_t pick[2] = {saved & 0xffu, (saved >> 8) & 0xffu};
+    for (unsigned pass = 0; pass < 2; ++pass) {
+        const int count = pass == 0 ? nr : nc;
+        for (int v = 0; v < count; ++v) {
+            dev.level.shape = pass == 0 ? (uint32_t(v) | (pick[1] << 8)) : (pick[0] | (uint32_t(v) << 8));
+            dev.note(ocean_hip_frame_passes(&dev.level, t2, dev.stream, 1u << pass));  // warm
+            hipEventRecord(e0, dev.stream);
+            for (uint32_t i = 0; i < reps; ++i)
+                dev.note(ocean_hip_frame_passes(&dev.level, t2, dev.stream, 1u << pass));
+            hipEventRecord(e1, dev.stream);
+            hipEventSynchronize(e1);
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, e0, e1);
+            ms /= float(reps);
+            if (ms < best[pass]) {
+                best[pass] = ms;
+                pick[pass] = uint32_t(v);
+            }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    dev.level.shape = pick[0] | (pick[1] << 8);
+    dev.note(hipMemcpyAsync(dev.d_level, &dev.level, sizeof(dev.level), hipMemcpyHostToDevice, dev.stream));
+    dev.note(ocean_hip_frame(&dev.level, t2, dev.stream));  // a complete frame at the current time
+    dev.note(hipStreamSynchronize(dev.stream));
+    if (row_variant) *row_variant = pick[0];
+    if (col_variant) *col_variant = pick[1];
+    if (row_ms) *row_ms = best[0];
+    if (col_ms) *col_ms = best[1];
+    return dev.status == 0 ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
+}
+
 void ocean_amd_synchronize(const ocean_sim* sim)
 {
     if (sim != nullptr) to_cpp(sim)->synchronize();

@@ -42,47 +42,134 @@ struct ColShape {
     static constexpr int BLOCK = C * T;
 };
 
-template <int N, int F>
-__global__ __launch_bounds__(RowShape<N>::BLOCK) void row_pass_kernel(ocean_hip_level L,
-                                                                      double time_over_two_pi)
-{
-    __shared__ float2 tw[N / 2];
-    __shared__ float2 xch[RowShape<N>::TEAMS * team_stride<N>()];
-    row_block<N, F, RowShape<N>::TEAMS>(L, blockIdx.x, time_over_two_pi, tw, xch);
-}
-
 // Workgroups go to the 8 XCDs round-robin by block id, and each XCD has its
 // own L2. Column blocks read C*8 bytes of every row, so blocks of adjacent
-// column groups share 128-byte lines; mapping block b to group
-// (b % 8) * (G / 8) + b / 8 keeps each XCD on a contiguous range of groups, so
-// a shared line is fetched from HBM once instead of once per XCD.
+// column groups share 128-byte lines (and row blocks read each other's mirror
+// rows); mapping block b to group (b % 8) * (G / 8) + b / 8 keeps each XCD on
+// a contiguous range of groups, so a shared line is fetched from HBM once
+// instead of once per XCD.
 __device__ __forceinline__ uint32_t xcd_group(uint32_t b, uint32_t g)
 {
     if (g % 8u != 0u) return b;
     return (b % 8u) * (g / 8u) + b / 8u;
 }
 
-template <int N, int F>
-__global__ __launch_bounds__(ColShape<N>::BLOCK) void col_pass_kernel(ocean_hip_level L)
+template <int N, int F, int TEAMS>
+__global__ __launch_bounds__(TEAMS * FftShape<N>::T) void row_pass_kernel(ocean_hip_level L,
+                                                                          double time_over_two_pi)
 {
     __shared__ float2 tw[N / 2];
-    __shared__ float2 xch[ColShape<N>::C * team_stride<N>()];
-    col_block<N, F, ColShape<N>::C>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch);
+    __shared__ float2 xch[TEAMS * team_stride<N>()];
+    row_block<N, F, TEAMS>(L, xcd_group(blockIdx.x, gridDim.x), time_over_two_pi, tw, xch);
+}
+
+template <int N, int F, int C>
+__global__ __launch_bounds__(C * FftShape<N>::T) void col_pass_kernel(ocean_hip_level L)
+{
+    __shared__ float2 tw[N / 2];
+    __shared__ float2 xch[C * team_stride<N>()];
+    col_block<N, F, C>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch);
+}
+
+// Launch shapes of the per-level passes. Variant 0 is the default shape;
+// the others (rows per block TEAMS, columns per block C, each 1..32) are what
+// ocean_hip_frame_shapes lists for the autotuner. Every shape runs the same
+// row_block / col_block code on the same cells, so results do not depend on
+// the shape.
+template <int N, int TEAMS>
+constexpr bool row_ok()
+{
+    constexpr int lanes = TEAMS * FftShape<N>::T;
+    return TEAMS <= N && lanes <= 512 && lanes >= 64 && lanes % 64 == 0;  // 1024 lanes spill at F = 6
+}
+template <int N, int C>
+constexpr bool col_ok()
+{
+    constexpr int lanes = C * FftShape<N>::T;
+    return C <= N && lanes <= 512 && lanes >= 64 && lanes % 64 == 0 &&
+           C * team_stride<N>() * 8 + N * 4 <= 65536;
+}
+
+struct Pick {
+    int want, seen;
+};
+
+template <int N, int F, int TEAMS>
+bool try_row(Pick& p, const ocean_hip_level& L, double t2, hipStream_t s, bool is_default)
+{
+    if constexpr (!row_ok<N, TEAMS>() && TEAMS != RowShape<N>::TEAMS) {
+        return false;
+    } else {
+        if (!is_default && TEAMS == RowShape<N>::TEAMS) return false;  // listed first already
+        if (p.seen++ != p.want) return false;
+        if (s != nullptr || L.n != 0)
+            hipLaunchKernelGGL((row_pass_kernel<N, F, TEAMS>), dim3(N / TEAMS),
+                               dim3(TEAMS * FftShape<N>::T), 0, s, L, t2);
+        return true;
+    }
+}
+
+template <int N, int F, int C>
+bool try_col(Pick& p, const ocean_hip_level& L, hipStream_t s, bool is_default)
+{
+    if constexpr (!col_ok<N, C>() && C != ColShape<N>::C) {
+        return false;
+    } else {
+        if (!is_default && C == ColShape<N>::C) return false;
+        if (p.seen++ != p.want) return false;
+        if (s != nullptr || L.n != 0)
+            hipLaunchKernelGGL((col_pass_kernel<N, F, C>), dim3(N / C), dim3(C * FftShape<N>::T), 0,
+                               s, L);
+        return true;
+    }
+}
+
+template <int N, int F>
+bool row_variant(int v, const ocean_hip_level& L, double t2, hipStream_t s)
+{
+    Pick p{v, 0};
+    return try_row<N, F, RowShape<N>::TEAMS>(p, L, t2, s, true) || try_row<N, F, 1>(p, L, t2, s, false) ||
+           try_row<N, F, 2>(p, L, t2, s, false) || try_row<N, F, 4>(p, L, t2, s, false) ||
+           try_row<N, F, 8>(p, L, t2, s, false) || try_row<N, F, 16>(p, L, t2, s, false) ||
+           try_row<N, F, 32>(p, L, t2, s, false);
+}
+
+template <int N, int F>
+bool col_variant(int v, const ocean_hip_level& L, hipStream_t s)
+{
+    Pick p{v, 0};
+    return try_col<N, F, ColShape<N>::C>(p, L, s, true) || try_col<N, F, 1>(p, L, s, false) ||
+           try_col<N, F, 2>(p, L, s, false) || try_col<N, F, 4>(p, L, s, false) ||
+           try_col<N, F, 8>(p, L, s, false) || try_col<N, F, 16>(p, L, s, false) ||
+           try_col<N, F, 32>(p, L, s, false);
+}
+
+// Number of variants of each pass (counted without launching: a null level
+// view with n = 0 and no stream).
+template <int N>
+void count_variants(int& rows, int& cols)
+{
+    ocean_hip_level none{};
+    rows = 0;
+    while (row_variant<N, 4>(rows, none, 0.0, nullptr)) ++rows;
+    cols = 0;
+    while (col_variant<N, 4>(cols, none, nullptr)) ++cols;
 }
 
 template <int N>
 hipError_t launch_frame_n(const ocean_hip_level* L, double time_over_two_pi, hipStream_t s,
                           unsigned passes)
 {
-    const dim3 rg(N / RowShape<N>::TEAMS), rb(RowShape<N>::BLOCK);
-    const dim3 cg(N / ColShape<N>::C), cb(ColShape<N>::BLOCK);
+    const int rv = int(L->shape & 0xffu), cv = int((L->shape >> 8) & 0xffu);
+    bool ok = true;
     if (L->vel != nullptr) {
-        if (passes & 1u) hipLaunchKernelGGL((row_pass_kernel<N, 6>), rg, rb, 0, s, *L, time_over_two_pi);
-        if (passes & 2u) hipLaunchKernelGGL((col_pass_kernel<N, 6>), cg, cb, 0, s, *L);
+        if (passes & 1u) ok = ok && row_variant<N, 6>(rv, *L, time_over_two_pi, s);
+        if (passes & 2u) ok = ok && col_variant<N, 6>(cv, *L, s);
     } else {
-        if (passes & 1u) hipLaunchKernelGGL((row_pass_kernel<N, 4>), rg, rb, 0, s, *L, time_over_two_pi);
-        if (passes & 2u) hipLaunchKernelGGL((col_pass_kernel<N, 4>), cg, cb, 0, s, *L);
+        if (passes & 1u) ok = ok && row_variant<N, 4>(rv, *L, time_over_two_pi, s);
+        if (passes & 2u) ok = ok && col_variant<N, 4>(cv, *L, s);
     }
+    if (!ok) return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -191,6 +278,26 @@ extern "C" hipError_t ocean_hip_frame(const ocean_hip_level* L, double time_over
                                       hipStream_t s)
 {
     return ocean_hip_frame_passes(L, time_over_two_pi, s, 3u);
+}
+
+extern "C" int ocean_hip_frame_shapes(uint32_t n, int* row_variants, int* col_variants)
+{
+    int r = 0, c = 0;
+    switch (n) {
+        case 16: count_variants<16>(r, c); break;
+        case 32: count_variants<32>(r, c); break;
+        case 64: count_variants<64>(r, c); break;
+        case 128: count_variants<128>(r, c); break;
+        case 256: count_variants<256>(r, c); break;
+        case 512: count_variants<512>(r, c); break;
+        case 1024: count_variants<1024>(r, c); break;
+        case 2048: count_variants<2048>(r, c); break;
+        case 4096: count_variants<4096>(r, c); break;
+        default: return -1;
+    }
+    if (row_variants) *row_variants = r;
+    if (col_variants) *col_variants = c;
+    return 0;
 }
 
 extern "C" int ocean_hip_batchable(uint32_t n) { return batch_kernel(n, 4, true) != nullptr; }

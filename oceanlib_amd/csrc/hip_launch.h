@@ -17,6 +17,8 @@ typedef struct ocean_hip_level {
     float choppiness;
     float foam_threshold;
     float inv_foam_threshold;   /* host-computed as in evolve.cpp:247-248      */
+    uint32_t shape;             /* launch shape of the per-level passes: row
+                                   variant | column variant << 8 (0 = default) */
     const float* h0;            /* N*N float2 (re, im)                         */
     const float* omega;         /* N*N float                                   */
     const float* twiddles;      /* N/2 float2 (cos, +sin)                      */
@@ -42,9 +44,14 @@ typedef struct ocean_hip_foam_params {
 hipError_t ocean_hip_frame(const ocean_hip_level* level, double time_over_two_pi,
                            hipStream_t s);
 /* The same with a pass mask (1 = row pass, 2 = column pass), for timing the
- * two kernels separately. */
+ * two kernels separately. The level's `shape` picks the launch shapes. */
 hipError_t ocean_hip_frame_passes(const ocean_hip_level* level, double time_over_two_pi,
                                   hipStream_t s, unsigned passes);
+
+/* Launch-shape variants of the per-level passes for grid size n (variant 0
+ * is the default): how many row and column variants exist; -1 if n is not
+ * a supported size. */
+int ocean_hip_frame_shapes(uint32_t n, int* row_variants, int* col_variants);
 
 /* Batched sample_at over `count` (x, z) pairs (device float2). `out` and/or
  * `heights` may be NULL. */

@@ -79,6 +79,13 @@ int ocean_amd_sample_batch_device(const ocean_sim* sim, const float* xz_device,
                                   uint32_t count, ocean_surface_ex* out_device,
                                   float* heights_device);
 
+/* The ocean's constant tables as the frame kernels read them (copied from
+ * the device): h0 (2*N*N floats), omega (N*N), and how many entries the
+ * device build flagged and the host recomputed at creation. The device build
+ * (reference src/core/spectrum.cpp:264-339, src/core/rng.cpp:7-17 on the
+ * GPU) equals ocean_amd_build_spectrum bit for bit. Any pointer may be NULL. */
+int ocean_amd_get_tables(const ocean_sim* sim, float* h0, float* omega, uint32_t* repaired);
+
 /* ---- host-side tables (no device needed) -------------------------------
  * The constant tables the frame kernels read, as the library builds them:
  * h0 (2*N*N floats, re/im interleaved) and omega (N*N), reference

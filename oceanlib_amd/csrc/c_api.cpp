@@ -256,6 +256,21 @@ int ocean_amd_profile_frame(ocean_sim* sim, double t0, double dt, uint32_t iters
     return dev.status == 0 ? rc : OCEAN_ERROR_UNKNOWN;
 }
 
+int ocean_amd_get_tables(const ocean_sim* sim, float* h0, float* omega, uint32_t* repaired)
+{
+    if (sim == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    auto& dev = to_cpp(sim)->device();
+    dev.make_current();
+    const std::size_t cells = std::size_t(dev.level.n) * dev.level.n;
+    if (h0 != nullptr)
+        dev.note(hipMemcpyAsync(h0, dev.level.h0, cells * 8, hipMemcpyDeviceToHost, dev.stream));
+    if (omega != nullptr)
+        dev.note(hipMemcpyAsync(omega, dev.level.omega, cells * 4, hipMemcpyDeviceToHost, dev.stream));
+    dev.note(hipStreamSynchronize(dev.stream));
+    if (repaired != nullptr) *repaired = dev.init_repaired;
+    return dev.status == 0 ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
+}
+
 int ocean_amd_frame_shapes(uint32_t n, uint32_t* row_variants, uint32_t* col_variants)
 {
     int r = 0, c = 0;

@@ -40,6 +40,25 @@ typedef struct ocean_hip_foam_params {
     float dt, inv_cell, decay, gain, scale, drift_x, drift_z;
 } ocean_hip_foam_params;
 
+/* ---- the constant tables on the device -----------------------------------
+ * h0 (N*N float2), omega (N*N) and the twiddles (N/2 float2) by the device
+ * libm; flags[i] = 1 marks a cell (twiddle) whose float results could differ
+ * from the host libm's and must be recomputed on the host. */
+typedef struct ocean_hip_spectrum_params {
+    uint32_t n;
+    float patch_length;
+    uint64_t seed;
+    double dk;
+    double gravity, depth, omega_p, alpha, gamma, wind, swell, wind_direction, cutoff_l, mu, q_den;
+} ocean_hip_spectrum_params;
+hipError_t ocean_hip_spectrum(const ocean_hip_spectrum_params* p, float* h0, float* omega,
+                              uint8_t* flags, hipStream_t s);
+hipError_t ocean_hip_twiddles(uint32_t n, float* tw, uint8_t* flags, hipStream_t s);
+/* Writes host-computed cells: `fixes` is a device array of count records
+ * {uint32 index; float h0_re, h0_im, omega}. */
+hipError_t ocean_hip_spectrum_fix(const void* fixes, uint32_t count, float* h0, float* omega,
+                                  hipStream_t s);
+
 /* Enqueue one frame (row pass + column pass) of one level on stream s. */
 hipError_t ocean_hip_frame(const ocean_hip_level* level, double time_over_two_pi,
                            hipStream_t s);

@@ -16,7 +16,6 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
-#include <thread>
 #include <vector>
 
 namespace ocean {
@@ -45,22 +44,80 @@ void validate(const OceanDesc& d)
 
 std::size_t align_up(std::size_t v, std::size_t a) { return (v + a - 1) / a * a; }
 
-// Spectrum rows are independent (one PCG32 stream per row), so the host build
-// splits them across threads with bit-identical results.
-void build_tables_threaded(const OceanDesc& d, float* h0, float* omega)
+// One cell the device could not certify, recomputed by the host chain.
+struct TableFix {
+    std::uint32_t index;
+    float h0r, h0i, omega;
+};
+
+// h0, omega and the twiddles by the device kernels (spectrum_kernels.hip);
+// the cells and twiddles they flag are recomputed by the host's libm chain
+// (spectrum_host.cpp) and written over the device's values, so the tables
+// equal the host build bit for bit. Returns the number of repaired entries.
+std::uint32_t build_tables_on_device(const OceanDesc& desc, float* h0, float* omega, float* tw,
+                                     unsigned char* scratch, hipStream_t stream)
 {
-    const std::uint32_t n = d.size;
-    unsigned workers = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (static_cast<std::size_t>(n) * n < 16384) workers = 1;
-    std::vector<std::thread> pool;
-    const std::uint32_t chunk = (n + workers - 1) / workers;
-    for (unsigned w = 0; w < workers; ++w) {
-        const std::uint32_t b = w * chunk, e = std::min(n, b + chunk);
-        if (b >= e) break;
-        pool.emplace_back([&, b, e] { oceanamd::build_spectrum_tables(d, h0, omega, b, e); });
+    using detail::hip_check;
+    const std::uint32_t n = desc.size;
+    const std::size_t cells = std::size_t(n) * n;
+    const oceanamd::SpectrumConstants c = oceanamd::spectrum_constants(desc);
+    ocean_hip_spectrum_params p{};
+    p.n = n;
+    p.patch_length = desc.patch_length;
+    p.seed = desc.seed;
+    p.dk = c.dk;
+    p.gravity = c.gravity;
+    p.depth = c.depth;
+    p.omega_p = c.omega_p;
+    p.alpha = c.alpha;
+    p.gamma = c.gamma;
+    p.wind = c.wind;
+    p.swell = c.swell;
+    p.wind_direction = c.wind_direction;
+    p.cutoff_l = c.cutoff_l;
+    p.mu = c.mu;
+    p.q_den = c.q_den;
+    unsigned char* flags = scratch;
+    unsigned char* tflags = scratch + cells;
+    hip_check(ocean_hip_spectrum(&p, h0, omega, flags, stream), "device spectrum");
+    hip_check(ocean_hip_twiddles(n, tw, tflags, stream), "device twiddles");
+    std::vector<unsigned char> hf(cells + n / 2);
+    hip_check(hipMemcpyAsync(hf.data(), flags, hf.size(), hipMemcpyDeviceToHost, stream), "flags");
+    hip_check(hipStreamSynchronize(stream), "device tables");
+    std::vector<TableFix> fixes;
+    for (std::size_t i = 0; i < cells; ++i) {
+        if (!hf[i]) continue;
+        TableFix f{std::uint32_t(i), 0.0f, 0.0f, 0.0f};
+        float hv[2];
+        oceanamd::spectrum_cell(desc, std::uint32_t(i % n), std::uint32_t(i / n), hv, &f.omega);
+        f.h0r = hv[0];
+        f.h0i = hv[1];
+        fixes.push_back(f);
     }
-    for (auto& t : pool) t.join();
+    std::vector<float> twfix;
+    std::vector<std::uint32_t> twidx;
+    for (std::uint32_t j = 0; j < n / 2; ++j) {
+        if (!hf[cells + j]) continue;
+        float v[2];
+        oceanamd::twiddle_at(n, j, v);
+        twidx.push_back(j);
+        twfix.push_back(v[0]);
+        twfix.push_back(v[1]);
+    }
+    if (!fixes.empty()) {
+        hip_check(hipMemcpyAsync(scratch, fixes.data(), fixes.size() * sizeof(TableFix),
+                                 hipMemcpyHostToDevice, stream), "upload fixes");
+        hip_check(ocean_hip_spectrum_fix(scratch, std::uint32_t(fixes.size()), h0, omega, stream),
+                  "apply fixes");
+    }
+    for (std::size_t k = 0; k < twidx.size(); ++k)
+        hip_check(hipMemcpyAsync(tw + 2 * std::size_t(twidx[k]), twfix.data() + 2 * k, 8,
+                                 hipMemcpyHostToDevice, stream), "upload twiddle fix");
+    hip_check(hipStreamSynchronize(stream), "table fixes");
+    return std::uint32_t(fixes.size() + twidx.size());
 }
+
+
 
 }  // namespace
 
@@ -143,31 +200,19 @@ void OceanDevice::create(const OceanDesc& desc, hipStream_t shared_stream, bool 
     }
     d_level = reinterpret_cast<ocean_hip_level*>(base + o_lvl);
 
-    std::vector<float> staging;
-    float* h0_host;
-    float* omega_host;
     if (!lazy_host) {
         alloc_host_mirrors();
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_q_xz), std::size_t(kQueryStage) * 8, hipHostMallocDefault), "hipHostMalloc");
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_q_out), std::size_t(kQueryStage) * sizeof(ocean_hip_surface), hipHostMallocDefault), "hipHostMalloc");
         if (status != 0) throw std::bad_alloc();
-        // staged through the (not yet used) pinned mirrors
-        h0_host = h_disp;               // cells*8 bytes fit in cells*16
-        omega_host = h_normal;
-    } else {
-        staging.resize(cells * 3);
-        h0_host = staging.data();
-        omega_host = staging.data() + cells * 2;
     }
 
-    // Constant tables: built on the host with the reference's libm chain and
-    // uploaded once.
-    build_tables_threaded(desc, h0_host, omega_host);
-    std::vector<float> tw(std::max<std::size_t>(n, 2));
-    oceanamd::build_twiddles(n, tw.data());
-    hip_check(hipMemcpyAsync(base + o_h0, h0_host, cells * 8, hipMemcpyHostToDevice, stream), "upload h0");
-    hip_check(hipMemcpyAsync(base + o_omega, omega_host, cells * 4, hipMemcpyHostToDevice, stream), "upload omega");
-    hip_check(hipMemcpyAsync(base + o_tw, tw.data(), (n / 2) * 8, hipMemcpyHostToDevice, stream), "upload twiddles");
+    // Constant tables, built on the device; the spec scratch (unused until the
+    // first frame) holds the certification flags and the repaired cells.
+    init_repaired = build_tables_on_device(desc, reinterpret_cast<float*>(base + o_h0),
+                                           reinterpret_cast<float*>(base + o_omega),
+                                           reinterpret_cast<float*>(base + o_tw),
+                                           reinterpret_cast<unsigned char*>(base + o_spec), stream);
     hip_check(hipMemcpyAsync(d_level, &level, sizeof(level), hipMemcpyHostToDevice, stream), "upload level");
     hip_check(hipStreamSynchronize(stream), "upload");
 }

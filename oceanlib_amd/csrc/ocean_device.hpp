@@ -50,6 +50,7 @@ struct OceanDevice {
 
     bool mirror_stale = true;
     int status = 0;            // first HIP error seen (sticky)
+    std::uint32_t init_repaired = 0;  // table entries the host recomputed at creation
 
     OceanDevice() = default;
     OceanDevice(const OceanDevice&) = delete;

@@ -188,6 +188,56 @@ inline float wave_component(std::uint32_t i, std::uint32_t n, float patch_length
 
 }  // namespace
 
+SpectrumConstants spectrum_constants(const ocean::OceanDesc& desc) noexcept
+{
+    const ocean::SpectrumDesc& d = desc.spectrum;
+    SpectrumConstants c{};
+    c.gravity = d.gravity;
+    c.depth = static_cast<double>(d.depth);
+    c.omega_p = peak_omega(d);
+    c.alpha = alpha_of(d);
+    c.gamma = static_cast<double>(d.peak_enhancement);
+    c.wind = static_cast<double>(d.wind_speed);
+    c.swell = static_cast<double>(d.swell);
+    c.wind_direction = static_cast<double>(d.wind_direction);
+    c.cutoff_l = static_cast<double>(d.small_wave_cutoff) / kTwoPi;
+    // the per-cell expression of directional_spread, hoisted (same operations)
+    c.mu = -2.33 - 1.45 * ((c.wind * c.omega_p / c.gravity) - 1.17);
+    c.q_den = 2.0 * std::sqrt(kPi);
+    c.dk = kTwoPi / static_cast<double>(desc.patch_length);
+    return c;
+}
+
+void spectrum_cell(const ocean::OceanDesc& desc, std::uint32_t x, std::uint32_t y, float* h0,
+                   float* omega) noexcept
+{
+    const std::uint32_t n = desc.size;
+    const float L = desc.patch_length;
+    const double dk = kTwoPi / static_cast<double>(L);
+    Pcg32 rng = row_rng(desc.seed, y);
+    for (std::uint32_t k = 0; k < 2u * x; ++k) next_u32(rng);
+    const float kz_row = wave_component(y, n, L);
+    const float kx_val = wave_component(x, n, L);
+    const double k2 = static_cast<double>(kx_val) * kx_val + static_cast<double>(kz_row) * kz_row;
+    const double k = std::sqrt(k2);
+    *omega = static_cast<float>(
+        dispersion_omega(desc.spectrum.gravity, k, static_cast<double>(desc.spectrum.depth)));
+    const double psi = wave_density(desc.spectrum, kx_val, kz_row);
+    const double amp = 0.5 * std::sqrt(psi) * dk;
+    float ga, gb;
+    gaussian_pair(rng, ga, gb);
+    h0[0] = static_cast<float>(ga * amp);
+    h0[1] = static_cast<float>(gb * amp);
+}
+
+void twiddle_at(std::uint32_t n, std::uint32_t j, float* tw) noexcept
+{
+    constexpr double k2pi = 6.283185307179586476925286766559;
+    const double angle = k2pi * static_cast<double>(j) / static_cast<double>(n);
+    tw[0] = static_cast<float>(std::cos(angle));
+    tw[1] = static_cast<float>(std::sin(angle));
+}
+
 void build_spectrum_tables(const ocean::OceanDesc& desc, float* h0, float* omega,
                            std::uint32_t row_begin, std::uint32_t row_end) noexcept
 {

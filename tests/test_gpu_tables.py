@@ -1,0 +1,74 @@
+"""The constant tables built on the device equal the host build bit for bit.
+
+The device kernels (oceanlib_amd/csrc/spectrum_kernels.hip) restate the
+spectrum, PCG32 rows and Box-Muller draws (reference src/core/spectrum.cpp,
+src/core/rng.hpp, src/core/rng.cpp) with the device libm and flag every cell
+whose float results a libm difference could change; the host recomputes
+those. Compared here with ocean_amd_build_spectrum (the host chain, itself
+equal to the oracle and the reference: tests/test_abi.py) over the workload
+descriptors and 30 random ones covering depth, swell, wind direction, peak
+enhancement, cutoff, fetch and calm seas.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oceanlib_amd import capi
+from oceanlib_amd.workloads import (CONFIG1, CONFIG2, CONFIG3_LEVELS, SMALL_CASES, config4_levels,
+                                    desc_from)
+from oracle_ffi import assert_bitexact, fptr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return capi.load()
+
+
+def random_descs(count=30, seed=21):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        out.append(dict(size=int(rng.choice([16, 32, 64, 128, 256, 512, 1024])),
+                        patch_length=float(np.float32(10 ** rng.uniform(0.5, 3.3))),
+                        wind_speed=float(np.float32(0.0 if i == 0 else rng.uniform(0.5, 40.0))),
+                        fetch=float(np.float32(10 ** rng.uniform(3, 6))),
+                        wind_direction=float(np.float32(rng.uniform(-7, 7))),
+                        peak_enhancement=float(np.float32(rng.uniform(1.0, 7.0))),
+                        swell=float(np.float32(rng.uniform(0, 1) if i % 3 == 0 else 0.0)),
+                        small_wave_cutoff=float(np.float32(rng.uniform(0, 5))),
+                        water_depth=float(np.float32(0.0 if i % 2 else rng.uniform(0.5, 500))),
+                        seed=int(rng.integers(0, 2 ** 63)), choppiness=1.0))
+    return out
+
+
+CASES = ([("config1", CONFIG1), ("config2", CONFIG2)] +
+         [(f"config3_{k}", kw) for k, kw in enumerate(CONFIG3_LEVELS)] +
+         [(name, kw) for name, kw, _ in SMALL_CASES] +
+         [(f"config4_{k}", kw) for k, kw in enumerate(config4_levels(20)[0])] +
+         [(f"random_{k}", kw) for k, kw in enumerate(random_descs())])
+
+
+@pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
+def test_device_tables_equal_host_build(lib, name, kw):
+    d = capi.default_desc(lib, **kw)
+    n = d.size
+    want_h0 = np.zeros(2 * n * n, np.float32)
+    want_om = np.zeros(n * n, np.float32)
+    assert lib.ocean_amd_build_spectrum(C.byref(d), fptr(want_h0), fptr(want_om)) == 0
+    st = C.c_int(-1)
+    sim = lib.ocean_create(C.byref(d), C.byref(st))
+    assert sim, st.value
+    try:
+        h0 = np.zeros(2 * n * n, np.float32)
+        om = np.zeros(n * n, np.float32)
+        rep = C.c_uint32()
+        assert lib.ocean_amd_get_tables(sim, fptr(h0), fptr(om), C.byref(rep)) == 0
+        print(f"{name}: N={n} repaired {rep.value} of {n * n} cells")
+        assert_bitexact(h0, want_h0, f"{name} h0")
+        assert_bitexact(om, want_om, f"{name} omega")
+        assert rep.value <= max(16, n * n // 100)
+    finally:
+        lib.ocean_destroy(sim)

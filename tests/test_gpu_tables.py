@@ -69,6 +69,9 @@ def test_device_tables_equal_host_build(lib, name, kw):
         print(f"{name}: N={n} repaired {rep.value} of {n * n} cells")
         assert_bitexact(h0, want_h0, f"{name} h0")
         assert_bitexact(om, want_om, f"{name} omega")
-        assert rep.value <= max(16, n * n // 100)
+        # cells on the axis opposite the wind (theta = pi exactly: the spreading
+        # cosine is 6e-17, a branch no margin can certify) and the rare values
+        # next to a float rounding midpoint: about one row's worth
+        assert rep.value <= n + n * n // 200
     finally:
         lib.ocean_destroy(sim)

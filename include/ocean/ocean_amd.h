@@ -57,11 +57,13 @@ int ocean_amd_profile_frame(ocean_sim* sim, double t0, double dt, uint32_t iters
  * src/ocean.cpp:117-161). The two frame kernels of one ocean can run with
  * several launch shapes (rows / columns per workgroup); every shape computes
  * the same bits. _frame_shapes: how many row and column variants exist for
- * grid size n (variant 0 = default). _set_frame_shape selects one;
+ * this ocean's size and field count (variant 0 = default; the last ones
+ * spread the fields over several teams per row / column). _set_frame_shape
+ * selects one;
  * _autotune_frame times every variant of each pass on the ocean's own stream
  * (reps launches each, HIP events), keeps the fastest, leaves the ocean at
  * its current time and returns the choice and the times. */
-int ocean_amd_frame_shapes(uint32_t n, uint32_t* row_variants, uint32_t* col_variants);
+int ocean_amd_frame_shapes(const ocean_sim* sim, uint32_t* row_variants, uint32_t* col_variants);
 int ocean_amd_set_frame_shape(ocean_sim* sim, uint32_t row_variant, uint32_t col_variant);
 int ocean_amd_autotune_frame(ocean_sim* sim, uint32_t reps, uint32_t* row_variant,
                              uint32_t* col_variant, float* row_ms, float* col_ms);

@@ -271,10 +271,13 @@ int ocean_amd_get_tables(const ocean_sim* sim, float* h0, float* omega, uint32_t
     return dev.status == 0 ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
 }
 
-int ocean_amd_frame_shapes(uint32_t n, uint32_t* row_variants, uint32_t* col_variants)
+int ocean_amd_frame_shapes(const ocean_sim* sim, uint32_t* row_variants, uint32_t* col_variants)
 {
+    if (sim == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    const auto& dev = to_cpp(sim)->device();
     int r = 0, c = 0;
-    if (ocean_hip_frame_shapes(n, &r, &c) != 0) return OCEAN_ERROR_INVALID_ARG;
+    if (ocean_hip_frame_shapes(dev.level.n, uint32_t(dev.fields), &r, &c) != 0)
+        return OCEAN_ERROR_INVALID_ARG;
     if (row_variants) *row_variants = static_cast<uint32_t>(r);
     if (col_variants) *col_variants = static_cast<uint32_t>(c);
     return OCEAN_OK;
@@ -286,7 +289,8 @@ int ocean_amd_set_frame_shape(ocean_sim* sim, uint32_t row_variant, uint32_t col
     ocean::Ocean& o = *to_cpp(sim);
     auto& dev = o.device();
     int r = 0, c = 0;
-    if (ocean_hip_frame_shapes(dev.level.n, &r, &c) != 0 || row_variant >= uint32_t(r) ||
+    if (ocean_hip_frame_shapes(dev.level.n, uint32_t(dev.fields), &r, &c) != 0 ||
+        row_variant >= uint32_t(r) ||
         col_variant >= uint32_t(c))
         return OCEAN_ERROR_INVALID_ARG;
     dev.make_current();
@@ -303,7 +307,8 @@ int ocean_amd_autotune_frame(ocean_sim* sim, uint32_t reps, uint32_t* row_varian
     ocean::Ocean& o = *to_cpp(sim);
     auto& dev = o.device();
     int nr = 0, nc = 0;
-    if (ocean_hip_frame_shapes(dev.level.n, &nr, &nc) != 0) return OCEAN_ERROR_INVALID_ARG;
+    if (ocean_hip_frame_shapes(dev.level.n, uint32_t(dev.fields), &nr, &nc) != 0)
+        return OCEAN_ERROR_INVALID_ARG;
     dev.make_current();
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return OCEAN_ERROR_UNKNOWN;

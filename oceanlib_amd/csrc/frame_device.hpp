@@ -199,4 +199,203 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
     }
 }
 
+// ---- field-split blocks ------------------------------------------------------
+// The same passes with the F fields spread over G = F / 2 teams per row (or
+// column): team group g transforms fields 2g and 2g + 1, so a frame has G
+// times the wavefronts, each with a third (F = 6) or half (F = 4) of the
+// serial FFT work. Every field is computed by the same expressions as above.
+// The column finalize needs fields 0-3 of a cell together: group 1 hands its
+// two fields to group 0 through LDS (`xfer`); group 2 (velocity) writes its
+// output itself. Requires T = N/8 >= 64, so a team is whole wavefronts and the
+// group branches are uniform.
+
+template <int N, int F, int TEAMS>
+__device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32_t blk,
+                                                double time_over_two_pi, float2* __restrict__ tw,
+                                                float2* __restrict__ xch)
+{
+    constexpr int T = FftShape<N>::T;
+    constexpr int G = F / 2;
+    constexpr int BLOCK = TEAMS * G * T;
+    constexpr uint32_t CELLS = uint32_t(N) * N;
+    static_assert(T >= 64 && F % 2 == 0, "field split needs whole-wavefront teams");
+    const float2* __restrict__ h0 = reinterpret_cast<const float2*>(L.h0);
+    const float* __restrict__ omega = L.omega;
+    const float2* __restrict__ twg = reinterpret_cast<const float2*>(L.twiddles);
+    float2* __restrict__ spec = reinterpret_cast<float2*>(L.spec);
+    const float patch_length = L.patch_length;
+
+    __syncthreads();
+    for (int i = threadIdx.x; i < N / 2; i += BLOCK) tw[i] = twg[i];
+    __syncthreads();
+
+    const uint32_t tt = threadIdx.x / T;
+    const uint32_t g = tt % G;
+    const uint32_t rteam = tt / G;
+    const uint32_t J = threadIdx.x % T;
+    const uint32_t z = row_of_team<N>(blk * TEAMS + rteam);
+    const uint32_t mz = (uint32_t(N) - z) & uint32_t(N - 1);
+
+    float fr[2][8], fi[2][8];
+    const float kz = wave_component(z, N, patch_length);
+
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const uint32_t x = J + uint32_t(a) * T;
+        const uint32_t mx = (uint32_t(N) - x) & uint32_t(N - 1);
+        const uint32_t i = z * N + x;
+        const float w = omega[i];
+        const float2 hv = h0[i];
+        const float2 hm = h0[mz * N + mx];
+
+        // reference src/core/evolve.cpp:45-121 (h0c = conj(h0(-k)), spectrum.cpp:329-338)
+        const float phase = reduced_phase(w, time_over_two_pi);
+        float s, c;
+        sincos_poly(phase, s, c);
+        const float a_re = hv.x, a_im = hv.y;
+        const float b_re = hm.x, b_im = -hm.y;
+        const float fwd_re = a_re * c - a_im * s;
+        const float fwd_im = a_re * s + a_im * c;
+        const float bwd_re = b_re * c + b_im * s;
+        const float bwd_im = b_im * c - b_re * s;
+        const float hr = fwd_re + bwd_re;
+        const float hi = fwd_im + bwd_im;
+
+        const float kx = wave_component(x, N, patch_length);
+        const float ki = inv_wavenumber(kx, kz);
+        const float sx = kx * ki;
+        const float sz = kz * ki;
+        if (g == 0u) {
+            const float dx_re = sx * hi, dx_im = -sx * hr;
+            const float dz_re = sz * hi, dz_im = -sz * hr;
+            const float hx_re = -kx * hi, hx_im = kx * hr;
+            fr[0][a] = hr - dx_im;      fi[0][a] = hi + dx_re;
+            fr[1][a] = dz_re - hx_im;   fi[1][a] = dz_im + hx_re;
+        } else if (g == 1u) {
+            const float hz_re = -kz * hi, hz_im = kz * hr;
+            const float jxx = kx * sx, jzz = kz * sz, jxz = kx * sz;
+            const float jxx_re = jxx * hr, jxx_im = jxx * hi;
+            const float jzz_re = jzz * hr, jzz_im = jzz * hi;
+            const float jxz_re = jxz * hr, jxz_im = jxz * hi;
+            fr[0][a] = hz_re - jxx_im;  fi[0][a] = hz_im + jxx_re;
+            fr[1][a] = jzz_re - jxz_im; fi[1][a] = jzz_im + jxz_re;
+        } else {
+            // reference src/core/evolve.cpp:150-182
+            const float dr = fwd_re - bwd_re;
+            const float di = fwd_im - bwd_im;
+            const float wsx = w * sx, wsz = w * sz;
+            const float ux_re = -wsx * dr, ux_im = -wsx * di;
+            const float uz_re = -wsz * dr, uz_im = -wsz * di;
+            const float uy_re = -w * di, uy_im = w * dr;
+            fr[0][a] = ux_re - uy_im; fi[0][a] = ux_im + uy_re;
+            fr[1][a] = uz_re;         fi[1][a] = uz_im;
+        }
+    }
+
+    float2* x_team = xch + tt * team_stride<N>();
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+        uint32_t idx[8];
+        fft_team<N>(fr[f], fi[f], idx, J, x_team, tw, BlockSync{});
+        float2* dst = spec + size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
+#pragma unroll
+        for (int a = 0; a < 8; ++a) dst[idx[a]] = make_float2(fr[f][a], fi[f][a]);
+    }
+}
+
+template <int N, int F, int C>
+__device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32_t blk,
+                                                float2* __restrict__ tw, float2* __restrict__ xch,
+                                                float2* __restrict__ xfer)
+{
+    constexpr int T = FftShape<N>::T;
+    constexpr int G = F / 2;
+    constexpr int BLOCK = C * T * G;
+    constexpr uint32_t CELLS = uint32_t(N) * N;
+    static_assert(T >= 64 && F % 2 == 0, "field split needs whole-wavefront teams");
+    const float2* __restrict__ spec = reinterpret_cast<const float2*>(L.spec);
+    const float2* __restrict__ twg = reinterpret_cast<const float2*>(L.twiddles);
+    float4* __restrict__ disp = reinterpret_cast<float4*>(L.disp);
+    float4* __restrict__ normal = reinterpret_cast<float4*>(L.normal);
+    float4* __restrict__ vel = reinterpret_cast<float4*>(L.vel);
+    const float lambda = L.choppiness;
+    const float foam_threshold = L.foam_threshold;
+    const float inv_threshold = L.inv_foam_threshold;
+
+    __syncthreads();
+    for (int i = threadIdx.x; i < N / 2; i += BLOCK) tw[i] = twg[i];
+    __syncthreads();
+
+    const uint32_t g = threadIdx.x / uint32_t(C * T);
+    const uint32_t l = threadIdx.x % uint32_t(C * T);
+    const uint32_t c = l % C;
+    const uint32_t J = l / C;
+    const uint32_t x = blk * C + c;
+
+    float fr[2][8], fi[2][8];
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const float2 v = spec[size_t(2u * g + uint32_t(f)) * CELLS + size_t(J + a * T) * N + x];
+            fr[f][a] = v.x;
+            fi[f][a] = v.y;
+        }
+    }
+
+    uint32_t idx[8];
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+        fft_team<N>(fr[f], fi[f], idx, J, xch + (g * C + c) * team_stride<N>(), tw, BlockSync{});
+
+    if (g == 1u) {  // fields 2 and 3 to group 0
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            xfer[(c * 2u + 0u) * N + idx[a]] = make_float2(fr[0][a], fi[0][a]);
+            xfer[(c * 2u + 1u) * N + idx[a]] = make_float2(fr[1][a], fi[1][a]);
+        }
+    }
+    if constexpr (F == 6) {
+        if (g == 2u) {
+            // reference src/core/evolve.cpp:186-203
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+                vel[size_t(idx[a]) * N + x] = make_float4(fr[0][a], fi[0][a], fr[1][a], 0.0f);
+        }
+    }
+    __syncthreads();
+    if (g != 0u) return;
+
+    // reference src/core/evolve.cpp:256-325
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const float2 v2 = xfer[(c * 2u + 0u) * N + idx[a]];
+        const float2 v3 = xfer[(c * 2u + 1u) * N + idx[a]];
+        const float h = fr[0][a], d_x = fi[0][a];
+        const float d_z = fr[1][a], hx = fi[1][a];
+        const float hz = v2.x, jxx = v2.y;
+        const float jzz = v3.x, jxz = v3.y;
+        const float disp_x = lambda * d_x;
+        const float disp_z = lambda * d_z;
+        const float axx = 1.0f + lambda * jxx;
+        const float azz = 1.0f + lambda * jzz;
+        const float axz = lambda * jxz;
+        const float jac = axx * azz - axz * axz;
+        float nx = -(hx * azz - axz * hz);
+        float ny = jac;
+        float nz = -(axx * hz - hx * axz);
+        const float len2 = nx * nx + ny * ny + nz * nz;
+        const float inv_len = (len2 > 0.0f) ? 1.0f / sqrtf(len2) : 0.0f;
+        nx *= inv_len;
+        ny *= inv_len;
+        nz *= inv_len;
+        float foam = (foam_threshold - jac) * inv_threshold;
+        if (foam < 0.0f) foam = 0.0f;
+        if (foam > 1.0f) foam = 1.0f;
+        const size_t o = size_t(idx[a]) * N + x;
+        disp[o] = make_float4(disp_x, h, disp_z, foam);
+        normal[o] = make_float4(nx, ny, nz, jac);
+    }
+}
+
 }  // namespace oceanamd

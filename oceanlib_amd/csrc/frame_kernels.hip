@@ -71,6 +71,24 @@ __global__ __launch_bounds__(C * FftShape<N>::T) void col_pass_kernel(ocean_hip_
     col_block<N, F, C>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch);
 }
 
+template <int N, int F, int TEAMS>
+__global__ __launch_bounds__(TEAMS * (F / 2) * FftShape<N>::T) void row_split_kernel(
+    ocean_hip_level L, double time_over_two_pi)
+{
+    __shared__ float2 tw[N / 2];
+    __shared__ float2 xch[TEAMS * (F / 2) * team_stride<N>()];
+    row_block_split<N, F, TEAMS>(L, xcd_group(blockIdx.x, gridDim.x), time_over_two_pi, tw, xch);
+}
+
+template <int N, int F, int C>
+__global__ __launch_bounds__(C * (F / 2) * FftShape<N>::T) void col_split_kernel(ocean_hip_level L)
+{
+    __shared__ float2 tw[N / 2];
+    __shared__ float2 xch[C * (F / 2) * team_stride<N>()];
+    __shared__ float2 xfer[C * 2 * N];
+    col_block_split<N, F, C>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch, xfer);
+}
+
 // Launch shapes of the per-level passes. Variant 0 is the default shape;
 // the others (rows per block TEAMS, columns per block C, each 1..32) are what
 // ocean_hip_frame_shapes lists for the autotuner. Every shape runs the same
@@ -90,9 +108,54 @@ constexpr bool col_ok()
            C * team_stride<N>() * 8 + N * 4 <= 65536;
 }
 
+// Field-split shapes (row_split_kernel / col_split_kernel): whole-wavefront
+// teams, at most 512 lanes, static LDS within 64 KB.
+template <int N, int F, int TEAMS>
+constexpr bool row_split_ok()
+{
+    constexpr int lanes = TEAMS * (F / 2) * FftShape<N>::T;
+    return FftShape<N>::T >= 64 && TEAMS <= N && lanes <= 512 &&
+           (N / 2 + TEAMS * (F / 2) * team_stride<N>()) * 8 <= 65536;
+}
+template <int N, int F, int C>
+constexpr bool col_split_ok()
+{
+    constexpr int lanes = C * (F / 2) * FftShape<N>::T;
+    return FftShape<N>::T >= 64 && C <= N && lanes <= 512 &&
+           (N / 2 + C * (F / 2) * team_stride<N>() + C * 2 * N) * 8 <= 65536;
+}
+
 struct Pick {
     int want, seen;
 };
+
+template <int N, int F, int TEAMS>
+bool try_row_split(Pick& p, const ocean_hip_level& L, double t2, hipStream_t s)
+{
+    if constexpr (!row_split_ok<N, F, TEAMS>()) {
+        return false;
+    } else {
+        if (p.seen++ != p.want) return false;
+        if (s != nullptr || L.n != 0)
+            hipLaunchKernelGGL((row_split_kernel<N, F, TEAMS>), dim3(N / TEAMS),
+                               dim3(TEAMS * (F / 2) * FftShape<N>::T), 0, s, L, t2);
+        return true;
+    }
+}
+
+template <int N, int F, int C>
+bool try_col_split(Pick& p, const ocean_hip_level& L, hipStream_t s)
+{
+    if constexpr (!col_split_ok<N, F, C>()) {
+        return false;
+    } else {
+        if (p.seen++ != p.want) return false;
+        if (s != nullptr || L.n != 0)
+            hipLaunchKernelGGL((col_split_kernel<N, F, C>), dim3(N / C),
+                               dim3(C * (F / 2) * FftShape<N>::T), 0, s, L);
+        return true;
+    }
+}
 
 template <int N, int F, int TEAMS>
 bool try_row(Pick& p, const ocean_hip_level& L, double t2, hipStream_t s, bool is_default)
@@ -131,7 +194,8 @@ bool row_variant(int v, const ocean_hip_level& L, double t2, hipStream_t s)
     return try_row<N, F, RowShape<N>::TEAMS>(p, L, t2, s, true) || try_row<N, F, 1>(p, L, t2, s, false) ||
            try_row<N, F, 2>(p, L, t2, s, false) || try_row<N, F, 4>(p, L, t2, s, false) ||
            try_row<N, F, 8>(p, L, t2, s, false) || try_row<N, F, 16>(p, L, t2, s, false) ||
-           try_row<N, F, 32>(p, L, t2, s, false);
+           try_row<N, F, 32>(p, L, t2, s, false) || try_row_split<N, F, 1>(p, L, t2, s) ||
+           try_row_split<N, F, 2>(p, L, t2, s) || try_row_split<N, F, 4>(p, L, t2, s);
 }
 
 template <int N, int F>
@@ -141,19 +205,29 @@ bool col_variant(int v, const ocean_hip_level& L, hipStream_t s)
     return try_col<N, F, ColShape<N>::C>(p, L, s, true) || try_col<N, F, 1>(p, L, s, false) ||
            try_col<N, F, 2>(p, L, s, false) || try_col<N, F, 4>(p, L, s, false) ||
            try_col<N, F, 8>(p, L, s, false) || try_col<N, F, 16>(p, L, s, false) ||
-           try_col<N, F, 32>(p, L, s, false);
+           try_col<N, F, 32>(p, L, s, false) || try_col_split<N, F, 1>(p, L, s) ||
+           try_col_split<N, F, 2>(p, L, s) || try_col_split<N, F, 4>(p, L, s);
 }
 
 // Number of variants of each pass (counted without launching: a null level
 // view with n = 0 and no stream).
-template <int N>
-void count_variants(int& rows, int& cols)
+template <int N, int F>
+void count_variants_f(int& rows, int& cols)
 {
     ocean_hip_level none{};
     rows = 0;
-    while (row_variant<N, 4>(rows, none, 0.0, nullptr)) ++rows;
+    while (row_variant<N, F>(rows, none, 0.0, nullptr)) ++rows;
     cols = 0;
-    while (col_variant<N, 4>(cols, none, nullptr)) ++cols;
+    while (col_variant<N, F>(cols, none, nullptr)) ++cols;
+}
+
+template <int N>
+void count_variants(int fields, int& rows, int& cols)
+{
+    if (fields == 6)
+        count_variants_f<N, 6>(rows, cols);
+    else
+        count_variants_f<N, 4>(rows, cols);
 }
 
 template <int N>
@@ -280,19 +354,21 @@ extern "C" hipError_t ocean_hip_frame(const ocean_hip_level* L, double time_over
     return ocean_hip_frame_passes(L, time_over_two_pi, s, 3u);
 }
 
-extern "C" int ocean_hip_frame_shapes(uint32_t n, int* row_variants, int* col_variants)
+extern "C" int ocean_hip_frame_shapes(uint32_t n, uint32_t fields, int* row_variants,
+                                      int* col_variants)
 {
     int r = 0, c = 0;
+    const int f = int(fields);
     switch (n) {
-        case 16: count_variants<16>(r, c); break;
-        case 32: count_variants<32>(r, c); break;
-        case 64: count_variants<64>(r, c); break;
-        case 128: count_variants<128>(r, c); break;
-        case 256: count_variants<256>(r, c); break;
-        case 512: count_variants<512>(r, c); break;
-        case 1024: count_variants<1024>(r, c); break;
-        case 2048: count_variants<2048>(r, c); break;
-        case 4096: count_variants<4096>(r, c); break;
+        case 16: count_variants<16>(f, r, c); break;
+        case 32: count_variants<32>(f, r, c); break;
+        case 64: count_variants<64>(f, r, c); break;
+        case 128: count_variants<128>(f, r, c); break;
+        case 256: count_variants<256>(f, r, c); break;
+        case 512: count_variants<512>(f, r, c); break;
+        case 1024: count_variants<1024>(f, r, c); break;
+        case 2048: count_variants<2048>(f, r, c); break;
+        case 4096: count_variants<4096>(f, r, c); break;
         default: return -1;
     }
     if (row_variants) *row_variants = r;

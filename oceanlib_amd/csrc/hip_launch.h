@@ -67,10 +67,10 @@ hipError_t ocean_hip_frame(const ocean_hip_level* level, double time_over_two_pi
 hipError_t ocean_hip_frame_passes(const ocean_hip_level* level, double time_over_two_pi,
                                   hipStream_t s, unsigned passes);
 
-/* Launch-shape variants of the per-level passes for grid size n (variant 0
- * is the default): how many row and column variants exist; -1 if n is not
- * a supported size. */
-int ocean_hip_frame_shapes(uint32_t n, int* row_variants, int* col_variants);
+/* Launch-shape variants of the per-level passes for grid size n and F fields
+ * (4, or 6 with velocity; variant 0 is the default): how many row and column
+ * variants exist; -1 if n is not a supported size. */
+int ocean_hip_frame_shapes(uint32_t n, uint32_t fields, int* row_variants, int* col_variants);
 
 /* Batched sample_at over `count` (x, z) pairs (device float2). `out` and/or
  * `heights` may be NULL. */

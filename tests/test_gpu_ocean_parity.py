@@ -177,17 +177,17 @@ def test_queries_bitexact(lib):
         g.close()
 
 
-@pytest.mark.parametrize("n", [32, 64, 256, 512, 1024, 2048])
+@pytest.mark.parametrize("n", [32, 64, 256, 512, 1024, 2048, 4096])
 def test_every_launch_shape_is_bitexact(lib, n):
     """Each launch-shape variant of both passes (rows / columns per
     workgroup) gives the oracle's bits; the autotuner's pick does too."""
-    rows, cols = C.c_uint32(), C.c_uint32()
-    assert lib.ocean_amd_frame_shapes(n, C.byref(rows), C.byref(cols)) == 0
-    assert rows.value >= 1 and cols.value >= 1
-    vel = 1 if n in (256, 1024) else 0
+    vel = 1 if n in (256, 512, 1024) else 0
     d = capi.default_desc(lib, size=n, patch_length=float(n) * 0.7, wind_speed=17.0, seed=n + 3,
                           choppiness=1.2, compute_velocity=vel, water_depth=30.0 if vel else 0.0)
     g, o = GpuOcean(lib, d), OracleOcean(d)
+    rows, cols = C.c_uint32(), C.c_uint32()
+    assert lib.ocean_amd_frame_shapes(g.h, C.byref(rows), C.byref(cols)) == 0
+    assert rows.value >= 1 and cols.value >= 1
     try:
         t = 4.25
         o.update(t)

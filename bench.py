@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "updates/s at N=512 (1 and batched) vs CPU; height_at queries/s @0.01mm"
 HBM_PEAK_GBS = 8000.0
+TIMING_EVERY = 16
 QUERIES_PER_FRAME = 4096
 
 
@@ -315,6 +316,9 @@ def main():
                     help="skip the converged / learned height query leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched leg: keep the default grid sizes")
+    ap.add_argument("--frame-shape", default=None, metavar="ROW,COL",
+                    help="fix the frame kernels' launch shapes instead of autotuning "
+                         "(counter runs, where timing is perturbed)")
     ap.add_argument("--profile-only", action="store_true",
                     help="run only warmup + timed frames (for rocprofv3 runs)")
     args = ap.parse_args()
@@ -348,11 +352,17 @@ def main():
     # autotuner; every shape gives the same bits).
     shape_r, shape_c = C.c_uint32(), C.c_uint32()
     tune_r, tune_c = C.c_float(), C.c_float()
-    if lib.ocean_amd_autotune_frame(sim, 20, C.byref(shape_r), C.byref(shape_c), C.byref(tune_r),
-                                    C.byref(tune_c)) != 0:
-        raise SystemExit("frame autotune failed")
-    log(f"bench: frame shapes row {shape_r.value} col {shape_c.value} "
-        f"({tune_r.value * 1e3:.1f} / {tune_c.value * 1e3:.1f} us)")
+    if args.frame_shape:
+        shape_r.value, shape_c.value = (int(v) for v in args.frame_shape.split(","))
+        if lib.ocean_amd_set_frame_shape(sim, shape_r.value, shape_c.value) != 0:
+            raise SystemExit(f"bad --frame-shape {args.frame_shape}")
+        log(f"bench: frame shapes row {shape_r.value} col {shape_c.value} (fixed)")
+    else:
+        if lib.ocean_amd_autotune_frame(sim, 20, C.byref(shape_r), C.byref(shape_c),
+                                        C.byref(tune_r), C.byref(tune_c)) != 0:
+            raise SystemExit("frame autotune failed")
+        log(f"bench: frame shapes row {shape_r.value} col {shape_c.value} "
+            f"({tune_r.value * 1e3:.1f} / {tune_c.value * 1e3:.1f} us)")
 
     total = args.warmup + args.steps
     frames_xz = 64
@@ -377,12 +387,22 @@ def main():
     lib.ocean_amd_synchronize(sim)
     torch.cuda.synchronize()
     log(f"bench: timing {args.steps} frames")
+    # The two frame kernels of every 16th frame of the timed loop record HIP
+    # start/end events in their own dispatches, on the ocean's stream: the
+    # kernels' durations inside this loop.
+    if lib.ocean_amd_frame_timing(sim, TIMING_EVERY) != 0:
+        raise SystemExit("frame timing failed")
     t0 = time.perf_counter()
     for k in range(args.warmup, total):
         frame(k)
     lib.ocean_amd_synchronize(sim)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    row_ms, col_ms, samples = C.c_float(), C.c_float(), C.c_uint32()
+    if lib.ocean_amd_frame_timing_read(sim, C.byref(row_ms), C.byref(col_ms),
+                                       C.byref(samples)) != 0:
+        raise SystemExit("frame timing read failed")
+    lib.ocean_amd_frame_timing(sim, 0)
     if lib.ocean_amd_last_error(sim) != 0:
         raise SystemExit(f"device error {lib.ocean_amd_last_error(sim)}")
     ms_per_step = el * 1000.0 / args.steps
@@ -401,19 +421,22 @@ def main():
         print(json.dumps(rec), flush=True)
         return
 
-    # Per-kernel device time (HIP events on the ocean's stream).
-    row_ms, col_ms = C.c_float(), C.c_float()
-    assert lib.ocean_amd_profile_frame(sim, 100.0, 1.0 / 60.0, 200, C.byref(row_ms),
-                                       C.byref(col_ms)) == 0
+    # The same kernels launched back to back with nothing between them.
+    iso_r, iso_c = C.c_float(), C.c_float()
+    assert lib.ocean_amd_profile_frame(sim, 100.0, 1.0 / 60.0, 200, C.byref(iso_r),
+                                       C.byref(iso_c)) == 0
     fields = 6
     row_b, col_b = algorithmic_bytes(n, fields, True)
     kernels = {
         "row_pass": {"ms": row_ms.value, "alg_bytes": row_b,
-                     "gbs": row_b / (row_ms.value * 1e-3) / 1e9},
+                     "gbs": row_b / (row_ms.value * 1e-3) / 1e9,
+                     "ms_back_to_back": iso_r.value},
         "col_pass": {"ms": col_ms.value, "alg_bytes": col_b,
-                     "gbs": col_b / (col_ms.value * 1e-3) / 1e9},
+                     "gbs": col_b / (col_ms.value * 1e-3) / 1e9,
+                     "ms_back_to_back": iso_c.value},
+        "timed_frames_sampled": samples.value,
     }
-    dom = max(kernels, key=lambda k: kernels[k]["ms"])
+    dom = max(("row_pass", "col_pass"), key=lambda k: kernels[k]["ms"])
     achieved = kernels[dom]["gbs"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),

@@ -52,6 +52,16 @@ int  ocean_amd_last_error(const ocean_sim* sim);
 int ocean_amd_profile_frame(ocean_sim* sim, double t0, double dt, uint32_t iters,
                             float* row_ms, float* col_ms);
 
+/* Timing of the frame kernels inside the caller's own update loop:
+ * _frame_timing(sim, k): from now on the row and column dispatches of every
+ * k-th ocean_update record HIP start/end events themselves
+ * (hipExtLaunchKernel, no marker packets between kernels; k = 0 stops); _frame_timing_read
+ * waits for the ocean's stream and returns the mean ms of each pass over the
+ * recorded frames (the last 64 at most), then clears them. Diagnostic; no
+ * reference counterpart. */
+int ocean_amd_frame_timing(ocean_sim* sim, uint32_t every);
+int ocean_amd_frame_timing_read(ocean_sim* sim, float* row_ms, float* col_ms, uint32_t* samples);
+
 /* ---- launch shapes of the frame kernels ---------------------------------
  * No reference counterpart (the reference's work split is its task count,
  * src/ocean.cpp:117-161). The two frame kernels of one ocean can run with

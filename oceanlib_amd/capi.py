@@ -190,6 +190,9 @@ AMD_SIGNATURES = {
     "ocean_amd_get_tables": (C.c_int, [_P, _F, _F, C.POINTER(C.c_uint32)]),
     "ocean_amd_frame_shapes": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "ocean_amd_set_frame_shape": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
+    "ocean_amd_frame_timing": (C.c_int, [_P, C.c_uint32]),
+    "ocean_amd_frame_timing_read": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                              C.POINTER(C.c_uint32)]),
     "ocean_amd_autotune_frame": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint32),
                                            C.POINTER(C.c_uint32), _F, _F]),
     "ocean_amd_sample_batch": (C.c_int, [_P, _F, C.c_uint32, C.POINTER(OceanSurfaceEx)]),

@@ -256,6 +256,20 @@ int ocean_amd_profile_frame(ocean_sim* sim, double t0, double dt, uint32_t iters
     return dev.status == 0 ? rc : OCEAN_ERROR_UNKNOWN;
 }
 
+int ocean_amd_frame_timing(ocean_sim* sim, uint32_t every)
+{
+    if (sim == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    return to_cpp(sim)->device().set_frame_timing(every) == 0 ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
+}
+
+int ocean_amd_frame_timing_read(ocean_sim* sim, float* row_ms, float* col_ms, uint32_t* samples)
+{
+    if (sim == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    auto& dev = to_cpp(sim)->device();
+    if (dev.read_frame_timing(row_ms, col_ms, samples) != 0) return OCEAN_ERROR_UNKNOWN;
+    return dev.status == 0 ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
+}
+
 int ocean_amd_get_tables(const ocean_sim* sim, float* h0, float* omega, uint32_t* repaired)
 {
     if (sim == nullptr) return OCEAN_ERROR_INVALID_ARG;

@@ -16,6 +16,7 @@
 #include "hip_launch.h"
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <vector>
 
@@ -129,84 +130,100 @@ struct Pick {
     int want, seen;
 };
 
+// Where a pass is enqueued: the stream, and optionally a pair of events that
+// the dispatch itself records at its start and end (hipExtLaunchKernel), so
+// the kernel is timed without marker packets between kernels.
+struct Launch {
+    hipStream_t s = nullptr;
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
+template <typename Kernel, typename... Args>
+void launch_on(const Launch& k, Kernel kernel, dim3 grid, dim3 block, Args... args)
+{
+    if (k.start != nullptr)
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, k.s, k.start, k.stop, 0u, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, 0, k.s, args...);
+}
+
 template <int N, int F, int TEAMS>
-bool try_row_split(Pick& p, const ocean_hip_level& L, double t2, hipStream_t s)
+bool try_row_split(Pick& p, const ocean_hip_level& L, double t2, const Launch& k)
 {
     if constexpr (!row_split_ok<N, F, TEAMS>()) {
         return false;
     } else {
         if (p.seen++ != p.want) return false;
-        if (s != nullptr || L.n != 0)
-            hipLaunchKernelGGL((row_split_kernel<N, F, TEAMS>), dim3(N / TEAMS),
-                               dim3(TEAMS * (F / 2) * FftShape<N>::T), 0, s, L, t2);
+        if (k.s != nullptr || L.n != 0)
+            launch_on(k, row_split_kernel<N, F, TEAMS>, dim3(N / TEAMS),
+                      dim3(TEAMS * (F / 2) * FftShape<N>::T), L, t2);
         return true;
     }
 }
 
 template <int N, int F, int C>
-bool try_col_split(Pick& p, const ocean_hip_level& L, hipStream_t s)
+bool try_col_split(Pick& p, const ocean_hip_level& L, const Launch& k)
 {
     if constexpr (!col_split_ok<N, F, C>()) {
         return false;
     } else {
         if (p.seen++ != p.want) return false;
-        if (s != nullptr || L.n != 0)
-            hipLaunchKernelGGL((col_split_kernel<N, F, C>), dim3(N / C),
-                               dim3(C * (F / 2) * FftShape<N>::T), 0, s, L);
+        if (k.s != nullptr || L.n != 0)
+            launch_on(k, col_split_kernel<N, F, C>, dim3(N / C), dim3(C * (F / 2) * FftShape<N>::T),
+                      L);
         return true;
     }
 }
 
 template <int N, int F, int TEAMS>
-bool try_row(Pick& p, const ocean_hip_level& L, double t2, hipStream_t s, bool is_default)
+bool try_row(Pick& p, const ocean_hip_level& L, double t2, const Launch& k, bool is_default)
 {
     if constexpr (!row_ok<N, TEAMS>() && TEAMS != RowShape<N>::TEAMS) {
         return false;
     } else {
         if (!is_default && TEAMS == RowShape<N>::TEAMS) return false;  // listed first already
         if (p.seen++ != p.want) return false;
-        if (s != nullptr || L.n != 0)
-            hipLaunchKernelGGL((row_pass_kernel<N, F, TEAMS>), dim3(N / TEAMS),
-                               dim3(TEAMS * FftShape<N>::T), 0, s, L, t2);
+        if (k.s != nullptr || L.n != 0)
+            launch_on(k, row_pass_kernel<N, F, TEAMS>, dim3(N / TEAMS), dim3(TEAMS * FftShape<N>::T),
+                      L, t2);
         return true;
     }
 }
 
 template <int N, int F, int C>
-bool try_col(Pick& p, const ocean_hip_level& L, hipStream_t s, bool is_default)
+bool try_col(Pick& p, const ocean_hip_level& L, const Launch& k, bool is_default)
 {
     if constexpr (!col_ok<N, C>() && C != ColShape<N>::C) {
         return false;
     } else {
         if (!is_default && C == ColShape<N>::C) return false;
         if (p.seen++ != p.want) return false;
-        if (s != nullptr || L.n != 0)
-            hipLaunchKernelGGL((col_pass_kernel<N, F, C>), dim3(N / C), dim3(C * FftShape<N>::T), 0,
-                               s, L);
+        if (k.s != nullptr || L.n != 0)
+            launch_on(k, col_pass_kernel<N, F, C>, dim3(N / C), dim3(C * FftShape<N>::T), L);
         return true;
     }
 }
 
 template <int N, int F>
-bool row_variant(int v, const ocean_hip_level& L, double t2, hipStream_t s)
+bool row_variant(int v, const ocean_hip_level& L, double t2, const Launch& k)
 {
     Pick p{v, 0};
-    return try_row<N, F, RowShape<N>::TEAMS>(p, L, t2, s, true) || try_row<N, F, 1>(p, L, t2, s, false) ||
-           try_row<N, F, 2>(p, L, t2, s, false) || try_row<N, F, 4>(p, L, t2, s, false) ||
-           try_row<N, F, 8>(p, L, t2, s, false) || try_row<N, F, 16>(p, L, t2, s, false) ||
-           try_row<N, F, 32>(p, L, t2, s, false) || try_row_split<N, F, 1>(p, L, t2, s) ||
-           try_row_split<N, F, 2>(p, L, t2, s) || try_row_split<N, F, 4>(p, L, t2, s);
+    return try_row<N, F, RowShape<N>::TEAMS>(p, L, t2, k, true) || try_row<N, F, 1>(p, L, t2, k, false) ||
+           try_row<N, F, 2>(p, L, t2, k, false) || try_row<N, F, 4>(p, L, t2, k, false) ||
+           try_row<N, F, 8>(p, L, t2, k, false) || try_row<N, F, 16>(p, L, t2, k, false) ||
+           try_row<N, F, 32>(p, L, t2, k, false) || try_row_split<N, F, 1>(p, L, t2, k) ||
+           try_row_split<N, F, 2>(p, L, t2, k) || try_row_split<N, F, 4>(p, L, t2, k);
 }
 
 template <int N, int F>
-bool col_variant(int v, const ocean_hip_level& L, hipStream_t s)
+bool col_variant(int v, const ocean_hip_level& L, const Launch& k)
 {
     Pick p{v, 0};
-    return try_col<N, F, ColShape<N>::C>(p, L, s, true) || try_col<N, F, 1>(p, L, s, false) ||
-           try_col<N, F, 2>(p, L, s, false) || try_col<N, F, 4>(p, L, s, false) ||
-           try_col<N, F, 8>(p, L, s, false) || try_col<N, F, 16>(p, L, s, false) ||
-           try_col<N, F, 32>(p, L, s, false) || try_col_split<N, F, 1>(p, L, s) ||
-           try_col_split<N, F, 2>(p, L, s) || try_col_split<N, F, 4>(p, L, s);
+    return try_col<N, F, ColShape<N>::C>(p, L, k, true) || try_col<N, F, 1>(p, L, k, false) ||
+           try_col<N, F, 2>(p, L, k, false) || try_col<N, F, 4>(p, L, k, false) ||
+           try_col<N, F, 8>(p, L, k, false) || try_col<N, F, 16>(p, L, k, false) ||
+           try_col<N, F, 32>(p, L, k, false) || try_col_split<N, F, 1>(p, L, k) ||
+           try_col_split<N, F, 2>(p, L, k) || try_col_split<N, F, 4>(p, L, k);
 }
 
 // Number of variants of each pass (counted without launching: a null level
@@ -216,9 +233,9 @@ void count_variants_f(int& rows, int& cols)
 {
     ocean_hip_level none{};
     rows = 0;
-    while (row_variant<N, F>(rows, none, 0.0, nullptr)) ++rows;
+    while (row_variant<N, F>(rows, none, 0.0, Launch{})) ++rows;
     cols = 0;
-    while (col_variant<N, F>(cols, none, nullptr)) ++cols;
+    while (col_variant<N, F>(cols, none, Launch{})) ++cols;
 }
 
 template <int N>
@@ -232,16 +249,18 @@ void count_variants(int fields, int& rows, int& cols)
 
 template <int N>
 hipError_t launch_frame_n(const ocean_hip_level* L, double time_over_two_pi, hipStream_t s,
-                          unsigned passes)
+                          unsigned passes, hipEvent_t* events)
 {
     const int rv = int(L->shape & 0xffu), cv = int((L->shape >> 8) & 0xffu);
+    const Launch row{s, events ? events[0] : nullptr, events ? events[1] : nullptr};
+    const Launch col{s, events ? events[2] : nullptr, events ? events[3] : nullptr};
     bool ok = true;
     if (L->vel != nullptr) {
-        if (passes & 1u) ok = ok && row_variant<N, 6>(rv, *L, time_over_two_pi, s);
-        if (passes & 2u) ok = ok && col_variant<N, 6>(cv, *L, s);
+        if (passes & 1u) ok = ok && row_variant<N, 6>(rv, *L, time_over_two_pi, row);
+        if (passes & 2u) ok = ok && col_variant<N, 6>(cv, *L, col);
     } else {
-        if (passes & 1u) ok = ok && row_variant<N, 4>(rv, *L, time_over_two_pi, s);
-        if (passes & 2u) ok = ok && col_variant<N, 4>(cv, *L, s);
+        if (passes & 1u) ok = ok && row_variant<N, 4>(rv, *L, time_over_two_pi, row);
+        if (passes & 2u) ok = ok && col_variant<N, 4>(cv, *L, col);
     }
     if (!ok) return hipErrorInvalidValue;
     return hipGetLastError();
@@ -331,27 +350,43 @@ hipKernelNodeParams group_params(ocean_hip_batch_group& g, bool row, double* t2,
 
 using namespace oceanamd;
 
+namespace {
+
+hipError_t launch_frame(const ocean_hip_level* L, double t2, hipStream_t s, unsigned passes,
+                        hipEvent_t* events)
+{
+    switch (L->n) {
+        case 16: return launch_frame_n<16>(L, t2, s, passes, events);
+        case 32: return launch_frame_n<32>(L, t2, s, passes, events);
+        case 64: return launch_frame_n<64>(L, t2, s, passes, events);
+        case 128: return launch_frame_n<128>(L, t2, s, passes, events);
+        case 256: return launch_frame_n<256>(L, t2, s, passes, events);
+        case 512: return launch_frame_n<512>(L, t2, s, passes, events);
+        case 1024: return launch_frame_n<1024>(L, t2, s, passes, events);
+        case 2048: return launch_frame_n<2048>(L, t2, s, passes, events);
+        case 4096: return launch_frame_n<4096>(L, t2, s, passes, events);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
 extern "C" hipError_t ocean_hip_frame_passes(const ocean_hip_level* L, double time_over_two_pi,
                                              hipStream_t s, unsigned passes)
 {
-    switch (L->n) {
-        case 16: return launch_frame_n<16>(L, time_over_two_pi, s, passes);
-        case 32: return launch_frame_n<32>(L, time_over_two_pi, s, passes);
-        case 64: return launch_frame_n<64>(L, time_over_two_pi, s, passes);
-        case 128: return launch_frame_n<128>(L, time_over_two_pi, s, passes);
-        case 256: return launch_frame_n<256>(L, time_over_two_pi, s, passes);
-        case 512: return launch_frame_n<512>(L, time_over_two_pi, s, passes);
-        case 1024: return launch_frame_n<1024>(L, time_over_two_pi, s, passes);
-        case 2048: return launch_frame_n<2048>(L, time_over_two_pi, s, passes);
-        case 4096: return launch_frame_n<4096>(L, time_over_two_pi, s, passes);
-        default: return hipErrorInvalidValue;
-    }
+    return launch_frame(L, time_over_two_pi, s, passes, nullptr);
 }
 
 extern "C" hipError_t ocean_hip_frame(const ocean_hip_level* L, double time_over_two_pi,
                                       hipStream_t s)
 {
-    return ocean_hip_frame_passes(L, time_over_two_pi, s, 3u);
+    return launch_frame(L, time_over_two_pi, s, 3u, nullptr);
+}
+
+extern "C" hipError_t ocean_hip_frame_timed(const ocean_hip_level* L, double time_over_two_pi,
+                                            hipStream_t s, hipEvent_t* events)
+{
+    return launch_frame(L, time_over_two_pi, s, 3u, events);
 }
 
 extern "C" int ocean_hip_frame_shapes(uint32_t n, uint32_t fields, int* row_variants,

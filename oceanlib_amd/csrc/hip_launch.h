@@ -67,6 +67,11 @@ hipError_t ocean_hip_frame(const ocean_hip_level* level, double time_over_two_pi
 hipError_t ocean_hip_frame_passes(const ocean_hip_level* level, double time_over_two_pi,
                                   hipStream_t s, unsigned passes);
 
+/* The same frame with each pass's dispatch recording its own start and end:
+ * events[0..1] bracket the row pass, events[2..3] the column pass. */
+hipError_t ocean_hip_frame_timed(const ocean_hip_level* level, double time_over_two_pi,
+                                 hipStream_t s, hipEvent_t* events);
+
 /* Launch-shape variants of the per-level passes for grid size n and F fields
  * (4, or 6 with velocity; variant 0 is the default): how many row and column
  * variants exist; -1 if n is not a supported size. */

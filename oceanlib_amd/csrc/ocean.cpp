@@ -133,6 +133,9 @@ OceanDevice::~OceanDevice()
     if (h_vel) hipHostFree(h_vel);
     if (h_q_xz) hipHostFree(h_q_xz);
     if (h_q_out) hipHostFree(h_q_out);
+    for (auto& slot : timing_ev)
+        for (auto& e : slot)
+            if (e) hipEventDestroy(e);
     if (own_stream && stream) hipStreamDestroy(stream);
 }
 
@@ -222,8 +225,52 @@ void OceanDevice::enqueue_frame(double time) noexcept
     make_current();
     // Hoisted exactly as the reference hoists it (src/core/evolve.cpp:33).
     const double t2 = time * kInvTwoPiD;
-    note(ocean_hip_frame(&level, t2, stream));
+    if (timing_every != 0 && timing_counter++ % timing_every == 0) {
+        hipEvent_t* ev = timing_ev[timing_next];
+        timing_next = (timing_next + 1) % kTimingSlots;
+        if (timing_used < kTimingSlots) ++timing_used;
+        note(ocean_hip_frame_timed(&level, t2, stream, ev));
+    } else {
+        note(ocean_hip_frame(&level, t2, stream));
+    }
     mirror_stale = true;
+}
+
+int OceanDevice::set_frame_timing(std::uint32_t every) noexcept
+{
+    make_current();
+    if (every != 0 && timing_ev[0][0] == nullptr) {
+        for (auto& slot : timing_ev)
+            for (auto& e : slot)
+                if (hipEventCreate(&e) != hipSuccess) return -1;
+    }
+    timing_every = every;
+    timing_counter = 0;
+    timing_next = 0;
+    timing_used = 0;
+    return 0;
+}
+
+int OceanDevice::read_frame_timing(float* row_ms, float* col_ms, std::uint32_t* samples) noexcept
+{
+    make_current();
+    if (hipStreamSynchronize(stream) != hipSuccess) return -1;
+    double row = 0.0, col = 0.0;
+    for (std::uint32_t i = 0; i < timing_used; ++i) {
+        float a = 0.0f, b = 0.0f;
+        if (hipEventElapsedTime(&a, timing_ev[i][0], timing_ev[i][1]) != hipSuccess ||
+            hipEventElapsedTime(&b, timing_ev[i][2], timing_ev[i][3]) != hipSuccess)
+            return -1;
+        row += a;
+        col += b;
+    }
+    const std::uint32_t n = timing_used;
+    if (row_ms) *row_ms = n ? static_cast<float>(row / n) : 0.0f;
+    if (col_ms) *col_ms = n ? static_cast<float>(col / n) : 0.0f;
+    if (samples) *samples = n;
+    timing_used = 0;
+    timing_next = 0;
+    return 0;
 }
 
 void OceanDevice::alloc_host_mirrors() noexcept

@@ -48,6 +48,16 @@ struct OceanDevice {
     float* h_q_xz = nullptr;
     ocean_hip_surface* h_q_out = nullptr;
 
+    // Optional timing of the frame kernels inside an update loop: the row and
+    // column dispatches of every `timing_every`-th frame record start and end
+    // events, in a ring of kTimingSlots samples (the oldest are overwritten).
+    static constexpr std::uint32_t kTimingSlots = 64;
+    std::uint32_t timing_every = 0;
+    std::uint32_t timing_counter = 0;
+    std::uint32_t timing_next = 0;
+    std::uint32_t timing_used = 0;
+    hipEvent_t timing_ev[kTimingSlots][4] = {};
+
     bool mirror_stale = true;
     int status = 0;            // first HIP error seen (sticky)
     std::uint32_t init_repaired = 0;  // table entries the host recomputed at creation
@@ -69,6 +79,10 @@ struct OceanDevice {
     }
     void make_current() const noexcept;
     void enqueue_frame(double time) noexcept;
+    // every = 0 stops timing; otherwise creates the events (once) and starts.
+    int set_frame_timing(std::uint32_t every) noexcept;
+    // Waits for the stream, averages the recorded samples and clears them.
+    int read_frame_timing(float* row_ms, float* col_ms, std::uint32_t* samples) noexcept;
     // Copies the device outputs into the pinned mirrors if they are stale.
     void refresh_mirror() noexcept;
     bool sample_host(const float* xz, std::uint32_t count, Surface* out) noexcept;

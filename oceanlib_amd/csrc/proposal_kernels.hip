@@ -97,24 +97,41 @@ __device__ __forceinline__ Base base_cell(float X, float Z, const ocean_hip_leve
     return b;
 }
 
-// Bilinear (dx, dz[, h]) at cell position (ix + px, iz + pz), periodic.
+// Bilinear (dx, dz[, h]) at cell position (ix + px, iz + pz), periodic. The
+// four texels of the last cell are kept in `c`: once the iteration settles
+// into its final cell, later taps blend registers (the same expression on the
+// same values) instead of loading again.
+struct TapCache {
+    uint32_t key = 0xffffffffu;  // z0 * n + x0 of the cached cell
+    // Zero-initialised: the first tap always loads (key < n * n), but an
+    // indeterminate member would let the compiler drop the components that
+    // only the final tap reads (the height) from the loop's cached cell.
+    float4 a{}, c{}, d{}, e{};
+};
+
 template <bool HEIGHT>
 __device__ __forceinline__ void tap(const float4* __restrict__ disp, uint32_t n, const Base& b,
-                                    float px, float pz, float& dx, float& dz, float& h)
+                                    float px, float pz, TapCache& k, float& dx, float& dz, float& h)
 {
     const float flx = floorf(px), flz = floorf(pz);
     const float tu = px - flx, tv = pz - flz;
     const uint32_t mask = n - 1u;
     const uint32_t x0 = uint32_t(b.ix + int32_t(flx)) & mask;
     const uint32_t z0 = uint32_t(b.iz + int32_t(flz)) & mask;
-    const uint32_t x1 = (x0 + 1u) & mask, z1 = (z0 + 1u) & mask;
-    const float4 a = disp[z0 * n + x0], c = disp[z0 * n + x1];
-    const float4 d = disp[z1 * n + x0], e = disp[z1 * n + x1];
+    const uint32_t key = z0 * n + x0;
+    if (key != k.key) {
+        const uint32_t x1 = (x0 + 1u) & mask, z1 = (z0 + 1u) & mask;
+        k.a = disp[key];
+        k.c = disp[z0 * n + x1];
+        k.d = disp[z1 * n + x0];
+        k.e = disp[z1 * n + x1];
+        k.key = key;
+    }
     const float w00 = (1.0f - tu) * (1.0f - tv), w10 = tu * (1.0f - tv);
     const float w01 = (1.0f - tu) * tv, w11 = tu * tv;
-    dx = a.x * w00 + c.x * w10 + d.x * w01 + e.x * w11;
-    dz = a.z * w00 + c.z * w10 + d.z * w01 + e.z * w11;
-    if (HEIGHT) h = a.y * w00 + c.y * w10 + d.y * w01 + e.y * w11;
+    dx = k.a.x * w00 + k.c.x * w10 + k.d.x * w01 + k.e.x * w11;
+    dz = k.a.z * w00 + k.c.z * w10 + k.d.z * w01 + k.e.z * w11;
+    if (HEIGHT) h = k.a.y * w00 + k.c.y * w10 + k.d.y * w01 + k.e.y * w11;
 }
 
 struct Solve {
@@ -126,7 +143,8 @@ struct Solve {
 
 // Fixed-point inversion of one level from the start offset (du0, dv0) cells.
 __device__ __forceinline__ Solve solve_level(const ocean_hip_level& L, const Base& b, float du0,
-                                             float dv0, float tol_cells, int max_iter, bool learned)
+                                             float dv0, float tol_cells, int max_iter, bool learned,
+                                             TapCache& cache)
 {
     const float4* __restrict__ disp = reinterpret_cast<const float4*>(L.disp);
     const float inv_cell = float(L.n) / L.patch_length;
@@ -135,7 +153,7 @@ __device__ __forceinline__ Solve solve_level(const ocean_hip_level& L, const Bas
     float h;
     while (s.iters < max_iter) {
         float dx, dz;
-        tap<false>(disp, L.n, b, b.fx + s.du, b.fz + s.dv, dx, dz, h);
+        tap<false>(disp, L.n, b, b.fx + s.du, b.fz + s.dv, cache, dx, dz, h);
         ++s.iters;
         const float nu = -dx * inv_cell, nv = -dz * inv_cell;
         const float step = fmaxf(fabsf(nu - s.du), fabsf(nv - s.dv));
@@ -487,9 +505,11 @@ __global__ __launch_bounds__(256) void converged_kernel(
                 fl |= 2u;
             }
         }
-        const Solve s = solve_level(L, b, du0, dv0, tol * inv_cell, kMaxIter, use);
+        TapCache cache;
+        const Solve s = solve_level(L, b, du0, dv0, tol * inv_cell, kMaxIter, use, cache);
         float dx, dz, h;
-        tap<true>(reinterpret_cast<const float4*>(L.disp), L.n, b, b.fx + s.du, b.fz + s.dv, dx, dz, h);
+        tap<true>(reinterpret_cast<const float4*>(L.disp), L.n, b, b.fx + s.du, b.fz + s.dv, cache,
+                  dx, dz, h);
         height += h;
         if (iters) iters[size_t(q) * level_count + lv] = uint8_t(s.iters > 255 ? 255 : s.iters);
         if (!s.converged) fl |= 1u;
@@ -522,7 +542,8 @@ __global__ __launch_bounds__(256) void pairs_kernel(const ocean_hip_level* __res
         const int s = spacing[lv];
         level_features(L, b, s, f + lv * kFeatPerLevel);
         const float inv_cell = float(L.n) / L.patch_length;
-        const Solve r = solve_level(L, b, 0.0f, 0.0f, tol * inv_cell, 255, false);
+        TapCache cache;
+        const Solve r = solve_level(L, b, 0.0f, 0.0f, tol * inv_cell, 255, false, cache);
         good = good && r.converged;
         t[2 * lv] = r.du / float(s);
         t[2 * lv + 1] = r.dv / float(s);

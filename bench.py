@@ -43,24 +43,25 @@ def log(msg):
 def algorithmic_bytes(n: int, fields: int, velocity: bool):
     """Per-launch algorithmic HBM bytes of the two frame kernels.
 
-    row pass : reads h0 (8 B) + omega (4 B) per cell once, writes the F
-               complex fields (8F B per cell);
+    row pass : reads h0 (8 B) + omega (4 B) + 1/|k| (4 B) per cell once,
+               writes the F complex fields (8F B per cell);
     col pass : reads the F fields (8F B), writes displacement + normal
                (+ velocity) RGBA32F (16 B each per cell).
     """
     cells = n * n
-    row = cells * (12 + 8 * fields)
+    row = cells * (16 + 8 * fields)
     col = cells * (8 * fields + 16 * (3 if velocity else 2))
     return row, col
 
 
 def frame_bytes(n: int, fields: int, velocity: bool) -> int:
-    """Algorithmic HBM bytes of one level's frame: h0 + omega read once
-    (12 B/cell), RGBA32F outputs written once (32 B/cell, +16 with velocity),
-    and, for N >= 256 only, one write + read of the F complex fields between
-    the passes (16F B/cell; a smaller level's fields could stay on chip)."""
+    """Algorithmic HBM bytes of one level's frame: h0 + omega + 1/|k| read
+    once (16 B/cell), RGBA32F outputs written once (32 B/cell, +16 with
+    velocity), and, for N >= 256 only, one write + read of the F complex
+    fields between the passes (16F B/cell; a smaller level's fields could stay
+    on chip)."""
     inter = 16 * fields if n >= 256 else 0
-    return n * n * (12 + 32 + (16 if velocity else 0) + inter)
+    return n * n * (16 + 32 + (16 if velocity else 0) + inter)
 
 
 def traffic_from_profiles(kernels):

@@ -43,11 +43,12 @@ constexpr int lds_span() { return N + N / 16; }
 template <int N>
 constexpr int team_stride() { return lds_span<N>() + ((2 - lds_span<N>() % 16) + 16) % 16; }
 
-// r butterfly stages (r <= 3) starting at stage t0, on a lane's 8 registers.
+// r butterfly stages (r <= 3) starting at stage t0, on a lane's 8 registers
+// of each of K independent sequences (same indices and twiddles for all K).
 // idx[a] tracks the reference-array index register a currently represents.
-template <int T0, int RST>
-__host__ __device__ __forceinline__ void fft_pass(float (&re)[8], float (&im)[8], uint32_t (&idx)[8],
-                                         const float2* __restrict__ tw)
+template <int T0, int RST, int K>
+__host__ __device__ __forceinline__ void fft_pass(float (&re)[K][8], float (&im)[K][8],
+                                                  uint32_t (&idx)[8], const float2* __restrict__ tw)
 {
 #pragma unroll
     for (int st = 0; st < RST; ++st) {
@@ -60,14 +61,17 @@ __host__ __device__ __forceinline__ void fft_pass(float (&re)[8], float (&im)[8]
             const int b = a | m;
             const uint32_t jl = idx[a];
             const float2 w = tw[jl & ~(s - 1u)];
-            const float ar = re[a], ai = im[a];
-            const float br = re[b], bi = im[b];
-            re[a] = ar + br;
-            im[a] = ai + bi;
-            const float dr = ar - br;
-            const float di = ai - bi;
-            re[b] = dr * w.x - di * w.y;
-            im[b] = dr * w.y + di * w.x;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const float ar = re[k][a], ai = im[k][a];
+                const float br = re[k][b], bi = im[k][b];
+                re[k][a] = ar + br;
+                im[k][a] = ai + bi;
+                const float dr = ar - br;
+                const float di = ai - bi;
+                re[k][b] = dr * w.x - di * w.y;
+                im[k][b] = dr * w.y + di * w.x;
+            }
             const uint32_t base = ((jl >> t) << (t + 1)) | (jl & (s - 1u));
             idx[a] = base;
             idx[b] = base | s;
@@ -81,66 +85,90 @@ struct PassPlan {
     static constexpr int RST = (FftShape<N>::LOGN - T0) < 3 ? (FftShape<N>::LOGN - T0) : 3;
 };
 
-// Full inverse transform of one complex sequence held by a team.
+// One LDS exchange between passes: every register goes to its current index
+// and comes back as element J + a*T, for all K sequences between the same
+// two barriers (sequence k uses the area xch + k * kstride).
+template <int N, int K, typename Sync>
+__host__ __device__ __forceinline__ void exchange(float (&re)[K][8], float (&im)[K][8],
+                                                  uint32_t (&idx)[8], uint32_t J,
+                                                  float2* __restrict__ xch, uint32_t kstride,
+                                                  Sync sync)
+{
+    constexpr int T = FftShape<N>::T;
+    sync();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+#pragma unroll
+        for (int a = 0; a < 8; ++a) xch[k * kstride + lds_pad(idx[a])] = make_float2(re[k][a], im[k][a]);
+    }
+    sync();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const float2 v = xch[k * kstride + lds_pad(J + a * T)];
+            re[k][a] = v.x;
+            im[k][a] = v.y;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) idx[a] = J | (static_cast<uint32_t>(a) * T);
+}
+
+// Full inverse transform of K complex sequences held by a team.
 //   re/im   : in: register a = element J + a*T; out: element idx[a]
-//   xch     : this team's LDS exchange area (lds_span<N>() float2; teams of
-//             a block are team_stride<N>() apart)
+//   xch     : this team's LDS exchange areas, K of them kstride float2 apart
+//             (each lds_span<N>() float2; teams of a block are
+//             team_stride<N>() * K apart)
 //   tw      : LDS twiddles, N/2 entries (cos, +sin) of 2*pi*j/N
 //   sync    : barrier callable (a block barrier in practice)
-template <int N, typename Sync>
-__host__ __device__ __forceinline__ void fft_team(float (&re)[8], float (&im)[8], uint32_t (&idx)[8],
-                                         uint32_t J, float2* __restrict__ xch,
-                                         const float2* __restrict__ tw, Sync sync)
+// Transforming K sequences together shares the index arithmetic and twiddle
+// reads, and gives each barrier K times the independent work.
+template <int N, int K, typename Sync>
+__host__ __device__ __forceinline__ void fft_team_k(float (&re)[K][8], float (&im)[K][8],
+                                                    uint32_t (&idx)[8], uint32_t J,
+                                                    float2* __restrict__ xch, uint32_t kstride,
+                                                    const float2* __restrict__ tw, Sync sync)
 {
     constexpr int T = FftShape<N>::T;
     constexpr int PASSES = FftShape<N>::PASSES;
 #pragma unroll
     for (int a = 0; a < 8; ++a) idx[a] = J | (static_cast<uint32_t>(a) * T);
 
-    fft_pass<PassPlan<N, 0>::T0, PassPlan<N, 0>::RST>(re, im, idx, tw);
+    fft_pass<PassPlan<N, 0>::T0, PassPlan<N, 0>::RST, K>(re, im, idx, tw);
     if constexpr (PASSES > 1) {
-        sync();
-#pragma unroll
-        for (int a = 0; a < 8; ++a) xch[lds_pad(idx[a])] = make_float2(re[a], im[a]);
-        sync();
-#pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            const float2 v = xch[lds_pad(J + a * T)];
-            re[a] = v.x;
-            im[a] = v.y;
-            idx[a] = J | (static_cast<uint32_t>(a) * T);
-        }
-        fft_pass<PassPlan<N, 1>::T0, PassPlan<N, 1>::RST>(re, im, idx, tw);
+        exchange<N, K>(re, im, idx, J, xch, kstride, sync);
+        fft_pass<PassPlan<N, 1>::T0, PassPlan<N, 1>::RST, K>(re, im, idx, tw);
     }
     if constexpr (PASSES > 2) {
-        sync();
-#pragma unroll
-        for (int a = 0; a < 8; ++a) xch[lds_pad(idx[a])] = make_float2(re[a], im[a]);
-        sync();
-#pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            const float2 v = xch[lds_pad(J + a * T)];
-            re[a] = v.x;
-            im[a] = v.y;
-            idx[a] = J | (static_cast<uint32_t>(a) * T);
-        }
-        fft_pass<PassPlan<N, 2>::T0, PassPlan<N, 2>::RST>(re, im, idx, tw);
+        exchange<N, K>(re, im, idx, J, xch, kstride, sync);
+        fft_pass<PassPlan<N, 2>::T0, PassPlan<N, 2>::RST, K>(re, im, idx, tw);
     }
     if constexpr (PASSES > 3) {
-        sync();
-#pragma unroll
-        for (int a = 0; a < 8; ++a) xch[lds_pad(idx[a])] = make_float2(re[a], im[a]);
-        sync();
-#pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            const float2 v = xch[lds_pad(J + a * T)];
-            re[a] = v.x;
-            im[a] = v.y;
-            idx[a] = J | (static_cast<uint32_t>(a) * T);
-        }
-        fft_pass<PassPlan<N, 3>::T0, PassPlan<N, 3>::RST>(re, im, idx, tw);
+        exchange<N, K>(re, im, idx, J, xch, kstride, sync);
+        fft_pass<PassPlan<N, 3>::T0, PassPlan<N, 3>::RST, K>(re, im, idx, tw);
     }
     static_assert(PASSES <= 4, "N > 4096 is outside the reference's range");
+}
+
+// One sequence (K = 1).
+template <int N, typename Sync>
+__host__ __device__ __forceinline__ void fft_team(float (&re)[8], float (&im)[8], uint32_t (&idx)[8],
+                                                  uint32_t J, float2* __restrict__ xch,
+                                                  const float2* __restrict__ tw, Sync sync)
+{
+    float r1[1][8], i1[1][8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        r1[0][a] = re[a];
+        i1[0][a] = im[a];
+    }
+    fft_team_k<N, 1>(r1, i1, idx, J, xch, 0u, tw, sync);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        re[a] = r1[0][a];
+        im[a] = i1[0][a];
+    }
 }
 
 }  // namespace oceanamd

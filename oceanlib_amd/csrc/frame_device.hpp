@@ -47,6 +47,8 @@ __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk
     constexpr uint32_t CELLS = uint32_t(N) * N;
     const float2* __restrict__ h0 = reinterpret_cast<const float2*>(L.h0);
     const float* __restrict__ omega = L.omega;
+    const float* __restrict__ kinv = L.kinv;
+    const float* __restrict__ kxs = L.kx;
     const float2* __restrict__ twg = reinterpret_cast<const float2*>(L.twiddles);
     float2* __restrict__ spec = reinterpret_cast<float2*>(L.spec);
     const float patch_length = L.patch_length;
@@ -85,8 +87,8 @@ __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk
         const float hr = fwd_re + bwd_re;
         const float hi = fwd_im + bwd_im;
 
-        const float kx = wave_component(x, N, patch_length);
-        const float ki = inv_wavenumber(kx, kz);
+        const float kx = kxs[x];
+        const float ki = kinv[i];
         const float sx = kx * ki;
         const float sz = kz * ki;
         const float dx_re = sx * hi, dx_im = -sx * hr;
@@ -207,9 +209,11 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
 // The column finalize needs fields 0-3 of a cell together: group 1 hands its
 // two fields to group 0 through LDS (`xfer`); group 2 (velocity) writes its
 // output itself. Requires T = N/8 >= 64, so a team is whole wavefronts and the
-// group branches are uniform.
+// group branches are uniform. KX = 2 transforms a team's two fields together
+// (fft_team_k: shared index arithmetic and twiddle reads, one pair of
+// barriers per exchange for both), KX = 1 one after the other.
 
-template <int N, int F, int TEAMS>
+template <int N, int F, int TEAMS, int KX>
 __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32_t blk,
                                                 double time_over_two_pi, float2* __restrict__ tw,
                                                 float2* __restrict__ xch)
@@ -221,6 +225,8 @@ __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32
     static_assert(T >= 64 && F % 2 == 0, "field split needs whole-wavefront teams");
     const float2* __restrict__ h0 = reinterpret_cast<const float2*>(L.h0);
     const float* __restrict__ omega = L.omega;
+    const float* __restrict__ kinv = L.kinv;
+    const float* __restrict__ kxs = L.kx;
     const float2* __restrict__ twg = reinterpret_cast<const float2*>(L.twiddles);
     float2* __restrict__ spec = reinterpret_cast<float2*>(L.spec);
     const float patch_length = L.patch_length;
@@ -261,8 +267,8 @@ __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32
         const float hr = fwd_re + bwd_re;
         const float hi = fwd_im + bwd_im;
 
-        const float kx = wave_component(x, N, patch_length);
-        const float ki = inv_wavenumber(kx, kz);
+        const float kx = kxs[x];
+        const float ki = kinv[i];
         const float sx = kx * ki;
         const float sz = kz * ki;
         if (g == 0u) {
@@ -292,18 +298,29 @@ __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32
         }
     }
 
-    float2* x_team = xch + tt * team_stride<N>();
-#pragma unroll
-    for (int f = 0; f < 2; ++f) {
+    float2* x_team = xch + tt * KX * team_stride<N>();
+    if constexpr (KX == 2) {
         uint32_t idx[8];
-        fft_team<N>(fr[f], fi[f], idx, J, x_team, tw, BlockSync{});
-        float2* dst = spec + size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
+        fft_team_k<N, 2>(fr, fi, idx, J, x_team, team_stride<N>(), tw, BlockSync{});
 #pragma unroll
-        for (int a = 0; a < 8; ++a) dst[idx[a]] = make_float2(fr[f][a], fi[f][a]);
+        for (int f = 0; f < 2; ++f) {
+            float2* dst = spec + size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
+#pragma unroll
+            for (int a = 0; a < 8; ++a) dst[idx[a]] = make_float2(fr[f][a], fi[f][a]);
+        }
+    } else {
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+            uint32_t idx[8];
+            fft_team<N>(fr[f], fi[f], idx, J, x_team, tw, BlockSync{});
+            float2* dst = spec + size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
+#pragma unroll
+            for (int a = 0; a < 8; ++a) dst[idx[a]] = make_float2(fr[f][a], fi[f][a]);
+        }
     }
 }
 
-template <int N, int F, int C>
+template <int N, int F, int C, int KX>
 __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32_t blk,
                                                 float2* __restrict__ tw, float2* __restrict__ xch,
                                                 float2* __restrict__ xfer)
@@ -344,9 +361,14 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
     }
 
     uint32_t idx[8];
+    if constexpr (KX == 2) {
+        fft_team_k<N, 2>(fr, fi, idx, J, xch + (g * C + c) * 2 * team_stride<N>(), team_stride<N>(),
+                         tw, BlockSync{});
+    } else {
 #pragma unroll
-    for (int f = 0; f < 2; ++f)
-        fft_team<N>(fr[f], fi[f], idx, J, xch + (g * C + c) * team_stride<N>(), tw, BlockSync{});
+        for (int f = 0; f < 2; ++f)
+            fft_team<N>(fr[f], fi[f], idx, J, xch + (g * C + c) * team_stride<N>(), tw, BlockSync{});
+    }
 
     if (g == 1u) {  // fields 2 and 3 to group 0
 #pragma unroll

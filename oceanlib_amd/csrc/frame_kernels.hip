@@ -72,22 +72,23 @@ __global__ __launch_bounds__(C * FftShape<N>::T) void col_pass_kernel(ocean_hip_
     col_block<N, F, C>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch);
 }
 
-template <int N, int F, int TEAMS>
+template <int N, int F, int TEAMS, int KX>
 __global__ __launch_bounds__(TEAMS * (F / 2) * FftShape<N>::T) void row_split_kernel(
     ocean_hip_level L, double time_over_two_pi)
 {
     __shared__ float2 tw[N / 2];
-    __shared__ float2 xch[TEAMS * (F / 2) * team_stride<N>()];
-    row_block_split<N, F, TEAMS>(L, xcd_group(blockIdx.x, gridDim.x), time_over_two_pi, tw, xch);
+    __shared__ float2 xch[TEAMS * (F / 2) * KX * team_stride<N>()];
+    row_block_split<N, F, TEAMS, KX>(L, xcd_group(blockIdx.x, gridDim.x), time_over_two_pi, tw,
+                                     xch);
 }
 
-template <int N, int F, int C>
+template <int N, int F, int C, int KX>
 __global__ __launch_bounds__(C * (F / 2) * FftShape<N>::T) void col_split_kernel(ocean_hip_level L)
 {
     __shared__ float2 tw[N / 2];
-    __shared__ float2 xch[C * (F / 2) * team_stride<N>()];
+    __shared__ float2 xch[C * (F / 2) * KX * team_stride<N>()];
     __shared__ float2 xfer[C * 2 * N];
-    col_block_split<N, F, C>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch, xfer);
+    col_block_split<N, F, C, KX>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch, xfer);
 }
 
 // Launch shapes of the per-level passes. Variant 0 is the default shape;
@@ -110,20 +111,21 @@ constexpr bool col_ok()
 }
 
 // Field-split shapes (row_split_kernel / col_split_kernel): whole-wavefront
-// teams, at most 512 lanes, static LDS within 64 KB.
-template <int N, int F, int TEAMS>
+// teams, at most 512 lanes, static LDS within 64 KB. KX = fields a team
+// exchanges between the same barriers (1: one field after the other, 2: both).
+template <int N, int F, int TEAMS, int KX>
 constexpr bool row_split_ok()
 {
     constexpr int lanes = TEAMS * (F / 2) * FftShape<N>::T;
     return FftShape<N>::T >= 64 && TEAMS <= N && lanes <= 512 &&
-           (N / 2 + TEAMS * (F / 2) * team_stride<N>()) * 8 <= 65536;
+           (N / 2 + TEAMS * (F / 2) * KX * team_stride<N>()) * 8 <= 65536;
 }
-template <int N, int F, int C>
+template <int N, int F, int C, int KX>
 constexpr bool col_split_ok()
 {
     constexpr int lanes = C * (F / 2) * FftShape<N>::T;
     return FftShape<N>::T >= 64 && C <= N && lanes <= 512 &&
-           (N / 2 + C * (F / 2) * team_stride<N>() + C * 2 * N) * 8 <= 65536;
+           (N / 2 + C * (F / 2) * KX * team_stride<N>() + C * 2 * N) * 8 <= 65536;
 }
 
 struct Pick {
@@ -147,29 +149,29 @@ void launch_on(const Launch& k, Kernel kernel, dim3 grid, dim3 block, Args... ar
         hipLaunchKernelGGL(kernel, grid, block, 0, k.s, args...);
 }
 
-template <int N, int F, int TEAMS>
+template <int N, int F, int TEAMS, int KX>
 bool try_row_split(Pick& p, const ocean_hip_level& L, double t2, const Launch& k)
 {
-    if constexpr (!row_split_ok<N, F, TEAMS>()) {
+    if constexpr (!row_split_ok<N, F, TEAMS, KX>()) {
         return false;
     } else {
         if (p.seen++ != p.want) return false;
         if (k.s != nullptr || L.n != 0)
-            launch_on(k, row_split_kernel<N, F, TEAMS>, dim3(N / TEAMS),
+            launch_on(k, row_split_kernel<N, F, TEAMS, KX>, dim3(N / TEAMS),
                       dim3(TEAMS * (F / 2) * FftShape<N>::T), L, t2);
         return true;
     }
 }
 
-template <int N, int F, int C>
+template <int N, int F, int C, int KX>
 bool try_col_split(Pick& p, const ocean_hip_level& L, const Launch& k)
 {
-    if constexpr (!col_split_ok<N, F, C>()) {
+    if constexpr (!col_split_ok<N, F, C, KX>()) {
         return false;
     } else {
         if (p.seen++ != p.want) return false;
         if (k.s != nullptr || L.n != 0)
-            launch_on(k, col_split_kernel<N, F, C>, dim3(N / C), dim3(C * (F / 2) * FftShape<N>::T),
+            launch_on(k, col_split_kernel<N, F, C, KX>, dim3(N / C), dim3(C * (F / 2) * FftShape<N>::T),
                       L);
         return true;
     }
@@ -211,8 +213,9 @@ bool row_variant(int v, const ocean_hip_level& L, double t2, const Launch& k)
     return try_row<N, F, RowShape<N>::TEAMS>(p, L, t2, k, true) || try_row<N, F, 1>(p, L, t2, k, false) ||
            try_row<N, F, 2>(p, L, t2, k, false) || try_row<N, F, 4>(p, L, t2, k, false) ||
            try_row<N, F, 8>(p, L, t2, k, false) || try_row<N, F, 16>(p, L, t2, k, false) ||
-           try_row<N, F, 32>(p, L, t2, k, false) || try_row_split<N, F, 1>(p, L, t2, k) ||
-           try_row_split<N, F, 2>(p, L, t2, k) || try_row_split<N, F, 4>(p, L, t2, k);
+           try_row<N, F, 32>(p, L, t2, k, false) || try_row_split<N, F, 1, 1>(p, L, t2, k) ||
+           try_row_split<N, F, 2, 1>(p, L, t2, k) || try_row_split<N, F, 4, 1>(p, L, t2, k) ||
+           try_row_split<N, F, 1, 2>(p, L, t2, k) || try_row_split<N, F, 2, 2>(p, L, t2, k);
 }
 
 template <int N, int F>
@@ -222,8 +225,9 @@ bool col_variant(int v, const ocean_hip_level& L, const Launch& k)
     return try_col<N, F, ColShape<N>::C>(p, L, k, true) || try_col<N, F, 1>(p, L, k, false) ||
            try_col<N, F, 2>(p, L, k, false) || try_col<N, F, 4>(p, L, k, false) ||
            try_col<N, F, 8>(p, L, k, false) || try_col<N, F, 16>(p, L, k, false) ||
-           try_col<N, F, 32>(p, L, k, false) || try_col_split<N, F, 1>(p, L, k) ||
-           try_col_split<N, F, 2>(p, L, k) || try_col_split<N, F, 4>(p, L, k);
+           try_col<N, F, 32>(p, L, k, false) || try_col_split<N, F, 1, 1>(p, L, k) ||
+           try_col_split<N, F, 2, 1>(p, L, k) || try_col_split<N, F, 4, 1>(p, L, k) ||
+           try_col_split<N, F, 1, 2>(p, L, k) || try_col_split<N, F, 2, 2>(p, L, k);
 }
 
 // Number of variants of each pass (counted without launching: a null level
@@ -352,6 +356,18 @@ using namespace oceanamd;
 
 namespace {
 
+__global__ __launch_bounds__(kBlock) void wavenumber_kernel(uint32_t n, float patch_length,
+                                                            float* __restrict__ kinv,
+                                                            float* __restrict__ kx)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n * n) return;
+    const uint32_t x = i % n, z = i / n;
+    const float kxv = wave_component(x, n, patch_length);
+    kinv[i] = inv_wavenumber(kxv, wave_component(z, n, patch_length));
+    if (z == 0u) kx[x] = kxv;
+}
+
 hipError_t launch_frame(const ocean_hip_level* L, double t2, hipStream_t s, unsigned passes,
                         hipEvent_t* events)
 {
@@ -370,6 +386,15 @@ hipError_t launch_frame(const ocean_hip_level* L, double t2, hipStream_t s, unsi
 }
 
 }  // namespace
+
+extern "C" hipError_t ocean_hip_wavenumbers(uint32_t n, float patch_length, float* kinv,
+                                            float* kx, hipStream_t s)
+{
+    const uint32_t cells = n * n;
+    hipLaunchKernelGGL(wavenumber_kernel, dim3((cells + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                       n, patch_length, kinv, kx);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t ocean_hip_frame_passes(const ocean_hip_level* L, double time_over_two_pi,
                                              hipStream_t s, unsigned passes)

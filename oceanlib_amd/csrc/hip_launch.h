@@ -26,6 +26,8 @@ typedef struct ocean_hip_level {
     float* disp;                /* N*N float4                                  */
     float* normal;              /* N*N float4                                  */
     float* vel;                 /* N*N float4, or NULL when velocity is off    */
+    const float* kinv;          /* N*N float: 1/|k| (0 at DC)                  */
+    const float* kx;            /* N float: signed wavenumber of each column   */
 } ocean_hip_level;
 
 /* Same layout as ocean_surface_ex. */
@@ -58,6 +60,13 @@ hipError_t ocean_hip_twiddles(uint32_t n, float* tw, uint8_t* flags, hipStream_t
  * {uint32 index; float h0_re, h0_im, omega}. */
 hipError_t ocean_hip_spectrum_fix(const void* fixes, uint32_t count, float* h0, float* omega,
                                   hipStream_t s);
+
+/* The per-cell 1/|k| and per-column kx planes of an N grid (time-independent;
+ * the row pass reads them instead of recomputing a double sqrt and divide per
+ * cell), by the reference's expressions (src/core/spectrum.hpp:85-92,
+ * src/core/spectrum.cpp:300-306). */
+hipError_t ocean_hip_wavenumbers(uint32_t n, float patch_length, float* kinv, float* kx,
+                                 hipStream_t s);
 
 /* Enqueue one frame (row pass + column pass) of one level on stream s. */
 hipError_t ocean_hip_frame(const ocean_hip_level* level, double time_over_two_pi,

@@ -168,6 +168,8 @@ void OceanDevice::create(const OceanDesc& desc, hipStream_t shared_stream, bool 
     std::size_t off = 0;
     const std::size_t o_h0 = off;     off = align_up(off + cells * 8, a);
     const std::size_t o_omega = off;  off = align_up(off + cells * 4, a);
+    const std::size_t o_kinv = off;   off = align_up(off + cells * 4, a);
+    const std::size_t o_kx = off;     off = align_up(off + std::size_t(n) * 4, a);
     const std::size_t o_tw = off;     off = align_up(off + std::max<std::size_t>(n / 2, 1) * 8, a);
     const std::size_t o_spec = off;   off = align_up(off + cells * 8 * fields, a);
     const std::size_t o_disp = off;   off = align_up(off + cells * 16, a);
@@ -193,6 +195,8 @@ void OceanDevice::create(const OceanDesc& desc, hipStream_t shared_stream, bool 
     level.h0 = reinterpret_cast<const float*>(base + o_h0);
     level.omega = reinterpret_cast<const float*>(base + o_omega);
     level.twiddles = reinterpret_cast<const float*>(base + o_tw);
+    level.kinv = reinterpret_cast<const float*>(base + o_kinv);
+    level.kx = reinterpret_cast<const float*>(base + o_kx);
     level.spec = reinterpret_cast<float*>(base + o_spec);
     level.disp = reinterpret_cast<float*>(base + o_disp);
     level.normal = reinterpret_cast<float*>(base + o_norm);
@@ -216,6 +220,9 @@ void OceanDevice::create(const OceanDesc& desc, hipStream_t shared_stream, bool 
                                            reinterpret_cast<float*>(base + o_omega),
                                            reinterpret_cast<float*>(base + o_tw),
                                            reinterpret_cast<unsigned char*>(base + o_spec), stream);
+    hip_check(ocean_hip_wavenumbers(n, desc.patch_length, reinterpret_cast<float*>(base + o_kinv),
+                                    reinterpret_cast<float*>(base + o_kx), stream),
+              "wavenumber planes");
     hip_check(hipMemcpyAsync(d_level, &level, sizeof(level), hipMemcpyHostToDevice, stream), "upload level");
     hip_check(hipStreamSynchronize(stream), "upload");
 }

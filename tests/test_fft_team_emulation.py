@@ -31,6 +31,8 @@ def emul(tmp_path_factory):
     lib = C.CDLL(out)
     lib.emul_fft_team.argtypes = [C.c_uint32] + [C.POINTER(C.c_float)] * 4
     lib.emul_fft_team.restype = C.c_int
+    lib.emul_fft_team_k.argtypes = [C.c_uint32, C.c_uint32] + [C.POINTER(C.c_float)] * 4
+    lib.emul_fft_team_k.restype = C.c_int
     return lib
 
 
@@ -54,3 +56,26 @@ def test_team_fft_equals_oracle(emul, n):
         assert emul.emul_fft_team(n, fptr(c), fptr(d), fptr(twr), fptr(twi)) == 0
         assert_bitexact(c, a, f"re n={n}")
         assert_bitexact(d, b, f"im n={n}")
+
+
+@pytest.mark.parametrize("n", [16, 64, 512, 4096])
+@pytest.mark.parametrize("k", [2, 3])
+def test_team_fft_of_several_sequences_equals_oracle(emul, n, k):
+    """fft_team_k: K sequences through the same exchanges, each as if alone."""
+    twr = np.zeros(n // 2, np.float32)
+    twi = np.zeros(n // 2, np.float32)
+    oracle().orc_twiddles(n, fptr(twr), fptr(twi))
+    rng = np.random.default_rng(100 * k + n)
+    re = rng.standard_normal(k * n).astype(np.float32)
+    im = rng.standard_normal(k * n).astype(np.float32)
+    re[:n] = 0.0  # one sequence of signed zeros and an impulse
+    im[:n] = -0.0
+    im[n // 2 + 1] = 1.0
+    want_re, want_im = re.copy(), im.copy()
+    for j in range(k):
+        a, b = want_re[j * n:(j + 1) * n].copy(), want_im[j * n:(j + 1) * n].copy()
+        oracle().orc_fft_inverse(n, fptr(a), fptr(b))
+        want_re[j * n:(j + 1) * n], want_im[j * n:(j + 1) * n] = a, b
+    assert emul.emul_fft_team_k(n, k, fptr(re), fptr(im), fptr(twr), fptr(twi)) == 0
+    assert_bitexact(re, want_re, f"re n={n} k={k}")
+    assert_bitexact(im, want_im, f"im n={n} k={k}")

@@ -23,6 +23,17 @@
 
 namespace oceanamd {
 
+// Streaming store of the row pass's fields: whole rows, read next by the
+// column pass mostly on other XCDs, so keeping the lines in this XCD's L2
+// only displaces the constant tables the next frame reads again. (The column
+// pass's outputs are C-column strips of each row, partial lines that need the
+// L2 to merge them, so they are stored normally.)
+typedef float nt_f32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_stream(float2* p, float x, float y)
+{
+    __builtin_nontemporal_store(nt_f32x2{x, y}, reinterpret_cast<nt_f32x2*>(p));
+}
+
 struct BlockSync {
     __device__ void operator()() const { __syncthreads(); }
 };
@@ -123,8 +134,37 @@ __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk
         fft_team<N>(fr[f], fi[f], idx, J, x_team, tw, BlockSync{});
         float2* dst = spec + size_t(f) * CELLS + size_t(z) * N;
 #pragma unroll
-        for (int a = 0; a < 8; ++a) dst[idx[a]] = make_float2(fr[f][a], fi[f][a]);
+        for (int a = 0; a < 8; ++a) store_stream(dst + idx[a], fr[f][a], fi[f][a]);
     }
+}
+
+// The finalize of one cell from its eight real fields (h, Dx, Dz, dh/dx,
+// dh/dz, dDx/dx, dDz/dz, dDx/dz): reference src/core/evolve.cpp:256-325.
+__device__ __forceinline__ void pack_cell(float h, float d_x, float d_z, float hx, float hz,
+                                          float jxx, float jzz, float jxz, float lambda,
+                                          float foam_threshold, float inv_threshold,
+                                          float4* __restrict__ disp, float4* __restrict__ normal,
+                                          size_t o)
+{
+    const float disp_x = lambda * d_x;
+    const float disp_z = lambda * d_z;
+    const float axx = 1.0f + lambda * jxx;
+    const float azz = 1.0f + lambda * jzz;
+    const float axz = lambda * jxz;
+    const float jac = axx * azz - axz * axz;
+    float nx = -(hx * azz - axz * hz);
+    float ny = jac;
+    float nz = -(axx * hz - hx * axz);
+    const float len2 = nx * nx + ny * ny + nz * nz;
+    const float inv_len = (len2 > 0.0f) ? 1.0f / sqrtf(len2) : 0.0f;
+    nx *= inv_len;
+    ny *= inv_len;
+    nz *= inv_len;
+    float foam = (foam_threshold - jac) * inv_threshold;
+    if (foam < 0.0f) foam = 0.0f;
+    if (foam > 1.0f) foam = 1.0f;
+    disp[o] = make_float4(disp_x, h, disp_z, foam);
+    normal[o] = make_float4(nx, ny, nz, jac);
 }
 
 template <int N, int F, int C>
@@ -167,33 +207,11 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
     for (int f = 0; f < F; ++f)
         fft_team<N>(fr[f], fi[f], idx, J, xch + c * team_stride<N>(), tw, BlockSync{});
 
-    // reference src/core/evolve.cpp:256-325
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-        const float h = fr[0][a], d_x = fi[0][a];
-        const float d_z = fr[1][a], hx = fi[1][a];
-        const float hz = fr[2][a], jxx = fi[2][a];
-        const float jzz = fr[3][a], jxz = fi[3][a];
-        const float disp_x = lambda * d_x;
-        const float disp_z = lambda * d_z;
-        const float axx = 1.0f + lambda * jxx;
-        const float azz = 1.0f + lambda * jzz;
-        const float axz = lambda * jxz;
-        const float jac = axx * azz - axz * axz;
-        float nx = -(hx * azz - axz * hz);
-        float ny = jac;
-        float nz = -(axx * hz - hx * axz);
-        const float len2 = nx * nx + ny * ny + nz * nz;
-        const float inv_len = (len2 > 0.0f) ? 1.0f / sqrtf(len2) : 0.0f;
-        nx *= inv_len;
-        ny *= inv_len;
-        nz *= inv_len;
-        float foam = (foam_threshold - jac) * inv_threshold;
-        if (foam < 0.0f) foam = 0.0f;
-        if (foam > 1.0f) foam = 1.0f;
         const size_t o = size_t(idx[a]) * N + x;
-        disp[o] = make_float4(disp_x, h, disp_z, foam);
-        normal[o] = make_float4(nx, ny, nz, jac);
+        pack_cell(fr[0][a], fi[0][a], fr[1][a], fi[1][a], fr[2][a], fi[2][a], fr[3][a], fi[3][a],
+                  lambda, foam_threshold, inv_threshold, disp, normal, o);
         if constexpr (F == 6) {
             // reference src/core/evolve.cpp:186-203
             vel[o] = make_float4(fr[4][a], fi[4][a], fr[5][a], 0.0f);
@@ -206,9 +224,9 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
 // column): team group g transforms fields 2g and 2g + 1, so a frame has G
 // times the wavefronts, each with a third (F = 6) or half (F = 4) of the
 // serial FFT work. Every field is computed by the same expressions as above.
-// The column finalize needs fields 0-3 of a cell together: group 1 hands its
-// two fields to group 0 through LDS (`xfer`); group 2 (velocity) writes its
-// output itself. Requires T = N/8 >= 64, so a team is whole wavefronts and the
+// The column finalize needs fields 0-3 of a cell together: groups 0 and 1
+// swap half of their fields through LDS (`xfer`) and each packs half of the
+// cells; group 2 (velocity) writes its output itself. Requires T = N/8 >= 64, so a team is whole wavefronts and the
 // group branches are uniform. KX = 2 transforms a team's two fields together
 // (fft_team_k: shared index arithmetic and twiddle reads, one pair of
 // barriers per exchange for both), KX = 1 one after the other.
@@ -306,7 +324,7 @@ __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32
         for (int f = 0; f < 2; ++f) {
             float2* dst = spec + size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
 #pragma unroll
-            for (int a = 0; a < 8; ++a) dst[idx[a]] = make_float2(fr[f][a], fi[f][a]);
+            for (int a = 0; a < 8; ++a) store_stream(dst + idx[a], fr[f][a], fi[f][a]);
         }
     } else {
 #pragma unroll
@@ -315,7 +333,7 @@ __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32
             fft_team<N>(fr[f], fi[f], idx, J, x_team, tw, BlockSync{});
             float2* dst = spec + size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
 #pragma unroll
-            for (int a = 0; a < 8; ++a) dst[idx[a]] = make_float2(fr[f][a], fi[f][a]);
+            for (int a = 0; a < 8; ++a) store_stream(dst + idx[a], fr[f][a], fi[f][a]);
         }
     }
 }
@@ -370,11 +388,24 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
             fft_team<N>(fr[f], fi[f], idx, J, xch + (g * C + c) * team_stride<N>(), tw, BlockSync{});
     }
 
-    if (g == 1u) {  // fields 2 and 3 to group 0
+    // Groups 0 and 1 share the finalize: group 0 packs the cells of registers
+    // a = 0..3, group 1 those of a = 4..7. Lanes with the same (c, J) hold the
+    // same indices in both groups, and the cells of a = 0..3 and of a = 4..7
+    // are disjoint, so one LDS area per column carries both hand-overs:
+    // group 1's fields 2, 3 of a = 0..3 and group 0's fields 0, 1 of a = 4..7.
+    float2* xf0 = xfer + (c * 2u + 0u) * N;
+    float2* xf1 = xfer + (c * 2u + 1u) * N;
+    if (g == 0u) {
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            xfer[(c * 2u + 0u) * N + idx[a]] = make_float2(fr[0][a], fi[0][a]);
-            xfer[(c * 2u + 1u) * N + idx[a]] = make_float2(fr[1][a], fi[1][a]);
+        for (int a = 4; a < 8; ++a) {
+            xf0[idx[a]] = make_float2(fr[0][a], fi[0][a]);
+            xf1[idx[a]] = make_float2(fr[1][a], fi[1][a]);
+        }
+    } else if (g == 1u) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            xf0[idx[a]] = make_float2(fr[0][a], fi[0][a]);
+            xf1[idx[a]] = make_float2(fr[1][a], fi[1][a]);
         }
     }
     if constexpr (F == 6) {
@@ -386,37 +417,22 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
         }
     }
     __syncthreads();
-    if (g != 0u) return;
-
-    // reference src/core/evolve.cpp:256-325
+    if (g == 0u) {
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        const float2 v2 = xfer[(c * 2u + 0u) * N + idx[a]];
-        const float2 v3 = xfer[(c * 2u + 1u) * N + idx[a]];
-        const float h = fr[0][a], d_x = fi[0][a];
-        const float d_z = fr[1][a], hx = fi[1][a];
-        const float hz = v2.x, jxx = v2.y;
-        const float jzz = v3.x, jxz = v3.y;
-        const float disp_x = lambda * d_x;
-        const float disp_z = lambda * d_z;
-        const float axx = 1.0f + lambda * jxx;
-        const float azz = 1.0f + lambda * jzz;
-        const float axz = lambda * jxz;
-        const float jac = axx * azz - axz * axz;
-        float nx = -(hx * azz - axz * hz);
-        float ny = jac;
-        float nz = -(axx * hz - hx * axz);
-        const float len2 = nx * nx + ny * ny + nz * nz;
-        const float inv_len = (len2 > 0.0f) ? 1.0f / sqrtf(len2) : 0.0f;
-        nx *= inv_len;
-        ny *= inv_len;
-        nz *= inv_len;
-        float foam = (foam_threshold - jac) * inv_threshold;
-        if (foam < 0.0f) foam = 0.0f;
-        if (foam > 1.0f) foam = 1.0f;
-        const size_t o = size_t(idx[a]) * N + x;
-        disp[o] = make_float4(disp_x, h, disp_z, foam);
-        normal[o] = make_float4(nx, ny, nz, jac);
+        for (int a = 0; a < 4; ++a) {
+            const float2 v2 = xf0[idx[a]];
+            const float2 v3 = xf1[idx[a]];
+            pack_cell(fr[0][a], fi[0][a], fr[1][a], fi[1][a], v2.x, v2.y, v3.x, v3.y, lambda,
+                      foam_threshold, inv_threshold, disp, normal, size_t(idx[a]) * N + x);
+        }
+    } else if (g == 1u) {
+#pragma unroll
+        for (int a = 4; a < 8; ++a) {
+            const float2 v0 = xf0[idx[a]];
+            const float2 v1 = xf1[idx[a]];
+            pack_cell(v0.x, v0.y, v1.x, v1.y, fr[0][a], fi[0][a], fr[1][a], fi[1][a], lambda,
+                      foam_threshold, inv_threshold, disp, normal, size_t(idx[a]) * N + x);
+        }
     }
 }
 

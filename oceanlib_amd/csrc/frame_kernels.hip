@@ -281,6 +281,9 @@ struct BatchShape {
     static_assert(PER * FftShape<N>::T == kBlock, "batched blocks are 256 lanes");
 };
 
+// Field-split items (two teams per row or column, as in the single-level
+// split kernels) were measured slower here: the batch is throughput-bound,
+// and the split repeats the per-cell phase and sincos in both teams.
 template <int N, int F>
 __global__ __launch_bounds__(kBlock) void batch_row_kernel(const ocean_hip_level* __restrict__ levels,
                                                            const uint32_t* __restrict__ items,

@@ -114,12 +114,21 @@ __global__ void coverage_final_kernel(const double* __restrict__ partial, uint32
 
 using namespace oceanamd;
 
+namespace {
+
+// Query launches: a query is a chain of dependent taps, so a small batch (a
+// frame's few thousand points) is latency-bound; one wavefront per workgroup
+// spreads it over four times as many CUs (and their L1s) as 256-lane blocks.
+dim3 query_block(uint32_t count) { return dim3(count <= 65536u ? 64u : uint32_t(kBlock)); }
+
+}  // namespace
+
 extern "C" hipError_t ocean_hip_sample(const ocean_hip_level* L, const float* xz, uint32_t count,
                                        ocean_hip_surface* out, float* heights, hipStream_t s)
 {
     if (count == 0) return hipSuccess;
     const float inv_cell = static_cast<float>(L->n) / L->patch_length;
-    const dim3 g((count + kBlock - 1) / kBlock), b(kBlock);
+    const dim3 b = query_block(count), g((count + b.x - 1) / b.x);
     hipLaunchKernelGGL(sample_kernel, g, b, 0, s, reinterpret_cast<const float4*>(L->disp),
                        reinterpret_cast<const float4*>(L->normal),
                        reinterpret_cast<const float4*>(L->vel), L->n, inv_cell,
@@ -133,7 +142,7 @@ extern "C" hipError_t ocean_hip_cascade_sample(const ocean_hip_level* levels,
                                                float* heights, hipStream_t s)
 {
     if (count == 0) return hipSuccess;
-    const dim3 g((count + kBlock - 1) / kBlock), b(kBlock);
+    const dim3 b = query_block(count), g((count + b.x - 1) / b.x);
     hipLaunchKernelGGL(cascade_sample_kernel, g, b, 0, s, levels, level_count,
                        reinterpret_cast<const float2*>(xz), count, out, heights);
     return hipGetLastError();

@@ -51,7 +51,7 @@ __device__ __forceinline__ uint32_t row_of_team(uint32_t g)
 template <int N, int F, int TEAMS>
 __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk,
                                           double time_over_two_pi, float2* __restrict__ tw,
-                                          float2* __restrict__ xch)
+                                          float2* __restrict__ xch, uint32_t tid = threadIdx.x)
 {
     constexpr int T = FftShape<N>::T;
     constexpr int BLOCK = TEAMS * T;
@@ -65,11 +65,11 @@ __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk
     const float patch_length = L.patch_length;
 
     __syncthreads();
-    for (int i = threadIdx.x; i < N / 2; i += BLOCK) tw[i] = twg[i];
+    for (int i = tid; i < N / 2; i += BLOCK) tw[i] = twg[i];
     __syncthreads();
 
-    const uint32_t team = threadIdx.x / T;
-    const uint32_t J = threadIdx.x % T;
+    const uint32_t team = tid / T;
+    const uint32_t J = tid % T;
     const uint32_t z = row_of_team<N>(blk * TEAMS + team);
     const uint32_t mz = (uint32_t(N) - z) & uint32_t(N - 1);
 
@@ -169,7 +169,8 @@ __device__ __forceinline__ void pack_cell(float h, float d_x, float d_z, float h
 
 template <int N, int F, int C>
 __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk,
-                                          float2* __restrict__ tw, float2* __restrict__ xch)
+                                          float2* __restrict__ tw, float2* __restrict__ xch,
+                                          uint32_t tid = threadIdx.x)
 {
     constexpr int T = FftShape<N>::T;
     constexpr int BLOCK = C * T;
@@ -184,11 +185,11 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
     const float inv_threshold = L.inv_foam_threshold;
 
     __syncthreads();
-    for (int i = threadIdx.x; i < N / 2; i += BLOCK) tw[i] = twg[i];
+    for (int i = tid; i < N / 2; i += BLOCK) tw[i] = twg[i];
     __syncthreads();
 
-    const uint32_t c = threadIdx.x % C;
-    const uint32_t J = threadIdx.x / C;
+    const uint32_t c = tid % C;
+    const uint32_t J = tid / C;
     const uint32_t x = blk * C + c;
 
     float fr[F][8], fi[F][8];

@@ -28,10 +28,17 @@ namespace oceanamd {
 // only displaces the constant tables the next frame reads again. (The column
 // pass's outputs are C-column strips of each row, partial lines that need the
 // L2 to merge them, so they are stored normally.)
+// A team's lanes write T consecutive float2 per register: below T = 16
+// (N = 64 and smaller) a store covers part of a 128-byte line, and those are
+// left to the L2 to merge.
 typedef float nt_f32x2 __attribute__((ext_vector_type(2)));
+template <int N>
 __device__ __forceinline__ void store_stream(float2* p, float x, float y)
 {
-    __builtin_nontemporal_store(nt_f32x2{x, y}, reinterpret_cast<nt_f32x2*>(p));
+    if constexpr (FftShape<N>::T * 8 >= 128)
+        __builtin_nontemporal_store(nt_f32x2{x, y}, reinterpret_cast<nt_f32x2*>(p));
+    else
+        *p = make_float2(x, y);
 }
 
 struct BlockSync {
@@ -134,7 +141,7 @@ __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk
         fft_team<N>(fr[f], fi[f], idx, J, x_team, tw, BlockSync{});
         float2* dst = spec + size_t(f) * CELLS + size_t(z) * N;
 #pragma unroll
-        for (int a = 0; a < 8; ++a) store_stream(dst + idx[a], fr[f][a], fi[f][a]);
+        for (int a = 0; a < 8; ++a) store_stream<N>(dst + idx[a], fr[f][a], fi[f][a]);
     }
 }
 
@@ -325,7 +332,7 @@ __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32
         for (int f = 0; f < 2; ++f) {
             float2* dst = spec + size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
 #pragma unroll
-            for (int a = 0; a < 8; ++a) store_stream(dst + idx[a], fr[f][a], fi[f][a]);
+            for (int a = 0; a < 8; ++a) store_stream<N>(dst + idx[a], fr[f][a], fi[f][a]);
         }
     } else {
 #pragma unroll
@@ -334,7 +341,7 @@ __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32
             fft_team<N>(fr[f], fi[f], idx, J, x_team, tw, BlockSync{});
             float2* dst = spec + size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
 #pragma unroll
-            for (int a = 0; a < 8; ++a) store_stream(dst + idx[a], fr[f][a], fi[f][a]);
+            for (int a = 0; a < 8; ++a) store_stream<N>(dst + idx[a], fr[f][a], fi[f][a]);
         }
     }
 }

@@ -300,6 +300,40 @@ def learned_leg(lib, reps: int = 10):
     return out
 
 
+def handoff_leg(lib, sim, foam, n: int, reps: int = 200):
+    """The renderer handoff of the same patch: the displacement (alpha = the
+    persistent foam) and the normal as RGBA16F, into device arrays and into
+    host memory (the D2H copy included). Wall time per call over back-to-back
+    calls; the kernels' own durations are in the rocprofv3 trace."""
+    import numpy as np
+    import torch
+
+    cells = n * n
+    dd = torch.empty(4 * cells, dtype=torch.int16, device="cuda")
+    dn = torch.empty(4 * cells, dtype=torch.int16, device="cuda")
+    pd, pn = C.c_void_p(dd.data_ptr()), C.c_void_p(dn.data_ptr())
+    for _ in range(10):
+        assert lib.ocean_amd_pack_rgba16_device(sim, foam, pd, pn) == 0
+    lib.ocean_amd_synchronize(sim)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        lib.ocean_amd_pack_rgba16_device(sim, foam, pd, pn)
+    lib.ocean_amd_synchronize(sim)
+    dev_ms = (time.perf_counter() - t0) * 1e3 / reps
+    hd = np.empty(4 * cells, np.uint16)
+    hn = np.empty(4 * cells, np.uint16)
+    lib.ocean_amd_pack_rgba16(sim, foam, hd.ctypes.data, hn.ctypes.data)
+    t0 = time.perf_counter()
+    for _ in range(20):
+        lib.ocean_amd_pack_rgba16(sim, foam, hd.ctypes.data, hn.ctypes.data)
+    host_ms = (time.perf_counter() - t0) * 1e3 / 20
+    alg = cells * (16 + 4 + 16 + 16)  # disp + foam + normal read, 2 x 8 B written
+    return {"what": "displacement (alpha = persistent foam) + normal of the 512^2 patch to "
+                    "RGBA16F, the viewer's upload format",
+            "device_wall_ms": round(dev_ms, 5), "alg_bytes": alg,
+            "host_wall_ms": round(host_ms, 4), "host_bytes": cells * 16}
+
+
 def capi_mod():
     from oceanlib_amd import capi
     return capi
@@ -466,6 +500,7 @@ def main():
     lib.ocean_amd_synchronize(sim)
     qps = reps * nq / (time.perf_counter() - tq)
 
+    handoff = handoff_leg(lib, sim, foam, n)
     batched = batched_leg(lib, args.batch_frames, 5) if args.batch_frames > 0 else None
     learned = learned_leg(lib) if not args.no_learned else None
     cpu = None if args.no_cpu else cpu_baseline(desc)
@@ -499,6 +534,7 @@ def main():
         "update_device_ms": round(row_ms.value + col_ms.value, 5),
         "frame_shape": {"row_variant": shape_r.value, "col_variant": shape_c.value},
         "height_at_queries_per_s": round(qps, 1),
+        "rgba16_handoff": handoff,
         "batched": batched,
         "converged_height_at": learned,
     }

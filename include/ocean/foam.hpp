@@ -45,6 +45,8 @@ public:
     // Host mirror of the N*N coverage, refreshed from the device on request.
     [[nodiscard]] const float* data() const noexcept;
     [[nodiscard]] const float* device_data() const noexcept;
+    // The ocean this field follows (no reference counterpart).
+    [[nodiscard]] const Ocean& ocean() const noexcept;
     [[nodiscard]] std::uint32_t size() const noexcept;
     [[nodiscard]] float foam_at(float world_x, float world_z) const noexcept;
     [[nodiscard]] const FoamDesc& desc() const noexcept;

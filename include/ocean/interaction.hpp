@@ -117,6 +117,11 @@ public:
     // levels combined as a cascade) plus this field, composed as WaterSurface.
     bool sample_combined_batch(const void* device_levels, std::uint32_t level_count, bool as_cascade,
                                const float* xz, std::uint32_t count, Surface* out) const noexcept;
+    // The field's RGBA32F buffer as RGBA16F (the viewer's upload, reference
+    // examples/viewer/ocean_view.cpp:864-870): 4 halves per cell into `out`
+    // (8-byte aligned), a device array enqueued on the field's stream, or a
+    // host array filled before returning.
+    bool pack_rgba16(std::uint16_t* out, bool on_device) const noexcept;
     void synchronize() const noexcept;
     [[nodiscard]] int last_error() const noexcept;
     [[nodiscard]] detail::InteractionDevice& device() const noexcept;

@@ -87,6 +87,8 @@ struct OceanDevice;
 class CascadeStack;
 class OceanBatch;
 
+class FoamField;
+
 class Ocean {
 public:
     // Throws std::invalid_argument for a malformed desc (reference
@@ -119,6 +121,14 @@ public:
     // Batched queries over device arrays, enqueued on the ocean's stream.
     bool sample_batch_device(const float* xz_dev, std::uint32_t count, Surface* out_dev,
                              float* heights_dev) const noexcept;
+    // Renderer handoff (reference examples/viewer/half.hpp:58-130,
+    // ocean_view.cpp:818-862): RGBA16F copies of the displacement (alpha =
+    // foam's persistent coverage when foam is given, which must follow this
+    // ocean; else its own w) and of the normal, 4 halves per cell, either
+    // pointer may be null (8-byte aligned). on_device: device arrays, enqueued
+    // on the ocean's stream; else host arrays, filled before returning.
+    bool pack_rgba16(const FoamField* foam, std::uint16_t* displacement, std::uint16_t* normal,
+                     bool on_device) const noexcept;
     void synchronize() const noexcept;
     // Sticky: the first HIP error seen since construction (0 = none).
     [[nodiscard]] int last_error() const noexcept;

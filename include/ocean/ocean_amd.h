@@ -270,6 +270,33 @@ int  ocean_amd_proposal_pairs(const ocean_amd_batch* batch, uint32_t first, cons
                               uint32_t count, float tol, float* feats_dev, float* targets_dev,
                               uint8_t* ok_dev, int32_t* spacing_dev);
 
+/* ---- renderer handoff: RGBA16F textures -----------------------------------
+ * The viewer's upload format (reference examples/viewer/half.hpp:58-130,
+ * examples/viewer/ocean_view.cpp:818-870, ARCHITECTURE.md ADR-026), made on
+ * the device: 4 halves per cell, the viewer's float_to_half (round to nearest
+ * even, half subnormals, overflow to infinity, NaN to the quiet NaN
+ * sign | 0x7E00). The displacement texture's alpha is the persistent foam of
+ * `foam` (which must follow this ocean) when foam is non-NULL, else the
+ * displacement's own w; the normal is converted as is. Either output may be
+ * NULL; outputs are 8-byte aligned arrays of 4*N*N uint16.
+ *   _device : device arrays (e.g. memory a renderer exported and HIP
+ *             imported), enqueued on the ocean's stream; returns at once.
+ *   host    : host arrays (e.g. a mapped staging buffer); blocks until
+ *             written. Half the bytes of ocean_get_buffers.               */
+int ocean_amd_pack_rgba16(const ocean_sim* sim, const ocean_amd_foam* foam,
+                          uint16_t* displacement, uint16_t* normal);
+int ocean_amd_pack_rgba16_device(const ocean_sim* sim, const ocean_amd_foam* foam,
+                                 uint16_t* displacement_dev, uint16_t* normal_dev);
+/* Any device array of `count` floats (16-byte aligned) to halves (8-byte
+ * aligned), the same conversion, enqueued on the ocean's stream: for other
+ * RGBA32F data a renderer uploads, e.g. ocean_get_velocity's. */
+int ocean_amd_pack_half_device(const ocean_sim* sim, const float* src_dev, uint16_t* dst_dev,
+                               uint64_t count);
+/* The interaction field's RGBA buffer (ocean_interaction_get_buffers) the same
+ * way, on the field's own stream (_device) or into host memory. */
+int ocean_amd_interaction_pack_rgba16(const ocean_interaction* field, uint16_t* out);
+int ocean_amd_interaction_pack_rgba16_device(const ocean_interaction* field, uint16_t* out_dev);
+
 #if defined(__cplusplus)
 }
 #endif

@@ -1102,3 +1102,57 @@ int ocean_amd_proposal_pairs(const ocean_amd_batch* batch, uint32_t first, const
 }
 
 }  // extern "C"
+
+// ---- renderer handoff --------------------------------------------------------
+
+extern "C" {
+
+int ocean_amd_pack_rgba16(const ocean_sim* sim, const ocean_amd_foam* foam,
+                          uint16_t* displacement, uint16_t* normal)
+{
+    if (sim == nullptr || (displacement == nullptr && normal == nullptr))
+        return OCEAN_ERROR_INVALID_ARG;
+    return to_cpp(sim)->pack_rgba16(to_foam(foam), displacement, normal, false)
+               ? OCEAN_OK
+               : OCEAN_ERROR_INVALID_ARG;
+}
+
+int ocean_amd_pack_rgba16_device(const ocean_sim* sim, const ocean_amd_foam* foam,
+                                 uint16_t* displacement_dev, uint16_t* normal_dev)
+{
+    if (sim == nullptr || (displacement_dev == nullptr && normal_dev == nullptr))
+        return OCEAN_ERROR_INVALID_ARG;
+    return to_cpp(sim)->pack_rgba16(to_foam(foam), displacement_dev, normal_dev, true)
+               ? OCEAN_OK
+               : OCEAN_ERROR_INVALID_ARG;
+}
+
+int ocean_amd_pack_half_device(const ocean_sim* sim, const float* src_dev, uint16_t* dst_dev,
+                               uint64_t count)
+{
+    if (sim == nullptr || src_dev == nullptr || dst_dev == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    if (reinterpret_cast<std::uintptr_t>(src_dev) % 16 != 0 ||
+        reinterpret_cast<std::uintptr_t>(dst_dev) % 8 != 0)
+        return OCEAN_ERROR_INVALID_ARG;
+    if (count == 0) return OCEAN_OK;
+    auto& dev = to_cpp(sim)->device();
+    dev.make_current();
+    const hipError_t e = ocean_hip_pack_half(src_dev, dst_dev, count, dev.stream);
+    if (e == hipErrorInvalidValue) return OCEAN_ERROR_INVALID_ARG;
+    dev.note(e);
+    return dev.status == 0 ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
+}
+
+int ocean_amd_interaction_pack_rgba16(const ocean_interaction* field, uint16_t* out)
+{
+    if (field == nullptr || out == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    return to_cfield(field)->pack_rgba16(out, false) ? OCEAN_OK : OCEAN_ERROR_INVALID_ARG;
+}
+
+int ocean_amd_interaction_pack_rgba16_device(const ocean_interaction* field, uint16_t* out_dev)
+{
+    if (field == nullptr || out_dev == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    return to_cfield(field)->pack_rgba16(out_dev, true) ? OCEAN_OK : OCEAN_ERROR_INVALID_ARG;
+}
+
+}  // extern "C"

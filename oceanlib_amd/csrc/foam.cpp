@@ -163,6 +163,7 @@ const float* FoamField::data() const noexcept
 }
 
 const float* FoamField::device_data() const noexcept { return impl_->cur; }
+const Ocean& FoamField::ocean() const noexcept { return *impl_->ocean; }
 std::uint32_t FoamField::size() const noexcept { return impl_->n; }
 const FoamDesc& FoamField::desc() const noexcept { return impl_->desc; }
 std::uint32_t FoamField::last_substeps() const noexcept { return impl_->substeps_last; }

@@ -86,6 +86,14 @@ hipError_t ocean_hip_frame_timed(const ocean_hip_level* level, double time_over_
  * variants exist; -1 if n is not a supported size. */
 int ocean_hip_frame_shapes(uint32_t n, uint32_t fields, int* row_variants, int* col_variants);
 
+/* RGBA16F handoff (reference examples/viewer/half.hpp:58-130): cells RGBA32F
+ * texels of src to 4*cells halves in dst, the alpha from alpha[cell] when
+ * alpha is non-NULL; src 16-byte and dst 8-byte aligned. _half: count floats
+ * to count halves (src 16-byte, dst 8-byte aligned). */
+hipError_t ocean_hip_pack_rgba16(const float* src, const float* alpha, uint16_t* dst,
+                                 uint32_t cells, hipStream_t s);
+hipError_t ocean_hip_pack_half(const float* src, uint16_t* dst, uint64_t count, hipStream_t s);
+
 /* Batched sample_at over `count` (x, z) pairs (device float2). `out` and/or
  * `heights` may be NULL. */
 hipError_t ocean_hip_sample(const ocean_hip_level* level, const float* xz, uint32_t count,

@@ -94,6 +94,7 @@ struct InteractionField::Impl {
     float* d_qxz = nullptr;
     float* d_qout = nullptr;
     double* d_energy = nullptr;  // 2*cells scratch + 1 result
+    void* d_pack = nullptr;      // RGBA16F staging for host-array packs (on first use)
     std::size_t stamp_cap = 0;
 
     // pinned host
@@ -122,6 +123,7 @@ struct InteractionField::Impl {
     {
         if (dev.stream) hipStreamSynchronize(dev.stream);
         if (arena) hipFree(arena);
+        if (d_pack) hipFree(d_pack);
         if (h_out) hipHostFree(h_out);
         if (h_stamps) hipHostFree(h_stamps);
         if (h_qxz) hipHostFree(h_qxz);
@@ -500,6 +502,29 @@ void InteractionField::set_obstruction(const std::uint8_t* m8) noexcept
     std::memcpy(m.mask.data(), m8, m.cells);
     m.has_mask = true;
     m.rebuild_mask_tables();
+}
+
+bool InteractionField::pack_rgba16(std::uint16_t* out, bool on_device) const noexcept
+{
+    if (out == nullptr || reinterpret_cast<std::uintptr_t>(out) % 8 != 0) return false;
+    Impl& m = *impl_;
+    m.dev.make_current();
+    const std::uint32_t cells = m.n * m.n;
+    const std::size_t bytes = std::size_t(cells) * 8;
+    std::uint16_t* dst = out;
+    if (!on_device) {
+        if (m.d_pack == nullptr && hipMalloc(&m.d_pack, bytes) != hipSuccess) {
+            m.d_pack = nullptr;
+            return false;
+        }
+        dst = static_cast<std::uint16_t*>(m.d_pack);
+    }
+    m.dev.note(ocean_hip_pack_rgba16(m.out, nullptr, dst, cells, m.dev.stream));
+    if (!on_device) {
+        m.dev.note(hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, m.dev.stream));
+        m.dev.note(hipStreamSynchronize(m.dev.stream));
+    }
+    return m.dev.status == 0;
 }
 
 bool InteractionField::sample_batch(const float* xz, std::uint32_t count,

@@ -5,6 +5,7 @@
 // and allocation-free. The difference is that the frame is two HIP kernels
 // enqueued on the ocean's stream (ocean_hip_frame) and the outputs live in
 // device memory, copied to a pinned host mirror only when buffers() is asked.
+#include "ocean/foam.hpp"
 #include "ocean/ocean.hpp"
 
 #include "ocean_device.hpp"
@@ -128,6 +129,7 @@ OceanDevice::~OceanDevice()
     if (stream) hipStreamSynchronize(stream);
     if (arena) hipFree(arena);
     if (q_stage) hipFree(q_stage);
+    if (pack_stage) hipFree(pack_stage);
     if (h_disp) hipHostFree(h_disp);
     if (h_normal) hipHostFree(h_normal);
     if (h_vel) hipHostFree(h_vel);
@@ -420,6 +422,40 @@ bool Ocean::sample_batch_device(const float* xz_dev, std::uint32_t count, Surfac
                                      reinterpret_cast<ocean_hip_surface*>(out_dev), heights_dev,
                                      impl_->dev.stream));
     return impl_->dev.status == 0;
+}
+
+bool Ocean::pack_rgba16(const FoamField* foam, std::uint16_t* displacement, std::uint16_t* normal,
+                        bool on_device) const noexcept
+{
+    if (displacement == nullptr && normal == nullptr) return false;
+    if (foam != nullptr && &foam->ocean() != this) return false;
+    auto aligned = [](const void* p) { return reinterpret_cast<std::uintptr_t>(p) % 8 == 0; };
+    if (!aligned(displacement) || !aligned(normal)) return false;
+    detail::OceanDevice& d = impl_->dev;
+    d.make_current();
+    const std::uint32_t cells = d.level.n * d.level.n;
+    const std::size_t bytes = std::size_t(cells) * 8;
+    const float* alpha = foam != nullptr ? foam->device_data() : nullptr;
+    std::uint16_t* dd = displacement;
+    std::uint16_t* dn = normal;
+    if (!on_device) {
+        if (d.pack_stage == nullptr && hipMalloc(&d.pack_stage, 2 * bytes) != hipSuccess) {
+            d.pack_stage = nullptr;
+            return false;
+        }
+        dd = static_cast<std::uint16_t*>(d.pack_stage);
+        dn = dd + std::size_t(cells) * 4;
+    }
+    if (displacement != nullptr)
+        d.note(ocean_hip_pack_rgba16(d.level.disp, alpha, dd, cells, d.stream));
+    if (normal != nullptr) d.note(ocean_hip_pack_rgba16(d.level.normal, nullptr, dn, cells, d.stream));
+    if (!on_device) {
+        if (displacement != nullptr)
+            d.note(hipMemcpyAsync(displacement, dd, bytes, hipMemcpyDeviceToHost, d.stream));
+        if (normal != nullptr) d.note(hipMemcpyAsync(normal, dn, bytes, hipMemcpyDeviceToHost, d.stream));
+        d.note(hipStreamSynchronize(d.stream));
+    }
+    return d.status == 0;
 }
 
 void Ocean::synchronize() const noexcept

@@ -42,6 +42,8 @@ struct OceanDevice {
     float* q_xz = nullptr;     // device query staging
     ocean_hip_surface* q_out = nullptr;
 
+    void* pack_stage = nullptr;  // lazily allocated RGBA16F staging (host-array packs)
+
     float* h_disp = nullptr;   // pinned host mirrors (hipHostMalloc)
     float* h_normal = nullptr;
     float* h_vel = nullptr;

@@ -280,3 +280,26 @@ OREF void oref_height_batch(const ocean_sim* s, const float* xz, std::uint32_t c
     const Ocean& o = *reinterpret_cast<const Ocean*>(s);
     for (std::uint32_t i = 0; i < count; ++i) heights[i] = o.height_at(xz[2 * i], xz[2 * i + 1]);
 }
+
+// ---- the viewer's half conversion (examples/viewer/half.hpp, header-only;
+// its scalar path, which the header proves equal to its F16C path) ---------
+
+#include "../examples/viewer/half.hpp"
+
+OREF std::uint16_t oref_float_to_half(float v) { return viewer::float_to_half(v); }
+
+OREF void oref_half_range(std::uint32_t first, std::uint32_t count, std::uint16_t* out)
+{
+    for (std::uint32_t i = 0; i < count; ++i) {
+        const std::uint32_t bits = first + i;
+        float f;
+        std::memcpy(&f, &bits, sizeof f);
+        out[i] = viewer::float_to_half(f);
+    }
+}
+
+OREF void oref_pack_rgba16(std::uint16_t* dst, const float* src, const float* alpha,
+                           std::size_t cells)
+{
+    viewer::pack_rgba16_scalar(dst, src, alpha, cells);
+}

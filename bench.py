@@ -413,6 +413,15 @@ def main():
     dt = np.float32(1.0 / 60.0)
 
     def frame(k):
+        # the foam step and the frame's queries share one launch
+        lib.ocean_update(sim, k / 60.0)
+        j = k % frames_xz
+        if lib.ocean_amd_foam_update_sample_device(foam, dt, xz_ptr[j], QUERIES_PER_FRAME, None,
+                                                   h_ptr[j]) != 0:
+            raise SystemExit("foam step / queries failed")
+
+    def frame_separate(k):
+        # the reference's call sequence: update, foam update, queries
         lib.ocean_update(sim, k / 60.0)
         lib.ocean_amd_foam_update(foam, dt)
         j = k % frames_xz
@@ -459,6 +468,16 @@ def main():
             rec["batched"] = batched_leg(lib, args.batch_frames, 2, tune=not args.no_autotune)
         print(json.dumps(rec), flush=True)
         return
+
+    # The same frames with the three separate calls, for comparison.
+    for k in range(total, total + args.warmup):
+        frame_separate(k)
+    lib.ocean_amd_synchronize(sim)
+    t0 = time.perf_counter()
+    for k in range(total + args.warmup, total + args.warmup + args.steps):
+        frame_separate(k)
+    lib.ocean_amd_synchronize(sim)
+    separate_per_s = args.steps / (time.perf_counter() - t0)
 
     # The same kernels launched back to back with nothing between them.
     iso_r, iso_c = C.c_float(), C.c_float()
@@ -525,13 +544,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": "single heavy-chop patch: N=512, 250 m, 22 m/s, chop 1.3, "
                                "velocity + persistent advected foam, depth 50 m, seed 7, "
-                               "dt 1/60 s, 4096 height_at queries per frame",
+                               "dt 1/60 s, 4096 height_at queries per frame (foam step and "
+                               "queries in one launch)",
                    "size": n, "patch_length": 250.0, "queries_per_frame": QUERIES_PER_FRAME,
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
         "update_device_ms": round(row_ms.value + col_ms.value, 5),
+        "frames_per_s_separate_calls": round(separate_per_s, 1),
         "frame_shape": {"row_variant": shape_r.value, "col_variant": shape_c.value},
         "height_at_queries_per_s": round(qps, 1),
         "rgba16_handoff": handoff,

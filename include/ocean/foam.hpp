@@ -40,6 +40,11 @@ public:
     FoamField& operator=(const FoamField&) = delete;
 
     void update(float dt) noexcept;
+    // update(dt) followed by the ocean's sample_batch_device on the same
+    // frame, with the last foam substep and the queries in one launch (no
+    // reference counterpart; the same results as the two calls).
+    bool update_and_sample(float dt, const float* xz_dev, std::uint32_t count, Surface* out_dev,
+                           float* heights_dev) noexcept;
     void clear() noexcept;
 
     // Host mirror of the N*N coverage, refreshed from the device on request.

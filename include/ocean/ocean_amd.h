@@ -133,6 +133,13 @@ ocean_amd_foam* ocean_amd_foam_create(const ocean_sim* sim, const ocean_amd_foam
                                       ocean_status* out_status);
 void            ocean_amd_foam_destroy(ocean_amd_foam* foam);
 void            ocean_amd_foam_update(ocean_amd_foam* foam, float dt);
+/* ocean_amd_foam_update(foam, dt) then ocean_amd_sample_batch_device on the
+ * foam's ocean, with the last foam substep and the queries in one launch: the
+ * same results as the two calls, one kernel boundary fewer per frame. */
+int             ocean_amd_foam_update_sample_device(ocean_amd_foam* foam, float dt,
+                                                    const float* xz_device, uint32_t count,
+                                                    ocean_surface_ex* out_device,
+                                                    float* heights_device);
 void            ocean_amd_foam_clear(ocean_amd_foam* foam);
 /* N*N floats: host mirror (copied on request) / device pointer. */
 const float*    ocean_amd_foam_data(const ocean_amd_foam* foam);

@@ -209,6 +209,7 @@ AMD_SIGNATURES = {
     "ocean_amd_foam_clear": (None, [_P]),
     "ocean_amd_foam_data": (_F, [_P]),
     "ocean_amd_foam_device_data": (C.c_void_p, [_P]),
+    "ocean_amd_foam_update_sample_device": (C.c_int, [_P, C.c_float, _P, C.c_uint32, _P, _P]),
     # renderer handoff (RGBA16F); outputs as void* (host or device arrays of uint16)
     "ocean_amd_pack_rgba16": (C.c_int, [_P, _P, _P, _P]),
     "ocean_amd_pack_rgba16_device": (C.c_int, [_P, _P, _P, _P]),

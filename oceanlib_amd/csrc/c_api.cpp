@@ -504,6 +504,17 @@ void ocean_amd_foam_update(ocean_amd_foam* foam, float dt)
 {
     if (foam != nullptr) to_foam(foam)->update(dt);
 }
+int ocean_amd_foam_update_sample_device(ocean_amd_foam* foam, float dt, const float* xz_device,
+                                        uint32_t count, ocean_surface_ex* out_device,
+                                        float* heights_device)
+{
+    if (foam == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    return to_foam(foam)->update_and_sample(dt, xz_device, count,
+                                            reinterpret_cast<ocean::Surface*>(out_device),
+                                            heights_device)
+               ? OCEAN_OK
+               : OCEAN_ERROR_INVALID_ARG;
+}
 void ocean_amd_foam_clear(ocean_amd_foam* foam)
 {
     if (foam != nullptr) to_foam(foam)->clear();

@@ -31,6 +31,8 @@ void validate(const FoamDesc& d)
 }  // namespace
 
 struct FoamField::Impl {
+    std::uint32_t plan(float dt, ocean_hip_foam_params& p) noexcept;
+
     const Ocean* ocean = nullptr;
     FoamDesc desc;
     std::uint32_t n = 0;
@@ -98,10 +100,54 @@ FoamField::~FoamField() = default;
 FoamField::FoamField(FoamField&&) noexcept = default;
 FoamField& FoamField::operator=(FoamField&&) noexcept = default;
 
-// reference src/foam.cpp:218-252 (same accumulator arithmetic in double)
 void FoamField::update(float dt) noexcept
 {
     Impl& m = *impl_;
+    ocean_hip_foam_params p;
+    const std::uint32_t steps = m.plan(dt, p);
+    if (steps == 0) return;
+    auto& od = m.ocean->device();
+    for (std::uint32_t s = 0; s < steps; ++s) {
+        od.note(ocean_hip_foam_step(&od.level, m.cur, m.next, &p, od.stream));
+        std::swap(m.cur, m.next);
+    }
+    m.stale = true;
+}
+
+bool FoamField::update_and_sample(float dt, const float* xz_dev, std::uint32_t count,
+                                  Surface* out_dev, float* heights_dev) noexcept
+{
+    if (count != 0 && (xz_dev == nullptr || (out_dev == nullptr && heights_dev == nullptr)))
+        return false;
+    Impl& m = *impl_;
+    ocean_hip_foam_params p;
+    const std::uint32_t steps = m.plan(dt, p);
+    auto& od = m.ocean->device();
+    if (steps == 0 || count == 0) {
+        for (std::uint32_t s = 0; s < steps; ++s) {
+            od.note(ocean_hip_foam_step(&od.level, m.cur, m.next, &p, od.stream));
+            std::swap(m.cur, m.next);
+        }
+        if (steps != 0) m.stale = true;
+        return count == 0 || m.ocean->sample_batch_device(xz_dev, count, out_dev, heights_dev);
+    }
+    for (std::uint32_t s = 0; s + 1 < steps; ++s) {
+        od.note(ocean_hip_foam_step(&od.level, m.cur, m.next, &p, od.stream));
+        std::swap(m.cur, m.next);
+    }
+    od.note(ocean_hip_foam_step_sample(&od.level, m.cur, m.next, &p, xz_dev, count,
+                                       reinterpret_cast<ocean_hip_surface*>(out_dev), heights_dev,
+                                       od.stream));
+    std::swap(m.cur, m.next);
+    m.stale = true;
+    return od.status == 0;
+}
+
+// reference src/foam.cpp:218-252 (same accumulator arithmetic in double): the
+// substeps `dt` asks for, with the accumulator advanced, and their constants.
+std::uint32_t FoamField::Impl::plan(float dt, ocean_hip_foam_params& p) noexcept
+{
+    Impl& m = *this;
     if (!(dt > 0.0f)) dt = 0.0f;
     m.accum += static_cast<double>(dt);
     const double step = static_cast<double>(m.desc.fixed_dt);
@@ -116,14 +162,13 @@ void FoamField::update(float dt) noexcept
         if (m.accum < 0.0) m.accum = 0.0;
     }
     m.substeps_last = steps;
-    if (steps == 0) return;
+    if (steps == 0) return 0;
 
     auto& od = m.ocean->device();
-    if (od.level.vel == nullptr) return;
+    if (od.level.vel == nullptr) return 0;
     od.make_current();
     // Per-step constants, computed on the host exactly as foam.cpp:94-100.
     const float fdt = m.desc.fixed_dt;
-    ocean_hip_foam_params p;
     p.dt = fdt;
     p.inv_cell = 1.0f / m.cell;
     p.decay = std::exp(-m.desc.decay * fdt);
@@ -131,11 +176,7 @@ void FoamField::update(float dt) noexcept
     p.scale = m.desc.advect_scale;
     p.drift_x = m.desc.wind_drift_x;
     p.drift_z = m.desc.wind_drift_z;
-    for (std::uint32_t s = 0; s < steps; ++s) {
-        od.note(ocean_hip_foam_step(&od.level, m.cur, m.next, &p, od.stream));
-        std::swap(m.cur, m.next);
-    }
-    m.stale = true;
+    return steps;
 }
 
 void FoamField::clear() noexcept

@@ -102,6 +102,12 @@ hipError_t ocean_hip_sample(const ocean_hip_level* level, const float* xz, uint3
 /* One semi-Lagrangian foam step: cur -> next (N*N floats each). */
 hipError_t ocean_hip_foam_step(const ocean_hip_level* level, const float* cur, float* next,
                                const ocean_hip_foam_params* p, hipStream_t s);
+/* One foam step and `count` queries of the same frame (ocean_hip_sample's
+ * arguments) in one launch; the same arithmetic as the two separate calls. */
+hipError_t ocean_hip_foam_step_sample(const ocean_hip_level* level, const float* cur, float* next,
+                                      const ocean_hip_foam_params* p, const float* xz,
+                                      uint32_t count, ocean_hip_surface* out, float* heights,
+                                      hipStream_t s);
 
 /* Cascade query: per point, each level's own inversion and sample, combined
  * in level order. `levels` is a device array of `level_count` descriptors. */

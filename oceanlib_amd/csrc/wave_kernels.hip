@@ -17,18 +17,28 @@ namespace {
 
 constexpr int kBlock = 256;
 
+__device__ __forceinline__ void sample_point(uint32_t q, const float4* __restrict__ disp,
+                                             const float4* __restrict__ normal,
+                                             const float4* __restrict__ vel, uint32_t n,
+                                             float inv_cell, const float2* __restrict__ xz,
+                                             uint32_t count, ocean_hip_surface* __restrict__ out,
+                                             float* __restrict__ heights)
+{
+    if (q >= count) return;
+    const float2 p = xz[q];
+    const ocean_hip_surface s = sample_level(disp, normal, vel, n, inv_cell, p.x, p.y);
+    if (heights != nullptr) heights[q] = s.height;
+    if (out != nullptr) out[q] = s;
+}
+
 __global__ __launch_bounds__(kBlock) void sample_kernel(
     const float4* __restrict__ disp, const float4* __restrict__ normal,
     const float4* __restrict__ vel, uint32_t n, float inv_cell,
     const float2* __restrict__ xz, uint32_t count, ocean_hip_surface* __restrict__ out,
     float* __restrict__ heights)
 {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= count) return;
-    const float2 p = xz[q];
-    const ocean_hip_surface s = sample_level(disp, normal, vel, n, inv_cell, p.x, p.y);
-    if (heights != nullptr) heights[q] = s.height;
-    if (out != nullptr) out[q] = s;
+    sample_point(blockIdx.x * blockDim.x + threadIdx.x, disp, normal, vel, n, inv_cell, xz, count,
+                 out, heights);
 }
 
 __global__ __launch_bounds__(kBlock) void cascade_sample_kernel(
@@ -45,12 +55,12 @@ __global__ __launch_bounds__(kBlock) void cascade_sample_kernel(
 }
 
 // reference src/foam.cpp:89-139
-__global__ __launch_bounds__(kBlock) void foam_step_kernel(
-    const float4* __restrict__ disp, const float4* __restrict__ vel,
-    const float* __restrict__ cur, float* __restrict__ next, uint32_t n,
-    ocean_hip_foam_params p)
+__device__ __forceinline__ void foam_step_cell(uint32_t i, const float4* __restrict__ disp,
+                                               const float4* __restrict__ vel,
+                                               const float* __restrict__ cur,
+                                               float* __restrict__ next, uint32_t n,
+                                               const ocean_hip_foam_params& p)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * n) return;
     const uint32_t x = i % n, z = i / n;
     const float4 w = vel[i];
@@ -64,6 +74,32 @@ __global__ __launch_bounds__(kBlock) void foam_step_kernel(
     if (f > 1.0f) f = 1.0f;
     if (f < 0.0f) f = 0.0f;
     next[i] = f;
+}
+
+__global__ __launch_bounds__(kBlock) void foam_step_kernel(
+    const float4* __restrict__ disp, const float4* __restrict__ vel,
+    const float* __restrict__ cur, float* __restrict__ next, uint32_t n,
+    ocean_hip_foam_params p)
+{
+    foam_step_cell(blockIdx.x * blockDim.x + threadIdx.x, disp, vel, cur, next, n, p);
+}
+
+// One foam step and a batch of queries of the same frame in one launch: the
+// first `qblocks` workgroups answer the queries (their dependent tap chains
+// start first), the rest step the foam. Neither reads what the other writes.
+__global__ __launch_bounds__(kBlock) void foam_step_sample_kernel(
+    const float4* __restrict__ disp, const float4* __restrict__ normal,
+    const float4* __restrict__ vel, uint32_t n, float inv_cell, const float* __restrict__ cur,
+    float* __restrict__ next, ocean_hip_foam_params p, const float2* __restrict__ xz,
+    uint32_t count, ocean_hip_surface* __restrict__ out, float* __restrict__ heights,
+    uint32_t qblocks)
+{
+    if (blockIdx.x < qblocks)
+        sample_point(blockIdx.x * blockDim.x + threadIdx.x, disp, normal, vel, n, inv_cell, xz,
+                     count, out, heights);
+    else
+        foam_step_cell((blockIdx.x - qblocks) * blockDim.x + threadIdx.x, disp, vel, cur, next, n,
+                       p);
 }
 
 // FoamField::foam_at: reference src/foam.cpp:291-305.
@@ -173,6 +209,25 @@ extern "C" hipError_t ocean_hip_coverage(const float* v, uint32_t n, double* par
     const uint32_t blocks = ocean_hip_coverage_blocks(n);
     hipLaunchKernelGGL(coverage_partial_kernel, dim3(blocks), dim3(kBlock), 0, s, v, n, partial);
     hipLaunchKernelGGL(coverage_final_kernel, dim3(1), dim3(64), 0, s, partial, blocks, n, out);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ocean_hip_foam_step_sample(const ocean_hip_level* L, const float* cur,
+                                                 float* next, const ocean_hip_foam_params* p,
+                                                 const float* xz, uint32_t count,
+                                                 ocean_hip_surface* out, float* heights,
+                                                 hipStream_t s)
+{
+    const uint32_t cells = L->n * L->n;
+    const uint32_t b = 64;  // one wavefront per workgroup: the queries' latency chains spread
+    const uint32_t qblocks = (count + b - 1) / b;
+    const uint32_t fblocks = (cells + b - 1) / b;
+    const float inv_cell = static_cast<float>(L->n) / L->patch_length;
+    hipLaunchKernelGGL(foam_step_sample_kernel, dim3(qblocks + fblocks), dim3(b), 0, s,
+                       reinterpret_cast<const float4*>(L->disp),
+                       reinterpret_cast<const float4*>(L->normal),
+                       reinterpret_cast<const float4*>(L->vel), L->n, inv_cell, cur, next, *p,
+                       reinterpret_cast<const float2*>(xz), count, out, heights, qblocks);
     return hipGetLastError();
 }
 

@@ -205,3 +205,55 @@ def test_cascade_rejects_empty_and_bad_levels(lib):
     arr = (capi.OceanDesc * 2)(*levels)
     assert not lib.ocean_amd_cascade_create(arr, 2, C.byref(st))
     assert st.value == capi.OCEAN_ERROR_INVALID_ARG
+
+
+def test_fused_foam_step_and_queries_equal_the_two_calls(lib):
+    """ocean_amd_foam_update_sample_device (the last foam substep and the
+    frame's queries in one launch) gives the same foam field and the same
+    query results, bit for bit, as ocean_amd_foam_update followed by
+    ocean_amd_sample_batch_device, with 0, 1 and several substeps per call."""
+    from oceanlib_amd.hipmem import DeviceArray
+    desc = desc_from(CONFIG2)
+    n = desc.size
+    st = C.c_int(-1)
+    sims = [lib.ocean_create(C.byref(desc), C.byref(st)) for _ in range(2)]
+    assert all(sims)
+    fd = capi.FoamDesc()
+    lib.ocean_amd_foam_desc_init(C.byref(fd))
+    foams = [lib.ocean_amd_foam_create(s, C.byref(fd), C.byref(st)) for s in sims]
+    assert all(foams)
+    q = 3000
+    xz = DeviceArray.from_numpy(query_points(1, q, desc.patch_length).reshape(-1))
+    outs = [DeviceArray((q, 10), np.float32) for _ in range(2)]
+    hs = [DeviceArray((q,), np.float32) for _ in range(2)]
+    dts = [1 / 60, 1 / 60, 0.0, 3 / 60, 1 / 120, 1 / 120, 1 / 60]
+    try:
+        t = 0.0
+        for k in range(24):
+            dt = np.float32(dts[k % len(dts)])
+            t += float(dt)
+            for s in sims:
+                lib.ocean_update(s, t)
+            lib.ocean_amd_foam_update(foams[0], dt)
+            assert lib.ocean_amd_sample_batch_device(sims[0], xz.ptr, q, outs[0].ptr, hs[0].ptr) == 0
+            assert lib.ocean_amd_foam_update_sample_device(foams[1], dt, xz.ptr, q, outs[1].ptr,
+                                                           hs[1].ptr) == 0
+            assert (lib.ocean_amd_foam_last_substeps(foams[0]) ==
+                    lib.ocean_amd_foam_last_substeps(foams[1]))
+            if k % 4 == 3:
+                f0 = as_array(lib.ocean_amd_foam_data(foams[0]), n * n)
+                f1 = as_array(lib.ocean_amd_foam_data(foams[1]), n * n)
+                assert_bitexact(f1, f0, f"foam after frame {k}")
+                assert_bitexact(outs[1].numpy().reshape(-1), outs[0].numpy().reshape(-1),
+                                f"surfaces frame {k}")
+                assert_bitexact(hs[1].numpy(), hs[0].numpy(), f"heights frame {k}")
+        for s in sims:
+            lib.ocean_amd_synchronize(s)
+            assert lib.ocean_amd_last_error(s) == 0
+        assert lib.ocean_amd_foam_update_sample_device(foams[1], np.float32(0.01), None, 5,
+                                                       None, None) != 0
+    finally:
+        for f in foams:
+            lib.ocean_amd_foam_destroy(f)
+        for s in sims:
+            lib.ocean_destroy(s)

@@ -67,21 +67,27 @@ def frame_bytes(n: int, fields: int, velocity: bool) -> int:
 def traffic_from_profiles(kernels):
     """HBM bytes per launch (summed over `kernels`) measured with rocprofv3
     --pmc FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for
-    a 256 MiB copy (profiles/traffic_pmc.md); None if not recorded."""
+    a 256 MiB copy (profiles/traffic_pmc.md); None if not recorded. An entry
+    may be a tuple of names of the same pass (the first one recorded counts)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             rec = json.load(f)["kernels"]
-        return float(sum(rec[k]["traffic_bytes"] for k in kernels))
-    except (OSError, KeyError, ValueError, TypeError):
+        total = 0.0
+        for k in kernels:
+            names = k if isinstance(k, tuple) else (k,)
+            hit = next(n for n in names if n in rec)
+            total += rec[hit]["traffic_bytes"]
+        return total
+    except (OSError, KeyError, ValueError, TypeError, StopIteration):
         return None
 
 
-def cpu_baseline(desc, frames_budget_s: float = 12.0):
+def cpu_baseline(desc, frames_budget_s: float = 12.0, threads: int | None = None):
     """The reference library (oracle/_ref) on this host, same frames."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    cores = max(1, min(16, os.cpu_count() or 1))
+    cores = threads or max(1, min(16, os.cpu_count() or 1))
     kind = "reference"
     try:
         from oracle_ffi import ref, default_foam_params
@@ -178,7 +184,8 @@ def batched_leg(lib, frames: int, warmup: int, tune: bool = True):
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5),
                      "traffic": traffic_from_profiles(
-                         [f"batch_{p}_kernel<{g['size']}, {6 if g['velocity'] else 4}>"
+                         [tuple(f"batch_{p}{v}_kernel<{g['size']}, {6 if g['velocity'] else 4}>"
+                                for v in ("", "_item"))
                           for g in groups for p in ("row", "col")])},
         "groups": groups, "build_s": round(build_s, 2), "frames": frames,
     }
@@ -523,6 +530,9 @@ def main():
     batched = batched_leg(lib, args.batch_frames, 5) if args.batch_frames > 0 else None
     learned = learned_leg(lib) if not args.no_learned else None
     cpu = None if args.no_cpu else cpu_baseline(desc)
+    if cpu is not None:
+        one = cpu_baseline(desc, 4.0, threads=1)
+        cpu["value_1_thread"] = one["value"] if one else None
     if batched is not None and not args.no_cpu:
         batched["cpu_baseline"] = batched_cpu_baseline()
 

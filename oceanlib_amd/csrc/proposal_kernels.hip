@@ -482,16 +482,19 @@ __global__ __launch_bounds__(256) void proposal_kernel(const ocean_hip_level* __
 // kLevels levels from prop (cells) and falls back to the query point.
 template <bool LEARNED>
 __global__ __launch_bounds__(256) void converged_kernel(
-    const ocean_hip_level* __restrict__ levels, uint32_t level_count, const float2* __restrict__ xz,
-    uint32_t count, float tol, const float* __restrict__ prop, float* __restrict__ heights,
-    uint8_t* __restrict__ iters, uint8_t* __restrict__ flags)
+    const ocean_hip_level* __restrict__ levels, uint32_t level_count, uint32_t lv_begin,
+    uint32_t lv_end, const float2* __restrict__ xz, uint32_t count, float tol,
+    const float* __restrict__ prop, float* __restrict__ heights, uint8_t* __restrict__ iters,
+    uint8_t* __restrict__ flags)
 {
     const uint32_t q = blockIdx.x * 256u + threadIdx.x;
     if (q >= count) return;
     const float2 p = xz[q];
-    float height = 0.0f;
-    uint8_t fl = 0;
-    for (uint32_t lv = 0; lv < level_count; ++lv) {
+    // Levels [lv_begin, lv_end) of level_count; a later range continues the
+    // earlier ranges' sum and flags (the same float additions in level order).
+    float height = lv_begin == 0 ? 0.0f : heights[q];
+    uint8_t fl = (lv_begin == 0 || flags == nullptr) ? uint8_t(0) : flags[q];
+    for (uint32_t lv = lv_begin; lv < lv_end; ++lv) {
         const ocean_hip_level L = levels[lv];
         const Base b = base_cell(p.x, p.y, L);
         const float inv_cell = float(L.n) / L.patch_length;
@@ -840,11 +843,16 @@ extern "C" hipError_t ocean_hip_height_converged(const ocean_hip_level* levels, 
         if (spacing == nullptr || prop == nullptr) return hipErrorInvalidValue;
         hipLaunchKernelGGL(proposal_kernel, grid, dim3(256), kInferSmem, s, levels, level_count, p,
                            count, mlp->bf16, mlp->master, spacing, prop);
-        hipLaunchKernelGGL(converged_kernel<true>, grid, dim3(256), 0, s, levels, level_count, p, count,
-                           tol, prop, heights, iters, flags);
-    } else {
-        hipLaunchKernelGGL(converged_kernel<false>, grid, dim3(256), 0, s, levels, level_count, p, count,
-                           tol, nullptr, heights, iters, flags);
+    }
+    // One launch per level: all queries tap one level's 4 MB (at 512^2) at a
+    // time, which an XCD's L2 holds, instead of every level's at once.
+    for (uint32_t lv = 0; lv < level_count; ++lv) {
+        if (mlp != nullptr)
+            hipLaunchKernelGGL(converged_kernel<true>, grid, dim3(256), 0, s, levels, level_count, lv,
+                               lv + 1, p, count, tol, prop, heights, iters, flags);
+        else
+            hipLaunchKernelGGL(converged_kernel<false>, grid, dim3(256), 0, s, levels, level_count,
+                               lv, lv + 1, p, count, tol, nullptr, heights, iters, flags);
     }
     return hipGetLastError();
 }

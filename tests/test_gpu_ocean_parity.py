@@ -214,3 +214,35 @@ def test_every_launch_shape_is_bitexact(lib, n):
         assert_bitexact(nr, want[1], f"n={n} tuned normal")
     finally:
         g.close()
+
+
+def test_frame_timing_inside_a_loop(lib):
+    """ocean_amd_frame_timing: the timed frames' dispatches record their own
+    events and compute the same bits; every k-th update is sampled, the ring
+    keeps the last 64, reading clears it."""
+    d = capi.default_desc(lib, size=256, patch_length=200.0, wind_speed=12.0, seed=1337)
+    g, o = GpuOcean(lib, d), OracleOcean(d)
+    try:
+        r, c, k = C.c_float(), C.c_float(), C.c_uint32()
+        assert lib.ocean_amd_frame_timing(g.h, 3) == 0
+        for i in range(10):
+            g.update(i / 60.0)
+        assert lib.ocean_amd_frame_timing_read(g.h, C.byref(r), C.byref(c), C.byref(k)) == 0
+        assert k.value == 4 and r.value > 0 and c.value > 0  # frames 0, 3, 6, 9
+        o.update(9 / 60.0)
+        dd, nr, _ = g.buffers()
+        assert_bitexact(dd, o.displacement(), "timed frame displacement")
+        assert_bitexact(nr, o.normal(), "timed frame normal")
+        assert lib.ocean_amd_frame_timing_read(g.h, C.byref(r), C.byref(c), C.byref(k)) == 0
+        assert k.value == 0
+        lib.ocean_amd_frame_timing(g.h, 1)
+        for i in range(70):
+            g.update(i / 60.0)
+        assert lib.ocean_amd_frame_timing_read(g.h, C.byref(r), C.byref(c), C.byref(k)) == 0
+        assert k.value == 64
+        assert lib.ocean_amd_frame_timing(g.h, 0) == 0
+        g.update(1.0)
+        assert lib.ocean_amd_frame_timing_read(g.h, C.byref(r), C.byref(c), C.byref(k)) == 0
+        assert k.value == 0
+    finally:
+        g.close()

@@ -179,3 +179,30 @@ def test_interaction_field_pack(lib):
         np.testing.assert_array_equal(dev.numpy(), want)  # numpy() waits for the device
     finally:
         g.close()
+
+
+def test_pack_of_batch_levels(lib):
+    """The levels of a batch are oceans too: their packs equal the oracle's
+    pack of their buffers."""
+    descs = [desc_from(dict(size=n, patch_length=L, wind_speed=11.0, seed=s))
+             for n, L, s in ((128, 90.0, 5), (256, 300.0, 6))]
+    arr = (capi.OceanDesc * 2)(*descs)
+    st = C.c_int(-1)
+    b = lib.ocean_amd_batch_create(arr, 2, C.byref(st))
+    assert b, st.value
+    try:
+        lib.ocean_amd_batch_update(b, 7.5)
+        for i, d in enumerate(descs):
+            lvl = lib.ocean_amd_batch_level(b, i)
+            n = d.size
+            bufs = lib.ocean_get_buffers(lvl)
+            disp = np.ctypeslib.as_array(bufs.displacement, shape=(4 * n * n,)).copy()
+            nrm = np.ctypeslib.as_array(bufs.normal, shape=(4 * n * n,)).copy()
+            got_d = np.zeros(4 * n * n, np.uint16)
+            got_n = np.zeros(4 * n * n, np.uint16)
+            assert lib.ocean_amd_pack_rgba16(lvl, None, got_d.ctypes.data, got_n.ctypes.data) == 0
+            np.testing.assert_array_equal(got_d, orc_pack(disp, None))
+            np.testing.assert_array_equal(got_n, orc_pack(nrm, None))
+    finally:
+        assert lib.ocean_amd_batch_synchronize(b) == 0
+        lib.ocean_amd_batch_destroy(b)

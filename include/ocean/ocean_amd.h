@@ -12,6 +12,12 @@
  * Status returns: 0 on success, otherwise an ocean_status (ocean.h) or, for
  * a device failure, OCEAN_ERROR_UNKNOWN with the HIP error available from
  * ocean_amd_last_error().
+ *
+ * Threads: the entry points that take host arrays stage them through
+ * buffers the ocean (or batch, or field) owns, so on one handle they must not
+ * run concurrently with each other; device-array entry points only enqueue
+ * on the handle's stream. As with the reference, nothing runs concurrently
+ * with an update of the same handle.
  */
 #ifndef OCEANLIB_OCEAN_AMD_H
 #define OCEANLIB_OCEAN_AMD_H

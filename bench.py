@@ -361,6 +361,8 @@ def main():
     ap.add_argument("--frame-shape", default=None, metavar="ROW,COL",
                     help="fix the frame kernels' launch shapes instead of autotuning "
                          "(counter runs, where timing is perturbed)")
+    ap.add_argument("--queries-only", action="store_true",
+                    help="run only the converged / learned height query leg (for rocprofv3)")
     ap.add_argument("--profile-only", action="store_true",
                     help="run only warmup + timed frames (for rocprofv3 runs)")
     args = ap.parse_args()
@@ -378,6 +380,9 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a HIP device")
     lib = capi.load()
+    if args.queries_only:
+        print(json.dumps({"converged_height_at": learned_leg(lib)}), flush=True)
+        return
     desc = desc_from(CONFIG2)
     n = desc.size
     st = C.c_int(-1)

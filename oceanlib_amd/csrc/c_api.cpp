@@ -509,11 +509,11 @@ int ocean_amd_foam_update_sample_device(ocean_amd_foam* foam, float dt, const fl
                                         float* heights_device)
 {
     if (foam == nullptr) return OCEAN_ERROR_INVALID_ARG;
-    return to_foam(foam)->update_and_sample(dt, xz_device, count,
-                                            reinterpret_cast<ocean::Surface*>(out_device),
-                                            heights_device)
-               ? OCEAN_OK
-               : OCEAN_ERROR_INVALID_ARG;
+    ocean::FoamField* f = to_foam(foam);
+    if (f->update_and_sample(dt, xz_device, count, reinterpret_cast<ocean::Surface*>(out_device),
+                             heights_device))
+        return OCEAN_OK;
+    return f->ocean().last_error() != 0 ? OCEAN_ERROR_UNKNOWN : OCEAN_ERROR_INVALID_ARG;
 }
 void ocean_amd_foam_clear(ocean_amd_foam* foam)
 {
@@ -1123,9 +1123,8 @@ int ocean_amd_pack_rgba16(const ocean_sim* sim, const ocean_amd_foam* foam,
 {
     if (sim == nullptr || (displacement == nullptr && normal == nullptr))
         return OCEAN_ERROR_INVALID_ARG;
-    return to_cpp(sim)->pack_rgba16(to_foam(foam), displacement, normal, false)
-               ? OCEAN_OK
-               : OCEAN_ERROR_INVALID_ARG;
+    if (to_cpp(sim)->pack_rgba16(to_foam(foam), displacement, normal, false)) return OCEAN_OK;
+    return to_cpp(sim)->last_error() != 0 ? OCEAN_ERROR_UNKNOWN : OCEAN_ERROR_INVALID_ARG;
 }
 
 int ocean_amd_pack_rgba16_device(const ocean_sim* sim, const ocean_amd_foam* foam,
@@ -1133,9 +1132,8 @@ int ocean_amd_pack_rgba16_device(const ocean_sim* sim, const ocean_amd_foam* foa
 {
     if (sim == nullptr || (displacement_dev == nullptr && normal_dev == nullptr))
         return OCEAN_ERROR_INVALID_ARG;
-    return to_cpp(sim)->pack_rgba16(to_foam(foam), displacement_dev, normal_dev, true)
-               ? OCEAN_OK
-               : OCEAN_ERROR_INVALID_ARG;
+    if (to_cpp(sim)->pack_rgba16(to_foam(foam), displacement_dev, normal_dev, true)) return OCEAN_OK;
+    return to_cpp(sim)->last_error() != 0 ? OCEAN_ERROR_UNKNOWN : OCEAN_ERROR_INVALID_ARG;
 }
 
 int ocean_amd_pack_half_device(const ocean_sim* sim, const float* src_dev, uint16_t* dst_dev,
@@ -1157,13 +1155,15 @@ int ocean_amd_pack_half_device(const ocean_sim* sim, const float* src_dev, uint1
 int ocean_amd_interaction_pack_rgba16(const ocean_interaction* field, uint16_t* out)
 {
     if (field == nullptr || out == nullptr) return OCEAN_ERROR_INVALID_ARG;
-    return to_cfield(field)->pack_rgba16(out, false) ? OCEAN_OK : OCEAN_ERROR_INVALID_ARG;
+    if (to_cfield(field)->pack_rgba16(out, false)) return OCEAN_OK;
+    return to_cfield(field)->last_error() != 0 ? OCEAN_ERROR_UNKNOWN : OCEAN_ERROR_INVALID_ARG;
 }
 
 int ocean_amd_interaction_pack_rgba16_device(const ocean_interaction* field, uint16_t* out_dev)
 {
     if (field == nullptr || out_dev == nullptr) return OCEAN_ERROR_INVALID_ARG;
-    return to_cfield(field)->pack_rgba16(out_dev, true) ? OCEAN_OK : OCEAN_ERROR_INVALID_ARG;
+    if (to_cfield(field)->pack_rgba16(out_dev, true)) return OCEAN_OK;
+    return to_cfield(field)->last_error() != 0 ? OCEAN_ERROR_UNKNOWN : OCEAN_ERROR_INVALID_ARG;
 }
 
 }  // extern "C"

@@ -77,8 +77,9 @@ int ocean_amd_frame_timing_read(ocean_sim* sim, float* row_ms, float* col_ms, ui
  * spread the fields over several teams per row / column). _set_frame_shape
  * selects one;
  * _autotune_frame times every variant of each pass on the ocean's own stream
- * (reps launches each, HIP events), keeps the fastest, leaves the ocean at
- * its current time and returns the choice and the times. */
+ * inside reps whole frames (the pass's dispatches record their own events;
+ * reps <= 256), keeps the fastest, leaves the ocean at its current time and
+ * returns the choice and the times. */
 int ocean_amd_frame_shapes(const ocean_sim* sim, uint32_t* row_variants, uint32_t* col_variants);
 int ocean_amd_set_frame_shape(ocean_sim* sim, uint32_t row_variant, uint32_t col_variant);
 int ocean_amd_autotune_frame(ocean_sim* sim, uint32_t reps, uint32_t* row_variant,

@@ -317,44 +317,59 @@ int ocean_amd_set_frame_shape(ocean_sim* sim, uint32_t row_variant, uint32_t col
 int ocean_amd_autotune_frame(ocean_sim* sim, uint32_t reps, uint32_t* row_variant,
                              uint32_t* col_variant, float* row_ms, float* col_ms)
 {
-    if (sim == nullptr || reps == 0) return OCEAN_ERROR_INVALID_ARG;
+    if (sim == nullptr || reps == 0 || reps > 256) return OCEAN_ERROR_INVALID_ARG;
     ocean::Ocean& o = *to_cpp(sim);
     auto& dev = o.device();
     int nr = 0, nc = 0;
     if (ocean_hip_frame_shapes(dev.level.n, uint32_t(dev.fields), &nr, &nc) != 0)
         return OCEAN_ERROR_INVALID_ARG;
     dev.make_current();
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess) return OCEAN_ERROR_UNKNOWN;
-    if (hipEventCreate(&e1) != hipSuccess) {
-        hipEventDestroy(e0);
+    // Each candidate is timed inside whole frames (row pass then column pass,
+    // the other pass at its current pick), by the dispatches' own events, so
+    // it sees the caches a frame leaves: the column pass reads fields the
+    // row pass just wrote on other XCDs, the row pass follows a column pass.
+    std::vector<hipEvent_t> ev(std::size_t(reps) * 4, nullptr);
+    bool ok = true;
+    for (auto& e : ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+    auto release = [&] {
+        for (auto& e : ev)
+            if (e) hipEventDestroy(e);
+    };
+    if (!ok) {
+        release();
         return OCEAN_ERROR_UNKNOWN;
     }
     const double t2 = o.time() * 0.159154943091895335769;
     const uint32_t saved = dev.level.shape;
     float best[2] = {1e30f, 1e30f};
     uint32_t pick[2] = {saved & 0xffu, (saved >> 8) & 0xffu};
-    for (unsigned pass = 0; pass < 2; ++pass) {
+    for (unsigned pass = 0; pass < 2 && ok; ++pass) {
         const int count = pass == 0 ? nr : nc;
-        for (int v = 0; v < count; ++v) {
+        for (int v = 0; v < count && ok; ++v) {
             dev.level.shape = pass == 0 ? (uint32_t(v) | (pick[1] << 8)) : (pick[0] | (uint32_t(v) << 8));
-            dev.note(ocean_hip_frame_passes(&dev.level, t2, dev.stream, 1u << pass));  // warm
-            hipEventRecord(e0, dev.stream);
+            dev.note(ocean_hip_frame(&dev.level, t2, dev.stream));  // warm
             for (uint32_t i = 0; i < reps; ++i)
-                dev.note(ocean_hip_frame_passes(&dev.level, t2, dev.stream, 1u << pass));
-            hipEventRecord(e1, dev.stream);
-            hipEventSynchronize(e1);
-            float ms = 0.0f;
-            hipEventElapsedTime(&ms, e0, e1);
-            ms /= float(reps);
-            if (ms < best[pass]) {
+                dev.note(ocean_hip_frame_timed(&dev.level, t2, dev.stream, &ev[std::size_t(i) * 4]));
+            ok = hipStreamSynchronize(dev.stream) == hipSuccess;
+            double sum = 0.0;
+            for (uint32_t i = 0; i < reps && ok; ++i) {
+                float ms = 0.0f;
+                hipEvent_t* e = &ev[std::size_t(i) * 4 + 2 * pass];
+                ok = hipEventElapsedTime(&ms, e[0], e[1]) == hipSuccess;
+                sum += ms;
+            }
+            const float ms = float(sum / reps);
+            if (ok && ms < best[pass]) {
                 best[pass] = ms;
                 pick[pass] = uint32_t(v);
             }
         }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    release();
+    if (!ok) {
+        dev.level.shape = saved;
+        return OCEAN_ERROR_UNKNOWN;
+    }
     dev.level.shape = pick[0] | (pick[1] << 8);
     dev.note(hipMemcpyAsync(dev.d_level, &dev.level, sizeof(dev.level), hipMemcpyHostToDevice, dev.stream));
     dev.note(ocean_hip_frame(&dev.level, t2, dev.stream));  // a complete frame at the current time

@@ -1,8 +1,9 @@
 // The reference's behavioural tests of the per-frame path, restated against
 // this library's C++ surface (ocean::Ocean, CascadeStack, FoamField) and its
-// C ABI, on the GPU. Each case names the reference test it follows
-// (tests/<file>.cpp:<lines>) and checks the same property at the same
-// tolerance. Two contracts differ by design and the cases say so where they
+// C ABI, on the GPU; interaction_cases.cpp does the same for the interaction
+// field, and this file's main runs both tables. Each case names the reference
+// test it follows (tests/<file>.cpp:<lines>) and checks the same property at
+// the same tolerance. Two contracts differ by design and the cases say so where they
 // meet them:
 //   - the host scheduler hook and thread_count are accepted and never used
 //     (the frame runs on the GPU), so where the reference counts the hook's
@@ -17,8 +18,10 @@
 //
 // Usage: reference_cases [--list | name-prefix]. Prints "RUN <case>" before
 // and "PASS <case>" or "FAIL <case>: <reason>" after each case, and
-// "INFO <case>: ..." for measured values; exits 1 if any case failed. Built and run by
-// tests/test_gpu_reference_cases.py.
+// "INFO <case>: ..." for measured values; exits 1 if any case failed. Built
+// and run by tests/test_gpu_reference_cases.py.
+#include "harness.hpp"
+
 #include "ocean/cascade.hpp"
 #include "ocean/foam.hpp"
 #include "ocean/ocean.h"
@@ -28,7 +31,6 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
-#include <cstdarg>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -39,60 +41,15 @@
 #include <vector>
 
 using namespace ocean;
+using namespace cases;
 
 namespace {
 
-struct CaseFailed {
-    std::string why;
-};
-
-std::string g_case;
-
-[[noreturn]] void fail(const char* fmt, ...)
-{
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    std::vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    throw CaseFailed{buf};
-}
-
-#define EXPECT(cond, ...)                                                            \
-    do {                                                                             \
-        if (!(cond)) fail("line %d: %s", __LINE__, #cond);                          \
-    } while (0)
-
-void info(const char* fmt, ...)
-{
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    std::vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    std::printf("INFO %s: %s\n", g_case.c_str(), buf);
-}
-
 std::size_t texels(const Buffers& b) { return std::size_t(b.size) * b.size * 4; }
-
-std::vector<float> copy_of(const float* p, std::size_t count) { return {p, p + count}; }
-
-bool same_bits(const float* a, const float* b, std::size_t count)
-{
-    return std::memcmp(a, b, count * sizeof(float)) == 0;
-}
 
 bool same_surface(const Surface& a, const Surface& b)
 {
     return std::memcmp(&a, &b, sizeof(Surface)) == 0;
-}
-
-double rel_close(double a, double b) { return std::fabs(a - b) / std::max(std::fabs(b), 1e-300); }
-
-// |a - b| <= eps (1 + max(|a|, |b|)): the reference tests' Approx(b).epsilon(eps)
-bool approx(double a, double b, double eps)
-{
-    return std::fabs(a - b) <= eps * (1.0 + std::max(std::fabs(a), std::fabs(b)));
 }
 
 double rms_height(const Buffers& b)
@@ -921,32 +878,6 @@ void foam_saturating_clamp()  // tests/test_foam.cpp:236-257
     EXPECT(foam.coverage() > 0.5);
 }
 
-std::atomic<int> g_hook_calls{0};
-
-void counting_hook(void*, TaskFn task, void* ctx, std::uint32_t count)
-{
-    g_hook_calls.fetch_add(1);
-    for (std::uint32_t i = 0; i < count; ++i) task(ctx, i);
-}
-
-void reverse_hook(void*, TaskFn task, void* ctx, std::uint32_t count)
-{
-    g_hook_calls.fetch_add(1);
-    for (std::uint32_t i = count; i-- > 0;) task(ctx, i);
-}
-
-void scattering_hook(void*, TaskFn task, void* ctx, std::uint32_t count)
-{
-    g_hook_calls.fetch_add(1);
-    std::atomic<std::uint32_t> next{0};
-    std::vector<std::thread> workers;
-    for (int w = 0; w < 4; ++w)
-        workers.emplace_back([&] {
-            for (std::uint32_t i; (i = next.fetch_add(1)) < count;) task(ctx, i);
-        });
-    for (auto& t : workers) t.join();
-}
-
 std::vector<float> foam_run(ParallelForFn hook, std::uint32_t threads)
 {
     Ocean sim{foam_ocean(128)};
@@ -1389,12 +1320,11 @@ void threads_repeated_stable()  // tests/test_threading.cpp:63-87
     }
 }
 
-struct Case {
-    const char* name;
-    void (*run)();
-};
+}  // namespace
 
-const Case kCases[] = {
+namespace cases {
+
+const Case kFrameCases[] = {
     {"api.buffers_aligned", api_buffers_aligned},
     {"api.constructed_at_t0", api_constructed_at_t0},
     {"api.zero_wind_flat", api_zero_wind_flat},
@@ -1455,33 +1385,39 @@ const Case kCases[] = {
     {"threads.bitidentical_thread_count", threads_bitidentical_thread_count},
     {"threads.repeated_stable", threads_repeated_stable},
 };
+const std::size_t kFrameCaseCount = sizeof(kFrameCases) / sizeof(kFrameCases[0]);
 
-}  // namespace
+}  // namespace cases
 
 int main(int argc, char** argv)
 {
     const std::string prefix = argc > 1 ? argv[1] : "";
-    if (prefix == "--list") {
-        for (const Case& c : kCases) std::printf("%s\n", c.name);
-        return 0;
-    }
+    const std::pair<const Case*, std::size_t> tables[] = {{kFrameCases, kFrameCaseCount},
+                                                          {kInteractionCases, kInteractionCaseCount}};
     int failed = 0;
-    for (const Case& c : kCases) {
-        if (std::string(c.name).rfind(prefix, 0) != 0) continue;
-        g_case = c.name;
-        std::printf("RUN %s\n", c.name);
-        std::fflush(stdout);
-        try {
-            c.run();
-            std::printf("PASS %s\n", c.name);
-        } catch (const CaseFailed& f) {
-            std::printf("FAIL %s: %s\n", c.name, f.why.c_str());
-            ++failed;
-        } catch (const std::exception& e) {
-            std::printf("FAIL %s: exception %s\n", c.name, e.what());
-            ++failed;
+    for (const auto& [table, count] : tables) {
+        for (std::size_t k = 0; k < count; ++k) {
+            const Case& c = table[k];
+            if (prefix == "--list") {
+                std::printf("%s\n", c.name);
+                continue;
+            }
+            if (std::string(c.name).rfind(prefix, 0) != 0) continue;
+            g_case = c.name;
+            std::printf("RUN %s\n", c.name);
+            std::fflush(stdout);
+            try {
+                c.run();
+                std::printf("PASS %s\n", c.name);
+            } catch (const CaseFailed& f) {
+                std::printf("FAIL %s: %s\n", c.name, f.why.c_str());
+                ++failed;
+            } catch (const std::exception& e) {
+                std::printf("FAIL %s: exception %s\n", c.name, e.what());
+                ++failed;
+            }
+            std::fflush(stdout);
         }
-        std::fflush(stdout);
     }
     return failed == 0 ? 0 : 1;
 }

@@ -1,13 +1,15 @@
 """The reference's own behavioural tests of the per-frame path, through this
 library's C++ API and C ABI on the GPU.
 
-tests/cpp_api/reference_cases.cpp restates, case by case, the reference's
-tests/test_api.cpp, test_query.cpp, test_simulation.cpp, test_velocity.cpp,
-test_foam.cpp, test_cascade.cpp, test_c_api.cpp and test_threading.cpp (the
+tests/cpp_api/reference_cases.cpp and interaction_cases.cpp restate, case by
+case, the reference's tests/test_api.cpp, test_query.cpp, test_simulation.cpp,
+test_velocity.cpp, test_foam.cpp, test_cascade.cpp, test_c_api.cpp,
+test_threading.cpp, test_interaction.cpp and test_c_interaction.cpp (the
 cases on this path; each cites its file:lines) against ocean::Ocean,
-CascadeStack, FoamField and the ocean_* functions, with the reference's
-tolerances: what a program written against the reference's headers checks
-when it is linked against this library instead. The driver is compiled with
+CascadeStack, FoamField, InteractionField, WaterSurface and the ocean_*
+functions, with the reference's tolerances: what a program written against
+the reference's headers checks when it is linked against this library
+instead. The driver is compiled with
 g++ from the public headers, the way such a program would be, and run once;
 each case is one test here.
 """
@@ -22,13 +24,16 @@ from oceanlib_amd import capi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCE = os.path.join(HERE, "cpp_api", "reference_cases.cpp")
+SOURCES = [os.path.join(HERE, "cpp_api", f) for f in ("reference_cases.cpp", "interaction_cases.cpp")]
 LIB_DIR = os.path.dirname(capi.LIB_PATH)
 
 
 def case_names():
-    with open(SOURCE) as f:
-        return re.findall(r'\{"([a-z_.0-9]+)", [a-z_0-9]+\}', f.read())
+    names = []
+    for path in SOURCES:  # the driver runs the files' case tables in this order
+        with open(path) as f:
+            names += re.findall(r'\{"([a-z_.0-9]+)", [a-z_0-9]+\}', f.read())
+    return names
 
 
 def build_driver(out_dir) -> str:
@@ -38,7 +43,7 @@ def build_driver(out_dir) -> str:
         pytest.fail(f"{capi.LIB_PATH} not built (python -c 'import __graft_entry__ as g; g.build()')")
     exe = os.path.join(str(out_dir), "reference_cases")
     subprocess.run(["g++", "-std=c++20", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
-                    SOURCE, "-L", LIB_DIR, "-loceanlib_amd", f"-Wl,-rpath,{LIB_DIR}", "-o", exe],
+                    *SOURCES, "-L", LIB_DIR, "-loceanlib_amd", f"-Wl,-rpath,{LIB_DIR}", "-o", exe],
                    check=True)
     return exe
 
@@ -56,7 +61,7 @@ def test_driver_builds_from_the_public_headers(tmp_path):
 @pytest.fixture(scope="module")
 def driver_results(tmp_path_factory):
     exe = build_driver(tmp_path_factory.mktemp("reference_cases"))
-    print("running tests/cpp_api/reference_cases.cpp on the GPU", flush=True)
+    print("running the tests/cpp_api reference cases on the GPU", flush=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
     results, last_run = {}, None
     for line in r.stdout.splitlines():

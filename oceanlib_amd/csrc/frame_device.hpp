@@ -25,13 +25,12 @@ namespace oceanamd {
 
 // Streaming store of the row pass's fields: whole rows, read next by the
 // column pass mostly on other XCDs, so keeping the lines in this XCD's L2
-// only displaces the constant tables the next frame reads again. (The column
-// pass's outputs are C-column strips of each row, partial lines that need the
-// L2 to merge them, so they are stored normally.)
+// only displaces the constant tables the next frame reads again.
 // A team's lanes write T consecutive float2 per register: below T = 16
 // (N = 64 and smaller) a store covers part of a 128-byte line, and those are
 // left to the L2 to merge.
 typedef float nt_f32x2 __attribute__((ext_vector_type(2)));
+typedef float nt_f32x4 __attribute__((ext_vector_type(4)));
 template <int N>
 __device__ __forceinline__ void store_stream(float2* p, float x, float y)
 {
@@ -39,6 +38,26 @@ __device__ __forceinline__ void store_stream(float2* p, float x, float y)
         __builtin_nontemporal_store(nt_f32x2{x, y}, reinterpret_cast<nt_f32x2*>(p));
     else
         *p = make_float2(x, y);
+}
+
+// The column pass's RGBA32F outputs: streamed in the batched passes, stored
+// normally in the single ocean's. A batch's outputs are read later, if at
+// all, and their lines in the writing XCD's L2 only displaced the row pass's
+// tables (measured A/B per group: column pass -9% to -22% at N <= 256). A
+// single ocean's are gathered right away by the foam step and the queries,
+// which then found them further away: streamed, its column pass was 1.5 us
+// faster and the whole frame with the foam step and queries in one launch
+// 1.6% slower. Where a store covers only part of a 128-byte line (4 columns
+// per item at N = 512) the streamed lines are written as partial lines
+// (WRITE_SIZE 1.27x); there only the normal is streamed, which measured
+// faster than both (-3%) or neither (-3.5%).
+template <bool STREAM>
+__device__ __forceinline__ void store_out(float4* p, float x, float y, float z, float w)
+{
+    if constexpr (STREAM)
+        __builtin_nontemporal_store(nt_f32x4{x, y, z, w}, reinterpret_cast<nt_f32x4*>(p));
+    else
+        *p = make_float4(x, y, z, w);
 }
 
 struct BlockSync {
@@ -147,6 +166,7 @@ __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk
 
 // The finalize of one cell from its eight real fields (h, Dx, Dz, dh/dx,
 // dh/dz, dDx/dx, dDz/dz, dDx/dz): reference src/core/evolve.cpp:256-325.
+template <bool STREAM_DISP, bool STREAM_NORMAL>
 __device__ __forceinline__ void pack_cell(float h, float d_x, float d_z, float hx, float hz,
                                           float jxx, float jzz, float jxz, float lambda,
                                           float foam_threshold, float inv_threshold,
@@ -170,11 +190,11 @@ __device__ __forceinline__ void pack_cell(float h, float d_x, float d_z, float h
     float foam = (foam_threshold - jac) * inv_threshold;
     if (foam < 0.0f) foam = 0.0f;
     if (foam > 1.0f) foam = 1.0f;
-    disp[o] = make_float4(disp_x, h, disp_z, foam);
-    normal[o] = make_float4(nx, ny, nz, jac);
+    store_out<STREAM_DISP>(disp + o, disp_x, h, disp_z, foam);
+    store_out<STREAM_NORMAL>(normal + o, nx, ny, nz, jac);
 }
 
-template <int N, int F, int C>
+template <int N, int F, int C, bool STREAM = false>
 __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk,
                                           float2* __restrict__ tw, float2* __restrict__ xch,
                                           uint32_t tid = threadIdx.x)
@@ -218,11 +238,12 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
         const size_t o = size_t(idx[a]) * N + x;
-        pack_cell(fr[0][a], fi[0][a], fr[1][a], fi[1][a], fr[2][a], fi[2][a], fr[3][a], fi[3][a],
-                  lambda, foam_threshold, inv_threshold, disp, normal, o);
+        pack_cell<(STREAM && C * 16 >= 128), STREAM>(fr[0][a], fi[0][a], fr[1][a], fi[1][a],
+                                                     fr[2][a], fi[2][a], fr[3][a], fi[3][a], lambda,
+                                                     foam_threshold, inv_threshold, disp, normal, o);
         if constexpr (F == 6) {
             // reference src/core/evolve.cpp:186-203
-            vel[o] = make_float4(fr[4][a], fi[4][a], fr[5][a], 0.0f);
+            store_out<STREAM>(vel + o, fr[4][a], fi[4][a], fr[5][a], 0.0f);
         }
     }
 }
@@ -421,7 +442,7 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
             // reference src/core/evolve.cpp:186-203
 #pragma unroll
             for (int a = 0; a < 8; ++a)
-                vel[size_t(idx[a]) * N + x] = make_float4(fr[0][a], fi[0][a], fr[1][a], 0.0f);
+                store_out<false>(vel + size_t(idx[a]) * N + x, fr[0][a], fi[0][a], fr[1][a], 0.0f);
         }
     }
     __syncthreads();
@@ -430,16 +451,18 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
         for (int a = 0; a < 4; ++a) {
             const float2 v2 = xf0[idx[a]];
             const float2 v3 = xf1[idx[a]];
-            pack_cell(fr[0][a], fi[0][a], fr[1][a], fi[1][a], v2.x, v2.y, v3.x, v3.y, lambda,
-                      foam_threshold, inv_threshold, disp, normal, size_t(idx[a]) * N + x);
+            pack_cell<false, false>(fr[0][a], fi[0][a], fr[1][a], fi[1][a], v2.x, v2.y, v3.x,
+                                    v3.y, lambda, foam_threshold, inv_threshold, disp, normal,
+                                    size_t(idx[a]) * N + x);
         }
     } else if (g == 1u) {
 #pragma unroll
         for (int a = 4; a < 8; ++a) {
             const float2 v0 = xf0[idx[a]];
             const float2 v1 = xf1[idx[a]];
-            pack_cell(v0.x, v0.y, v1.x, v1.y, fr[0][a], fi[0][a], fr[1][a], fi[1][a], lambda,
-                      foam_threshold, inv_threshold, disp, normal, size_t(idx[a]) * N + x);
+            pack_cell<false, false>(v0.x, v0.y, v1.x, v1.y, fr[0][a], fi[0][a], fr[1][a],
+                                    fi[1][a], lambda, foam_threshold, inv_threshold, disp, normal,
+                                    size_t(idx[a]) * N + x);
         }
     }
 }

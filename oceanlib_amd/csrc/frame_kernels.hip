@@ -318,8 +318,8 @@ __global__ __launch_bounds__(kBlock) void batch_col_kernel(const ocean_hip_level
     __shared__ float2 xch[BatchShape<N>::PER * team_stride<N>()];
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const uint32_t it = items[i];
-        col_block<N, F, BatchShape<N>::PER>(levels[it >> kItemBlockBits], it & kItemBlockMask,
-                                            tw, xch, item_lane());
+        col_block<N, F, BatchShape<N>::PER, true>(levels[it >> kItemBlockBits],
+                                                  it & kItemBlockMask, tw, xch, item_lane());
     }
 }
 
@@ -349,8 +349,8 @@ __global__ __launch_bounds__(kBlock) void batch_col_item_kernel(
     __shared__ float2 xch[BatchShape<N>::PER * team_stride<N>()];
     if (blockIdx.x >= n_items) return;
     const uint32_t it = items[blockIdx.x];
-    col_block<N, F, BatchShape<N>::PER>(levels[it >> kItemBlockBits], it & kItemBlockMask, tw,
-                                        xch);
+    col_block<N, F, BatchShape<N>::PER, true>(levels[it >> kItemBlockBits], it & kItemBlockMask,
+                                              tw, xch);
 }
 
 template <int N, int F>

@@ -367,9 +367,9 @@ def main():
                     help="run only warmup + timed frames (for rocprofv3 runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not "
-              "supported (independent oceans would shard by ocean across GPUs with no "
-              "communication; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU only; --gpus {args.gpus} is not supported "
+              "(independent oceans would shard by ocean across GPUs with no communication)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

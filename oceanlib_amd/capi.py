@@ -2,8 +2,8 @@
 
 The struct layouts are the reference's (reference include/ocean/ocean.h:35-415);
 `OceanDesc` etc. are byte-compatible with what a C caller passes. This module
-is the host-side Python binding a maintainer would use to drive the library
-(see INTEGRATION.md); the parity tests and bench.py call the library through it.
+is the host-side Python binding a maintainer would use to call the library;
+the parity tests and bench.py call the library through it.
 """
 from __future__ import annotations
 

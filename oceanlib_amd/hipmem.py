@@ -6,7 +6,7 @@ library itself links (libamdhip64.so.7, already loaded with it).
     host = dev.numpy()                          # synchronous copy back
     dev.free()
 
-Only what the training driver and the tests need: allocation, host<->device
+Only what the trainer and the tests need: allocation, host<->device
 copies and a device-wide synchronize. Copies use the null stream, which
 waits for the library's non-blocking streams only through an explicit
 synchronize (DeviceArray.numpy() calls hipDeviceSynchronize first).

@@ -1,4 +1,4 @@
-// A minimal case harness shared by the reference-case drivers: a case is a
+// A minimal case harness shared by the reference-case programs: a case is a
 // named function that throws CaseFailed on the first broken expectation.
 #pragma once
 

@@ -1,4 +1,4 @@
-"""Drivers for interaction-field runs shared by the oracle and GPU tests."""
+"""Helpers for interaction-field runs shared by the oracle and GPU tests."""
 import ctypes as C
 
 import numpy as np
@@ -22,7 +22,7 @@ def disturbance(rec) -> capi.Disturbance:
 
 
 class Field:
-    """Uniform driver over the three implementations' C-shaped interfaces.
+    """One Python interface over the three implementations' C functions.
     `kind` is 'orc' (oracle), 'oref' (reference build) or 'gpu' (library)."""
 
     def __init__(self, kind, lib, desc):

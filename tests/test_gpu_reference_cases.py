@@ -9,7 +9,7 @@ cases on this path; each cites its file:lines) against ocean::Ocean,
 CascadeStack, FoamField, InteractionField, WaterSurface and the ocean_*
 functions, with the reference's tolerances: what a program written against
 the reference's headers checks when it is linked against this library
-instead. The driver is compiled with
+instead. The case program is compiled with
 g++ from the public headers, the way such a program would be, and run once;
 each case is one test here.
 """
@@ -30,13 +30,13 @@ LIB_DIR = os.path.dirname(capi.LIB_PATH)
 
 def case_names():
     names = []
-    for path in SOURCES:  # the driver runs the files' case tables in this order
+    for path in SOURCES:  # the program runs the files' case tables in this order
         with open(path) as f:
             names += re.findall(r'\{"([a-z_.0-9]+)", [a-z_0-9]+\}', f.read())
     return names
 
 
-def build_driver(out_dir) -> str:
+def build_program(out_dir) -> str:
     if shutil.which("g++") is None:
         pytest.skip("g++ missing")
     if not os.path.exists(capi.LIB_PATH):
@@ -48,10 +48,10 @@ def build_driver(out_dir) -> str:
     return exe
 
 
-def test_driver_builds_from_the_public_headers(tmp_path):
-    """No device needed: the driver compiles against include/ocean as shipped,
+def test_case_program_builds_from_the_public_headers(tmp_path):
+    """No device needed: the program compiles against include/ocean as shipped,
     lists every case, and the validation case (no device work) passes."""
-    exe = build_driver(tmp_path)
+    exe = build_program(tmp_path)
     listed = subprocess.run([exe, "--list"], check=True, capture_output=True, text=True).stdout.split()
     assert listed == case_names() and len(listed) >= 50
     r = subprocess.run([exe, "api.bad_desc_throws"], capture_output=True, text=True, timeout=60)
@@ -59,8 +59,8 @@ def test_driver_builds_from_the_public_headers(tmp_path):
 
 
 @pytest.fixture(scope="module")
-def driver_results(tmp_path_factory):
-    exe = build_driver(tmp_path_factory.mktemp("reference_cases"))
+def case_results(tmp_path_factory):
+    exe = build_program(tmp_path_factory.mktemp("reference_cases"))
     print("running the tests/cpp_api reference cases on the GPU", flush=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
     results, last_run = {}, None
@@ -78,8 +78,8 @@ def driver_results(tmp_path_factory):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", case_names())
-def test_reference_case(driver_results, name):
-    results, rc, last_run, err = driver_results
-    assert name in results, f"no result: driver exited {rc} while running {last_run}\n{err}"
+def test_reference_case(case_results, name):
+    results, rc, last_run, err = case_results
+    assert name in results, f"no result: the program exited {rc} while running {last_run}\n{err}"
     kind, why = results[name]
     assert kind == "PASS", why

@@ -129,6 +129,14 @@ public:
     // on the ocean's stream; else host arrays, filled before returning.
     bool pack_rgba16(const FoamField* foam, std::uint16_t* displacement, std::uint16_t* normal,
                      bool on_device) const noexcept;
+    // Zero-copy outputs: the following frames write the RGBA32F outputs into
+    // caller-owned device memory (N*N float4 each, 16-byte aligned, e.g. a
+    // renderer's buffers imported into HIP), and every reader of this ocean
+    // reads them there; null keeps or restores the ocean's own buffer for
+    // that output. Re-evaluates the current frame into them. False (nothing
+    // changed) for a pointer HIP does not know as device memory of that size,
+    // velocity without compute_velocity, or a batch's or cascade's level.
+    bool bind_outputs(float* displacement, float* normal, float* velocity) noexcept;
     void synchronize() const noexcept;
     // Sticky: the first HIP error seen since construction (0 = none).
     [[nodiscard]] int last_error() const noexcept;

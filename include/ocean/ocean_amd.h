@@ -31,8 +31,9 @@ extern "C" {
 /* ---- device-resident outputs -------------------------------------------
  * Extends ocean_get_buffers (reference include/ocean/ocean.h:140-171,
  * src/c_api.cpp:180-192): the same N*N RGBA32F layout, but device pointers,
- * valid until ocean_destroy. `stream` is the hipStream_t the ocean enqueues
- * its frames on; work queued on it after ocean_update sees that frame. */
+ * valid until ocean_destroy or ocean_amd_bind_outputs. `stream` is the
+ * hipStream_t the ocean enqueues its frames on; work queued on it after
+ * ocean_update sees that frame. */
 typedef struct ocean_amd_device_buffers {
     const float* displacement;
     const float* normal;
@@ -43,6 +44,21 @@ typedef struct ocean_amd_device_buffers {
 } ocean_amd_device_buffers;
 
 int  ocean_amd_get_device_buffers(const ocean_sim* sim, ocean_amd_device_buffers* out);
+/* Zero-copy outputs for a renderer (no reference counterpart; the consumer
+ * the reference's viewer feeds by copying, examples/viewer/ocean_view.cpp:
+ * 818-870): the following frames' column pass writes displacement, normal
+ * (and velocity) straight into caller-owned device memory, e.g. a Vulkan
+ * buffer exported with VK_KHR_external_memory_fd and imported with
+ * hipImportExternalMemory + hipExternalMemoryGetMappedBuffer; the queries,
+ * the foam step, the packs, ocean_get_buffers and ocean_amd_get_device_buffers
+ * then read them there. Each pointer: N*N float4, 16-byte aligned, inside one
+ * allocation HIP knows; NULL keeps or restores the ocean's own buffer for that
+ * output. Re-evaluates the current frame into the bound buffers (enqueued).
+ * The caller keeps the memory alive until it rebinds or destroys the ocean.
+ * INVALID_ARG (binding unchanged): a pointer that fails those checks,
+ * velocity without compute_velocity, or a level of a batch or cascade. */
+int  ocean_amd_bind_outputs(ocean_sim* sim, float* displacement_dev, float* normal_dev,
+                            float* velocity_dev);
 /* Blocks until every frame and query enqueued on the ocean has finished. */
 void ocean_amd_synchronize(const ocean_sim* sim);
 /* First HIP error latched since creation (hipError_t value), 0 if none.

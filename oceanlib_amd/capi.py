@@ -184,6 +184,7 @@ SIGNATURES = {
 # include/ocean/ocean_amd.h (additive entry points).
 AMD_SIGNATURES = {
     "ocean_amd_get_device_buffers": (C.c_int, [_P, C.c_void_p]),
+    "ocean_amd_bind_outputs": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ocean_amd_synchronize": (None, [_P]),
     "ocean_amd_last_error": (C.c_int, [_P]),
     "ocean_amd_profile_frame": (C.c_int, [_P, C.c_double, C.c_double, C.c_uint32, _F, _F]),

@@ -223,6 +223,15 @@ int ocean_amd_get_device_buffers(const ocean_sim* sim, ocean_amd_device_buffers*
     return OCEAN_OK;
 }
 
+int ocean_amd_bind_outputs(ocean_sim* sim, float* displacement_dev, float* normal_dev,
+                           float* velocity_dev)
+{
+    if (sim == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    ocean::Ocean& o = *to_cpp(sim);
+    if (o.bind_outputs(displacement_dev, normal_dev, velocity_dev)) return OCEAN_OK;
+    return o.last_error() != 0 ? OCEAN_ERROR_UNKNOWN : OCEAN_ERROR_INVALID_ARG;
+}
+
 int ocean_amd_profile_frame(ocean_sim* sim, double t0, double dt, uint32_t iters, float* row_ms,
                             float* col_ms)
 {

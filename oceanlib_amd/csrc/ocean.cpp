@@ -203,6 +203,9 @@ void OceanDevice::create(const OceanDesc& desc, hipStream_t shared_stream, bool 
     level.disp = reinterpret_cast<float*>(base + o_disp);
     level.normal = reinterpret_cast<float*>(base + o_norm);
     level.vel = desc.compute_velocity ? reinterpret_cast<float*>(base + o_vel) : nullptr;
+    own_disp = level.disp;
+    own_normal = level.normal;
+    own_vel = level.vel;
     if (!lazy_host) {
         q_xz = reinterpret_cast<float*>(base + o_qxz);
         q_out = reinterpret_cast<ocean_hip_surface*>(base + o_qout);
@@ -455,6 +458,44 @@ bool Ocean::pack_rgba16(const FoamField* foam, std::uint16_t* displacement, std:
         if (normal != nullptr) d.note(hipMemcpyAsync(normal, dn, bytes, hipMemcpyDeviceToHost, d.stream));
         d.note(hipStreamSynchronize(d.stream));
     }
+    return d.status == 0;
+}
+
+namespace {
+
+// p is 16-byte aligned device memory with at least `bytes` from p to the end
+// of the allocation HIP knows it by.
+bool device_span_ok(const void* p, std::size_t bytes) noexcept
+{
+    if (reinterpret_cast<std::uintptr_t>(p) % 16 != 0) return false;
+    hipDeviceptr_t base = nullptr;
+    std::size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<void*>(p)) != hipSuccess) {
+        (void)hipGetLastError();  // a refused pointer is the caller's error, not the ocean's
+        return false;
+    }
+    const auto b = reinterpret_cast<std::uintptr_t>(base), q = reinterpret_cast<std::uintptr_t>(p);
+    return q >= b && q - b <= size && size - (q - b) >= bytes;
+}
+
+}  // namespace
+
+bool Ocean::bind_outputs(float* displacement, float* normal, float* velocity) noexcept
+{
+    detail::OceanDevice& d = impl_->dev;
+    if (!d.own_stream) return false;  // a batch's or a cascade's level: its group holds the views
+    d.make_current();
+    const std::size_t bytes = std::size_t(d.level.n) * d.level.n * 16;
+    if (velocity != nullptr && d.own_vel == nullptr) return false;
+    for (const float* p : {displacement, normal, velocity})
+        if (p != nullptr && !device_span_ok(p, bytes)) return false;
+    d.level.disp = displacement != nullptr ? displacement : d.own_disp;
+    d.level.normal = normal != nullptr ? normal : d.own_normal;
+    d.level.vel = velocity != nullptr ? velocity : d.own_vel;
+    d.note(hipMemcpyAsync(d.d_level, &d.level, sizeof(d.level), hipMemcpyHostToDevice, d.stream));
+    // The current frame again, into the buffers now bound (not a timed frame).
+    d.note(ocean_hip_frame(&d.level, impl_->time * kInvTwoPiD, d.stream));
+    d.mirror_stale = true;
     return d.status == 0;
 }
 

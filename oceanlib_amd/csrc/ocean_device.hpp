@@ -38,6 +38,11 @@ struct OceanDevice {
     int fields = 4;            // complex fields per frame: 4, or 6 with velocity
 
     void* arena = nullptr;     // one hipMalloc holds every device buffer
+    // The arena's own output buffers; level.disp / normal / vel point here
+    // unless the caller bound its own (Ocean::bind_outputs).
+    float* own_disp = nullptr;
+    float* own_normal = nullptr;
+    float* own_vel = nullptr;
     void* q_stage = nullptr;   // lazily allocated query staging (batch levels)
     float* q_xz = nullptr;     // device query staging
     ocean_hip_surface* q_out = nullptr;

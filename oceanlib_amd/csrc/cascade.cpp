@@ -1,9 +1,13 @@
-// CascadeStack on the GPU: reference src/cascade.cpp. Levels are ordinary
-// Oceans sharing the stack's stream, so update() enqueues every level's
-// frame back to back with no host synchronisation, and a query is one kernel
-// over all levels (ocean_hip_cascade_sample).
+// CascadeStack on the GPU: reference src/cascade.cpp. The levels are an
+// OceanBatch: update() steps all of them with one graph launch (one row and
+// one column launch per group of equal N, the groups overlapping) instead of
+// the reference's level-by-level loop, with the same bits per level; a query
+// is one kernel over all levels (ocean_hip_cascade_sample), also for a single
+// level, whose foam and normal then pass through the composite's own
+// arithmetic as in the reference.
 #include "ocean/cascade.hpp"
 
+#include "ocean/batch.hpp"
 #include "ocean_device.hpp"
 
 #include <hip/hip_runtime_api.h>
@@ -15,60 +19,54 @@
 
 namespace ocean {
 
+namespace {
+OceanBatch make_levels(std::span<const CascadeLevel> levels)
+{
+    if (levels.empty())
+        throw std::invalid_argument("CascadeStack: at least one level is required");
+    return OceanBatch(levels);
+}
+}  // namespace
+
 struct CascadeStack::Impl {
-    // Declared first so it is destroyed last: the levels synchronise on it.
-    struct Stream {
-        hipStream_t s = nullptr;
-        ~Stream()
-        {
-            if (s) hipStreamDestroy(s);
-        }
-    } stream;
-    std::vector<Ocean> levels;
+    OceanBatch batch;
     double time = 0.0;
-    ocean_hip_level* dev_levels = nullptr;  // device copy of every level's view
+    void* q_stage = nullptr;  // device query staging
     float* q_xz = nullptr;
     ocean_hip_surface* q_out = nullptr;
     float* h_xz = nullptr;
     ocean_hip_surface* h_out = nullptr;
 
+    explicit Impl(std::span<const CascadeLevel> levels) : batch(make_levels(levels)) {}
     ~Impl()
     {
-        if (stream.s) hipStreamSynchronize(stream.s);
-        if (dev_levels) hipFree(dev_levels);
+        batch.synchronize();
+        if (q_stage) hipFree(q_stage);
         if (h_xz) hipHostFree(h_xz);
         if (h_out) hipHostFree(h_out);
     }
+    const ocean_hip_level* views() const noexcept
+    {
+        return static_cast<const ocean_hip_level*>(batch.device_levels());
+    }
+    hipStream_t stream() const noexcept { return static_cast<hipStream_t>(batch.stream()); }
+    detail::OceanDevice& first() const noexcept { return batch.level(0).device(); }
 };
 
 CascadeStack::CascadeStack(std::span<const CascadeLevel> levels)
 {
-    if (levels.empty())
-        throw std::invalid_argument("CascadeStack: at least one level is required");
-    impl_ = std::make_unique<Impl>();
+    impl_ = std::make_unique<Impl>(levels);
     Impl& m = *impl_;
-    detail::hip_check(hipStreamCreateWithFlags(&m.stream.s, hipStreamNonBlocking),
-                      "hipStreamCreate");
-    m.levels.reserve(levels.size());
-    for (const CascadeLevel& d : levels) m.levels.push_back(Ocean(d, m.stream.s, false));
-
-    const std::size_t lv_bytes = (levels.size() * sizeof(ocean_hip_level) + 255) / 256 * 256;
     const std::size_t q = detail::kQueryStage;
-    detail::hip_check(hipMalloc(reinterpret_cast<void**>(&m.dev_levels),
-                                lv_bytes + q * 8 + q * sizeof(ocean_hip_surface)),
+    detail::hip_check(hipMalloc(&m.q_stage, q * 8 + q * sizeof(ocean_hip_surface)),
                       "hipMalloc cascade");
-    m.q_xz = reinterpret_cast<float*>(reinterpret_cast<char*>(m.dev_levels) + lv_bytes);
+    m.q_xz = static_cast<float*>(m.q_stage);
     m.q_out = reinterpret_cast<ocean_hip_surface*>(m.q_xz + 2 * q);
     detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m.h_xz), q * 8, hipHostMallocDefault),
                       "hipHostMalloc");
     detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m.h_out),
                                     q * sizeof(ocean_hip_surface), hipHostMallocDefault),
                       "hipHostMalloc");
-    std::vector<ocean_hip_level> views;
-    for (const Ocean& o : m.levels) views.push_back(o.device().level);
-    detail::hip_check(hipMemcpy(m.dev_levels, views.data(), views.size() * sizeof(ocean_hip_level),
-                                hipMemcpyHostToDevice),
-                      "upload cascade levels");
 }
 
 CascadeStack::CascadeStack(std::initializer_list<CascadeLevel> levels)
@@ -83,23 +81,26 @@ CascadeStack& CascadeStack::operator=(CascadeStack&&) noexcept = default;
 // reference src/cascade.cpp:42-55: every level to the same absolute time.
 void CascadeStack::update(double time) noexcept
 {
-    for (Ocean& o : impl_->levels) o.update(time);
+    impl_->batch.update(time);
     impl_->time = time;
 }
 
-std::size_t CascadeStack::level_count() const noexcept { return impl_->levels.size(); }
+std::size_t CascadeStack::level_count() const noexcept { return impl_->batch.level_count(); }
 Buffers CascadeStack::buffers(std::size_t level) const noexcept
 {
-    return impl_->levels[level].buffers();
+    return impl_->batch.level(level).buffers();
 }
 const OceanDesc& CascadeStack::desc(std::size_t level) const noexcept
 {
-    return impl_->levels[level].desc();
+    return impl_->batch.level(level).desc();
 }
-const Ocean& CascadeStack::level(std::size_t level) const noexcept { return impl_->levels[level]; }
+const Ocean& CascadeStack::level(std::size_t level) const noexcept
+{
+    return impl_->batch.level(level);
+}
 double CascadeStack::time() const noexcept { return impl_->time; }
-void* CascadeStack::stream() const noexcept { return impl_->stream.s; }
-const void* CascadeStack::device_levels() const noexcept { return impl_->dev_levels; }
+void* CascadeStack::stream() const noexcept { return impl_->batch.stream(); }
+const void* CascadeStack::device_levels() const noexcept { return impl_->batch.device_levels(); }
 
 float CascadeStack::height_at(float world_x, float world_z) const noexcept
 {
@@ -120,19 +121,19 @@ bool CascadeStack::sample_batch(const float* xz, std::uint32_t count, Surface* o
     if (count == 0) return true;
     if (xz == nullptr || out == nullptr) return false;
     Impl& m = *impl_;
-    auto& od = m.levels.front().device();
+    auto& od = m.first();
     od.make_current();
+    const auto levels = static_cast<std::uint32_t>(m.batch.level_count());
     for (std::uint32_t done = 0; done < count;) {
         const std::uint32_t k = std::min(detail::kQueryStage, count - done);
         std::memcpy(m.h_xz, xz + 2 * std::size_t(done), std::size_t(k) * 8);
         od.note(hipMemcpyAsync(m.q_xz, m.h_xz, std::size_t(k) * 8, hipMemcpyHostToDevice,
-                               m.stream.s));
-        od.note(ocean_hip_cascade_sample(m.dev_levels,
-                                         static_cast<std::uint32_t>(m.levels.size()), m.q_xz, k,
-                                         m.q_out, nullptr, m.stream.s));
+                               m.stream()));
+        od.note(ocean_hip_cascade_sample(m.views(), levels, m.q_xz, k, m.q_out, nullptr,
+                                         m.stream()));
         od.note(hipMemcpyAsync(m.h_out, m.q_out, std::size_t(k) * sizeof(ocean_hip_surface),
-                               hipMemcpyDeviceToHost, m.stream.s));
-        od.note(hipStreamSynchronize(m.stream.s));
+                               hipMemcpyDeviceToHost, m.stream()));
+        od.note(hipStreamSynchronize(m.stream()));
         std::memcpy(out + done, m.h_out, std::size_t(k) * sizeof(ocean_hip_surface));
         done += k;
     }
@@ -145,11 +146,11 @@ bool CascadeStack::sample_batch_device(const float* xz_dev, std::uint32_t count,
     if (count == 0) return true;
     if (xz_dev == nullptr || (out_dev == nullptr && heights_dev == nullptr)) return false;
     Impl& m = *impl_;
-    auto& od = m.levels.front().device();
+    auto& od = m.first();
     od.make_current();
-    od.note(ocean_hip_cascade_sample(m.dev_levels, static_cast<std::uint32_t>(m.levels.size()),
+    od.note(ocean_hip_cascade_sample(m.views(), static_cast<std::uint32_t>(m.batch.level_count()),
                                      xz_dev, count, reinterpret_cast<ocean_hip_surface*>(out_dev),
-                                     heights_dev, m.stream.s));
+                                     heights_dev, m.stream()));
     return od.status == 0;
 }
 

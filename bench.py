@@ -307,6 +307,127 @@ def learned_leg(lib, reps: int = 10):
     return out
 
 
+def scene_inputs(lib, frames: int):
+    """The three-cascade + interaction scene: cascade levels (256^2 of
+    800 / 150 / 25 m, 15 m/s, seed 42), the interaction descriptor (256^2 at
+    0.1 m, kernel radius 6), the static hull mask and, per frame, the eight
+    moving sources' disturbance records as C structs."""
+    import numpy as np
+    from oceanlib_amd.workloads import (CONFIG3_INTERACTION, CONFIG3_LEVELS, config3_disturbances,
+                                        config3_hull_mask, config3_sources, desc_from)
+    cm = capi_mod()
+    levels = (cm.OceanDesc * len(CONFIG3_LEVELS))(*[desc_from(kw) for kw in CONFIG3_LEVELS])
+    idesc = cm.InteractionDesc()
+    lib.ocean_interaction_desc_init(C.byref(idesc))
+    for k, v in CONFIG3_INTERACTION.items():
+        setattr(idesc, k, v)
+    srcs = config3_sources()
+    recs = []
+    for k in range(frames):
+        row = []
+        for r in config3_disturbances(srcs, k / 60.0):
+            d = cm.Disturbance()
+            for f, v in r.items():
+                setattr(d, f, v)
+            row.append(d)
+        recs.append(row)
+    return levels, idesc, np.ascontiguousarray(config3_hull_mask(), np.uint8), recs
+
+
+def scene_leg(lib, frames: int, warmup: int):
+    """Three cascades plus interaction on the device: a step is the cascade
+    update (three 256^2 levels in one batched launch per pass) and the
+    interaction update with that frame's eight sources (stamps, hull, one
+    leapfrog substep, RGBA finalize). Wall time of `frames` steps between two
+    device synchronizes, and of each half alone."""
+    import numpy as np
+    from oceanlib_amd import hipmem
+    levels, idesc, mask, recs = scene_inputs(lib, warmup + frames)
+    st = C.c_int(-1)
+    cas = lib.ocean_amd_cascade_create(levels, len(levels), C.byref(st))
+    field = lib.ocean_interaction_create(C.byref(idesc), C.byref(st))
+    if not cas or not field:
+        raise SystemExit(f"scene create failed: {st.value}")
+    lib.ocean_interaction_set_obstruction(field, mask.ctypes.data_as(C.POINTER(C.c_uint8)))
+    dt = np.float32(1.0 / 60.0)
+
+    def run(first, count, do_cascade, do_field):
+        for k in range(first, first + count):
+            if do_cascade:
+                lib.ocean_amd_cascade_update(cas, k / 60.0)
+            if do_field:
+                for d in recs[k]:
+                    lib.ocean_interaction_add(field, C.byref(d))
+                lib.ocean_interaction_update(field, dt)
+
+    out = {}
+    for name, c, f in (("scene", True, True), ("cascade_only", True, False),
+                       ("interaction_only", False, True)):
+        run(0, warmup, c, f)
+        hipmem.synchronize()
+        t0 = time.perf_counter()
+        run(warmup, frames, c, f)
+        hipmem.synchronize()
+        out[name] = frames / (time.perf_counter() - t0)
+    lib.ocean_interaction_destroy(field)
+    lib.ocean_amd_cascade_destroy(cas)
+    return {
+        "workload": "three cascades plus interaction: 3 x 256^2 levels (800/150/25 m, 15 m/s, "
+                    "seed 42) and a 256^2 iWave grid at 0.1 m, kernel radius 6, 8 continuous "
+                    "sources on circles, one static hull, 1/60 s steps",
+        "frames": frames,
+        "steps_per_s": round(out["scene"], 1),
+        "ms_per_step": round(1000.0 / out["scene"], 4),
+        "cascade_updates_per_s": round(out["cascade_only"], 1),
+        "interaction_updates_per_s": round(out["interaction_only"], 1),
+    }
+
+
+def scene_cpu_baseline(budget_s: float = 8.0):
+    """The reference library's CascadeStack and InteractionField on the same
+    scene, on this host's cores, for a bounded sample of steps."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    try:
+        from oracle_ffi import ref
+        r = ref()
+    except OSError:
+        r = None
+    if r is None:
+        return None
+    lib = capi_mod().load()
+    cores = max(1, min(16, os.cpu_count() or 1))
+    levels, idesc, mask, recs = scene_inputs(lib, 1200)
+    for d in levels:
+        d.thread_count = cores
+    idesc.thread_count = cores
+    cas = r.oref_cascade_create(levels, len(levels))
+    st = C.c_int(-1)
+    field = r.oref_interaction_create(C.byref(idesc), C.byref(st))
+    if not cas or not field:
+        return None
+    r.oref_interaction_set_obstruction(field, mask.ctypes.data_as(C.POINTER(C.c_uint8)))
+    dt = np.float32(1.0 / 60.0)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        k = steps % len(recs)
+        r.oref_cascade_update(cas, k / 60.0)
+        for d in recs[k]:
+            r.oref_interaction_add(field, C.byref(d))
+        r.oref_interaction_update(field, dt)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 1200:
+            break
+    r.oref_interaction_destroy(field)
+    r.oref_cascade_destroy(cas)
+    return {"value": round(steps / el, 1), "unit": "steps/s", "cores": cores, "kind": "reference",
+            "sample": f"{steps} steps of the same scene ({el:.1f} s): reference "
+                      f"CascadeStack::update and InteractionField add + update, "
+                      f"thread_count={cores}"}
+
+
 def handoff_leg(lib, sim, foam, n: int, reps: int = 200):
     """The renderer handoff of the same patch: the displacement (alpha = the
     persistent foam) and the normal as RGBA16F, into device arrays and into
@@ -363,6 +484,10 @@ def main():
                          "(counter runs, where timing is perturbed)")
     ap.add_argument("--queries-only", action="store_true",
                     help="run only the converged / learned height query leg (for rocprofv3)")
+    ap.add_argument("--scene-frames", type=int, default=600,
+                    help="timed steps of the three-cascade + interaction scene (0 skips it)")
+    ap.add_argument("--scene-only", action="store_true",
+                    help="run only the three-cascade + interaction scene leg (for rocprofv3)")
     ap.add_argument("--profile-only", action="store_true",
                     help="run only warmup + timed frames (for rocprofv3 runs)")
     args = ap.parse_args()
@@ -382,6 +507,9 @@ def main():
     lib = capi.load()
     if args.queries_only:
         print(json.dumps({"converged_height_at": learned_leg(lib)}), flush=True)
+        return
+    if args.scene_only:
+        print(json.dumps({"scene": scene_leg(lib, max(args.scene_frames, 1), 30)}), flush=True)
         return
     desc = desc_from(CONFIG2)
     n = desc.size
@@ -534,6 +662,9 @@ def main():
     handoff = handoff_leg(lib, sim, foam, n)
     batched = batched_leg(lib, args.batch_frames, 5) if args.batch_frames > 0 else None
     learned = learned_leg(lib) if not args.no_learned else None
+    scene = scene_leg(lib, args.scene_frames, 30) if args.scene_frames > 0 else None
+    if scene is not None and not args.no_cpu:
+        scene["cpu_baseline"] = scene_cpu_baseline()
     cpu = None if args.no_cpu else cpu_baseline(desc)
     if cpu is not None:
         one = cpu_baseline(desc, 4.0, threads=1)
@@ -573,6 +704,7 @@ def main():
         "rgba16_handoff": handoff,
         "batched": batched,
         "converged_height_at": learned,
+        "cascades_and_interaction": scene,
     }
     print(json.dumps(out), flush=True)
 

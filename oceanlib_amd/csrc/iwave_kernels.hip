@@ -12,6 +12,7 @@
 //     expf_libm, bit-identical to the host expf the reference calls;
 //   - overlapping stamps are applied per cell in queue order.
 #include "device_expf.hpp"
+#include "exact_sum.hpp"
 #include "hip_launch.h"
 #include "sample_device.hpp"
 
@@ -38,24 +39,52 @@ __device__ __forceinline__ float stamp_value(const ocean_hip_stamp& st, int x, i
     return ricker((px * px + pz * pz) * cell * cell, st.inv2s2);
 }
 
-// One block per stamp: the stamp's values are computed in parallel into LDS,
-// then summed in the reference's sequential order by one lane (the double
-// sum's rounding depends on that order, so it is not reassociated).
-__global__ __launch_bounds__(kBlock) void stamp_mean_kernel(const ocean_hip_stamp* __restrict__ stamps,
-                                                            float cell, float* __restrict__ means)
+// One block per stamp. The stamp's values are computed in parallel into LDS
+// (z-major, the reference's order); then every wavefront gathers the exactness
+// statistics of its 256-term chunks (exact_sum.hpp), and one lane runs the
+// reference's sequential double sum chunk by chunk: a chunk the statistics
+// prove rounding-free is added as its exact total in one step, any other term
+// by term. The result is the sequential sum bit for bit either way.
+constexpr int kSumChunk = 256;
+constexpr int kMaxChunks = kStampLds / kSumChunk;
+constexpr int kMeanBlock = 1024;  // the values and statistics of one stamp on one CU
+
+__global__ __launch_bounds__(kMeanBlock) void stamp_mean_kernel(const ocean_hip_stamp* __restrict__ stamps,
+                                                                float cell, float* __restrict__ means)
 {
     __shared__ float vals[kStampLds];
+    __shared__ ChunkStats chunk[kMaxChunks];
     const ocean_hip_stamp st = stamps[blockIdx.x];
     const int w = st.x1 - st.x0 + 1;
     const int h = st.z1 - st.z0 + 1;
     const int cnt = w * h;
     if (cnt <= kStampLds) {
-        for (int i = threadIdx.x; i < cnt; i += kBlock)
+        for (int i = threadIdx.x; i < cnt; i += kMeanBlock)
             vals[i] = stamp_value(st, st.x0 + i % w, st.z0 + i / w, cell);
+        __syncthreads();
+        const int chunks = (cnt + kSumChunk - 1) / kSumChunk;
+        const int lane = threadIdx.x % warpSize;
+        for (int c = threadIdx.x / warpSize; c < chunks; c += kMeanBlock / warpSize) {
+            ChunkAcc acc;
+            const int end = cnt < (c + 1) * kSumChunk ? cnt : (c + 1) * kSumChunk;
+            for (int i = c * kSumChunk + lane; i < end; i += warpSize) acc.add(vals[i]);
+            for (int off = warpSize / 2; off >= 1; off /= 2)
+                acc.merge(__shfl_xor(acc.total, off), __shfl_xor(acc.abs_sum, off),
+                          __shfl_xor(acc.q, off), __shfl_xor(int(acc.finite), off) != 0);
+            if (lane == 0) chunk[c] = acc.finish();
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
             double s = 0.0;
-            for (int i = 0; i < cnt; ++i) s += static_cast<double>(vals[i]);
+            for (int c = 0; c < chunks; ++c) {
+                const ChunkStats cs = chunk[c];
+                if (chunk_is_exact(s, cs)) {
+                    s += cs.total;
+                } else {
+                    const int end = cnt < (c + 1) * kSumChunk ? cnt : (c + 1) * kSumChunk;
+                    for (int i = c * kSumChunk; i < end; ++i) s += static_cast<double>(vals[i]);
+                }
+            }
             means[blockIdx.x] = static_cast<float>(s / (static_cast<double>(w) * static_cast<double>(h)));
         }
     } else if (threadIdx.x == 0) {
@@ -305,7 +334,8 @@ extern "C" hipError_t ocean_hip_iwave_inject(const ocean_hip_iwave* f, const oce
                                              uint32_t count, float* means, hipStream_t s)
 {
     if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(stamp_mean_kernel, dim3(count), dim3(kBlock), 0, s, stamps, f->cell, means);
+    hipLaunchKernelGGL(stamp_mean_kernel, dim3(count), dim3(kMeanBlock), 0, s, stamps, f->cell,
+                       means);
     hipLaunchKernelGGL(stamp_apply_kernel, cell_grid(f->n), dim3(kBlock), 0, s, *f, stamps, count,
                        static_cast<const float*>(means));
     return hipGetLastError();

@@ -99,14 +99,20 @@ struct InteractionField::Impl {
 
     // pinned host
     float* h_out = nullptr;
-    ocean_hip_stamp* h_stamps = nullptr;
+    // Two host staging buffers for the stamps, used in turn, so that an update
+    // does not wait for the previous update's upload to finish.
+    ocean_hip_stamp* h_stamps_buf[2] = {nullptr, nullptr};
+    ocean_hip_stamp* h_stamps = nullptr;  // the one this update fills
+    std::uint32_t stamp_turn = 0;
+    std::vector<std::uint32_t> first;     // per substep, its first stamp (max_substeps + 1)
     float* h_qxz = nullptr;
     float* h_qout = nullptr;
-    hipEvent_t stamps_free = nullptr;  // the last upload from h_stamps has completed
+    hipEvent_t stamps_free[2] = {nullptr, nullptr};  // the last upload from each buffer is done
     bool mirror_stale = true;
 
     // host bookkeeping (reference Impl members)
     std::vector<std::uint8_t> mask;
+    std::vector<std::uint8_t> mask_shift;  // recenter's copy of the mask (sized once)
     std::vector<std::uint32_t> nearest, solid_list, bfs;
     std::uint32_t solid_count = 0;
     bool has_mask = false;
@@ -125,10 +131,12 @@ struct InteractionField::Impl {
         if (arena) hipFree(arena);
         if (d_pack) hipFree(d_pack);
         if (h_out) hipHostFree(h_out);
-        if (h_stamps) hipHostFree(h_stamps);
+        for (ocean_hip_stamp* b : h_stamps_buf)
+            if (b) hipHostFree(b);
         if (h_qxz) hipHostFree(h_qxz);
         if (h_qout) hipHostFree(h_qout);
-        if (stamps_free) hipEventDestroy(stamps_free);
+        for (hipEvent_t e : stamps_free)
+            if (e) hipEventDestroy(e);
         if (dev.stream) hipStreamDestroy(dev.stream);
     }
 
@@ -295,7 +303,9 @@ InteractionField::InteractionField(const InteractionDesc& d)
         throw std::runtime_error("ocean: no HIP device available");
     detail::hip_check(hipGetDevice(&m.dev.device), "hipGetDevice");
     detail::hip_check(hipStreamCreateWithFlags(&m.dev.stream, hipStreamNonBlocking), "hipStreamCreate");
-    detail::hip_check(hipEventCreateWithFlags(&m.stamps_free, hipEventDisableTiming), "hipEventCreate");
+    for (hipEvent_t& e : m.stamps_free)
+        detail::hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    m.first.assign(std::size_t(d.max_substeps) + 1, 0u);
 
     m.stamp_cap = static_cast<std::size_t>(d.max_substeps) * d.max_sources;
     const std::size_t w = 2 * p + 1;
@@ -330,11 +340,14 @@ InteractionField::InteractionField(const InteractionDesc& d)
     m.d_energy = reinterpret_cast<double*>(base + o_energy);
 
     detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m.h_out), m.cells * 16, hipHostMallocDefault), "hipHostMalloc");
-    detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m.h_stamps), m.stamp_cap * sizeof(ocean_hip_stamp), hipHostMallocDefault), "hipHostMalloc");
+    for (ocean_hip_stamp*& b : m.h_stamps_buf)
+        detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&b), m.stamp_cap * sizeof(ocean_hip_stamp), hipHostMallocDefault), "hipHostMalloc");
+    m.h_stamps = m.h_stamps_buf[0];
     detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m.h_qxz), std::size_t(detail::kQueryStage) * 8, hipHostMallocDefault), "hipHostMalloc");
     detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m.h_qout), std::size_t(detail::kQueryStage) * sizeof(ocean_hip_surface), hipHostMallocDefault), "hipHostMalloc");
 
     m.mask.assign(m.cells, 0);
+    m.mask_shift.assign(m.cells, 0);
     m.nearest.assign(m.cells, 0);
     m.solid_list.assign(m.cells, 0);
     m.bfs.assign(m.cells, 0);
@@ -396,9 +409,12 @@ void InteractionField::update(float dt) noexcept
 
     m.dev.make_current();
     hipStream_t s = m.dev.stream;
-    // All substeps' stamps are staged at once and uploaded in one copy.
-    m.dev.note(hipEventSynchronize(m.stamps_free));
-    std::vector<std::uint32_t> first(steps + 1, 0);
+    // All substeps' stamps are staged at once and uploaded in one copy, from
+    // the staging buffer whose previous upload (two updates ago) is done.
+    m.stamp_turn ^= 1u;
+    m.h_stamps = m.h_stamps_buf[m.stamp_turn];
+    m.dev.note(hipEventSynchronize(m.stamps_free[m.stamp_turn]));
+    std::uint32_t* first = m.first.data();
     std::size_t k = 0;
     for (std::uint32_t sidx = 0; sidx < steps; ++sidx) {
         first[sidx] = static_cast<std::uint32_t>(k);
@@ -420,7 +436,7 @@ void InteractionField::update(float dt) noexcept
     if (k > 0) {
         m.dev.note(hipMemcpyAsync(m.d_stamps, m.h_stamps, k * sizeof(ocean_hip_stamp),
                                   hipMemcpyHostToDevice, s));
-        m.dev.note(hipEventRecord(m.stamps_free, s));
+        m.dev.note(hipEventRecord(m.stamps_free[m.stamp_turn], s));
     }
     const int dirichlet = (m.desc.obstruction == Obstruction::Dirichlet) ? 1 : 0;
     for (std::uint32_t sidx = 0; sidx < steps; ++sidx) {
@@ -472,7 +488,8 @@ void InteractionField::recenter(float world_x, float world_z) noexcept
     }
     m.shift_planes(dx, dz);
     if (m.has_mask) {
-        std::vector<std::uint8_t> src(m.mask);
+        std::copy(m.mask.begin(), m.mask.end(), m.mask_shift.begin());
+        const std::vector<std::uint8_t>& src = m.mask_shift;
         for (std::int32_t z = 0; z < n; ++z)
             for (std::int32_t x = 0; x < n; ++x) {
                 const std::int32_t sx = x + dx, sz = z + dz;

@@ -40,20 +40,27 @@ __device__ __forceinline__ float stamp_value(const ocean_hip_stamp& st, int x, i
 }
 
 // One block per stamp. The stamp's values are computed in parallel into LDS
-// (z-major, the reference's order); then every wavefront gathers the exactness
-// statistics of its 256-term chunks (exact_sum.hpp), and one lane runs the
-// reference's sequential double sum chunk by chunk: a chunk the statistics
-// prove rounding-free is added as its exact total in one step, any other term
-// by term. The result is the sequential sum bit for bit either way.
+// (z-major, the reference's order); the wavefronts gather the exactness
+// statistics (exact_sum.hpp) of every 256-value chunk, and of every four of
+// them; then one lane runs the reference's sequential double sum: a 1024-chunk
+// the statistics prove rounding-free is added as its exact total in one step,
+// else each of its 256-chunks that is, and any other term by term. The result
+// is the sequential sum bit for bit either way. One block per stamp keeps the
+// launch to a few CUs, so it overlaps the other streams' work (the cascade's
+// passes, in a scene).
 constexpr int kSumChunk = 256;
 constexpr int kMaxChunks = kStampLds / kSumChunk;
-constexpr int kMeanBlock = 1024;  // the values and statistics of one stamp on one CU
+constexpr int kBigChunk = 4;  // 256-chunks per 1024-chunk
+constexpr int kMeanBlock = 1024;
 
 __global__ __launch_bounds__(kMeanBlock) void stamp_mean_kernel(const ocean_hip_stamp* __restrict__ stamps,
                                                                 float cell, float* __restrict__ means)
 {
     __shared__ float vals[kStampLds];
-    __shared__ ChunkStats chunk[kMaxChunks];
+    __shared__ double c_total[kMaxChunks], c_abs[kMaxChunks];
+    __shared__ int c_q[kMaxChunks], c_finite[kMaxChunks];
+    __shared__ ChunkStats small[kMaxChunks];
+    __shared__ ChunkStats big[kMaxChunks / kBigChunk];
     const ocean_hip_stamp st = stamps[blockIdx.x];
     const int w = st.x1 - st.x0 + 1;
     const int h = st.z1 - st.z0 + 1;
@@ -63,6 +70,7 @@ __global__ __launch_bounds__(kMeanBlock) void stamp_mean_kernel(const ocean_hip_
             vals[i] = stamp_value(st, st.x0 + i % w, st.z0 + i / w, cell);
         __syncthreads();
         const int chunks = (cnt + kSumChunk - 1) / kSumChunk;
+        const int bigs = (chunks + kBigChunk - 1) / kBigChunk;
         const int lane = threadIdx.x % warpSize;
         for (int c = threadIdx.x / warpSize; c < chunks; c += kMeanBlock / warpSize) {
             ChunkAcc acc;
@@ -71,16 +79,35 @@ __global__ __launch_bounds__(kMeanBlock) void stamp_mean_kernel(const ocean_hip_
             for (int off = warpSize / 2; off >= 1; off /= 2)
                 acc.merge(__shfl_xor(acc.total, off), __shfl_xor(acc.abs_sum, off),
                           __shfl_xor(acc.q, off), __shfl_xor(int(acc.finite), off) != 0);
-            if (lane == 0) chunk[c] = acc.finish();
+            if (lane == 0) {
+                c_total[c] = acc.total;
+                c_abs[c] = acc.abs_sum;
+                c_q[c] = acc.q;
+                c_finite[c] = acc.finite ? 1 : 0;
+                small[c] = acc.finish();
+            }
+        }
+        __syncthreads();
+        for (int b = threadIdx.x; b < bigs; b += kMeanBlock) {
+            ChunkAcc acc;
+            const int end = chunks < (b + 1) * kBigChunk ? chunks : (b + 1) * kBigChunk;
+            for (int c = b * kBigChunk; c < end; ++c) acc.merge(c_total[c], c_abs[c], c_q[c], c_finite[c] != 0);
+            big[b] = acc.finish();
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             double s = 0.0;
-            for (int c = 0; c < chunks; ++c) {
-                const ChunkStats cs = chunk[c];
-                if (chunk_is_exact(s, cs)) {
-                    s += cs.total;
-                } else {
+            for (int b = 0; b < bigs; ++b) {
+                if (chunk_is_exact(s, big[b])) {
+                    s += big[b].total;
+                    continue;
+                }
+                const int c_end = chunks < (b + 1) * kBigChunk ? chunks : (b + 1) * kBigChunk;
+                for (int c = b * kBigChunk; c < c_end; ++c) {
+                    if (chunk_is_exact(s, small[c])) {
+                        s += small[c].total;
+                        continue;
+                    }
                     const int end = cnt < (c + 1) * kSumChunk ? cnt : (c + 1) * kSumChunk;
                     for (int i = c * kSumChunk; i < end; ++i) s += static_cast<double>(vals[i]);
                 }

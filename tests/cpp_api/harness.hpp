@@ -102,6 +102,12 @@ inline void scattering_hook(void*, ocean::TaskFn task, void* ctx, std::uint32_t 
     for (auto& t : workers) t.join();
 }
 
+// Heap allocations (operator new) made from this library's own code since
+// the program started: the program replaces the global operator new and
+// counts the calls whose caller lies in liboceanlib_amd (the HIP runtime's
+// own allocations inside each launch are not this library's).
+std::size_t library_allocations();
+
 extern const Case kFrameCases[];
 extern const std::size_t kFrameCaseCount;
 extern const Case kInteractionCases[];

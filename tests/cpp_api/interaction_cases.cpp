@@ -6,7 +6,7 @@
 // library's own tap fit (ocean_amd_build_iwave_taps), as the reference's
 // test takes it from its own. Not restated: the SIMD-vs-scalar step test
 // (the device step is compared with the oracle's scalar step bit for bit in
-// tests/test_gpu_interaction.py) and the allocation probe.
+// tests/test_gpu_interaction.py).
 #include "harness.hpp"
 
 #include "ocean/interaction.hpp"
@@ -586,6 +586,27 @@ void iwave_queries_agree_with_buffer()  // tests/test_interaction.cpp:1034-1064
     EXPECT(f.height_at(10000.0f, 10000.0f) == 0.0f && f.height_at(-10000.0f, 0.0f) == 0.0f);
 }
 
+void iwave_update_allocation_free()  // tests/test_interaction.cpp:1066-1084
+{
+    InteractionField f{base_desc()};
+    std::vector<std::uint8_t> mask(std::size_t(256) * 256, 0);
+    for (std::size_t z = 100; z < 120; ++z)
+        for (std::size_t x = 100; x < 140; ++x) mask[z * 256 + x] = 1;
+    f.set_obstruction(mask.data());
+    f.add(rock(32.0f, 32.0f));
+    f.update(kFrame);
+    const std::size_t before = library_allocations();
+    for (int i = 0; i < 30; ++i) {
+        f.add(rock(30.0f + 0.1f * i, 33.0f, 0.4f, 0.1f));
+        Disturbance c = rock(31.0f, 31.0f, 0.5f, 0.05f);
+        c.kind = SourceKind::Continuous;
+        f.add(c);
+        f.update(kFrame);
+        f.recenter(32.0f + 0.5f * i, 32.0f);  // whole-cell shifts, mask included
+    }
+    EXPECT(library_allocations() == before);
+}
+
 void iwave_exact_one_substep()  // tests/test_interaction.cpp:1086-1100
 {
     InteractionField f{base_desc()};
@@ -886,6 +907,7 @@ const Case kInteractionCases[] = {
     {"iwave.neumann_vs_dirichlet", iwave_neumann_vs_dirichlet},
     {"iwave.obstruction_changes_field", iwave_obstruction_changes_field},
     {"iwave.queries_agree_with_buffer", iwave_queries_agree_with_buffer},
+    {"iwave.update_allocation_free", iwave_update_allocation_free},
     {"iwave.exact_one_substep", iwave_exact_one_substep},
     {"cinter.defaults", cinter_defaults},
     {"cinter.c_equals_cpp", cinter_c_equals_cpp},

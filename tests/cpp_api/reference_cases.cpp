@@ -11,10 +11,10 @@
 //     results do not change;
 //   - ocean_simd_level() names the one device path ("gfx950"), and forcing a
 //     CPU level leaves it unchanged.
-// The reference's allocation-probe tests (operator new counting) have no
-// counterpart here: the HIP runtime allocates its own command objects inside
-// every launch, so a process-wide operator new count would measure the
-// runtime, not this library.
+// The reference's allocation probe counts every operator new of the process;
+// here it counts those made from this library's own code (the HIP runtime
+// allocates its own command objects inside every launch), see
+// library_allocations().
 //
 // Usage: reference_cases [--list | name-prefix]. Prints "RUN <case>" before
 // and "PASS <case>" or "FAIL <case>: <reason>" after each case, and
@@ -28,9 +28,13 @@
 #include "ocean/ocean.hpp"
 #include "ocean/ocean_amd.h"
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
+#include <new>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -42,6 +46,40 @@
 
 using namespace ocean;
 using namespace cases;
+
+// The replacement pairs operator new with malloc and operator delete with
+// free; g++ cannot see that pairing where it inlines the library's deletes.
+#pragma GCC diagnostic ignored "-Wmismatched-new-delete"
+
+namespace {
+
+std::atomic<std::size_t> g_library_allocations{0};
+
+bool called_from_library(void* ret) noexcept
+{
+    Dl_info info;
+    return dladdr(ret, &info) != 0 && info.dli_fname != nullptr &&
+           std::strstr(info.dli_fname, "liboceanlib_amd") != nullptr;
+}
+
+void* counted_alloc(std::size_t n, void* ret)
+{
+    if (called_from_library(ret)) g_library_allocations.fetch_add(1);
+    void* p = std::malloc(n != 0 ? n : 1);
+    if (p == nullptr) throw std::bad_alloc();
+    return p;
+}
+
+}  // namespace
+
+void* operator new(std::size_t n) { return counted_alloc(n, __builtin_return_address(0)); }
+void* operator new[](std::size_t n) { return counted_alloc(n, __builtin_return_address(0)); }
+void operator delete(void* p) noexcept { std::free(p); }
+void operator delete[](void* p) noexcept { std::free(p); }
+void operator delete(void* p, std::size_t) noexcept { std::free(p); }
+void operator delete[](void* p, std::size_t) noexcept { std::free(p); }
+
+std::size_t cases::library_allocations() { return g_library_allocations.load(); }
 
 namespace {
 
@@ -221,6 +259,29 @@ void api_movable_desc_roundtrip()  // tests/test_api.cpp:151-167
     Ocean again{d};
     again.update(2.0);
     EXPECT(same_bits(again.buffers().displacement, c.buffers().displacement, texels(c.buffers())));
+}
+
+void api_allocation_probe_installed()  // tests/test_api.cpp:108-117
+{
+    const std::size_t before = library_allocations();
+    OceanDesc d;
+    d.size = 64;
+    Ocean sim{d};
+    EXPECT(library_allocations() > before);
+}
+
+void api_update_allocation_free()  // tests/test_api.cpp:120-137, test_simulation.cpp:360-371
+{
+    for (std::uint32_t n : {16u, 64u, 128u, 256u}) {
+        OceanDesc d;
+        d.size = n;
+        Ocean sim{d};
+        sim.update(0.0);
+        const std::size_t before = library_allocations();
+        for (int i = 0; i < 16; ++i) sim.update(double(i) * 0.016);
+        (void)sim.buffers();  // the host mirror exists from creation
+        if (library_allocations() != before) fail("N=%u: %zu allocations", n, library_allocations() - before);
+    }
 }
 
 void api_bad_desc_throws()  // src/ocean.cpp:28-59
@@ -740,6 +801,20 @@ bool foam_throws(const Ocean& sim, F mutate)
     return false;
 }
 
+void foam_update_allocation_free()  // tests/test_foam.cpp:395-409
+{
+    Ocean sim{foam_ocean(128)};
+    FoamField foam{sim, FoamDesc{}};
+    sim.update(1.0);
+    foam.update(1.0f / 60.0f);
+    const std::size_t before = library_allocations();
+    for (int i = 0; i < 60; ++i) {
+        sim.update(2.0 + i / 60.0);
+        foam.update(1.0f / 60.0f);
+    }
+    EXPECT(library_allocations() == before);
+}
+
 void foam_refuses()  // tests/test_foam.cpp:47-69
 {
     OceanDesc d = foam_ocean(128);
@@ -1085,6 +1160,15 @@ void cascade_deterministic()  // tests/test_cascade.cpp:248-267
     }
 }
 
+void cascade_update_allocation_free()  // tests/test_cascade.cpp:285-300
+{
+    CascadeStack stack{std::span<const OceanDesc>(three_levels())};
+    stack.update(0.0);
+    const std::size_t before = library_allocations();
+    for (int i = 0; i < 16; ++i) stack.update(double(i) * 0.016);
+    EXPECT(library_allocations() == before);
+}
+
 void cascade_seek_backwards()  // tests/test_cascade.cpp:269-283
 {
     const std::vector<OceanDesc> levels = three_levels();
@@ -1330,6 +1414,8 @@ const Case kFrameCases[] = {
     {"api.zero_wind_flat", api_zero_wind_flat},
     {"api.absolute_time", api_absolute_time},
     {"api.movable_desc_roundtrip", api_movable_desc_roundtrip},
+    {"api.allocation_probe_installed", api_allocation_probe_installed},
+    {"api.update_allocation_free", api_update_allocation_free},
     {"api.bad_desc_throws", api_bad_desc_throws},
     {"query.agrees_with_drawn_grid", query_agrees_with_drawn_grid},
     {"query.chop0_exact_lookup", query_chop0_exact_lookup},
@@ -1358,6 +1444,7 @@ const Case kFrameCases[] = {
     {"velocity.query_agrees_with_buffer", vel_query_agrees_with_buffer},
     {"velocity.deterministic_seek", vel_deterministic_seek},
     {"velocity.cascade_sums", vel_cascade_sums},
+    {"foam.update_allocation_free", foam_update_allocation_free},
     {"foam.refuses", foam_refuses},
     {"foam.decay_rate", foam_decay_rate},
     {"foam.exact_advection", foam_exact_advection},
@@ -1374,6 +1461,7 @@ const Case kFrameCases[] = {
     {"cascade.unit_normal", cascade_unit_normal},
     {"cascade.variances_add", cascade_variances_add},
     {"cascade.deterministic", cascade_deterministic},
+    {"cascade.update_allocation_free", cascade_update_allocation_free},
     {"cascade.seek_backwards", cascade_seek_backwards},
     {"capi.c_equals_cpp", capi_c_equals_cpp},
     {"capi.invalid_desc", capi_invalid_desc},

@@ -34,17 +34,19 @@ def emul(tmp_path_factory):
     fp = C.POINTER(C.c_float)
     lib.emul_sum_plain.argtypes = [fp, C.c_int]
     lib.emul_sum_plain.restype = C.c_double
-    lib.emul_sum_chunked.argtypes = [fp, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    lib.emul_sum_chunked.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
     lib.emul_sum_chunked.restype = C.c_double
     return lib
 
 
-def check(lib, t, chunk=256):
+def check(lib, t, chunk=256, group=4):
+    """group: consecutive chunks whose merged statistics are tried first (the
+    kernel's 1024-term groups of four 256-chunks)."""
     t = np.ascontiguousarray(t, np.float32)
     p = t.ctypes.data_as(C.POINTER(C.c_float))
     want = lib.emul_sum_plain(p, t.size)
     sk = C.c_int(0)
-    got = lib.emul_sum_chunked(p, t.size, chunk, C.byref(sk))
+    got = lib.emul_sum_chunked(p, t.size, chunk, group, C.byref(sk))
     assert np.float64(got).tobytes() == np.float64(want).tobytes(), (got, want, t.size)
     return sk.value, (t.size + chunk - 1) // chunk
 
@@ -76,9 +78,9 @@ def test_stamps_of_every_size(emul):
     assert skipped > 0.95 * chunks, (skipped, chunks)
 
 
-@pytest.mark.parametrize("chunk", [1, 7, 64, 255, 256])
-def test_adversarial_sequences(emul, chunk):
-    rng = np.random.default_rng(chunk)
+@pytest.mark.parametrize("chunk,group", [(1, 1), (7, 3), (64, 4), (255, 1), (256, 1), (256, 4)])
+def test_adversarial_sequences(emul, chunk, group):
+    rng = np.random.default_rng(chunk * 10 + group)
     for trial in range(300):
         n = int(rng.integers(1, 3000))
         kind = trial % 6
@@ -99,15 +101,16 @@ def test_adversarial_sequences(emul, chunk):
         else:            # the running value grows past 2^53 ulps of the small terms
             t = np.full(n, 3.0 * 2.0 ** 20) + rng.uniform(0, 1, n) * 2.0 ** -20
         with np.errstate(over="ignore"):
-            check(emul, t.astype(np.float32), chunk)
+            check(emul, t.astype(np.float32), chunk, group)
 
 
-def test_non_finite_terms_fall_back(emul):
+@pytest.mark.parametrize("group", [1, 4])
+def test_non_finite_terms_fall_back(emul, group):
     t = np.ones(1000, np.float32)
     t[517] = np.float32(np.inf)
-    check(emul, t)
+    check(emul, t, group=group)
     t[300] = np.float32(-np.inf)
-    check(emul, t)  # nan, as the sequential sum gives
-    t = np.ones(600, np.float32)
+    check(emul, t, group=group)  # nan, as the sequential sum gives
+    t = np.ones(2600, np.float32)
     t[5] = np.float32(np.nan)
-    check(emul, t)
+    check(emul, t, group=group)

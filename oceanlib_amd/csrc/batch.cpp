@@ -1,11 +1,12 @@
-// OceanBatch: many ocean levels stepped by one hipGraph per frame.
+// OceanBatch: many ocean levels stepped by one batched launch per pass.
 //
 // Levels are ordinary Oceans built on the batch's stream with deferred host
 // mirrors. Levels whose N the batched kernels support are grouped by
 // (N, velocity); a group's work items are (level, block) pairs, ordered level
-// by level so consecutive items share a level's tables in cache. The graph
-// holds one row -> column chain per group; update() sets the frame time and
-// replays it.
+// by level so consecutive items share a level's tables in cache. With several
+// groups the frame is one hipGraph holding a row -> column chain per group;
+// update() sets the frame time and replays it. A single group's two passes
+// are launched directly.
 #include "ocean/batch.hpp"
 
 #include "ocean_device.hpp"
@@ -201,7 +202,13 @@ void OceanBatch::update(double time) noexcept
     Impl& m = *impl_;
     m.levels.front().device().make_current();
     const double t2 = time * kInvTwoPiD;  // hoisted as the reference hoists it (evolve.cpp:33)
-    if (m.graph) m.note(ocean_hip_batch_graph_launch(m.graph, t2, m.stream.s));
+    // One group (a cascade's levels, say): its two passes are launched
+    // directly, which costs the host less than launching a graph. Several
+    // groups: the graph, whose branches let the groups' passes overlap.
+    if (m.groups.size() == 1)
+        m.note(ocean_hip_batch_group_frame(&m.groups.front(), t2, 3u, m.stream.s));
+    else if (m.graph)
+        m.note(ocean_hip_batch_graph_launch(m.graph, t2, m.stream.s));
     for (std::size_t i : m.loose) m.levels[i].update(time);
     for (Ocean& o : m.levels) o.note_frame(time);
     m.time = time;

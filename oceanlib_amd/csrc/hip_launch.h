@@ -262,6 +262,12 @@ typedef struct ocean_hip_stamp {
  * `means` is device scratch of `count` floats. */
 hipError_t ocean_hip_iwave_inject(const ocean_hip_iwave* f, const ocean_hip_stamp* stamps,
                                   uint32_t count, float* means, hipStream_t s);
+/* The same with the stamps read from host memory at enqueue and passed inside
+ * the kernel arguments: count <= ocean_hip_iwave_arg_stamps(), else
+ * hipErrorInvalidValue and nothing is enqueued. */
+uint32_t ocean_hip_iwave_arg_stamps(void);
+hipError_t ocean_hip_iwave_inject_args(const ocean_hip_iwave* f, const ocean_hip_stamp* host_stamps,
+                                       uint32_t count, float* means, hipStream_t s);
 /* Solid cells: Dirichlet zero (dirichlet != 0) or copy from the nearest
  * fluid cell. `solid`, `nearest` are device arrays of cell indices. */
 hipError_t ocean_hip_iwave_obstruct(const ocean_hip_iwave* f, const uint32_t* solid,

@@ -108,6 +108,7 @@ struct InteractionField::Impl {
     float* h_qxz = nullptr;
     float* h_qout = nullptr;
     hipEvent_t stamps_free[2] = {nullptr, nullptr};  // the last upload from each buffer is done
+    bool stamps_pending[2] = {false, false};          // an upload from the buffer was enqueued
     bool mirror_stale = true;
 
     // host bookkeeping (reference Impl members)
@@ -409,11 +410,16 @@ void InteractionField::update(float dt) noexcept
 
     m.dev.make_current();
     hipStream_t s = m.dev.stream;
-    // All substeps' stamps are staged at once and uploaded in one copy, from
-    // the staging buffer whose previous upload (two updates ago) is done.
+    // All substeps' stamps are staged at once, into the staging buffer whose
+    // previous upload (two updates ago), if any, is done. When every substep
+    // has few enough stamps they travel inside the kernel arguments; else all
+    // of them are uploaded in one copy.
     m.stamp_turn ^= 1u;
     m.h_stamps = m.h_stamps_buf[m.stamp_turn];
-    m.dev.note(hipEventSynchronize(m.stamps_free[m.stamp_turn]));
+    if (m.stamps_pending[m.stamp_turn]) {
+        m.dev.note(hipEventSynchronize(m.stamps_free[m.stamp_turn]));
+        m.stamps_pending[m.stamp_turn] = false;
+    }
     std::uint32_t* first = m.first.data();
     std::size_t k = 0;
     for (std::uint32_t sidx = 0; sidx < steps; ++sidx) {
@@ -433,16 +439,23 @@ void InteractionField::update(float dt) noexcept
         }
     }
     first[steps] = static_cast<std::uint32_t>(k);
-    if (k > 0) {
+    std::uint32_t most = 0;
+    for (std::uint32_t sidx = 0; sidx < steps; ++sidx) most = std::max(most, first[sidx + 1] - first[sidx]);
+    const bool by_args = most <= ocean_hip_iwave_arg_stamps();
+    if (k > 0 && !by_args) {
         m.dev.note(hipMemcpyAsync(m.d_stamps, m.h_stamps, k * sizeof(ocean_hip_stamp),
                                   hipMemcpyHostToDevice, s));
         m.dev.note(hipEventRecord(m.stamps_free[m.stamp_turn], s));
+        m.stamps_pending[m.stamp_turn] = true;
     }
     const int dirichlet = (m.desc.obstruction == Obstruction::Dirichlet) ? 1 : 0;
     for (std::uint32_t sidx = 0; sidx < steps; ++sidx) {
         const ocean_hip_iwave f = m.view();
         const std::uint32_t c = first[sidx + 1] - first[sidx];
-        if (c > 0)
+        if (c > 0 && by_args)
+            m.dev.note(ocean_hip_iwave_inject_args(&f, m.h_stamps + first[sidx], c,
+                                                   m.d_means + first[sidx], s));
+        else if (c > 0)
             m.dev.note(ocean_hip_iwave_inject(&f, m.d_stamps + first[sidx], c,
                                               m.d_means + first[sidx], s));
         if (m.has_mask && m.solid_count > 0)

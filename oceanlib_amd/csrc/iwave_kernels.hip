@@ -23,7 +23,6 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kTile = 16;                 // step kernel: 16x16 outputs per block
-constexpr int kStampLds = 32768;          // stamp values staged in LDS up to this size
 
 // reference src/interaction.cpp:71-75
 __device__ __forceinline__ float ricker(float r2, float inv_two_sigma2)
@@ -39,43 +38,58 @@ __device__ __forceinline__ float stamp_value(const ocean_hip_stamp& st, int x, i
     return ricker((px * px + pz * pz) * cell * cell, st.inv2s2);
 }
 
-// One block per stamp. The stamp's values are computed in parallel into LDS
-// (z-major, the reference's order); the wavefronts gather the exactness
-// statistics (exact_sum.hpp) of every 256-value chunk, and of every four of
-// them; then one lane runs the reference's sequential double sum: a 1024-chunk
+// One block per stamp, over windows of up to kWindow 256-value chunks of the
+// stamp's clipped box in z-major order (the reference's order). Per window
+// the wavefronts compute the values and gather the exactness statistics
+// (exact_sum.hpp) of each chunk and of each group of four; then one lane
+// carries the reference's sequential double sum through the window: a group
 // the statistics prove rounding-free is added as its exact total in one step,
-// else each of its 256-chunks that is, and any other term by term. The result
-// is the sequential sum bit for bit either way. One block per stamp keeps the
-// launch to a few CUs, so it overlaps the other streams' work (the cascade's
-// passes, in a scene).
+// else each chunk that is, and any other chunk term by term, its values
+// recomputed by that lane (the same function of the cell, so the same bits).
+// The result is the sequential sum bit for bit either way. Only the
+// statistics are held in LDS, so the blocks share their CUs with the other
+// streams' passes (the cascade's, in a scene).
 constexpr int kSumChunk = 256;
-constexpr int kMaxChunks = kStampLds / kSumChunk;
-constexpr int kBigChunk = 4;  // 256-chunks per 1024-chunk
+constexpr int kGroup = 4;     // chunks per group
+constexpr int kWindow = 128;  // chunks per window
 constexpr int kMeanBlock = 1024;
 
-__global__ __launch_bounds__(kMeanBlock) void stamp_mean_kernel(const ocean_hip_stamp* __restrict__ stamps,
-                                                                float cell, float* __restrict__ means)
+// A substep's stamps reach the kernels either in device memory or, when there
+// are at most kArgStamps of them, inside the kernel arguments (no upload).
+struct DevStamps {
+    const ocean_hip_stamp* __restrict__ p;
+    __device__ __forceinline__ ocean_hip_stamp operator[](uint32_t i) const { return p[i]; }
+};
+constexpr uint32_t kArgStamps = 32;
+struct ArgStamps {
+    ocean_hip_stamp s[kArgStamps];
+    __device__ __forceinline__ ocean_hip_stamp operator[](uint32_t i) const { return s[i]; }
+};
+
+template <typename Stamps>
+__global__ __launch_bounds__(kMeanBlock) void stamp_mean_kernel(const Stamps stamps, float cell,
+                                                                float* __restrict__ means)
 {
-    __shared__ float vals[kStampLds];
-    __shared__ double c_total[kMaxChunks], c_abs[kMaxChunks];
-    __shared__ int c_q[kMaxChunks], c_finite[kMaxChunks];
-    __shared__ ChunkStats small[kMaxChunks];
-    __shared__ ChunkStats big[kMaxChunks / kBigChunk];
+    __shared__ double c_total[kWindow], c_abs[kWindow];
+    __shared__ int c_q[kWindow], c_finite[kWindow];
+    __shared__ ChunkStats small[kWindow];
+    __shared__ ChunkStats big[kWindow / kGroup];
     const ocean_hip_stamp st = stamps[blockIdx.x];
     const int w = st.x1 - st.x0 + 1;
     const int h = st.z1 - st.z0 + 1;
     const int cnt = w * h;
-    if (cnt <= kStampLds) {
-        for (int i = threadIdx.x; i < cnt; i += kMeanBlock)
-            vals[i] = stamp_value(st, st.x0 + i % w, st.z0 + i / w, cell);
-        __syncthreads();
-        const int chunks = (cnt + kSumChunk - 1) / kSumChunk;
-        const int bigs = (chunks + kBigChunk - 1) / kBigChunk;
-        const int lane = threadIdx.x % warpSize;
-        for (int c = threadIdx.x / warpSize; c < chunks; c += kMeanBlock / warpSize) {
+    const int chunks = (cnt + kSumChunk - 1) / kSumChunk;
+    const int lane = threadIdx.x % warpSize;
+    double s = 0.0;  // the running sum, carried by thread 0
+    for (int c0 = 0; c0 < chunks; c0 += kWindow) {
+        const int wc = chunks - c0 < kWindow ? chunks - c0 : kWindow;
+        const int groups = (wc + kGroup - 1) / kGroup;
+        for (int c = threadIdx.x / warpSize; c < wc; c += kMeanBlock / warpSize) {
+            const int begin = (c0 + c) * kSumChunk;
+            const int end = cnt - begin < kSumChunk ? cnt : begin + kSumChunk;
             ChunkAcc acc;
-            const int end = cnt < (c + 1) * kSumChunk ? cnt : (c + 1) * kSumChunk;
-            for (int i = c * kSumChunk + lane; i < end; i += warpSize) acc.add(vals[i]);
+            for (int i = begin + lane; i < end; i += warpSize)
+                acc.add(stamp_value(st, st.x0 + i % w, st.z0 + i / w, cell));
             for (int off = warpSize / 2; off >= 1; off /= 2)
                 acc.merge(__shfl_xor(acc.total, off), __shfl_xor(acc.abs_sum, off),
                           __shfl_xor(acc.q, off), __shfl_xor(int(acc.finite), off) != 0);
@@ -88,42 +102,40 @@ __global__ __launch_bounds__(kMeanBlock) void stamp_mean_kernel(const ocean_hip_
             }
         }
         __syncthreads();
-        for (int b = threadIdx.x; b < bigs; b += kMeanBlock) {
+        for (int g = threadIdx.x; g < groups; g += kMeanBlock) {
             ChunkAcc acc;
-            const int end = chunks < (b + 1) * kBigChunk ? chunks : (b + 1) * kBigChunk;
-            for (int c = b * kBigChunk; c < end; ++c) acc.merge(c_total[c], c_abs[c], c_q[c], c_finite[c] != 0);
-            big[b] = acc.finish();
+            const int end = wc < (g + 1) * kGroup ? wc : (g + 1) * kGroup;
+            for (int c = g * kGroup; c < end; ++c) acc.merge(c_total[c], c_abs[c], c_q[c], c_finite[c] != 0);
+            big[g] = acc.finish();
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            double s = 0.0;
-            for (int b = 0; b < bigs; ++b) {
-                if (chunk_is_exact(s, big[b])) {
-                    s += big[b].total;
+            for (int g = 0; g < groups; ++g) {
+                if (chunk_is_exact(s, big[g])) {
+                    s += big[g].total;
                     continue;
                 }
-                const int c_end = chunks < (b + 1) * kBigChunk ? chunks : (b + 1) * kBigChunk;
-                for (int c = b * kBigChunk; c < c_end; ++c) {
+                const int c_end = wc < (g + 1) * kGroup ? wc : (g + 1) * kGroup;
+                for (int c = g * kGroup; c < c_end; ++c) {
                     if (chunk_is_exact(s, small[c])) {
                         s += small[c].total;
                         continue;
                     }
-                    const int end = cnt < (c + 1) * kSumChunk ? cnt : (c + 1) * kSumChunk;
-                    for (int i = c * kSumChunk; i < end; ++i) s += static_cast<double>(vals[i]);
+                    const int begin = (c0 + c) * kSumChunk;
+                    const int end = cnt - begin < kSumChunk ? cnt : begin + kSumChunk;
+                    for (int i = begin; i < end; ++i)
+                        s += static_cast<double>(stamp_value(st, st.x0 + i % w, st.z0 + i / w, cell));
                 }
             }
-            means[blockIdx.x] = static_cast<float>(s / (static_cast<double>(w) * static_cast<double>(h)));
         }
-    } else if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int z = st.z0; z <= st.z1; ++z)
-            for (int x = st.x0; x <= st.x1; ++x) s += static_cast<double>(stamp_value(st, x, z, cell));
-        means[blockIdx.x] = static_cast<float>(s / (static_cast<double>(w) * static_cast<double>(h)));
+        __syncthreads();  // the window's statistics are read before the next window writes them
     }
+    if (threadIdx.x == 0)
+        means[blockIdx.x] = static_cast<float>(s / (static_cast<double>(w) * static_cast<double>(h)));
 }
 
-__global__ __launch_bounds__(kBlock) void stamp_apply_kernel(ocean_hip_iwave f,
-                                                             const ocean_hip_stamp* __restrict__ stamps,
+template <typename Stamps>
+__global__ __launch_bounds__(kBlock) void stamp_apply_kernel(ocean_hip_iwave f, const Stamps stamps,
                                                              uint32_t count,
                                                              const float* __restrict__ means)
 {
@@ -361,10 +373,28 @@ extern "C" hipError_t ocean_hip_iwave_inject(const ocean_hip_iwave* f, const oce
                                              uint32_t count, float* means, hipStream_t s)
 {
     if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(stamp_mean_kernel, dim3(count), dim3(kMeanBlock), 0, s, stamps, f->cell,
+    const DevStamps d{stamps};
+    hipLaunchKernelGGL(stamp_mean_kernel<DevStamps>, dim3(count), dim3(kMeanBlock), 0, s, d, f->cell,
                        means);
-    hipLaunchKernelGGL(stamp_apply_kernel, cell_grid(f->n), dim3(kBlock), 0, s, *f, stamps, count,
-                       static_cast<const float*>(means));
+    hipLaunchKernelGGL(stamp_apply_kernel<DevStamps>, cell_grid(f->n), dim3(kBlock), 0, s, *f, d,
+                       count, static_cast<const float*>(means));
+    return hipGetLastError();
+}
+
+extern "C" uint32_t ocean_hip_iwave_arg_stamps() { return kArgStamps; }
+
+extern "C" hipError_t ocean_hip_iwave_inject_args(const ocean_hip_iwave* f,
+                                                  const ocean_hip_stamp* host_stamps, uint32_t count,
+                                                  float* means, hipStream_t s)
+{
+    if (count == 0) return hipSuccess;
+    if (count > kArgStamps) return hipErrorInvalidValue;
+    ArgStamps a{};
+    for (uint32_t i = 0; i < count; ++i) a.s[i] = host_stamps[i];
+    hipLaunchKernelGGL(stamp_mean_kernel<ArgStamps>, dim3(count), dim3(kMeanBlock), 0, s, a, f->cell,
+                       means);
+    hipLaunchKernelGGL(stamp_apply_kernel<ArgStamps>, cell_grid(f->n), dim3(kBlock), 0, s, *f, a,
+                       count, static_cast<const float*>(means));
     return hipGetLastError();
 }
 

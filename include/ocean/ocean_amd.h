@@ -326,6 +326,11 @@ int ocean_amd_pack_half_device(const ocean_sim* sim, const float* src_dev, uint1
  * way, on the field's own stream (_device) or into host memory. */
 int ocean_amd_interaction_pack_rgba16(const ocean_interaction* field, uint16_t* out);
 int ocean_amd_interaction_pack_rgba16_device(const ocean_interaction* field, uint16_t* out_dev);
+/* InteractionField::energy() of the C++ API (reference
+ * src/interaction.cpp:803-846, which the reference's C API does not export):
+ * the scheme's discrete energy, computed on the device in the reference's
+ * summation order. NaN for a NULL field. */
+double ocean_amd_interaction_energy(const ocean_interaction* field);
 
 #if defined(__cplusplus)
 }

@@ -217,6 +217,7 @@ AMD_SIGNATURES = {
     "ocean_amd_pack_half_device": (C.c_int, [_P, _P, _P, C.c_uint64]),
     "ocean_amd_interaction_pack_rgba16": (C.c_int, [_P, _P]),
     "ocean_amd_interaction_pack_rgba16_device": (C.c_int, [_P, _P]),
+    "ocean_amd_interaction_energy": (C.c_double, [_P]),
     "ocean_amd_foam_last_substeps": (C.c_uint32, [_P]),
     "ocean_amd_foam_at": (C.c_float, [_P, C.c_float, C.c_float]),
     "ocean_amd_foam_coverage": (C.c_double, [_P]),

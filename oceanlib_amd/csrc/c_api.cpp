@@ -14,6 +14,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstring>
+#include <limits>
 #include <new>
 #include <stdexcept>
 
@@ -1181,6 +1182,12 @@ int ocean_amd_interaction_pack_rgba16(const ocean_interaction* field, uint16_t* 
     if (field == nullptr || out == nullptr) return OCEAN_ERROR_INVALID_ARG;
     if (to_cfield(field)->pack_rgba16(out, false)) return OCEAN_OK;
     return to_cfield(field)->last_error() != 0 ? OCEAN_ERROR_UNKNOWN : OCEAN_ERROR_INVALID_ARG;
+}
+
+double ocean_amd_interaction_energy(const ocean_interaction* field)
+{
+    if (field == nullptr) return std::numeric_limits<double>::quiet_NaN();
+    return to_cfield(field)->energy();
 }
 
 int ocean_amd_interaction_pack_rgba16_device(const ocean_interaction* field, uint16_t* out_dev)

@@ -662,6 +662,40 @@ EXPORT void orc_iw_sample_at(const orc_iw* m, float wx, float wz, ocean_interact
     s->velocity_y = v[3];
 }
 
+/* interaction.cpp:803-846: E = 1/2 (sum v^2 + g sum h^n (G h^{n-1})), one
+ * sequential double sum each in z-major cell order; the tap products are
+ * float products added to a double accumulator. */
+EXPORT double orc_iw_energy(const orc_iw* m)
+{
+    const int p = (int)m->desc.kernel_radius;
+    const int w = 2 * p + 1;
+    const size_t stride = m->stride;
+    const double inv_dt = 1.0 / (double)m->dt;
+    double kinetic = 0.0, potential = 0.0;
+    for (uint32_t z = 0; z < m->n; ++z) {
+        const size_t row = (size_t)z * stride;
+        for (uint32_t x = 0; x < m->n; ++x) {
+            const double hc = m->cur[row + x];
+            const double hp = m->old[row + x];
+            const double v = (hc - hp) * inv_dt;
+            kinetic += v * v;
+            const float* centre = m->old + row + x;
+            const float* tap = m->kernel.taps;
+            double acc = 0.0;
+            for (int j = -p; j <= p; ++j) {
+                const float* srow = centre + (ptrdiff_t)j * (ptrdiff_t)stride - p;
+                for (int i = 0; i < w; ++i) {
+                    const float prod = tap[i] * srow[i];
+                    acc += (double)prod;
+                }
+                tap += w;
+            }
+            potential += hc * acc;
+        }
+    }
+    return 0.5 * (kinetic + (double)m->desc.gravity * potential);
+}
+
 EXPORT const float* orc_iw_field(const orc_iw* m) { return m->out; }
 EXPORT float orc_iw_fixed_dt(const orc_iw* m) { return m->dt; }
 EXPORT float orc_iw_dt_limit(const orc_iw* m) { return m->dt_limit; }

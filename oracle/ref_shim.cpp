@@ -237,6 +237,11 @@ OREF void oref_interaction_sample_at(const ocean_interaction* f, float x, float 
 OREF float oref_interaction_fixed_dt(const ocean_interaction* f) { return ocean_interaction_fixed_dt(f); }
 OREF float oref_interaction_dt_limit(const ocean_interaction* f) { return ocean_interaction_dt_limit(f); }
 OREF std::uint64_t oref_interaction_dropped(const ocean_interaction* f) { return ocean_interaction_dropped(f); }
+// The C handle is the InteractionField itself (reference src/c_api.cpp:289-296).
+OREF double oref_interaction_energy(const ocean_interaction* f)
+{
+    return reinterpret_cast<const InteractionField*>(f)->energy();
+}
 OREF void oref_sample_combined(const ocean_sim* s, const ocean_interaction* f, float x, float z,
                                ocean_surface_ex* o)
 {

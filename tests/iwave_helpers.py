@@ -72,6 +72,11 @@ class Field:
     def dropped(self):
         return self._f("dropped")(self.h)
 
+    def energy(self) -> bytes:
+        """InteractionField::energy() as its 8 bytes (bit-for-bit comparison)."""
+        fn = self.lib.ocean_amd_interaction_energy if self.kind == "gpu" else self._f("energy")
+        return np.float64(fn(self.h)).tobytes()
+
     def fixed_dt(self):
         return self._f("fixed_dt")(self.h)
 

@@ -79,6 +79,7 @@ _ORACLE_ONLY = {
     "orc_iw_set_obstruction": (None, [_P, C.POINTER(C.c_uint8)]),
     "orc_iw_sample_at": (None, [_P, C.c_float, C.c_float, C.POINTER(InteractionSample)]),
     "orc_iw_field": (_F, [_P]),
+    "orc_iw_energy": (C.c_double, [_P]),
     "orc_iw_fixed_dt": (C.c_float, [_P]),
     "orc_iw_dt_limit": (C.c_float, [_P]),
     "orc_iw_origin_x": (C.c_float, [_P]),
@@ -120,6 +121,7 @@ _REF_ONLY = {
     "oref_interaction_fixed_dt": (C.c_float, [_P]),
     "oref_interaction_dt_limit": (C.c_float, [_P]),
     "oref_interaction_dropped": (C.c_uint64, [_P]),
+    "oref_interaction_energy": (C.c_double, [_P]),
     "oref_sample_combined": (None, [_P, _P, C.c_float, C.c_float, C.POINTER(OceanSurfaceEx)]),
     "oref_iwave_taps": (C.c_int, [C.c_uint32, C.c_double, C.c_double, _F]),
 }

@@ -54,6 +54,8 @@ def test_config3_interaction_scene_vs_golden_and_oracle(lib):
                 assert digest(fg) == want[str(k)], f"frame {k} vs reference digest"
                 if k % 200 == 0:
                     assert_bitexact(fg, o.field(), f"frame {k}")
+                    # energy(): the device's sequential sums in the reference's order
+                    assert g.energy() == o.energy(), f"energy, frame {k}"
         rng = np.random.default_rng(9)
         for x, z in rng.uniform(-3, 29, size=(300, 2)).astype(np.float32):
             assert bytes(g.sample(x, z)) == bytes(o.sample(x, z)), (x, z)
@@ -98,6 +100,7 @@ def test_dirichlet_recentre_impulses_clamp_vs_oracle(lib):
             assert lib.ocean_interaction_dropped(g.h) == oracle().orc_iw_dropped(o.h)
             if k % 3 == 0 or k == 89:
                 assert_bitexact(g.field(), o.field(), f"frame {k}")
+                assert g.energy() == o.energy(), f"energy, frame {k}"
         b = lib.ocean_interaction_get_buffers(g.h)
         assert b.origin_x == oracle().orc_iw_origin_x(o.h)
         assert b.origin_z == oracle().orc_iw_origin_z(o.h)

@@ -3,8 +3,9 @@
 Bit-for-bit against the reference's own InteractionField compiled into
 oracle/_ref (reference src/interaction.cpp, src/core/iwave_*.cpp): kernel
 taps from the least-squares fit, fields after every checked frame of the
-moving-source scene with a Neumann or Dirichlet hull, recentring, impulses
-kept over zero-length frames, the substep clamp and queue overflow. The
+moving-source scene with a Neumann or Dirichlet hull, and the scheme's
+energy() at each checked frame, recentring, impulses kept over zero-length
+frames, the substep clamp and queue overflow. The
 expf the Ricker stamp depends on is the host libm's in both.
 """
 import numpy as np
@@ -62,6 +63,7 @@ def run_pair(kind_a, lib_a, kind_b, lib_b, desc_kw, frames, mask=None, recenter_
             fb.update(dt)
             if k % check_every == 0 or k == frames - 1:
                 assert_bitexact(fa.field(), fb.field(), f"frame {k}")
+                assert fa.energy() == fb.energy(), f"energy, frame {k}"
         rng = np.random.default_rng(4)
         for x, z in rng.uniform(-2, 28, size=(200, 2)).astype(np.float32):
             assert bytes(fa.sample(x, z)) == bytes(fb.sample(x, z))

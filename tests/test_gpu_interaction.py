@@ -146,6 +146,37 @@ def test_many_stamps_of_every_size_vs_oracle(lib):
         o.close()
 
 
+def test_large_stamps_vs_oracle(lib):
+    """A 1024^2 field at 0.1 m: frames of 32 and of 8 sources of radius 5-6 m,
+    away from the edges (unclipped boxes of 401^2-481^2 cells, five windows of
+    the stamp mean's statistics each; the stamps ride in the kernel
+    arguments), and of 40 (more stamps than the arguments carry, so
+    uploaded): the field and energy() equal the oracle's every frame."""
+    rng = np.random.default_rng(23)
+    kw = dict(CONFIG3_INTERACTION, size=1024, extent=102.4, max_sources=40)
+    d = idesc(lib.ocean_interaction_desc_init, **kw)
+    g, o = Field("gpu", lib, d), Field("orc", oracle(), d)
+    try:
+        for k, count in enumerate((32, 32, 8, 40, 8)):
+            for _ in range(count):
+                rec = dict(world_x=np.float32(rng.uniform(30.0, 72.4)),
+                           world_z=np.float32(rng.uniform(30.0, 72.4)),
+                           radius=np.float32(rng.uniform(5.0, 6.0)),
+                           strength=np.float32(rng.uniform(-0.05, 0.05)),
+                           velocity_y=np.float32(rng.uniform(-1, 1)),
+                           velocity_x=np.float32(rng.uniform(-3, 3)),
+                           velocity_z=np.float32(rng.uniform(-3, 3)), kind=int(rng.integers(0, 2)))
+                g.add(rec)
+                o.add(rec)
+            g.update(1 / 60)
+            o.update(1 / 60)
+            assert_bitexact(g.field(), o.field(), f"frame {k} ({count} sources)")
+            assert g.energy() == o.energy(), f"energy, frame {k}"
+    finally:
+        g.close()
+        o.close()
+
+
 def test_combined_query_vs_oracle(lib):
     """ocean_sample_combined (WaterSurface, reference src/interaction.cpp:863-901)."""
     od = desc_from(dict(size=128, patch_length=64.0, wind_speed=14.0, choppiness=1.2,

@@ -269,9 +269,11 @@ uint32_t ocean_hip_iwave_arg_stamps(void);
 hipError_t ocean_hip_iwave_inject_args(const ocean_hip_iwave* f, const ocean_hip_stamp* host_stamps,
                                        uint32_t count, float* means, hipStream_t s);
 /* Solid cells: Dirichlet zero (dirichlet != 0) or copy from the nearest
- * fluid cell. `solid`, `nearest` are device arrays of cell indices. */
+ * fluid cell. `solid[k]` is the k-th solid cell's index and `source[k]` its
+ * nearest fluid cell (0xFFFFFFFF: none, the cell is left as it is); both are
+ * device arrays of `count` entries. */
 hipError_t ocean_hip_iwave_obstruct(const ocean_hip_iwave* f, const uint32_t* solid,
-                                    const uint32_t* nearest, uint32_t count, int dirichlet,
+                                    const uint32_t* source, uint32_t count, int dirichlet,
                                     hipStream_t s);
 /* One leapfrog step: old <- c1 (2 cur - c2 old - g dt^2 (G * cur)). */
 hipError_t ocean_hip_iwave_step(const ocean_hip_iwave* f, hipStream_t s);

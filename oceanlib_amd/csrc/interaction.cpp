@@ -88,7 +88,7 @@ struct InteractionField::Impl {
     float* c2 = nullptr;
     float* out = nullptr;
     std::uint32_t* d_solid = nullptr;
-    std::uint32_t* d_nearest = nullptr;
+    std::uint32_t* d_source = nullptr;  // per solid cell: its nearest fluid cell
     ocean_hip_stamp* d_stamps = nullptr;
     float* d_means = nullptr;
     float* d_qxz = nullptr;
@@ -114,7 +114,12 @@ struct InteractionField::Impl {
     // host bookkeeping (reference Impl members)
     std::vector<std::uint8_t> mask;
     std::vector<std::uint8_t> mask_shift;  // recenter's copy of the mask (sized once)
-    std::vector<std::uint32_t> nearest, solid_list, bfs;
+    std::vector<std::uint32_t> nearest, bfs, frontier;
+    std::vector<std::uint8_t> frontier_mark;
+    std::uint32_t* h_solid = nullptr;   // pinned: solid cell indices
+    std::uint32_t* h_source = nullptr;  // pinned: their nearest fluid cells
+    hipEvent_t mask_uploaded = nullptr;
+    bool mask_upload_pending = false;
     std::uint32_t solid_count = 0;
     bool has_mask = false;
     std::vector<Disturbance> queue;
@@ -138,6 +143,9 @@ struct InteractionField::Impl {
         if (h_qout) hipHostFree(h_qout);
         for (hipEvent_t e : stamps_free)
             if (e) hipEventDestroy(e);
+        if (h_solid) hipHostFree(h_solid);
+        if (h_source) hipHostFree(h_source);
+        if (mask_uploaded) hipEventDestroy(mask_uploaded);
         if (dev.stream) hipStreamDestroy(dev.stream);
     }
 
@@ -192,45 +200,75 @@ struct InteractionField::Impl {
         }
     }
 
-    // reference src/interaction.cpp:255-292 (multi-source BFS; ties resolved by
-    // queue order exactly as the reference resolves them)
+    // reference src/interaction.cpp:255-292: a multi-source BFS from every
+    // fluid cell in index order, each dequeued cell claiming its unclaimed
+    // 4-neighbours (+x, -x, +z, -z) for its nearest fluid cell. A fluid cell
+    // with no solid neighbour claims nothing, so the same table comes from
+    // visiting only the fluid cells next to a solid cell, in index order, then
+    // the claimed solid cells in the order they were claimed. Only the solid
+    // cells' entries are used, and only they are uploaded (with the solid
+    // list, from pinned staging; the copy is ordered before later kernels on
+    // the stream, and the staging is rewritten only once it has completed).
     void rebuild_mask_tables() noexcept
     {
         solid_count = 0;
         if (!has_mask) return;
-        std::uint32_t head = 0, tail = 0;
+        constexpr std::uint32_t kUnset = 0xFFFFFFFFu;
+        if (mask_upload_pending) {
+            dev.note(hipEventSynchronize(mask_uploaded));
+            mask_upload_pending = false;
+        }
         for (std::size_t i = 0; i < cells; ++i) {
-            if (mask[i] == 0) {
-                nearest[i] = static_cast<std::uint32_t>(i);
-                bfs[tail++] = static_cast<std::uint32_t>(i);
-            } else {
-                nearest[i] = 0xFFFFFFFFu;
-                solid_list[solid_count++] = static_cast<std::uint32_t>(i);
+            if (mask[i] != 0) {
+                nearest[i] = kUnset;
+                h_solid[solid_count++] = static_cast<std::uint32_t>(i);
             }
         }
-        while (head < tail) {
-            const std::uint32_t i = bfs[head++];
-            const std::uint32_t x = i % n, z = i / n;
-            const int ddx[4] = {1, -1, 0, 0};
-            const int ddz[4] = {0, 0, 1, -1};
+        const int ni = static_cast<int>(n);
+        const int ddx[4] = {1, -1, 0, 0};
+        const int ddz[4] = {0, 0, 1, -1};
+        // the fluid cells next to a solid cell, in index order
+        std::uint32_t edge = 0;
+        for (std::uint32_t k = 0; k < solid_count; ++k) {
+            const std::uint32_t i = h_solid[k];
+            const int x = static_cast<int>(i % n), z = static_cast<int>(i / n);
             for (int e = 0; e < 4; ++e) {
-                const int nx = static_cast<int>(x) + ddx[e];
-                const int nz = static_cast<int>(z) + ddz[e];
-                if (nx < 0 || nz < 0 || nx >= static_cast<int>(n) || nz >= static_cast<int>(n))
-                    continue;
+                const int nx = x + ddx[e], nz = z + ddz[e];
+                if (nx < 0 || nz < 0 || nx >= ni || nz >= ni) continue;
                 const std::uint32_t j = static_cast<std::uint32_t>(nz) * n + static_cast<std::uint32_t>(nx);
-                if (nearest[j] == 0xFFFFFFFFu) {
-                    nearest[j] = nearest[i];
-                    bfs[tail++] = j;
+                if (mask[j] == 0 && !frontier_mark[j]) {
+                    frontier_mark[j] = 1;
+                    frontier[edge++] = j;
                 }
             }
         }
-        // Synchronous upload: set_obstruction and recenter are not per frame.
-        dev.note(hipMemcpyAsync(d_solid, solid_list.data(), solid_count * sizeof(std::uint32_t),
+        std::sort(frontier.begin(), frontier.begin() + edge);
+        std::uint32_t tail = 0;
+        auto claim = [&](std::uint32_t i, std::uint32_t src) {
+            const int x = static_cast<int>(i % n), z = static_cast<int>(i / n);
+            for (int e = 0; e < 4; ++e) {
+                const int nx = x + ddx[e], nz = z + ddz[e];
+                if (nx < 0 || nz < 0 || nx >= ni || nz >= ni) continue;
+                const std::uint32_t j = static_cast<std::uint32_t>(nz) * n + static_cast<std::uint32_t>(nx);
+                if (mask[j] != 0 && nearest[j] == kUnset) {
+                    nearest[j] = src;
+                    bfs[tail++] = j;
+                }
+            }
+        };
+        for (std::uint32_t e = 0; e < edge; ++e) {
+            frontier_mark[frontier[e]] = 0;
+            claim(frontier[e], frontier[e]);
+        }
+        for (std::uint32_t head = 0; head < tail; ++head) claim(bfs[head], nearest[bfs[head]]);
+        for (std::uint32_t k = 0; k < solid_count; ++k) h_source[k] = nearest[h_solid[k]];
+        if (solid_count == 0) return;
+        dev.note(hipMemcpyAsync(d_solid, h_solid, solid_count * sizeof(std::uint32_t), hipMemcpyHostToDevice,
+                                dev.stream));
+        dev.note(hipMemcpyAsync(d_source, h_source, solid_count * sizeof(std::uint32_t),
                                 hipMemcpyHostToDevice, dev.stream));
-        dev.note(hipMemcpyAsync(d_nearest, nearest.data(), cells * sizeof(std::uint32_t),
-                                hipMemcpyHostToDevice, dev.stream));
-        dev.note(hipStreamSynchronize(dev.stream));
+        dev.note(hipEventRecord(mask_uploaded, dev.stream));
+        mask_upload_pending = true;
     }
 
     // The scalar part of inject() (reference src/interaction.cpp:322-348);
@@ -333,7 +371,7 @@ InteractionField::InteractionField(const InteractionDesc& d)
     m.c2 = reinterpret_cast<float*>(base + o_c2);
     m.out = reinterpret_cast<float*>(base + o_out);
     m.d_solid = reinterpret_cast<std::uint32_t*>(base + o_solid);
-    m.d_nearest = reinterpret_cast<std::uint32_t*>(base + o_near);
+    m.d_source = reinterpret_cast<std::uint32_t*>(base + o_near);
     m.d_stamps = reinterpret_cast<ocean_hip_stamp*>(base + o_stamps);
     m.d_means = reinterpret_cast<float*>(base + o_means);
     m.d_qxz = reinterpret_cast<float*>(base + o_qxz);
@@ -350,8 +388,12 @@ InteractionField::InteractionField(const InteractionDesc& d)
     m.mask.assign(m.cells, 0);
     m.mask_shift.assign(m.cells, 0);
     m.nearest.assign(m.cells, 0);
-    m.solid_list.assign(m.cells, 0);
     m.bfs.assign(m.cells, 0);
+    m.frontier.assign(m.cells, 0);
+    m.frontier_mark.assign(m.cells, 0);
+    for (std::uint32_t** b : {&m.h_solid, &m.h_source})
+        detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(b), m.cells * 4, hipHostMallocDefault), "hipHostMalloc");
+    detail::hip_check(hipEventCreateWithFlags(&m.mask_uploaded, hipEventDisableTiming), "hipEventCreate");
     m.queue.resize(d.max_sources);
 
     std::vector<float> h1(m.cells), h2(m.cells);
@@ -459,7 +501,7 @@ void InteractionField::update(float dt) noexcept
             m.dev.note(ocean_hip_iwave_inject(&f, m.d_stamps + first[sidx], c,
                                               m.d_means + first[sidx], s));
         if (m.has_mask && m.solid_count > 0)
-            m.dev.note(ocean_hip_iwave_obstruct(&f, m.d_solid, m.d_nearest, m.solid_count,
+            m.dev.note(ocean_hip_iwave_obstruct(&f, m.d_solid, m.d_source, m.solid_count,
                                                 dirichlet, s));
         m.dev.note(ocean_hip_iwave_step(&f, s));
         std::swap(m.cur_i, m.old_i);

@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void stamp_apply_kernel(ocean_hip_iwave f, 
 
 __global__ __launch_bounds__(kBlock) void obstruct_kernel(ocean_hip_iwave f,
                                                           const uint32_t* __restrict__ solid,
-                                                          const uint32_t* __restrict__ nearest,
+                                                          const uint32_t* __restrict__ source,
                                                           uint32_t count, int dirichlet)
 {
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void obstruct_kernel(ocean_hip_iwave f,
     if (dirichlet) {
         f.cur[dst] = 0.0f;
     } else {
-        const uint32_t src = nearest[i];
+        const uint32_t src = source[k];
         if (src == 0xFFFFFFFFu) return;
         f.cur[dst] = f.cur[size_t(src / f.n) * f.stride + (src % f.n)];
     }
@@ -399,12 +399,12 @@ extern "C" hipError_t ocean_hip_iwave_inject_args(const ocean_hip_iwave* f,
 }
 
 extern "C" hipError_t ocean_hip_iwave_obstruct(const ocean_hip_iwave* f, const uint32_t* solid,
-                                               const uint32_t* nearest, uint32_t count,
+                                               const uint32_t* source, uint32_t count,
                                                int dirichlet, hipStream_t s)
 {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(obstruct_kernel, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                       *f, solid, nearest, count, dirichlet);
+                       *f, solid, source, count, dirichlet);
     return hipGetLastError();
 }
 

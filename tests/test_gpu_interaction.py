@@ -177,6 +177,49 @@ def test_large_stamps_vs_oracle(lib):
         o.close()
 
 
+def test_random_blob_mask_neumann_vs_oracle(lib):
+    """Neumann walls on a mask of many concave blobs (about a third of the
+    cells solid, solid cells several cells from any fluid, nearest-fluid ties
+    everywhere) and an all-solid border row: the nearest-fluid table
+    (reference src/interaction.cpp:255-292) decides every solid cell's value,
+    so the field equals the oracle's only if its BFS order is the
+    reference's. Recentring shifts the mask; clearing it restores open
+    water."""
+    rng = np.random.default_rng(31)
+    n = 256
+    noise = rng.standard_normal((n // 8, n // 8))
+    blob = np.kron(noise, np.ones((8, 8)))
+    blob += 0.6 * rng.standard_normal((n, n))
+    mask = (blob > 0.45).astype(np.uint8)
+    mask[0, :] = 1
+    d = idesc(lib.ocean_interaction_desc_init, **dict(CONFIG3_INTERACTION, max_sources=16))
+    g, o = Field("gpu", lib, d), Field("orc", oracle(), d)
+    try:
+        g.set_obstruction(mask)
+        o.set_obstruction(mask)
+        srcs = config3_sources()
+        for k in range(24):
+            for rec in config3_disturbances(srcs, k / 60.0):
+                g.add(rec)
+                o.add(rec)
+            if k == 10:
+                assert g.recenter(14.1, 11.9) == o.recenter(14.1, 11.9)
+            if k == 16:
+                g.set_obstruction(mask[::-1].copy())
+                o.set_obstruction(mask[::-1].copy())
+            g.update(1 / 60)
+            o.update(1 / 60)
+            assert_bitexact(g.field(), o.field(), f"frame {k}")
+        g.set_obstruction(None)
+        o.set_obstruction(None)
+        g.update(1 / 60)
+        o.update(1 / 60)
+        assert_bitexact(g.field(), o.field(), "mask cleared")
+    finally:
+        g.close()
+        o.close()
+
+
 def test_combined_query_vs_oracle(lib):
     """ocean_sample_combined (WaterSurface, reference src/interaction.cpp:863-901)."""
     od = desc_from(dict(size=128, patch_length=64.0, wind_speed=14.0, choppiness=1.2,

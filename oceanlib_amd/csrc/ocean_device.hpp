@@ -28,6 +28,7 @@ void validate_desc(const OceanDesc& desc);
 
 // Largest batch one host-array query call stages at a time.
 inline constexpr std::uint32_t kQueryStage = 65536;
+inline constexpr std::uint32_t kMappedQueries = 64;
 
 struct OceanDevice {
     int device = 0;
@@ -54,6 +55,14 @@ struct OceanDevice {
     float* h_vel = nullptr;
     float* h_q_xz = nullptr;
     ocean_hip_surface* h_q_out = nullptr;
+    // Up to kMappedQueries points are sampled through mapped pinned memory:
+    // the query kernel reads the points from it and writes the results into
+    // it, so a single height_at is one launch and one wait. Allocated on the
+    // first such query; d_map stays null if the runtime cannot map it (then
+    // the staged copies are used).
+    void* h_map = nullptr;
+    void* d_map = nullptr;
+    bool map_tried = false;
 
     // Optional timing of the frame kernels inside an update loop: the row and
     // column dispatches of every `timing_every`-th frame record start and end

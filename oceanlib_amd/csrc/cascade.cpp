@@ -36,6 +36,7 @@ struct CascadeStack::Impl {
     ocean_hip_surface* q_out = nullptr;
     float* h_xz = nullptr;
     ocean_hip_surface* h_out = nullptr;
+    detail::MappedQueries mapped;  // a few points: no staged copies
 
     explicit Impl(std::span<const CascadeLevel> levels) : batch(make_levels(levels)) {}
     ~Impl()
@@ -124,6 +125,14 @@ bool CascadeStack::sample_batch(const float* xz, std::uint32_t count, Surface* o
     auto& od = m.first();
     od.make_current();
     const auto levels = static_cast<std::uint32_t>(m.batch.level_count());
+    if (count <= detail::kMappedQueries && m.mapped.ready()) {
+        std::memcpy(m.mapped.xz_host(), xz, std::size_t(count) * 8);
+        od.note(ocean_hip_cascade_sample(m.views(), levels, m.mapped.xz_dev(), count,
+                                         m.mapped.out_dev<ocean_hip_surface>(), nullptr, m.stream()));
+        od.note(hipStreamSynchronize(m.stream()));
+        std::memcpy(out, m.mapped.out_host(), std::size_t(count) * sizeof(ocean_hip_surface));
+        return od.status == 0;
+    }
     for (std::uint32_t done = 0; done < count;) {
         const std::uint32_t k = std::min(detail::kQueryStage, count - done);
         std::memcpy(m.h_xz, xz + 2 * std::size_t(done), std::size_t(k) * 8);

@@ -90,6 +90,7 @@ struct InteractionField::Impl {
     std::uint32_t* d_solid = nullptr;
     std::uint32_t* d_source = nullptr;  // per solid cell: its nearest fluid cell
     ocean_hip_stamp* d_stamps = nullptr;
+    detail::MappedQueries mapped;  // host queries of a few points: no staged copies
     float* d_means = nullptr;
     float* d_qxz = nullptr;
     float* d_qout = nullptr;
@@ -608,6 +609,14 @@ bool InteractionField::sample_batch(const float* xz, std::uint32_t count,
     Impl& m = *impl_;
     m.dev.make_current();
     const ocean_hip_iwave f = m.view();
+    if (count <= detail::kMappedQueries && m.mapped.ready()) {
+        std::memcpy(m.mapped.xz_host(), xz, std::size_t(count) * 8);
+        m.dev.note(ocean_hip_iwave_sample(&f, m.mapped.xz_dev(), count, m.mapped.out_dev<float>(),
+                                          m.dev.stream));
+        m.dev.note(hipStreamSynchronize(m.dev.stream));
+        std::memcpy(out, m.mapped.out_host(), std::size_t(count) * 16);
+        return m.dev.status == 0;
+    }
     for (std::uint32_t done = 0; done < count;) {
         const std::uint32_t k = std::min(detail::kQueryStage, count - done);
         std::memcpy(m.h_qxz, xz + 2 * std::size_t(done), std::size_t(k) * 8);
@@ -684,6 +693,15 @@ bool InteractionField::sample_combined_batch(const void* device_levels, std::uin
     m.dev.make_current();
     const ocean_hip_iwave f = m.view();
     auto* d_out = reinterpret_cast<ocean_hip_surface*>(m.d_qout);
+    if (count <= detail::kMappedQueries && m.mapped.ready()) {
+        std::memcpy(m.mapped.xz_host(), xz, std::size_t(count) * 8);
+        m.dev.note(ocean_hip_combined_sample(static_cast<const ocean_hip_level*>(device_levels),
+                                             level_count, as_cascade ? 1 : 0, &f, m.mapped.xz_dev(), count,
+                                             m.mapped.out_dev<ocean_hip_surface>(), m.dev.stream));
+        m.dev.note(hipStreamSynchronize(m.dev.stream));
+        std::memcpy(out, m.mapped.out_host(), std::size_t(count) * sizeof(ocean_hip_surface));
+        return m.dev.status == 0;
+    }
     for (std::uint32_t done = 0; done < count;) {
         const std::uint32_t k = std::min(detail::kQueryStage, count - done);
         std::memcpy(m.h_qxz, xz + 2 * std::size_t(done), std::size_t(k) * 8);

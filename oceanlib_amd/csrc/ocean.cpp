@@ -135,7 +135,6 @@ OceanDevice::~OceanDevice()
     if (h_vel) hipHostFree(h_vel);
     if (h_q_xz) hipHostFree(h_q_xz);
     if (h_q_out) hipHostFree(h_q_out);
-    if (h_map) hipHostFree(h_map);
     for (auto& slot : timing_ev)
         for (auto& e : slot)
             if (e) hipEventDestroy(e);
@@ -312,27 +311,13 @@ bool OceanDevice::sample_host(const float* xz, std::uint32_t count, Surface* out
 {
     static_assert(sizeof(Surface) == sizeof(ocean_hip_surface));
     make_current();
-    if (count <= kMappedQueries) {
-        constexpr std::size_t kOut = std::size_t(kMappedQueries) * 8;  // results follow the points
-        if (!map_tried) {
-            map_tried = true;
-            const std::size_t bytes = kOut + std::size_t(kMappedQueries) * sizeof(ocean_hip_surface);
-            if (hipHostMalloc(&h_map, bytes, hipHostMallocMapped) == hipSuccess) {
-                if (hipHostGetDevicePointer(&d_map, h_map, 0) != hipSuccess) d_map = nullptr;
-            } else {
-                h_map = nullptr;
-            }
-            (void)hipGetLastError();  // a refused mapping leaves the staged path; clear it
-        }
-        if (d_map != nullptr) {
-            std::memcpy(h_map, xz, std::size_t(count) * 8);
-            char* dev = static_cast<char*>(d_map);
-            note(ocean_hip_sample(&level, reinterpret_cast<const float*>(dev), count,
-                                  reinterpret_cast<ocean_hip_surface*>(dev + kOut), nullptr, stream));
-            note(hipStreamSynchronize(stream));
-            std::memcpy(out, static_cast<const char*>(h_map) + kOut, std::size_t(count) * sizeof(Surface));
-            return status == 0;
-        }
+    if (count <= kMappedQueries && mapped.ready()) {
+        std::memcpy(mapped.xz_host(), xz, std::size_t(count) * 8);
+        note(ocean_hip_sample(&level, mapped.xz_dev(), count, mapped.out_dev<ocean_hip_surface>(), nullptr,
+                              stream));
+        note(hipStreamSynchronize(stream));
+        std::memcpy(out, mapped.out_host(), std::size_t(count) * sizeof(Surface));
+        return status == 0;
     }
     if (q_xz == nullptr) {  // a batch level's first host query: its staging, once
         void* d = nullptr;

@@ -30,6 +30,51 @@ void validate_desc(const OceanDesc& desc);
 inline constexpr std::uint32_t kQueryStage = 65536;
 inline constexpr std::uint32_t kMappedQueries = 64;
 
+// Up to kMappedQueries host queries go through mapped pinned memory: the
+// query kernel reads the points from it and writes the results into it, so a
+// single query is one launch and one wait. Allocated on the first use;
+// ready() stays false if the runtime cannot map it (then the callers use
+// their staged copies). Points at the start, results from kOut on.
+struct MappedQueries {
+    static constexpr std::size_t kOut = std::size_t(kMappedQueries) * 8;
+    static constexpr std::size_t kResultBytes = 128;  // largest result struct per point
+    void* host = nullptr;
+    char* dev = nullptr;
+    bool tried = false;
+
+    bool ready() noexcept
+    {
+        if (!tried) {
+            tried = true;
+            if (hipHostMalloc(&host, kOut + std::size_t(kMappedQueries) * kResultBytes, hipHostMallocMapped) ==
+                hipSuccess) {
+                void* d = nullptr;
+                if (hipHostGetDevicePointer(&d, host, 0) == hipSuccess) dev = static_cast<char*>(d);
+            } else {
+                host = nullptr;
+            }
+            (void)hipGetLastError();  // a refused mapping leaves the staged path; clear it
+        }
+        return dev != nullptr;
+    }
+    float* xz_host() const noexcept { return static_cast<float*>(host); }
+    const float* xz_dev() const noexcept { return reinterpret_cast<const float*>(dev); }
+    template <typename T>
+    T* out_dev() const noexcept
+    {
+        static_assert(sizeof(T) <= kResultBytes);
+        return reinterpret_cast<T*>(dev + kOut);
+    }
+    const void* out_host() const noexcept { return static_cast<const char*>(host) + kOut; }
+    ~MappedQueries()
+    {
+        if (host) hipHostFree(host);
+    }
+    MappedQueries() = default;
+    MappedQueries(const MappedQueries&) = delete;
+    MappedQueries& operator=(const MappedQueries&) = delete;
+};
+
 struct OceanDevice {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -55,14 +100,7 @@ struct OceanDevice {
     float* h_vel = nullptr;
     float* h_q_xz = nullptr;
     ocean_hip_surface* h_q_out = nullptr;
-    // Up to kMappedQueries points are sampled through mapped pinned memory:
-    // the query kernel reads the points from it and writes the results into
-    // it, so a single height_at is one launch and one wait. Allocated on the
-    // first such query; d_map stays null if the runtime cannot map it (then
-    // the staged copies are used).
-    void* h_map = nullptr;
-    void* d_map = nullptr;
-    bool map_tried = false;
+    MappedQueries mapped;
 
     // Optional timing of the frame kernels inside an update loop: the row and
     // column dispatches of every `timing_every`-th frame record start and end

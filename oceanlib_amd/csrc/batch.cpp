@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <vector>
 
@@ -71,6 +72,7 @@ struct OceanBatch::Impl {
     unsigned long long* d_sums = nullptr;  // one checksum per level
     unsigned long long* h_sums = nullptr;
     int cus = 1;
+    std::mutex query_mu;  // concurrent const queries share the staging
     double time = 0.0;
     int status = 0;
 
@@ -277,6 +279,7 @@ bool OceanBatch::sample(std::size_t first, std::size_t count, const float* xz, s
     if (count == 0 || first + count > m.levels.size()) return false;
     if (n == 0) return true;
     if (xz == nullptr || out == nullptr) return false;
+    std::lock_guard<std::mutex> lock(m.query_mu);
     for (std::uint32_t done = 0; done < n;) {
         const std::uint32_t k = std::min(detail::kQueryStage, n - done);
         std::memcpy(m.h_xz, xz + 2 * std::size_t(done), std::size_t(k) * 8);

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <vector>
 
@@ -37,6 +38,7 @@ struct CascadeStack::Impl {
     float* h_xz = nullptr;
     ocean_hip_surface* h_out = nullptr;
     detail::MappedQueries mapped;  // a few points: no staged copies
+    std::mutex query_mu;           // concurrent const queries share the staging
 
     explicit Impl(std::span<const CascadeLevel> levels) : batch(make_levels(levels)) {}
     ~Impl()
@@ -122,6 +124,7 @@ bool CascadeStack::sample_batch(const float* xz, std::uint32_t count, Surface* o
     if (count == 0) return true;
     if (xz == nullptr || out == nullptr) return false;
     Impl& m = *impl_;
+    std::lock_guard<std::mutex> lock(m.query_mu);
     auto& od = m.first();
     od.make_current();
     const auto levels = static_cast<std::uint32_t>(m.batch.level_count());

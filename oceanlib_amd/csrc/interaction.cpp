@@ -15,6 +15,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -91,6 +92,7 @@ struct InteractionField::Impl {
     std::uint32_t* d_source = nullptr;  // per solid cell: its nearest fluid cell
     ocean_hip_stamp* d_stamps = nullptr;
     detail::MappedQueries mapped;  // host queries of a few points: no staged copies
+    std::mutex query_mu;           // concurrent const queries share the staging
     float* d_means = nullptr;
     float* d_qxz = nullptr;
     float* d_qout = nullptr;
@@ -607,6 +609,7 @@ bool InteractionField::sample_batch(const float* xz, std::uint32_t count,
     if (count == 0) return true;
     if (xz == nullptr || out == nullptr) return false;
     Impl& m = *impl_;
+    std::lock_guard<std::mutex> lock(m.query_mu);
     m.dev.make_current();
     const ocean_hip_iwave f = m.view();
     if (count <= detail::kMappedQueries && m.mapped.ready()) {
@@ -646,6 +649,7 @@ float InteractionField::height_at(float world_x, float world_z) const noexcept
 InteractionBuffers InteractionField::buffers() const noexcept
 {
     Impl& m = *impl_;
+    std::lock_guard<std::mutex> lock(m.query_mu);
     if (m.mirror_stale) {
         m.dev.make_current();
         m.dev.note(hipMemcpyAsync(m.h_out, m.out, m.cells * 16, hipMemcpyDeviceToHost, m.dev.stream));
@@ -690,6 +694,7 @@ bool InteractionField::sample_combined_batch(const void* device_levels, std::uin
     if (count == 0) return true;
     if (xz == nullptr || out == nullptr || device_levels == nullptr) return false;
     Impl& m = *impl_;
+    std::lock_guard<std::mutex> lock(m.query_mu);
     m.dev.make_current();
     const ocean_hip_iwave f = m.view();
     auto* d_out = reinterpret_cast<ocean_hip_surface*>(m.d_qout);

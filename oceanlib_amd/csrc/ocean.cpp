@@ -295,6 +295,7 @@ void OceanDevice::alloc_host_mirrors() noexcept
 
 void OceanDevice::refresh_mirror() noexcept
 {
+    std::lock_guard<std::mutex> lock(query_mu);
     if (!mirror_stale) return;
     make_current();
     alloc_host_mirrors();
@@ -310,6 +311,7 @@ void OceanDevice::refresh_mirror() noexcept
 bool OceanDevice::sample_host(const float* xz, std::uint32_t count, Surface* out) noexcept
 {
     static_assert(sizeof(Surface) == sizeof(ocean_hip_surface));
+    std::lock_guard<std::mutex> lock(query_mu);
     make_current();
     if (count <= kMappedQueries && mapped.ready()) {
         std::memcpy(mapped.xz_host(), xz, std::size_t(count) * 8);

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -101,6 +102,9 @@ struct OceanDevice {
     float* h_q_xz = nullptr;
     ocean_hip_surface* h_q_out = nullptr;
     MappedQueries mapped;
+    // Queries are const and may run concurrently on one handle (reference
+    // contract); they share the staging above, so each holds this lock.
+    std::mutex query_mu;
 
     // Optional timing of the frame kernels inside an update loop: the row and
     // column dispatches of every `timing_every`-th frame record start and end

@@ -1404,6 +1404,52 @@ void threads_repeated_stable()  // tests/test_threading.cpp:63-87
     }
 }
 
+// No reference counterpart as a test: the reference's queries are const and
+// may run concurrently on one handle. Here they share a handle's staging
+// (and its stream), so they take a lock; four threads querying one ocean and
+// one cascade at once, single points and batches, get the single-threaded
+// bits.
+void threads_concurrent_queries()
+{
+    Ocean sim{make_desc(128, 200.0f, 4242, 1.0f, 13.0f, 0.9f)};
+    sim.update(2.25);
+    std::vector<OceanDesc> levels{make_desc(64, 300.0f, 7, 1.0f, 12.0f, 0.8f),
+                                  make_desc(64, 60.0f, 8, 1.0f, 12.0f, 0.8f)};
+    CascadeStack stack{std::span<const OceanDesc>(levels)};
+    stack.update(2.25);
+    constexpr int kPoints = 600;
+    std::vector<float> xz(2 * kPoints);
+    for (int i = 0; i < kPoints; ++i) {
+        xz[2 * i] = 0.37f * float(i) - 40.0f;
+        xz[2 * i + 1] = 25.0f - 0.21f * float(i);
+    }
+    std::vector<float> want_h(kPoints), want_c(kPoints);
+    for (int i = 0; i < kPoints; ++i) {
+        want_h[i] = sim.height_at(xz[2 * i], xz[2 * i + 1]);
+        want_c[i] = stack.sample_at(xz[2 * i], xz[2 * i + 1]).height;
+    }
+    std::vector<Surface> want_b(kPoints);
+    EXPECT(sim.sample_batch(xz.data(), kPoints, want_b.data()));
+    std::atomic<int> bad{0};
+    auto worker = [&](int t) {
+        std::vector<Surface> got(kPoints);
+        for (int rep = 0; rep < 4; ++rep) {
+            for (int i = t; i < kPoints; i += 3) {
+                const float h = sim.height_at(xz[2 * i], xz[2 * i + 1]);
+                const float c = stack.sample_at(xz[2 * i], xz[2 * i + 1]).height;
+                if (!same_bits(&h, &want_h[i], 1) || !same_bits(&c, &want_c[i], 1)) ++bad;
+            }
+            if (!sim.sample_batch(xz.data(), kPoints, got.data()) ||
+                std::memcmp(got.data(), want_b.data(), sizeof(Surface) * kPoints) != 0)
+                ++bad;
+        }
+    };
+    std::vector<std::thread> threads;
+    for (int t = 0; t < 4; ++t) threads.emplace_back(worker, t);
+    for (std::thread& th : threads) th.join();
+    EXPECT(bad.load() == 0);
+}
+
 }  // namespace
 
 namespace cases {
@@ -1472,6 +1518,7 @@ const Case kFrameCases[] = {
     {"capi.status_version_simd", capi_status_version_simd},
     {"threads.bitidentical_thread_count", threads_bitidentical_thread_count},
     {"threads.repeated_stable", threads_repeated_stable},
+    {"threads.concurrent_queries", threads_concurrent_queries},
 };
 const std::size_t kFrameCaseCount = sizeof(kFrameCases) / sizeof(kFrameCases[0]);
 

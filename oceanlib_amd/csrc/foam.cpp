@@ -192,8 +192,9 @@ void FoamField::clear() noexcept
 const float* FoamField::data() const noexcept
 {
     Impl& m = *impl_;
+    auto& od = m.ocean->device();
+    std::lock_guard<std::mutex> lock(od.query_mu);  // shares the ocean's query staging and stream
     if (m.stale) {
-        auto& od = m.ocean->device();
         od.make_current();
         od.note(hipMemcpyAsync(m.host, m.cur, m.cells * sizeof(float), hipMemcpyDeviceToHost,
                                od.stream));
@@ -228,6 +229,7 @@ float FoamField::foam_at(float world_x, float world_z) const noexcept
 {
     const Impl& m = *impl_;
     auto& od = m.ocean->device();
+    std::lock_guard<std::mutex> lock(od.query_mu);
     od.make_current();
     float* hx = od.h_q_xz;
     hx[0] = world_x;
@@ -245,6 +247,7 @@ double FoamField::coverage() const noexcept
 {
     const Impl& m = *impl_;
     auto& od = m.ocean->device();
+    std::lock_guard<std::mutex> lock(od.query_mu);
     od.make_current();
     double* out = m.scratch + ocean_hip_coverage_blocks(static_cast<std::uint32_t>(m.cells));
     od.note(ocean_hip_coverage(m.cur, static_cast<std::uint32_t>(m.cells), m.scratch, out,

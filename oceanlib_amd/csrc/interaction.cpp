@@ -583,6 +583,7 @@ bool InteractionField::pack_rgba16(std::uint16_t* out, bool on_device) const noe
 {
     if (out == nullptr || reinterpret_cast<std::uintptr_t>(out) % 8 != 0) return false;
     Impl& m = *impl_;
+    std::lock_guard<std::mutex> lock(m.query_mu);
     m.dev.make_current();
     const std::uint32_t cells = m.n * m.n;
     const std::size_t bytes = std::size_t(cells) * 8;
@@ -676,6 +677,7 @@ float InteractionField::kernel_dispersion_error() const noexcept
 double InteractionField::energy() const noexcept
 {
     Impl& m = *impl_;
+    std::lock_guard<std::mutex> lock(m.query_mu);
     m.dev.make_current();
     const ocean_hip_iwave f = m.view();
     double* res = m.d_energy + 2 * m.cells;

@@ -445,6 +445,7 @@ bool Ocean::pack_rgba16(const FoamField* foam, std::uint16_t* displacement, std:
     auto aligned = [](const void* p) { return reinterpret_cast<std::uintptr_t>(p) % 8 == 0; };
     if (!aligned(displacement) || !aligned(normal)) return false;
     detail::OceanDevice& d = impl_->dev;
+    std::lock_guard<std::mutex> lock(d.query_mu);
     d.make_current();
     const std::uint32_t cells = d.level.n * d.level.n;
     const std::size_t bytes = std::size_t(cells) * 8;

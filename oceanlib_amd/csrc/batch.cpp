@@ -13,6 +13,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -74,11 +75,13 @@ struct OceanBatch::Impl {
     int cus = 1;
     std::mutex query_mu;  // concurrent const queries share the staging
     double time = 0.0;
-    int status = 0;
+    std::atomic<int> status{0};
 
     void note(hipError_t e) noexcept
     {
-        if (e != hipSuccess && status == 0) status = static_cast<int>(e);
+        // first error wins; queries may note concurrently on one handle
+        int none = 0;
+        if (e != hipSuccess) status.compare_exchange_strong(none, static_cast<int>(e));
     }
 
     ~Impl()
@@ -354,7 +357,7 @@ void OceanBatch::autotune(std::uint32_t reps) noexcept
     try {
         m.rebuild_graph();
     } catch (...) {
-        m.status = m.status ? m.status : static_cast<int>(hipErrorUnknown);
+        m.note(hipErrorUnknown);
     }
     m.note(hipStreamSynchronize(m.stream.s));
 }

@@ -231,15 +231,18 @@ float FoamField::foam_at(float world_x, float world_z) const noexcept
     auto& od = m.ocean->device();
     std::lock_guard<std::mutex> lock(od.query_mu);
     od.make_current();
-    float* hx = od.h_q_xz;
+    // The point and the result go through the field's own pinned words
+    // (host_cov[2], host_cov[3]): the ocean's host staging does not exist on
+    // the levels of a cascade or batch, which allocate it only on demand.
+    float* hx = reinterpret_cast<float*>(m.host_cov + 2);
     hx[0] = world_x;
     hx[1] = world_z;
     od.note(hipMemcpyAsync(m.q_xz, hx, 8, hipMemcpyHostToDevice, od.stream));
     od.note(ocean_hip_foam_query(&od.level, m.cur, m.cell, m.q_xz, 1, m.q_out, od.stream));
     float r = 0.0f;
-    od.note(hipMemcpyAsync(m.host_cov, m.q_out, 4, hipMemcpyDeviceToHost, od.stream));
+    od.note(hipMemcpyAsync(m.host_cov + 3, m.q_out, 4, hipMemcpyDeviceToHost, od.stream));
     od.note(hipStreamSynchronize(od.stream));
-    std::memcpy(&r, m.host_cov, 4);
+    std::memcpy(&r, m.host_cov + 3, 4);
     return r;
 }
 

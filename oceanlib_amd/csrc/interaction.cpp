@@ -14,6 +14,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
 #include <algorithm>
 #include <mutex>
 #include <cmath>
@@ -28,10 +29,12 @@ namespace detail {
 struct InteractionDevice {
     int device = 0;
     hipStream_t stream = nullptr;
-    int status = 0;
+    std::atomic<int> status{0};
     void note(hipError_t e) noexcept
     {
-        if (e != hipSuccess && status == 0) status = static_cast<int>(e);
+        // first error wins; queries may note concurrently on one handle
+        int none = 0;
+        if (e != hipSuccess) status.compare_exchange_strong(none, static_cast<int>(e));
     }
     void make_current() const noexcept
     {

@@ -322,12 +322,25 @@ bool OceanDevice::sample_host(const float* xz, std::uint32_t count, Surface* out
         return status == 0;
     }
     if (q_xz == nullptr) {  // a batch level's first host query: its staging, once
+        // All three or none: on a partial failure the ones that succeeded are
+        // released, so a later call starts from nothing again.
         void* d = nullptr;
-        note(hipMalloc(&d, std::size_t(kQueryStage) * (8 + sizeof(ocean_hip_surface))));
-        note(hipHostMalloc(reinterpret_cast<void**>(&h_q_xz), std::size_t(kQueryStage) * 8, hipHostMallocDefault));
-        note(hipHostMalloc(reinterpret_cast<void**>(&h_q_out), std::size_t(kQueryStage) * sizeof(ocean_hip_surface), hipHostMallocDefault));
-        if (status != 0) return false;
+        void* hx = nullptr;
+        void* ho = nullptr;
+        const hipError_t e0 = hipMalloc(&d, std::size_t(kQueryStage) * (8 + sizeof(ocean_hip_surface)));
+        const hipError_t e1 = hipHostMalloc(&hx, std::size_t(kQueryStage) * 8, hipHostMallocDefault);
+        const hipError_t e2 = hipHostMalloc(&ho, std::size_t(kQueryStage) * sizeof(ocean_hip_surface),
+                                            hipHostMallocDefault);
+        if (e0 != hipSuccess || e1 != hipSuccess || e2 != hipSuccess) {
+            if (e0 == hipSuccess) hipFree(d);
+            if (e1 == hipSuccess) hipHostFree(hx);
+            if (e2 == hipSuccess) hipHostFree(ho);
+            note(e0 != hipSuccess ? e0 : e1 != hipSuccess ? e1 : e2);
+            return false;
+        }
         q_stage = d;
+        h_q_xz = static_cast<float*>(hx);
+        h_q_out = static_cast<ocean_hip_surface*>(ho);
         q_xz = static_cast<float*>(d);
         q_out = reinterpret_cast<ocean_hip_surface*>(q_xz + 2 * std::size_t(kQueryStage));
     }
@@ -368,7 +381,7 @@ Ocean::Ocean(const OceanDesc& desc, void* shared_stream, bool lazy_host)
     detail::hip_check(hipStreamSynchronize(impl_->dev.stream), "initial frame");
     if (impl_->dev.status != 0)
         throw std::runtime_error("ocean: initial frame failed: " +
-                                 std::string(hipGetErrorString(hipError_t(impl_->dev.status))));
+                                 std::string(hipGetErrorString(hipError_t(impl_->dev.status.load()))));
 }
 
 Ocean::~Ocean() = default;

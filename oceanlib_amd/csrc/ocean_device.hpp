@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
 #include <cstdint>
 #include <mutex>
 #include <stdexcept>
@@ -117,7 +118,7 @@ struct OceanDevice {
     hipEvent_t timing_ev[kTimingSlots][4] = {};
 
     bool mirror_stale = true;
-    int status = 0;            // first HIP error seen (sticky)
+    std::atomic<int> status{0};  // first HIP error seen (sticky)
     std::uint32_t init_repaired = 0;  // table entries the host recomputed at creation
 
     OceanDevice() = default;
@@ -133,7 +134,9 @@ struct OceanDevice {
 
     void note(hipError_t e) noexcept
     {
-        if (e != hipSuccess && status == 0) status = static_cast<int>(e);
+        // first error wins; queries may note concurrently on one handle
+        int none = 0;
+        if (e != hipSuccess) status.compare_exchange_strong(none, static_cast<int>(e));
     }
     void make_current() const noexcept;
     void enqueue_frame(double time) noexcept;

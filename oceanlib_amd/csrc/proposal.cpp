@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -38,11 +39,13 @@ struct HeightProposal::Impl {
     hipEvent_t done = nullptr;         // last work enqueued on `stream`
     std::uint32_t max_queries = 0;
     std::uint32_t step = 0;            // Adam step count
-    int status = 0;
+    std::atomic<int> status{0};
 
     void note(hipError_t e) noexcept
     {
-        if (e != hipSuccess && status == 0) status = static_cast<int>(e);
+        // first error wins; queries may note concurrently on one handle
+        int none = 0;
+        if (e != hipSuccess) status.compare_exchange_strong(none, static_cast<int>(e));
     }
     void mark() noexcept { note(hipEventRecord(done, stream.s)); }
 

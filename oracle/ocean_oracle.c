@@ -858,3 +858,11 @@ EXPORT float orc_foam_at(const orc_foam* m, float wx, float wz)
     const float v = (wz - s.offset_z) / m->cell;
     return sample_wrap(m->cur, u, v, (int)m->n);
 }
+
+/* foam.cpp:307-313: the sequential double sum of the field over the cells. */
+EXPORT double orc_foam_coverage(const orc_foam* m)
+{
+    double sum = 0.0;
+    for (size_t i = 0; i < m->cells; ++i) sum += m->cur[i];
+    return sum / (double)m->cells;
+}

@@ -206,6 +206,7 @@ OREF void oref_foam_update(void* f, float dt) { static_cast<FoamField*>(f)->upda
 OREF const float* oref_foam_data(void* f) { return static_cast<FoamField*>(f)->data(); }
 OREF std::uint32_t oref_foam_last_substeps(void* f) { return static_cast<FoamField*>(f)->last_substeps(); }
 OREF float oref_foam_at(void* f, float x, float z) { return static_cast<FoamField*>(f)->foam_at(x, z); }
+OREF double oref_foam_coverage(void* f) { return static_cast<FoamField*>(f)->coverage(); }
 
 // --- interaction (iWave) ----------------------------------------------------
 

@@ -22,8 +22,10 @@
 // and run by tests/test_gpu_reference_cases.py.
 #include "harness.hpp"
 
+#include "ocean/batch.hpp"
 #include "ocean/cascade.hpp"
 #include "ocean/foam.hpp"
+#include "ocean/interaction.hpp"
 #include "ocean/ocean.h"
 #include "ocean/ocean.hpp"
 #include "ocean/ocean_amd.h"
@@ -1450,6 +1452,131 @@ void threads_concurrent_queries()
     EXPECT(bad.load() == 0);
 }
 
+// The other const queries on shared handles, four threads at once against
+// the single-threaded bits: FoamField::foam_at / coverage, InteractionField
+// and WaterSurface sample_at, and OceanBatch::sample over one level and over
+// a cascade of two. Every one of them shares its handle's staging and stream.
+void threads_concurrent_other_queries()
+{
+    OceanDesc od = make_desc(64, 80.0f, 4243, 1.2f, 18.0f, 0.9f);
+    od.compute_velocity = true;
+    Ocean sim{od};
+    sim.update(3.0);
+    FoamDesc fd;
+    fd.source_gain = 6.0f;
+    FoamField foam{sim, fd};
+    for (int i = 0; i < 20; ++i) foam.update(1.0f / 60.0f);
+    InteractionDesc idc;
+    idc.size = 64;
+    idc.extent = 12.8f;
+    InteractionField field{idc};
+    Disturbance dist;
+    dist.world_x = 6.0f;
+    dist.world_z = 6.5f;
+    dist.radius = 0.8f;
+    dist.strength = 0.2f;
+    dist.velocity_y = -1.0f;
+    for (int i = 0; i < 12; ++i) {
+        field.add(dist);
+        field.update(1.0f / 60.0f);
+    }
+    WaterSurface surface{sim, field};
+    std::vector<OceanDesc> levels{make_desc(64, 300.0f, 7, 1.0f, 12.0f, 0.8f),
+                                  make_desc(64, 60.0f, 8, 1.0f, 12.0f, 0.8f)};
+    OceanBatch batch{std::span<const OceanDesc>(levels)};
+    batch.update(2.5);
+    constexpr int kPoints = 300;
+    std::vector<float> xz(2 * kPoints);
+    for (int i = 0; i < kPoints; ++i) {
+        xz[2 * i] = 0.053f * float(i) - 1.0f;
+        xz[2 * i + 1] = 13.0f - 0.041f * float(i);
+    }
+    std::vector<float> want_f(kPoints);
+    std::vector<InteractionSample> want_i(kPoints);
+    std::vector<Surface> want_w(kPoints), want_b1(kPoints), want_b2(kPoints);
+    for (int i = 0; i < kPoints; ++i) {
+        want_f[i] = foam.foam_at(xz[2 * i], xz[2 * i + 1]);
+        want_i[i] = field.sample_at(xz[2 * i], xz[2 * i + 1]);
+        want_w[i] = surface.sample_at(xz[2 * i], xz[2 * i + 1]);
+    }
+    const double want_cov = foam.coverage();
+    EXPECT(want_cov > 0.0);
+    EXPECT(batch.sample(0, 1, xz.data(), kPoints, want_b1.data()));
+    EXPECT(batch.sample(0, 2, xz.data(), kPoints, want_b2.data()));
+    std::atomic<int> bad{0};
+    auto worker = [&](int t) {
+        std::vector<Surface> got(kPoints);
+        for (int rep = 0; rep < 3; ++rep) {
+            for (int i = t; i < kPoints; i += 5) {
+                const float f = foam.foam_at(xz[2 * i], xz[2 * i + 1]);
+                const InteractionSample e = field.sample_at(xz[2 * i], xz[2 * i + 1]);
+                const Surface w = surface.sample_at(xz[2 * i], xz[2 * i + 1]);
+                if (!same_bits(&f, &want_f[i], 1) ||
+                    std::memcmp(&e, &want_i[i], sizeof e) != 0 ||
+                    std::memcmp(&w, &want_w[i], sizeof w) != 0)
+                    ++bad;
+            }
+            const double c = foam.coverage();
+            if (std::memcmp(&c, &want_cov, sizeof c) != 0) ++bad;
+            if (!batch.sample(0, 1, xz.data(), kPoints, got.data()) ||
+                std::memcmp(got.data(), want_b1.data(), sizeof(Surface) * kPoints) != 0)
+                ++bad;
+            if (!batch.sample(0, 2, xz.data(), kPoints, got.data()) ||
+                std::memcmp(got.data(), want_b2.data(), sizeof(Surface) * kPoints) != 0)
+                ++bad;
+        }
+    };
+    std::vector<std::thread> threads;
+    for (int t = 0; t < 4; ++t) threads.emplace_back(worker, t);
+    for (std::thread& th : threads) th.join();
+    EXPECT(bad.load() == 0);
+    EXPECT(sim.last_error() == 0 && field.last_error() == 0);
+}
+
+// A FoamField on a cascade's level and on a batch's level (the reference
+// builds foam fields on stack.level(i): include/ocean/cascade.hpp:72-79).
+// Those levels hold no host query staging of their own; foam_at, coverage
+// and the field must equal a FoamField on a plain Ocean of the same desc.
+void foam_on_cascade_and_batch_levels()
+{
+    OceanDesc a = make_desc(64, 120.0f, 31, 1.3f, 20.0f, 0.9f);
+    OceanDesc b = make_desc(128, 30.0f, 32, 1.1f, 20.0f, 0.9f);
+    a.compute_velocity = b.compute_velocity = true;
+    std::vector<OceanDesc> levels{a, b};
+    CascadeStack stack{std::span<const OceanDesc>(levels)};
+    OceanBatch batch{std::span<const OceanDesc>(levels)};
+    Ocean plain_a{a}, plain_b{b};
+    FoamDesc fd;
+    fd.source_gain = 5.0f;
+    FoamField on_stack{stack.level(0), fd}, on_batch{batch.level(1), fd};
+    FoamField ref_a{plain_a, fd}, ref_b{plain_b, fd};
+    for (int k = 0; k < 30; ++k) {
+        const double t = 1.0 + k / 60.0;
+        stack.update(t);
+        batch.update(t);
+        plain_a.update(t);
+        plain_b.update(t);
+        on_stack.update(1.0f / 60.0f);
+        on_batch.update(1.0f / 60.0f);
+        ref_a.update(1.0f / 60.0f);
+        ref_b.update(1.0f / 60.0f);
+    }
+    const std::size_t ca = std::size_t(a.size) * a.size, cb = std::size_t(b.size) * b.size;
+    EXPECT(same_bits(on_stack.data(), ref_a.data(), ca));
+    EXPECT(same_bits(on_batch.data(), ref_b.data(), cb));
+    const double cs = on_stack.coverage(), cra = ref_a.coverage();
+    const double cbt = on_batch.coverage(), crb = ref_b.coverage();
+    EXPECT(cs > 0.0 && std::memcmp(&cs, &cra, sizeof cs) == 0);
+    EXPECT(cbt > 0.0 && std::memcmp(&cbt, &crb, sizeof cbt) == 0);
+    for (int i = 0; i < 200; ++i) {
+        const float x = 0.91f * float(i) - 40.0f, z = 70.0f - 0.63f * float(i);
+        const float fs = on_stack.foam_at(x, z), fa = ref_a.foam_at(x, z);
+        const float fb = on_batch.foam_at(x, z), fr = ref_b.foam_at(x, z);
+        if (!same_bits(&fs, &fa, 1) || !same_bits(&fb, &fr, 1)) fail("foam_at differs at point %d", i);
+    }
+    EXPECT(stack.level(0).last_error() == 0 && batch.level(1).last_error() == 0);
+}
+
 }  // namespace
 
 namespace cases {
@@ -1519,6 +1646,8 @@ const Case kFrameCases[] = {
     {"threads.bitidentical_thread_count", threads_bitidentical_thread_count},
     {"threads.repeated_stable", threads_repeated_stable},
     {"threads.concurrent_queries", threads_concurrent_queries},
+    {"threads.concurrent_other_queries", threads_concurrent_other_queries},
+    {"foam.on_cascade_and_batch_levels", foam_on_cascade_and_batch_levels},
 };
 const std::size_t kFrameCaseCount = sizeof(kFrameCases) / sizeof(kFrameCases[0]);
 

@@ -111,6 +111,28 @@ def run_config2(r, frames=600):
     return out
 
 
+def run_config2_final(r, frames=600, points=256):
+    """The heavy-chop patch's persistent foam after all 600 frames:
+    FoamField::foam_at at points on and off the patch, and coverage(), as raw
+    bits (the history-dependent state at the end of the config)."""
+    desc = desc_from(CONFIG2)
+    o = RefOcean(desc)
+    fp = default_foam_params()
+    foam = r.oref_foam_create(o.h, C.byref(fp))
+    assert foam
+    for k in range(frames):
+        o.update(k / 60.0)
+        r.oref_foam_update(foam, np.float32(1.0 / 60.0))
+    rng = np.random.default_rng(0xF0A)
+    xz = rng.uniform(-200.0, 450.0, size=(points, 2)).astype(np.float32)
+    vals = np.array([r.oref_foam_at(foam, float(x), float(z)) for x, z in xz], np.float32)
+    cov = np.float64(r.oref_foam_coverage(foam))
+    r.oref_foam_destroy(foam)
+    o.close()
+    return {"frames": frames, "points": xz.view(np.uint32).tolist(),
+            "foam_at": vals.view(np.uint32).tolist(), "coverage": int(cov.view(np.uint64))}
+
+
 def run_config3_cascades(r, frames=(0, 1, 2, 60, 600, 1199)):
     levels = [desc_from(kw) for kw in CONFIG3_LEVELS]
     arr = (type(levels[0]) * len(levels))(*levels)
@@ -157,6 +179,11 @@ def run_config3_interaction(r, frames=1200, every=50):
     return out
 
 
+def write_config2_final(r):
+    with open(os.path.join(HERE, "config2_final.json"), "w") as f:
+        json.dump(run_config2_final(r), f)
+
+
 def main():
     r = ref()
     if r is None:
@@ -181,8 +208,14 @@ def main():
     with open(os.path.join(HERE, "digests.json"), "w") as f:
         json.dump({"meta": meta, "config1": c1, "config2": c2, "config3_cascade": c3,
                    "config3_interaction": c3i}, f, indent=0)
-    print("wrote", os.path.join(HERE, "small_buffers.npz"), os.path.join(HERE, "digests.json"))
+    print("config2 final foam ...", flush=True)
+    write_config2_final(r)
+    print("wrote", os.path.join(HERE, "small_buffers.npz"), os.path.join(HERE, "digests.json"),
+          os.path.join(HERE, "config2_final.json"))
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["config2_final"]:
+        write_config2_final(ref())
+    else:
+        main()

@@ -67,6 +67,7 @@ _ORACLE_ONLY = {
     "orc_foam_data": (_F, [_P]),
     "orc_foam_last_substeps": (C.c_uint32, [_P]),
     "orc_foam_at": (C.c_float, [_P, C.c_float, C.c_float]),
+    "orc_foam_coverage": (C.c_double, [_P]),
     "orc_iwave_taps": (C.c_int, [C.c_uint32, C.c_double, C.c_double, _F,
                                  C.POINTER(C.c_double)]),
     "orc_iw_create": (_P, [C.POINTER(InteractionDesc)]),
@@ -108,6 +109,7 @@ _REF_ONLY = {
     "oref_foam_data": (_F, [_P]),
     "oref_foam_last_substeps": (C.c_uint32, [_P]),
     "oref_foam_at": (C.c_float, [_P, C.c_float, C.c_float]),
+    "oref_foam_coverage": (C.c_double, [_P]),
     "oref_interaction_desc_init": (None, [C.POINTER(InteractionDesc)]),
     "oref_interaction_create": (_P, [C.POINTER(InteractionDesc), C.POINTER(C.c_int)]),
     "oref_interaction_destroy": (None, [_P]),

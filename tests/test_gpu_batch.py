@@ -17,7 +17,8 @@ import pytest
 
 from oceanlib_amd import capi
 from oceanlib_amd.workloads import checksum_words, config4_levels, desc_from
-from oracle_ffi import OracleOcean, assert_bitexact, fptr, oracle
+from oracle_ffi import (OracleOcean, as_array, assert_bitexact, default_foam_params, fptr,
+                        oracle)
 
 pytestmark = pytest.mark.gpu
 
@@ -225,4 +226,56 @@ def test_config4_batched_seas_vs_reference(lib):
             assert not bad, (f"{name}: {len(bad)} of {count} levels differ, first "
                              f"(sea, level) {labels[bad[0]]}, N={flat[bad[0]]['size']}")
     finally:
+        b.close()
+
+
+def test_foam_on_batch_level_vs_oracle(lib):
+    """A FoamField on a batch level (whose host query staging is created only
+    on demand): the field after 40 substeps, foam_at at 128 points on and off
+    the patch, and coverage, against the oracle's FoamField on the same ocean
+    (reference src/foam.cpp:89-139,218-252,291-313)."""
+    descs = [desc_from(dict(size=64, patch_length=90.0, wind_speed=20.0, choppiness=1.3,
+                            compute_velocity=1, seed=31)),
+             desc_from(dict(size=128, patch_length=35.0, wind_speed=20.0, choppiness=1.1,
+                            compute_velocity=1, water_depth=12.0, seed=32))]
+    b = Batch(lib, descs)
+    olib = oracle()
+    fp = default_foam_params()
+    foams, ofoams, orcs = [], [], []
+    try:
+        fd = capi.FoamDesc()
+        lib.ocean_amd_foam_desc_init(C.byref(fd))
+        fd.source_gain = fp.source_gain
+        st = C.c_int(-1)
+        for i, d in enumerate(descs):
+            f = lib.ocean_amd_foam_create(b.level(i), C.byref(fd), C.byref(st))
+            assert f, st.value
+            foams.append(f)
+            o = OracleOcean(d)
+            orcs.append(o)
+            ofoams.append(olib.orc_foam_create(o.h, C.byref(fp)))
+        for k in range(40):
+            t = 2.0 + k / 60.0
+            b.update(t)
+            for f, o, of in zip(foams, orcs, ofoams):
+                o.update(t)
+                lib.ocean_amd_foam_update(f, np.float32(1.0 / 60.0))
+                olib.orc_foam_update(of, np.float32(1.0 / 60.0))
+        rng = np.random.default_rng(12)
+        for i, (f, of) in enumerate(zip(foams, ofoams)):
+            n = descs[i].size
+            fb = np.ctypeslib.as_array(lib.ocean_amd_foam_data(f), shape=(n * n,)).copy()
+            assert_bitexact(fb, as_array(olib.orc_foam_data(of), n * n), f"level {i} foam")
+            for x, z in rng.uniform(-150.0, 250.0, size=(128, 2)).astype(np.float32):
+                got = lib.ocean_amd_foam_at(f, float(x), float(z))
+                want = olib.orc_foam_at(of, float(x), float(z))
+                assert np.float32(got).tobytes() == np.float32(want).tobytes(), (i, x, z)
+            cov, want = lib.ocean_amd_foam_coverage(f), olib.orc_foam_coverage(of)
+            assert want > 0.0 and abs(cov - want) <= 1e-12 * want
+            assert lib.ocean_amd_last_error(b.level(i)) == 0
+    finally:
+        for f in foams:
+            lib.ocean_amd_foam_destroy(f)
+        for of in ofoams:
+            olib.orc_foam_destroy(of)
         b.close()

@@ -36,9 +36,13 @@ def golden():
 
 
 def test_config2_foam_and_queries_vs_golden_and_oracle(lib, golden):
-    """120 frames of the heavy-chop patch: surface, velocity, advected foam and
-    4096 queries per frame against the reference's digests; foam_at and
-    coverage against the oracle."""
+    """The whole heavy-chop config: all 600 frames of surface, velocity,
+    advected foam and the frame's 4096 queries against the reference run's
+    digests, every frame; the first 120 frames also in lockstep with the
+    oracle's foam field; after the last frame foam_at at 256 points and
+    coverage against the reference's own values (tests/golden/config2_final.json)."""
+    with open(os.path.join(GOLDEN, "config2_final.json")) as f:
+        final = json.load(f)
     desc = desc_from(CONFIG2)
     n = desc.size
     st = C.c_int(-1)
@@ -52,44 +56,39 @@ def test_config2_foam_and_queries_vs_golden_and_oracle(lib, golden):
     olib = oracle()
     fp = default_foam_params()
     ofoam = olib.orc_foam_create(o.h, C.byref(fp))
-    xz = query_points(600, 4096, desc.patch_length)
+    frames = final["frames"]
+    assert frames == len(golden["config2"]) == 600
+    xz = query_points(frames, 4096, desc.patch_length)
+    out = (capi.OceanSurfaceEx * 4096)()
     try:
-        for k in range(120):
+        for k in range(frames):
             t = k / 60.0
             lib.ocean_update(sim, t)
             lib.ocean_amd_foam_update(foam, np.float32(1.0 / 60.0))
-            o.update(t)
-            olib.orc_foam_update(ofoam, np.float32(1.0 / 60.0))
-            assert lib.ocean_amd_foam_last_substeps(foam) == olib.orc_foam_last_substeps(ofoam)
             want = golden["config2"][k]
             b = lib.ocean_get_buffers(sim)
             disp = np.ctypeslib.as_array(b.displacement, shape=(4 * n * n,))
-            assert digest(disp) == want[0], k
-            if k % 10 == 0 or k == 119:
-                nrm = np.ctypeslib.as_array(b.normal, shape=(4 * n * n,))
-                vel = np.ctypeslib.as_array(lib.ocean_get_velocity(sim), shape=(4 * n * n,))
-                fb = np.ctypeslib.as_array(lib.ocean_amd_foam_data(foam), shape=(n * n,))
-                assert digest(nrm) == want[1], k
-                assert digest(vel) == want[2], k
-                assert digest(fb) == want[3], (k, "foam")
-                assert_bitexact(fb.copy(), as_array(olib.orc_foam_data(ofoam), n * n), "foam")
-                pts = np.ascontiguousarray(xz[k].reshape(-1))
-                out = (capi.OceanSurfaceEx * 4096)()
-                assert lib.ocean_amd_sample_batch(sim, fptr(pts), 4096, out) == 0
-                q = np.frombuffer(bytes(out), np.float32).reshape(4096, 10)
-                assert digest(q) == want[4], (k, "queries")
-        rng = np.random.default_rng(2)
-        for x, z in rng.uniform(-100, 400, size=(64, 2)).astype(np.float32):
-            got = lib.ocean_amd_foam_at(foam, float(x), float(z))
-            ref = olib.orc_foam_at(ofoam, float(x), float(z))
-            assert np.float32(got).tobytes() == np.float32(ref).tobytes(), (x, z)
+            nrm = np.ctypeslib.as_array(b.normal, shape=(4 * n * n,))
+            vel = np.ctypeslib.as_array(lib.ocean_get_velocity(sim), shape=(4 * n * n,))
+            fb = np.ctypeslib.as_array(lib.ocean_amd_foam_data(foam), shape=(n * n,))
+            pts = np.ascontiguousarray(xz[k].reshape(-1))
+            assert lib.ocean_amd_sample_batch(sim, fptr(pts), 4096, out) == 0
+            q = np.frombuffer(bytes(out), np.float32).reshape(4096, 10)
+            assert [digest(disp), digest(nrm), digest(vel), digest(fb), digest(q)] == want, k
+            if k < 120:
+                o.update(t)
+                olib.orc_foam_update(ofoam, np.float32(1.0 / 60.0))
+                assert lib.ocean_amd_foam_last_substeps(foam) == olib.orc_foam_last_substeps(ofoam)
+                if k % 10 == 0 or k == 119:
+                    assert_bitexact(fb.copy(), as_array(olib.orc_foam_data(ofoam), n * n), "foam")
+        pts = np.array(final["points"], np.uint32).view(np.float32).reshape(-1, 2)
+        want_f = np.array(final["foam_at"], np.uint32)
+        got_f = np.array([lib.ocean_amd_foam_at(foam, float(x), float(z)) for x, z in pts],
+                         np.float32).view(np.uint32)
+        assert np.array_equal(got_f, want_f), np.nonzero(got_f != want_f)[0][:8]
         cov = lib.ocean_amd_foam_coverage(foam)
-        fb = as_array(olib.orc_foam_data(ofoam), n * n).astype(np.float64)
-        seq = 0.0
-        for v in fb:  # the reference's sequential double sum (src/foam.cpp:307-313)
-            seq += v
-        assert abs(cov - seq / fb.size) <= 1e-12 * max(1.0, abs(seq / fb.size))
-        assert cov > 0.0
+        want_cov = float(np.array([final["coverage"]], np.uint64).view(np.float64)[0])
+        assert abs(cov - want_cov) <= 1e-12 * want_cov and cov > 0.0
         # clear -> zero field, accumulator reset (reference src/foam.cpp:254-260)
         lib.ocean_amd_foam_clear(foam)
         assert lib.ocean_amd_foam_coverage(foam) == 0.0

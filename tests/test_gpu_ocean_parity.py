@@ -6,6 +6,9 @@ tests/test_c_api.cpp:51-95): here the HIP kernels must equal the oracle's
 restatement of the reference's scalar path, word for word.
 """
 import ctypes as C
+import hashlib
+import json
+import os
 
 import numpy as np
 import pytest
@@ -85,6 +88,28 @@ def test_config1_plain_patch(lib):
             check_frame(g, o, k / 60.0, "C1")
         for t in (3600.0, 36000.0, 1e-7, 123.456):
             check_frame(g, o, t, "C1 seek")
+    finally:
+        g.close()
+
+
+def test_config1_every_frame_vs_reference_digests(lib):
+    """The whole plain-patch config: every frame t = k/60, k = 0..600, of
+    displacement and normal against the reference run's digests
+    (tests/golden/digests.json, written from the reference build)."""
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                           "digests.json")) as f:
+        want = json.load(f)["config1"]
+    assert len(want) == 601
+    d = capi.default_desc(lib, **C1)
+    g = GpuOcean(lib, d)
+    n4 = 4 * d.size * d.size
+    try:
+        for k in range(601):
+            g.update(k / 60.0)
+            b = lib.ocean_get_buffers(g.h)
+            got = [hashlib.sha256(np.ctypeslib.as_array(p, shape=(n4,)).tobytes()).hexdigest()[:16]
+                   for p in (b.displacement, b.normal)]
+            assert got == want[k], f"frame {k}"
     finally:
         g.close()
 

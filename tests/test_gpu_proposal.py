@@ -277,3 +277,27 @@ def test_trained_proposal_saves_iterations_at_equal_error(lib, sea):
     assert np.all((fll & 1) == 0)
     assert itl.mean() < itp.mean()
     assert wl.sum() < wp.sum()
+
+
+@pytest.mark.skipif(not os.path.exists(PARAMS_FILE), reason="no trained proposal committed")
+def test_full_config_million_points_plain_and_learned(lib, sea):
+    """The learned-proposal config at its full size: 1M query points uniform
+    over the 2 km square on the three-level sea, the plain converged query and
+    the learned start point both within 0.01 mm of the float64 fixed point on
+    every point, no point left unconverged."""
+    b, bufs = sea
+    xz = points(1 << 20, 11)
+    want, ok = inv.height_converged(bufs, xz[:, 0], xz[:, 1])
+    assert ok.all()
+    hp, itp, flp = run_queries(lib, b, xz, None)
+    m = model(lib, np.load(PARAMS_FILE, allow_pickle=False), max_queries=len(xz))
+    try:
+        hl, itl, fll = run_queries(lib, b, xz, m)
+    finally:
+        lib.ocean_amd_proposal_destroy(m)
+    ep, el = np.abs(hp - want).max(), np.abs(hl - want).max()
+    print(f"1M points: plain max err {ep * 1e3:.4f} mm, taps/level {itp.mean(0)}; "
+          f"learned max err {el * 1e3:.4f} mm, taps/level {itl.mean(0)}, "
+          f"fallbacks {np.mean((fll & 2) != 0):.5f}")
+    assert np.all((flp & 1) == 0) and np.all((fll & 1) == 0)
+    assert ep <= MAX_ERR and el <= MAX_ERR

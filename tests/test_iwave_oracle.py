@@ -125,3 +125,62 @@ def test_library_stencil_equals_oracle(radius, cell, depth):
     assert oracle().orc_iwave_taps(radius, cell, depth, fptr(b), C.byref(mb)) == w * w
     assert_bitexact(a, b, "taps")
     assert ma.value == mb.value
+
+
+@needs_ref
+def test_compose_vs_reference_combined_query():
+    """orc_compose (WaterSurface::sample_at, reference src/interaction.cpp:863-901)
+    applied to the oracle's ocean sample and field sample, against the
+    reference's ocean_sample_combined on the same ocean and field history.
+    The chop is heavy enough that some queried normals are folded
+    (normal_y <= 1e-6), where the FFT normal must pass through unchanged."""
+    import ctypes as C
+    from oceanlib_amd.capi import InteractionSample, OceanSurfaceEx
+    from oceanlib_amd.workloads import desc_from
+    from oracle_ffi import OracleOcean, RefOcean
+
+    od = desc_from(dict(size=64, patch_length=16.0, wind_speed=30.0, choppiness=2.8,
+                        compute_velocity=1, seed=99))
+    oo, ro = OracleOcean(od), RefOcean(od)
+    init = ref().oref_interaction_desc_init
+    d = idesc(init, **dict(CONFIG3_INTERACTION, size=128, extent=12.8))
+    fa, fb = Field("orc", oracle(), d), Field("oref", ref(), d)
+    folded = 0
+    try:
+        srcs = config3_sources()
+        rng = np.random.default_rng(8)
+        for k in range(40):
+            for rec in config3_disturbances(srcs, k / 60.0, centre=(6.4, 6.4)):
+                fa.add(rec)
+                fb.add(rec)
+            fa.update(1 / 60)
+            fb.update(1 / 60)
+            if k % 8 != 7:
+                continue
+            t = 0.37 + k / 60.0
+            oo.update(t)
+            ro.update(t)
+            # plus the displaced positions of folded cells, where the query's
+            # solved point lands in or next to the fold
+            disp = oo.displacement().reshape(-1, 4)
+            nrm = oo.normal().reshape(-1, 4)
+            cell = od.patch_length / od.size
+            idx = np.nonzero(nrm[:, 1] <= 1.0e-6)[0][:60]
+            fx = (idx % od.size) * cell + disp[idx, 0]
+            fz = (idx // od.size) * cell + disp[idx, 2]
+            pts = np.concatenate([rng.uniform(-2.0, 15.0, size=(400, 2)),
+                                  rng.uniform(-80.0, 80.0, size=(50, 2)),
+                                  np.stack([fx, fz], axis=1)]).astype(np.float32)
+            for x, z in pts:
+                want = OceanSurfaceEx()
+                ref().oref_sample_combined(ro.h, fb.h, float(x), float(z), C.byref(want))
+                got = oo.sample(float(x), float(z))
+                if got.normal_y <= 1.0e-6:
+                    folded += 1
+                e = fa.sample(x, z)
+                oracle().orc_compose(C.byref(got), C.byref(e))
+                assert bytes(got) == bytes(want), (k, x, z)
+        assert folded > 0, "no folded crest was queried"
+    finally:
+        fa.close()
+        fb.close()

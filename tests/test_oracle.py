@@ -342,3 +342,53 @@ def test_parseval_energy():
     lhs = np.sum(h * h)
     rhs = n * n * np.sum(np.abs(H) ** 2)
     assert abs(lhs - rhs) / rhs < 1e-4
+
+
+@needs_ref
+@pytest.mark.parametrize("kw,dts", [
+    (dict(size=128, patch_length=120.0, wind_speed=22.0, choppiness=1.3, compute_velocity=1,
+          water_depth=50.0, seed=7), [1 / 60.0]),
+    (dict(size=64, patch_length=40.0, wind_speed=30.0, choppiness=2.5, compute_velocity=1,
+          seed=99), [1 / 60.0, 0.05, 0.0, 0.3, 1 / 30.0]),
+])
+def test_foam_at_and_coverage_vs_reference(kw, dts):
+    """orc_foam_at (FoamField::foam_at, reference src/foam.cpp:291-305) and
+    orc_foam_coverage (src/foam.cpp:307-313) against the reference's own
+    FoamField over a history of substeps (multi-substep and clamped dt
+    included), at points on, off and far outside the patch."""
+    d = desc_from(kw)
+    o, r = OracleOcean(d), RefOcean(d)
+    lib, rl = oracle(), ref()
+    fp = default_foam_params()
+    fo = lib.orc_foam_create(o.h, C.byref(fp))
+    fr = rl.oref_foam_create(r.h, C.byref(fp))
+    n = d.size
+    rng = np.random.default_rng(11)
+    pts = np.concatenate([
+        rng.uniform(0.0, d.patch_length, size=(150, 2)),
+        rng.uniform(-3.0 * d.patch_length, 4.0 * d.patch_length, size=(100, 2)),
+        np.array([[0.0, 0.0], [d.patch_length, d.patch_length], [-1e4, 2e4], [-0.0, 1e-7]]),
+    ]).astype(np.float32)
+    try:
+        t = 0.0
+        for k in range(24):
+            dt = dts[k % len(dts)]
+            t += dt
+            o.update(t)
+            r.update(t)
+            lib.orc_foam_update(fo, np.float32(dt))
+            rl.oref_foam_update(fr, np.float32(dt))
+            assert lib.orc_foam_last_substeps(fo) == rl.oref_foam_last_substeps(fr)
+            assert_bitexact(as_array(lib.orc_foam_data(fo), n * n),
+                            as_array(rl.oref_foam_data(fr), n * n), f"foam step {k}")
+            assert np.float64(lib.orc_foam_coverage(fo)).tobytes() == \
+                np.float64(rl.oref_foam_coverage(fr)).tobytes(), k
+            if k % 6 == 5:
+                for x, z in pts:
+                    a = lib.orc_foam_at(fo, float(x), float(z))
+                    b = rl.oref_foam_at(fr, float(x), float(z))
+                    assert np.float32(a).tobytes() == np.float32(b).tobytes(), (k, x, z)
+        assert lib.orc_foam_coverage(fo) > 0.0
+    finally:
+        lib.orc_foam_destroy(fo)
+        rl.oref_foam_destroy(fr)

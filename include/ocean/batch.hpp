@@ -60,6 +60,10 @@ public:
     // `reps` launches each), keeps the fastest and rebuilds the graph. The
     // choice changes only how work is spread, never the results.
     void autotune(std::uint32_t reps = 5) noexcept;
+    // Opt-in fast mode of every level (values as Ocean::set_precision),
+    // applied from the next update; the grid sizes are kept.
+    bool set_precision(std::uint32_t mode) noexcept;
+    [[nodiscard]] std::uint32_t precision() const noexcept;
     [[nodiscard]] std::size_t group_count() const noexcept;
     [[nodiscard]] BatchGroupInfo group(std::size_t g) const noexcept;
 

@@ -137,6 +137,15 @@ public:
     // changed) for a pointer HIP does not know as device memory of that size,
     // velocity without compute_velocity, or a batch's or cascade's level.
     bool bind_outputs(float* displacement, float* normal, float* velocity) noexcept;
+    // Opt-in fast mode (no reference counterpart; the reference's frame is
+    // only the bit-exact one): 0 = the reference's arithmetic bit for bit
+    // (the default), 1 = hardware sine/cosine of the phase, fused butterfly
+    // rotations and a hardware reciprocal square root for the normal,
+    // 2 = mode 1 with the fields between the two passes stored as fp16.
+    // Applies from the next update; the launch shape returns to the mode's
+    // default. False for another value or a batch's or cascade's level.
+    bool set_precision(std::uint32_t mode) noexcept;
+    [[nodiscard]] std::uint32_t precision() const noexcept;
     void synchronize() const noexcept;
     // Sticky: the first HIP error seen since construction (0 = none).
     [[nodiscard]] int last_error() const noexcept;

@@ -101,6 +101,27 @@ int ocean_amd_set_frame_shape(ocean_sim* sim, uint32_t row_variant, uint32_t col
 int ocean_amd_autotune_frame(ocean_sim* sim, uint32_t reps, uint32_t* row_variant,
                              uint32_t* col_variant, float* row_ms, float* col_ms);
 
+/* ---- opt-in fast mode ----------------------------------------------------
+ * No reference counterpart: the reference's frame is bit-exact only, and it
+ * names a real-to-complex FFT as its speed-up path (ARCHITECTURE.md ADR-008;
+ * the four complex fields here already carry eight real spectra). Never the
+ * default. Modes:
+ *   EXACT     : the reference's arithmetic bit for bit.
+ *   FAST      : the phase's sine and cosine from the hardware's sin/cos of the
+ *               phase in revolutions, fused (FMA) butterfly rotations, and the
+ *               normal's 1/|n| from a hardware reciprocal square root.
+ *   FAST_HALF : FAST with the complex fields between the row and the column
+ *               pass stored as fp16 pairs (half the bytes between the passes).
+ * _set_precision applies from the next update and returns the launch shape to
+ * the mode's default (the shapes of a fast mode are its own list; see
+ * _frame_shapes); OCEAN_ERROR_INVALID_ARG for another value or a batch's
+ * level. Queries, foam and packs read whatever the frame wrote. */
+#define OCEAN_AMD_PRECISION_EXACT 0u
+#define OCEAN_AMD_PRECISION_FAST 1u
+#define OCEAN_AMD_PRECISION_FAST_HALF 2u
+int      ocean_amd_set_precision(ocean_sim* sim, uint32_t mode);
+uint32_t ocean_amd_get_precision(const ocean_sim* sim);
+
 /* ---- batched queries ----------------------------------------------------
  * Batched form of ocean_sample_ex / ocean_height_at (reference
  * src/c_api.cpp:199-202,330-350; src/ocean.cpp:375-452): result q is exactly
@@ -230,6 +251,9 @@ int           ocean_amd_batch_sample_device(const ocean_amd_batch* batch, uint32
                                             float* heights_dev);
 /* Times candidate grid sizes (reps launches each) and keeps the fastest. */
 void          ocean_amd_batch_autotune(ocean_amd_batch* batch, uint32_t reps);
+/* Precision mode of every level (as ocean_amd_set_precision), from the next
+ * update; the grid sizes are kept. */
+int           ocean_amd_batch_set_precision(ocean_amd_batch* batch, uint32_t mode);
 uint32_t      ocean_amd_batch_group_count(const ocean_amd_batch* batch);
 int           ocean_amd_batch_group_info(const ocean_amd_batch* batch, uint32_t group,
                                          ocean_amd_batch_group* out);

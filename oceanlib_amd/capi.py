@@ -191,6 +191,8 @@ AMD_SIGNATURES = {
     "ocean_amd_get_tables": (C.c_int, [_P, _F, _F, C.POINTER(C.c_uint32)]),
     "ocean_amd_frame_shapes": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "ocean_amd_set_frame_shape": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
+    "ocean_amd_set_precision": (C.c_int, [_P, C.c_uint32]),
+    "ocean_amd_get_precision": (C.c_uint32, [_P]),
     "ocean_amd_frame_timing": (C.c_int, [_P, C.c_uint32]),
     "ocean_amd_frame_timing_read": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                               C.POINTER(C.c_uint32)]),
@@ -239,6 +241,7 @@ AMD_SIGNATURES = {
     "ocean_amd_batch_sample_device": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_void_p,
                                                 C.c_uint32, C.c_void_p, C.c_void_p]),
     "ocean_amd_batch_autotune": (None, [_P, C.c_uint32]),
+    "ocean_amd_batch_set_precision": (C.c_int, [_P, C.c_uint32]),
     "ocean_amd_batch_group_count": (C.c_uint32, [_P]),
     "ocean_amd_batch_group_info": (C.c_int, [_P, C.c_uint32, C.POINTER(BatchGroup)]),
     "ocean_amd_batch_checksums": (C.c_int, [_P, C.c_uint32, C.c_uint32,

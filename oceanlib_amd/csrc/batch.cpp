@@ -75,6 +75,7 @@ struct OceanBatch::Impl {
     int cus = 1;
     std::mutex query_mu;  // concurrent const queries share the staging
     double time = 0.0;
+    std::uint32_t mode = 0;  // precision mode of every level (0 = bit-exact)
     std::atomic<int> status{0};
 
     void note(hipError_t e) noexcept
@@ -218,6 +219,26 @@ void OceanBatch::update(double time) noexcept
     for (Ocean& o : m.levels) o.note_frame(time);
     m.time = time;
 }
+
+bool OceanBatch::set_precision(std::uint32_t mode) noexcept
+{
+    Impl& m = *impl_;
+    if (mode > 2u) return false;
+    m.levels.front().device().make_current();
+    m.note(hipStreamSynchronize(m.stream.s));  // the graph may still be running
+    for (ocean_hip_batch_group& g : m.groups) g.mode = mode;
+    for (std::size_t i : m.loose) m.levels[i].device().level.shape = mode << 16;
+    try {
+        m.rebuild_graph();
+    } catch (...) {
+        m.note(hipErrorUnknown);
+        return false;
+    }
+    m.mode = mode;
+    return m.status == 0;
+}
+
+std::uint32_t OceanBatch::precision() const noexcept { return impl_->mode; }
 
 double OceanBatch::time() const noexcept { return impl_->time; }
 std::size_t OceanBatch::level_count() const noexcept { return impl_->levels.size(); }

@@ -300,7 +300,8 @@ int ocean_amd_frame_shapes(const ocean_sim* sim, uint32_t* row_variants, uint32_
     if (sim == nullptr) return OCEAN_ERROR_INVALID_ARG;
     const auto& dev = to_cpp(sim)->device();
     int r = 0, c = 0;
-    if (ocean_hip_frame_shapes(dev.level.n, uint32_t(dev.fields), &r, &c) != 0)
+    if (ocean_hip_frame_shapes_mode(dev.level.n, uint32_t(dev.fields), (dev.level.shape >> 16) & 0xffu,
+                                    &r, &c) != 0)
         return OCEAN_ERROR_INVALID_ARG;
     if (row_variants) *row_variants = static_cast<uint32_t>(r);
     if (col_variants) *col_variants = static_cast<uint32_t>(c);
@@ -313,12 +314,13 @@ int ocean_amd_set_frame_shape(ocean_sim* sim, uint32_t row_variant, uint32_t col
     ocean::Ocean& o = *to_cpp(sim);
     auto& dev = o.device();
     int r = 0, c = 0;
-    if (ocean_hip_frame_shapes(dev.level.n, uint32_t(dev.fields), &r, &c) != 0 ||
+    const uint32_t mode_bits = dev.level.shape & 0xffff0000u;
+    if (ocean_hip_frame_shapes_mode(dev.level.n, uint32_t(dev.fields), mode_bits >> 16, &r, &c) != 0 ||
         row_variant >= uint32_t(r) ||
         col_variant >= uint32_t(c))
         return OCEAN_ERROR_INVALID_ARG;
     dev.make_current();
-    dev.level.shape = row_variant | (col_variant << 8);
+    dev.level.shape = mode_bits | row_variant | (col_variant << 8);
     dev.note(hipMemcpyAsync(dev.d_level, &dev.level, sizeof(dev.level), hipMemcpyHostToDevice, dev.stream));
     dev.note(hipStreamSynchronize(dev.stream));
     return dev.status == 0 ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
@@ -331,7 +333,8 @@ int ocean_amd_autotune_frame(ocean_sim* sim, uint32_t reps, uint32_t* row_varian
     ocean::Ocean& o = *to_cpp(sim);
     auto& dev = o.device();
     int nr = 0, nc = 0;
-    if (ocean_hip_frame_shapes(dev.level.n, uint32_t(dev.fields), &nr, &nc) != 0)
+    const uint32_t mode_bits = dev.level.shape & 0xffff0000u;
+    if (ocean_hip_frame_shapes_mode(dev.level.n, uint32_t(dev.fields), mode_bits >> 16, &nr, &nc) != 0)
         return OCEAN_ERROR_INVALID_ARG;
     dev.make_current();
     // Each candidate is timed inside whole frames (row pass then column pass,
@@ -356,7 +359,8 @@ int ocean_amd_autotune_frame(ocean_sim* sim, uint32_t reps, uint32_t* row_varian
     for (unsigned pass = 0; pass < 2 && ok; ++pass) {
         const int count = pass == 0 ? nr : nc;
         for (int v = 0; v < count && ok; ++v) {
-            dev.level.shape = pass == 0 ? (uint32_t(v) | (pick[1] << 8)) : (pick[0] | (uint32_t(v) << 8));
+            dev.level.shape =
+                mode_bits | (pass == 0 ? (uint32_t(v) | (pick[1] << 8)) : (pick[0] | (uint32_t(v) << 8)));
             dev.note(ocean_hip_frame(&dev.level, t2, dev.stream));  // warm
             for (uint32_t i = 0; i < reps; ++i)
                 dev.note(ocean_hip_frame_timed(&dev.level, t2, dev.stream, &ev[std::size_t(i) * 4]));
@@ -380,7 +384,7 @@ int ocean_amd_autotune_frame(ocean_sim* sim, uint32_t reps, uint32_t* row_varian
         dev.level.shape = saved;
         return OCEAN_ERROR_UNKNOWN;
     }
-    dev.level.shape = pick[0] | (pick[1] << 8);
+    dev.level.shape = mode_bits | pick[0] | (pick[1] << 8);
     dev.note(hipMemcpyAsync(dev.d_level, &dev.level, sizeof(dev.level), hipMemcpyHostToDevice, dev.stream));
     dev.note(ocean_hip_frame(&dev.level, t2, dev.stream));  // a complete frame at the current time
     dev.note(hipStreamSynchronize(dev.stream));
@@ -389,6 +393,19 @@ int ocean_amd_autotune_frame(ocean_sim* sim, uint32_t reps, uint32_t* row_varian
     if (row_ms) *row_ms = best[0];
     if (col_ms) *col_ms = best[1];
     return dev.status == 0 ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
+}
+
+int ocean_amd_set_precision(ocean_sim* sim, uint32_t mode)
+{
+    if (sim == nullptr || mode > OCEAN_AMD_PRECISION_FAST_HALF) return OCEAN_ERROR_INVALID_ARG;
+    ocean::Ocean& o = *to_cpp(sim);
+    if (!o.set_precision(mode)) return o.last_error() != 0 ? OCEAN_ERROR_UNKNOWN : OCEAN_ERROR_INVALID_ARG;
+    return OCEAN_OK;
+}
+
+uint32_t ocean_amd_get_precision(const ocean_sim* sim)
+{
+    return sim == nullptr ? 0u : to_cpp(sim)->precision();
 }
 
 void ocean_amd_synchronize(const ocean_sim* sim)
@@ -731,6 +748,12 @@ int ocean_amd_batch_sample_device(const ocean_amd_batch* batch, uint32_t first,
 void ocean_amd_batch_autotune(ocean_amd_batch* batch, uint32_t reps)
 {
     if (batch != nullptr) to_batch(batch)->autotune(reps);
+}
+
+int ocean_amd_batch_set_precision(ocean_amd_batch* batch, uint32_t mode)
+{
+    if (batch == nullptr || mode > OCEAN_AMD_PRECISION_FAST_HALF) return OCEAN_ERROR_INVALID_ARG;
+    return to_batch(batch)->set_precision(mode) ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
 }
 uint32_t ocean_amd_batch_group_count(const ocean_amd_batch* batch)
 {

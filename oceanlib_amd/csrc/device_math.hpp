@@ -53,6 +53,18 @@ __device__ __forceinline__ void sincos_poly(float x, float& out_s, float& out_c)
     out_c = neg_c ? -bc : bc;
 }
 
+// The opt-in fast mode's phase (not the reference's rounding): omega*t/2pi
+// folded into [0, 1) revolutions in double, narrowed, and the hardware's
+// sine and cosine, which take revolutions (v_sin_f32 / v_cos_f32).
+__device__ __forceinline__ void sincos_fast(float omega, double time_over_two_pi, float& out_s,
+                                            float& out_c)
+{
+    const double d = static_cast<double>(omega) * time_over_two_pi;
+    const float rev = static_cast<float>(d - floor(d));
+    out_s = __builtin_amdgcn_sinf(rev);
+    out_c = __builtin_amdgcn_cosf(rev);
+}
+
 // Signed wavenumber of grid index i (reference src/core/spectrum.hpp:85-92).
 __device__ __forceinline__ float wave_component(uint32_t i, uint32_t n, float patch_length)
 {

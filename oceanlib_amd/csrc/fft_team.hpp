@@ -46,7 +46,10 @@ constexpr int team_stride() { return lds_span<N>() + ((2 - lds_span<N>() % 16) +
 // r butterfly stages (r <= 3) starting at stage t0, on a lane's 8 registers
 // of each of K independent sequences (same indices and twiddles for all K).
 // idx[a] tracks the reference-array index register a currently represents.
-template <int T0, int RST, int K>
+// FMA (the opt-in fast mode only): the rotation's products are fused,
+// (dr*wr - di*wi) as fma(dr, wr, -(di*wi)), which is no longer the reference's
+// rounding.
+template <int T0, int RST, int K, bool FMA = false>
 __host__ __device__ __forceinline__ void fft_pass(float (&re)[K][8], float (&im)[K][8],
                                                   uint32_t (&idx)[8], const float2* __restrict__ tw)
 {
@@ -69,8 +72,13 @@ __host__ __device__ __forceinline__ void fft_pass(float (&re)[K][8], float (&im)
                 im[k][a] = ai + bi;
                 const float dr = ar - br;
                 const float di = ai - bi;
-                re[k][b] = dr * w.x - di * w.y;
-                im[k][b] = dr * w.y + di * w.x;
+                if constexpr (FMA) {
+                    re[k][b] = __builtin_fmaf(dr, w.x, -(di * w.y));
+                    im[k][b] = __builtin_fmaf(dr, w.y, di * w.x);
+                } else {
+                    re[k][b] = dr * w.x - di * w.y;
+                    im[k][b] = dr * w.y + di * w.x;
+                }
             }
             const uint32_t base = ((jl >> t) << (t + 1)) | (jl & (s - 1u));
             idx[a] = base;
@@ -124,7 +132,7 @@ __host__ __device__ __forceinline__ void exchange(float (&re)[K][8], float (&im)
 //   sync    : barrier callable (a block barrier in practice)
 // Transforming K sequences together shares the index arithmetic and twiddle
 // reads, and gives each barrier K times the independent work.
-template <int N, int K, typename Sync>
+template <int N, int K, typename Sync, bool FMA = false>
 __host__ __device__ __forceinline__ void fft_team_k(float (&re)[K][8], float (&im)[K][8],
                                                     uint32_t (&idx)[8], uint32_t J,
                                                     float2* __restrict__ xch, uint32_t kstride,
@@ -135,24 +143,24 @@ __host__ __device__ __forceinline__ void fft_team_k(float (&re)[K][8], float (&i
 #pragma unroll
     for (int a = 0; a < 8; ++a) idx[a] = J | (static_cast<uint32_t>(a) * T);
 
-    fft_pass<PassPlan<N, 0>::T0, PassPlan<N, 0>::RST, K>(re, im, idx, tw);
+    fft_pass<PassPlan<N, 0>::T0, PassPlan<N, 0>::RST, K, FMA>(re, im, idx, tw);
     if constexpr (PASSES > 1) {
         exchange<N, K>(re, im, idx, J, xch, kstride, sync);
-        fft_pass<PassPlan<N, 1>::T0, PassPlan<N, 1>::RST, K>(re, im, idx, tw);
+        fft_pass<PassPlan<N, 1>::T0, PassPlan<N, 1>::RST, K, FMA>(re, im, idx, tw);
     }
     if constexpr (PASSES > 2) {
         exchange<N, K>(re, im, idx, J, xch, kstride, sync);
-        fft_pass<PassPlan<N, 2>::T0, PassPlan<N, 2>::RST, K>(re, im, idx, tw);
+        fft_pass<PassPlan<N, 2>::T0, PassPlan<N, 2>::RST, K, FMA>(re, im, idx, tw);
     }
     if constexpr (PASSES > 3) {
         exchange<N, K>(re, im, idx, J, xch, kstride, sync);
-        fft_pass<PassPlan<N, 3>::T0, PassPlan<N, 3>::RST, K>(re, im, idx, tw);
+        fft_pass<PassPlan<N, 3>::T0, PassPlan<N, 3>::RST, K, FMA>(re, im, idx, tw);
     }
     static_assert(PASSES <= 4, "N > 4096 is outside the reference's range");
 }
 
 // One sequence (K = 1).
-template <int N, typename Sync>
+template <int N, typename Sync, bool FMA = false>
 __host__ __device__ __forceinline__ void fft_team(float (&re)[8], float (&im)[8], uint32_t (&idx)[8],
                                                   uint32_t J, float2* __restrict__ xch,
                                                   const float2* __restrict__ tw, Sync sync)
@@ -163,7 +171,7 @@ __host__ __device__ __forceinline__ void fft_team(float (&re)[8], float (&im)[8]
         r1[0][a] = re[a];
         i1[0][a] = im[a];
     }
-    fft_team_k<N, 1>(r1, i1, idx, J, xch, 0u, tw, sync);
+    fft_team_k<N, 1, Sync, FMA>(r1, i1, idx, J, xch, 0u, tw, sync);
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
         re[a] = r1[0][a];

@@ -42,7 +42,7 @@ struct FoamField::Impl {
     float* dev = nullptr;       // 2*cells floats: the ping-pong pair
     float* cur = nullptr;
     float* next = nullptr;
-    double* scratch = nullptr;  // coverage partials + result
+    double* scratch = nullptr;  // coverage result
     float* q_xz = nullptr;      // one-point query staging
     float* q_out = nullptr;
     float* host = nullptr;      // pinned mirror of `cur`
@@ -78,13 +78,12 @@ FoamField::FoamField(const Ocean& ocean, const FoamDesc& desc)
 
     auto& od = ocean.device();
     od.make_current();
-    const std::uint32_t blocks = ocean_hip_coverage_blocks(static_cast<std::uint32_t>(m.cells));
-    const std::size_t bytes = m.cells * 2 * sizeof(float) + (blocks + 2) * sizeof(double) + 64;
+    const std::size_t bytes = m.cells * 2 * sizeof(float) + 2 * sizeof(double) + 64;
     detail::hip_check(hipMalloc(reinterpret_cast<void**>(&m.dev), bytes), "hipMalloc foam");
     m.cur = m.dev;
     m.next = m.dev + m.cells;
     m.scratch = reinterpret_cast<double*>(m.dev + 2 * m.cells);
-    m.q_xz = reinterpret_cast<float*>(m.scratch + blocks + 2);
+    m.q_xz = reinterpret_cast<float*>(m.scratch + 2);
     m.q_out = m.q_xz + 2;
     detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m.host), m.cells * sizeof(float),
                                     hipHostMallocDefault),
@@ -252,8 +251,8 @@ double FoamField::coverage() const noexcept
     auto& od = m.ocean->device();
     std::lock_guard<std::mutex> lock(od.query_mu);
     od.make_current();
-    double* out = m.scratch + ocean_hip_coverage_blocks(static_cast<std::uint32_t>(m.cells));
-    od.note(ocean_hip_coverage(m.cur, static_cast<std::uint32_t>(m.cells), m.scratch, out,
+    double* out = m.scratch;
+    od.note(ocean_hip_coverage(m.cur, static_cast<std::uint32_t>(m.cells), out,
                                od.stream));
     od.note(hipMemcpyAsync(m.host_cov, out, sizeof(double), hipMemcpyDeviceToHost, od.stream));
     od.note(hipStreamSynchronize(od.stream));

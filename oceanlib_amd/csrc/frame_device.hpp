@@ -64,6 +64,54 @@ struct BlockSync {
     __device__ void operator()() const { __syncthreads(); }
 };
 
+// Precision modes of the two passes (template argument MODE).
+//   kModeExact    : the reference's arithmetic, bit for bit (the default).
+//   kModeFast     : the opt-in fast mode: the hardware's sine and cosine of
+//                   the phase in revolutions, fused butterfly rotations and
+//                   a hardware reciprocal square root for the normal.
+//   kModeFastHalf : kModeFast with the row pass's fields stored as fp16
+//                   pairs, halving the bytes between the passes.
+constexpr int kModeExact = 0;
+constexpr int kModeFast = 1;
+constexpr int kModeFastHalf = 2;
+
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+// One complex value of field f at cell `cell` (element offset within the
+// level's `spec`), in the mode's storage: float2, or fp16 pairs.
+template <int N, int MODE>
+__device__ __forceinline__ void store_field(void* spec, size_t at, float x, float y)
+{
+    if constexpr (MODE == kModeFastHalf) {
+        reinterpret_cast<f16x2*>(spec)[at] = f16x2{static_cast<_Float16>(x), static_cast<_Float16>(y)};
+    } else {
+        store_stream<N>(reinterpret_cast<float2*>(spec) + at, x, y);
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ float2 load_field(const void* spec, size_t at)
+{
+    if constexpr (MODE == kModeFastHalf) {
+        const f16x2 h = reinterpret_cast<const f16x2*>(spec)[at];
+        return make_float2(static_cast<float>(h.x), static_cast<float>(h.y));
+    } else {
+        return reinterpret_cast<const float2*>(spec)[at];
+    }
+}
+
+// The phase's sine and cosine in the mode's arithmetic.
+template <int MODE>
+__device__ __forceinline__ void phase_sincos(float w, double time_over_two_pi, float& s, float& c)
+{
+    if constexpr (MODE == kModeExact) {
+        const float phase = reduced_phase(w, time_over_two_pi);
+        sincos_poly(phase, s, c);
+    } else {
+        sincos_fast(w, time_over_two_pi, s, c);
+    }
+}
+
 // Team g handles row first(q) or second(q) of mirror pair q, so a block's
 // teams read each other's h0 rows as their mirrors.
 template <int N>
@@ -74,7 +122,7 @@ __device__ __forceinline__ uint32_t row_of_team(uint32_t g)
     return q == 0u ? uint32_t(N / 2) : uint32_t(N) - q;
 }
 
-template <int N, int F, int TEAMS>
+template <int N, int F, int TEAMS, int MODE = kModeExact>
 __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk,
                                           double time_over_two_pi, float2* __restrict__ tw,
                                           float2* __restrict__ xch, uint32_t tid = threadIdx.x)
@@ -87,7 +135,7 @@ __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk
     const float* __restrict__ kinv = L.kinv;
     const float* __restrict__ kxs = L.kx;
     const float2* __restrict__ twg = reinterpret_cast<const float2*>(L.twiddles);
-    float2* __restrict__ spec = reinterpret_cast<float2*>(L.spec);
+    void* __restrict__ spec = L.spec;
     const float patch_length = L.patch_length;
 
     __syncthreads();
@@ -112,9 +160,8 @@ __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk
         const float2 hm = h0[mz * N + mx];
 
         // reference src/core/evolve.cpp:45-121 (h0c = conj(h0(-k)), spectrum.cpp:329-338)
-        const float phase = reduced_phase(w, time_over_two_pi);
         float s, c;
-        sincos_poly(phase, s, c);
+        phase_sincos<MODE>(w, time_over_two_pi, s, c);
         const float a_re = hv.x, a_im = hv.y;
         const float b_re = hm.x, b_im = -hm.y;
         const float fwd_re = a_re * c - a_im * s;
@@ -157,16 +204,16 @@ __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk
 #pragma unroll
     for (int f = 0; f < F; ++f) {
         uint32_t idx[8];
-        fft_team<N>(fr[f], fi[f], idx, J, x_team, tw, BlockSync{});
-        float2* dst = spec + size_t(f) * CELLS + size_t(z) * N;
+        fft_team<N, BlockSync, MODE != kModeExact>(fr[f], fi[f], idx, J, x_team, tw, BlockSync{});
+        const size_t row0 = size_t(f) * CELLS + size_t(z) * N;
 #pragma unroll
-        for (int a = 0; a < 8; ++a) store_stream<N>(dst + idx[a], fr[f][a], fi[f][a]);
+        for (int a = 0; a < 8; ++a) store_field<N, MODE>(spec, row0 + idx[a], fr[f][a], fi[f][a]);
     }
 }
 
 // The finalize of one cell from its eight real fields (h, Dx, Dz, dh/dx,
 // dh/dz, dDx/dx, dDz/dz, dDx/dz): reference src/core/evolve.cpp:256-325.
-template <bool STREAM_DISP, bool STREAM_NORMAL>
+template <bool STREAM_DISP, bool STREAM_NORMAL, int MODE = kModeExact>
 __device__ __forceinline__ void pack_cell(float h, float d_x, float d_z, float hx, float hz,
                                           float jxx, float jzz, float jxz, float lambda,
                                           float foam_threshold, float inv_threshold,
@@ -183,7 +230,11 @@ __device__ __forceinline__ void pack_cell(float h, float d_x, float d_z, float h
     float ny = jac;
     float nz = -(axx * hz - hx * axz);
     const float len2 = nx * nx + ny * ny + nz * nz;
-    const float inv_len = (len2 > 0.0f) ? 1.0f / sqrtf(len2) : 0.0f;
+    float inv_len;
+    if constexpr (MODE == kModeExact)
+        inv_len = (len2 > 0.0f) ? 1.0f / sqrtf(len2) : 0.0f;
+    else
+        inv_len = (len2 > 0.0f) ? __builtin_amdgcn_rsqf(len2) : 0.0f;
     nx *= inv_len;
     ny *= inv_len;
     nz *= inv_len;
@@ -194,7 +245,7 @@ __device__ __forceinline__ void pack_cell(float h, float d_x, float d_z, float h
     store_out<STREAM_NORMAL>(normal + o, nx, ny, nz, jac);
 }
 
-template <int N, int F, int C, bool STREAM = false>
+template <int N, int F, int C, bool STREAM = false, int MODE = kModeExact>
 __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk,
                                           float2* __restrict__ tw, float2* __restrict__ xch,
                                           uint32_t tid = threadIdx.x)
@@ -202,7 +253,7 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
     constexpr int T = FftShape<N>::T;
     constexpr int BLOCK = C * T;
     constexpr uint32_t CELLS = uint32_t(N) * N;
-    const float2* __restrict__ spec = reinterpret_cast<const float2*>(L.spec);
+    const void* __restrict__ spec = L.spec;
     const float2* __restrict__ twg = reinterpret_cast<const float2*>(L.twiddles);
     float4* __restrict__ disp = reinterpret_cast<float4*>(L.disp);
     float4* __restrict__ normal = reinterpret_cast<float4*>(L.normal);
@@ -224,7 +275,7 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
     for (int f = 0; f < F; ++f) {
 #pragma unroll
         for (int a = 0; a < 8; ++a) {
-            const float2 v = spec[size_t(f) * CELLS + size_t(J + a * T) * N + x];
+            const float2 v = load_field<MODE>(spec, size_t(f) * CELLS + size_t(J + a * T) * N + x);
             fr[f][a] = v.x;
             fi[f][a] = v.y;
         }
@@ -233,12 +284,13 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
     uint32_t idx[8];
 #pragma unroll
     for (int f = 0; f < F; ++f)
-        fft_team<N>(fr[f], fi[f], idx, J, xch + c * team_stride<N>(), tw, BlockSync{});
+        fft_team<N, BlockSync, MODE != kModeExact>(fr[f], fi[f], idx, J, xch + c * team_stride<N>(),
+                                                   tw, BlockSync{});
 
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
         const size_t o = size_t(idx[a]) * N + x;
-        pack_cell<(STREAM && C * 16 >= 128), STREAM>(fr[0][a], fi[0][a], fr[1][a], fi[1][a],
+        pack_cell<(STREAM && C * 16 >= 128), STREAM, MODE>(fr[0][a], fi[0][a], fr[1][a], fi[1][a],
                                                      fr[2][a], fi[2][a], fr[3][a], fi[3][a], lambda,
                                                      foam_threshold, inv_threshold, disp, normal, o);
         if constexpr (F == 6) {
@@ -260,7 +312,7 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
 // (fft_team_k: shared index arithmetic and twiddle reads, one pair of
 // barriers per exchange for both), KX = 1 one after the other.
 
-template <int N, int F, int TEAMS, int KX>
+template <int N, int F, int TEAMS, int KX, int MODE = kModeExact>
 __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32_t blk,
                                                 double time_over_two_pi, float2* __restrict__ tw,
                                                 float2* __restrict__ xch)
@@ -275,7 +327,7 @@ __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32
     const float* __restrict__ kinv = L.kinv;
     const float* __restrict__ kxs = L.kx;
     const float2* __restrict__ twg = reinterpret_cast<const float2*>(L.twiddles);
-    float2* __restrict__ spec = reinterpret_cast<float2*>(L.spec);
+    void* __restrict__ spec = L.spec;
     const float patch_length = L.patch_length;
 
     __syncthreads();
@@ -302,9 +354,8 @@ __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32
         const float2 hm = h0[mz * N + mx];
 
         // reference src/core/evolve.cpp:45-121 (h0c = conj(h0(-k)), spectrum.cpp:329-338)
-        const float phase = reduced_phase(w, time_over_two_pi);
         float s, c;
-        sincos_poly(phase, s, c);
+        phase_sincos<MODE>(w, time_over_two_pi, s, c);
         const float a_re = hv.x, a_im = hv.y;
         const float b_re = hm.x, b_im = -hm.y;
         const float fwd_re = a_re * c - a_im * s;
@@ -348,26 +399,27 @@ __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32
     float2* x_team = xch + tt * KX * team_stride<N>();
     if constexpr (KX == 2) {
         uint32_t idx[8];
-        fft_team_k<N, 2>(fr, fi, idx, J, x_team, team_stride<N>(), tw, BlockSync{});
+        fft_team_k<N, 2, BlockSync, MODE != kModeExact>(fr, fi, idx, J, x_team, team_stride<N>(),
+                                                        tw, BlockSync{});
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
-            float2* dst = spec + size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
+            const size_t row0 = size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
 #pragma unroll
-            for (int a = 0; a < 8; ++a) store_stream<N>(dst + idx[a], fr[f][a], fi[f][a]);
+            for (int a = 0; a < 8; ++a) store_field<N, MODE>(spec, row0 + idx[a], fr[f][a], fi[f][a]);
         }
     } else {
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
             uint32_t idx[8];
-            fft_team<N>(fr[f], fi[f], idx, J, x_team, tw, BlockSync{});
-            float2* dst = spec + size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
+            fft_team<N, BlockSync, MODE != kModeExact>(fr[f], fi[f], idx, J, x_team, tw, BlockSync{});
+            const size_t row0 = size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
 #pragma unroll
-            for (int a = 0; a < 8; ++a) store_stream<N>(dst + idx[a], fr[f][a], fi[f][a]);
+            for (int a = 0; a < 8; ++a) store_field<N, MODE>(spec, row0 + idx[a], fr[f][a], fi[f][a]);
         }
     }
 }
 
-template <int N, int F, int C, int KX>
+template <int N, int F, int C, int KX, int MODE = kModeExact>
 __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32_t blk,
                                                 float2* __restrict__ tw, float2* __restrict__ xch,
                                                 float2* __restrict__ xfer)
@@ -377,7 +429,7 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
     constexpr int BLOCK = C * T * G;
     constexpr uint32_t CELLS = uint32_t(N) * N;
     static_assert(T >= 64 && F % 2 == 0, "field split needs whole-wavefront teams");
-    const float2* __restrict__ spec = reinterpret_cast<const float2*>(L.spec);
+    const void* __restrict__ spec = L.spec;
     const float2* __restrict__ twg = reinterpret_cast<const float2*>(L.twiddles);
     float4* __restrict__ disp = reinterpret_cast<float4*>(L.disp);
     float4* __restrict__ normal = reinterpret_cast<float4*>(L.normal);
@@ -401,7 +453,8 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
     for (int f = 0; f < 2; ++f) {
 #pragma unroll
         for (int a = 0; a < 8; ++a) {
-            const float2 v = spec[size_t(2u * g + uint32_t(f)) * CELLS + size_t(J + a * T) * N + x];
+            const float2 v =
+                load_field<MODE>(spec, size_t(2u * g + uint32_t(f)) * CELLS + size_t(J + a * T) * N + x);
             fr[f][a] = v.x;
             fi[f][a] = v.y;
         }
@@ -409,12 +462,15 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
 
     uint32_t idx[8];
     if constexpr (KX == 2) {
-        fft_team_k<N, 2>(fr, fi, idx, J, xch + (g * C + c) * 2 * team_stride<N>(), team_stride<N>(),
-                         tw, BlockSync{});
+        fft_team_k<N, 2, BlockSync, MODE != kModeExact>(fr, fi, idx, J,
+                                                        xch + (g * C + c) * 2 * team_stride<N>(),
+                                                        team_stride<N>(), tw, BlockSync{});
     } else {
 #pragma unroll
         for (int f = 0; f < 2; ++f)
-            fft_team<N>(fr[f], fi[f], idx, J, xch + (g * C + c) * team_stride<N>(), tw, BlockSync{});
+            fft_team<N, BlockSync, MODE != kModeExact>(fr[f], fi[f], idx, J,
+                                                       xch + (g * C + c) * team_stride<N>(), tw,
+                                                       BlockSync{});
     }
 
     // Groups 0 and 1 share the finalize: group 0 packs the cells of registers
@@ -451,7 +507,7 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
         for (int a = 0; a < 4; ++a) {
             const float2 v2 = xf0[idx[a]];
             const float2 v3 = xf1[idx[a]];
-            pack_cell<false, false>(fr[0][a], fi[0][a], fr[1][a], fi[1][a], v2.x, v2.y, v3.x,
+            pack_cell<false, false, MODE>(fr[0][a], fi[0][a], fr[1][a], fi[1][a], v2.x, v2.y, v3.x,
                                     v3.y, lambda, foam_threshold, inv_threshold, disp, normal,
                                     size_t(idx[a]) * N + x);
         }
@@ -460,7 +516,7 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
         for (int a = 4; a < 8; ++a) {
             const float2 v0 = xf0[idx[a]];
             const float2 v1 = xf1[idx[a]];
-            pack_cell<false, false>(v0.x, v0.y, v1.x, v1.y, fr[0][a], fi[0][a], fr[1][a],
+            pack_cell<false, false, MODE>(v0.x, v0.y, v1.x, v1.y, fr[0][a], fi[0][a], fr[1][a],
                                     fi[1][a], lambda, foam_threshold, inv_threshold, disp, normal,
                                     size_t(idx[a]) * N + x);
         }

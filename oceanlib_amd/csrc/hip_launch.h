@@ -18,7 +18,9 @@ typedef struct ocean_hip_level {
     float foam_threshold;
     float inv_foam_threshold;   /* host-computed as in evolve.cpp:247-248      */
     uint32_t shape;             /* launch shape of the per-level passes: row
-                                   variant | column variant << 8 (0 = default) */
+                                   variant | column variant << 8 (0 = default),
+                                   | precision mode << 16 (0 = bit-exact,
+                                   1 = fast, 2 = fast with fp16 fields)      */
     const float* h0;            /* N*N float2 (re, im)                         */
     const float* omega;         /* N*N float                                   */
     const float* twiddles;      /* N/2 float2 (cos, +sin)                      */
@@ -120,11 +122,9 @@ hipError_t ocean_hip_cascade_sample(const ocean_hip_level* levels, uint32_t leve
 hipError_t ocean_hip_foam_query(const ocean_hip_level* level, const float* foam, float cell,
                                 const float* xz, uint32_t count, float* out, hipStream_t s);
 
-/* Mean of n floats in double: `partial` holds ocean_hip_coverage_blocks(n)
- * doubles of scratch; the mean lands in *out (device). */
-uint32_t ocean_hip_coverage_blocks(uint32_t n);
-hipError_t ocean_hip_coverage(const float* v, uint32_t n, double* partial, double* out,
-                              hipStream_t s);
+/* Mean of n > 0 floats as the sequential double sum in index order over n
+ * (foam.cpp:307-313), bit for bit; lands in *out (device). */
+hipError_t ocean_hip_coverage(const float* v, uint32_t n, double* out, hipStream_t s);
 
 /* ---- batched frames ------------------------------------------------------
  * Levels with N in [64, 2048] can be stepped together: every level is cut
@@ -143,8 +143,23 @@ typedef struct ocean_hip_batch_group {
     uint32_t n_items;
     uint32_t grid_row;             /* persistent grid sizes (autotuned)        */
     uint32_t grid_col;
-    uint32_t pad_;
+    uint32_t mode;                 /* precision mode of the group's kernels    */
 } ocean_hip_batch_group;
+/* The opt-in fast modes' kernels (mode 1: frame_fast_kernels.hip, mode 2:
+ * frame_half_kernels.hip): the per-level passes of a level whose shape
+ * carries that mode, their variant counts, and the batched kernels of a
+ * group of that mode (NULL if none). */
+hipError_t ocean_hip_frame_fast_1(const ocean_hip_level* level, double time_over_two_pi,
+                                  hipStream_t s, unsigned passes, hipEvent_t* events);
+hipError_t ocean_hip_frame_fast_2(const ocean_hip_level* level, double time_over_two_pi,
+                                  hipStream_t s, unsigned passes, hipEvent_t* events);
+int ocean_hip_frame_shapes_fast_1(uint32_t n, uint32_t fields, int* row_variants, int* col_variants);
+int ocean_hip_frame_shapes_fast_2(uint32_t n, uint32_t fields, int* row_variants, int* col_variants);
+void* ocean_hip_batch_kernel_fast_1(uint32_t n, int fields, int row, int per_item);
+void* ocean_hip_batch_kernel_fast_2(uint32_t n, int fields, int row, int per_item);
+/* Variant counts of the per-level passes of one precision mode. */
+int ocean_hip_frame_shapes_mode(uint32_t n, uint32_t fields, uint32_t mode, int* row_variants,
+                                int* col_variants);
 /* Direct launches of one group (passes: 1 row, 2 column, 3 both), for tuning. */
 hipError_t ocean_hip_batch_group_frame(const ocean_hip_batch_group* group,
                                        double time_over_two_pi, unsigned passes, hipStream_t s);

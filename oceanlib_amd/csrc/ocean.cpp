@@ -504,6 +504,19 @@ bool device_span_ok(const void* p, std::size_t bytes) noexcept
 
 }  // namespace
 
+bool Ocean::set_precision(std::uint32_t mode) noexcept
+{
+    detail::OceanDevice& d = impl_->dev;
+    if (!d.own_stream || mode > 2u) return false;  // a batch's or cascade's level: its owner's mode
+    d.make_current();
+    d.level.shape = mode << 16;  // the mode's default launch shapes
+    d.note(hipMemcpyAsync(d.d_level, &d.level, sizeof(d.level), hipMemcpyHostToDevice, d.stream));
+    d.note(hipStreamSynchronize(d.stream));
+    return d.status == 0;
+}
+
+std::uint32_t Ocean::precision() const noexcept { return (impl_->dev.level.shape >> 16) & 0xffu; }
+
 bool Ocean::bind_outputs(float* displacement, float* normal, float* velocity) noexcept
 {
     detail::OceanDevice& d = impl_->dev;

@@ -5,9 +5,11 @@
 //       src/cascade.cpp:82-153), one lane per point;
 //   foam_step_kernel / foam_query_kernel : FoamField's semi-Lagrangian step
 //       and foam_at (reference src/foam.cpp:22-139,291-305);
-//   coverage kernels : FoamField::coverage as a device reduction.
+//   coverage_kernel : FoamField::coverage, the reference's sequential double
+//       sum bit for bit, by one workgroup (ordered_sum.hpp).
 // The frame kernels themselves are in frame_kernels.hip.
 #include "hip_launch.h"
+#include "ordered_sum.hpp"
 #include "sample_device.hpp"
 
 #include <hip/hip_runtime.h>
@@ -117,32 +119,130 @@ __global__ __launch_bounds__(kBlock) void foam_query_kernel(
     out[q] = sample_wrap(foam, u, v, static_cast<int>(n));
 }
 
-constexpr uint32_t kCoverageBlocks = 256;
+// FoamField::coverage (reference src/foam.cpp:307-313): s = 0; s += v[i] in
+// index order, in double, then s / n. One workgroup walks the array in chunks
+// of 4096 terms (4 consecutive per lane); within a chunk, runs of terms that
+// keep s in one binade are added in one exact step from a scan of the terms
+// rounded to that binade's spacing, and the term that ends a run is added the
+// plain way (ordered_sum.hpp). Same bits as the sequential loop.
+constexpr uint32_t kCovLanes = 1024;
+constexpr uint32_t kCovPer = 4;
+constexpr uint32_t kCovChunk = kCovLanes * kCovPer;
 
-__global__ __launch_bounds__(kBlock) void coverage_partial_kernel(const float* __restrict__ v,
-                                                                  uint32_t n,
-                                                                  double* __restrict__ partial)
+__device__ __forceinline__ void coverage_load(const float* __restrict__ v, uint32_t n,
+                                              uint32_t first, float (&t)[kCovPer])
 {
-    __shared__ double red[kBlock];
-    double acc = 0.0;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-        acc += static_cast<double>(v[i]);
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < uint32_t(w)) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
+    if (first + kCovPer <= n) {
+        const float4 q = *reinterpret_cast<const float4*>(v + first);
+        t[0] = q.x; t[1] = q.y; t[2] = q.z; t[3] = q.w;
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < kCovPer; ++j) t[j] = first + j < n ? v[first + j] : 0.0f;
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-__global__ void coverage_final_kernel(const double* __restrict__ partial, uint32_t blocks,
-                                      uint32_t n, double* __restrict__ out)
+__global__ __launch_bounds__(kCovLanes) void coverage_kernel(const float* __restrict__ v,
+                                                             uint32_t n, double* __restrict__ out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double wave_sum[kCovLanes / 64];
+    __shared__ uint32_t wave_end[kCovLanes / 64];
+    __shared__ double bc_prefix;
+    __shared__ float bc_term;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const bool vec = (reinterpret_cast<uintptr_t>(v) & 15u) == 0u;
     double s = 0.0;
-    for (uint32_t b = 0; b < blocks; ++b) s += partial[b];
-    *out = s / static_cast<double>(n);
+    float t[kCovPer], next[kCovPer];
+    if (vec) {
+        coverage_load(v, n, tid * kCovPer, next);
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < kCovPer; ++j) next[j] = tid * kCovPer + j < n ? v[tid * kCovPer + j] : 0.0f;
+    }
+    for (uint32_t base = 0; base < n; base += kCovChunk) {
+#pragma unroll
+        for (uint32_t j = 0; j < kCovPer; ++j) t[j] = next[j];
+        const uint32_t nb = base + kCovChunk + tid * kCovPer;  // the next chunk, in flight
+        if (vec) {
+            coverage_load(v, n, nb, next);
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < kCovPer; ++j) next[j] = nb + j < n ? v[nb + j] : 0.0f;
+        }
+        // Terms past n read as +0.0: they neither end a run nor change s.
+        uint32_t pos = 0;  // chunk-local index of the first term not yet in s
+        while (pos < kCovChunk) {
+            const Binade bin = binade_of(s);
+            if (bin.mode == 2) {
+                // s is negative or not finite: the rest term by term, one lane
+                if (tid == 0) {
+                    for (uint32_t i = base + pos; i < n; ++i) s += static_cast<double>(v[i]);
+                    *out = s / static_cast<double>(n);
+                }
+                return;
+            }
+            double p[kCovPer];
+            bool stop[kCovPer];
+            double run = 0.0;
+#pragma unroll
+            for (uint32_t j = 0; j < kCovPer; ++j) {
+                bool st = false;
+                double r = 0.0;
+                if (tid * kCovPer + j >= pos) r = rounded_term(t[j], bin, st);
+                run += r;
+                p[j] = run;
+                stop[j] = st;
+            }
+            double x = run;  // inclusive scan of the lanes' sums over the wavefront
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const double y = __shfl_up(x, off, 64);
+                if (lane >= uint32_t(off)) x += y;
+            }
+            double lane_before = __shfl_up(x, 1, 64);
+            if (lane == 0) lane_before = 0.0;
+            if (lane == 63) wave_sum[wave] = x;
+            __syncthreads();
+            double before = 0.0;
+            for (uint32_t w = 0; w < wave; ++w) before += wave_sum[w];
+            before += lane_before;  // this chunk's r before this lane's terms
+            uint32_t end = kCovChunk;
+#pragma unroll
+            for (int j = kCovPer - 1; j >= 0; --j)
+                if (tid * kCovPer + j >= pos && ends_run(stop[j], before + p[j], bin))
+                    end = tid * kCovPer + j;
+            uint32_t m = end;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) m = min(m, uint32_t(__shfl_xor(int(m), off, 64)));
+            if (lane == 0) wave_end[wave] = m;
+            __syncthreads();
+            uint32_t e = kCovChunk;
+            for (uint32_t w = 0; w < kCovLanes / 64; ++w) e = min(e, wave_end[w]);
+            if (e == kCovChunk) {
+                double total = 0.0;
+                for (uint32_t w = 0; w < kCovLanes / 64; ++w) total += wave_sum[w];
+                s += total;  // exact: the whole chunk stays in the binade
+                pos = kCovChunk;
+            } else {
+                if (tid == e / kCovPer) {
+                    const uint32_t j = e % kCovPer;
+                    double pre = before;
+#pragma unroll
+                    for (uint32_t k = 0; k < kCovPer; ++k)
+                        if (k + 1 == j) pre = before + p[k];
+                    bc_prefix = pre;
+#pragma unroll
+                    for (uint32_t k = 0; k < kCovPer; ++k)
+                        if (k == j) bc_term = t[k];
+                }
+                __syncthreads();
+                s += bc_prefix;                       // exact: the run before the end
+                s += static_cast<double>(bc_term);    // the ending term, rounded as usual
+                pos = e + 1;
+            }
+            __syncthreads();  // wave_sum / wave_end / bc_* are rewritten next
+        }
+    }
+    if (tid == 0) *out = s / static_cast<double>(n);
 }
 
 }  // namespace
@@ -197,18 +297,10 @@ extern "C" hipError_t ocean_hip_foam_query(const ocean_hip_level* L, const float
     return hipGetLastError();
 }
 
-extern "C" uint32_t ocean_hip_coverage_blocks(uint32_t n)
+extern "C" hipError_t ocean_hip_coverage(const float* v, uint32_t n, double* out, hipStream_t s)
 {
-    const uint32_t want = (n + kBlock - 1) / kBlock;
-    return want < kCoverageBlocks ? (want == 0 ? 1 : want) : kCoverageBlocks;
-}
-
-extern "C" hipError_t ocean_hip_coverage(const float* v, uint32_t n, double* partial,
-                                         double* out, hipStream_t s)
-{
-    const uint32_t blocks = ocean_hip_coverage_blocks(n);
-    hipLaunchKernelGGL(coverage_partial_kernel, dim3(blocks), dim3(kBlock), 0, s, v, n, partial);
-    hipLaunchKernelGGL(coverage_final_kernel, dim3(1), dim3(64), 0, s, partial, blocks, n, out);
+    if (n == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(coverage_kernel, dim3(1), dim3(kCovLanes), 0, s, v, n, out);
     return hipGetLastError();
 }
 

@@ -86,9 +86,9 @@ def test_config2_foam_and_queries_vs_golden_and_oracle(lib, golden):
         got_f = np.array([lib.ocean_amd_foam_at(foam, float(x), float(z)) for x, z in pts],
                          np.float32).view(np.uint32)
         assert np.array_equal(got_f, want_f), np.nonzero(got_f != want_f)[0][:8]
-        cov = lib.ocean_amd_foam_coverage(foam)
-        want_cov = float(np.array([final["coverage"]], np.uint64).view(np.float64)[0])
-        assert abs(cov - want_cov) <= 1e-12 * want_cov and cov > 0.0
+        # coverage: the reference's sequential double sum, bit for bit
+        cov = np.float64(lib.ocean_amd_foam_coverage(foam))
+        assert int(cov.view(np.uint64)) == final["coverage"] and cov > 0.0
         # clear -> zero field, accumulator reset (reference src/foam.cpp:254-260)
         lib.ocean_amd_foam_clear(foam)
         assert lib.ocean_amd_foam_coverage(foam) == 0.0

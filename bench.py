@@ -83,24 +83,55 @@ def traffic_from_profiles(kernels):
         return None
 
 
-def cpu_baseline(desc, frames_budget_s: float = 12.0, threads: int | None = None):
-    """The reference library (oracle/_ref) on this host, same frames."""
-    import numpy as np
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    cores = threads or max(1, min(16, os.cpu_count() or 1))
-    kind = "reference"
+def cpu_model() -> str:
     try:
-        from oracle_ffi import ref, default_foam_params
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def ref_lib():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    try:
+        from oracle_ffi import ref
         r = ref()
     except OSError:
-        r = None
+        return None
+    if r is not None:
+        r.oref_hardware_concurrency.restype = C.c_uint32
+        for name in ("oref_height_batch_mt", "oref_cascade_height_batch_mt"):
+            getattr(r, name).argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32,
+                                         C.POINTER(C.c_float), C.c_uint32]
+    return r
+
+
+def host_threads() -> int:
+    """std::thread::hardware_concurrency() of this host: the reference's
+    thread count for thread_count = 0 (src/ocean.cpp:119-120)."""
+    r = ref_lib()
+    return int(r.oref_hardware_concurrency()) if r is not None else (os.cpu_count() or 1)
+
+
+def cpu_baseline(desc, frames_budget_s: float = 12.0, threads: int = 0):
+    """The reference library (oracle/_ref) on this host, same frames.
+    threads = 0 is the reference's own default, one worker per hardware
+    thread (std::thread::hardware_concurrency())."""
+    import numpy as np
+    from oracle_ffi import default_foam_params
+    r = ref_lib()
     if r is None:
         return None
+    cores = threads if threads > 0 else host_threads()
+    kind = "reference"
     r.oref_sample_batch.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_void_p]
     r.oref_height_batch.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32,
                                     C.POINTER(C.c_float)]
     d = type(desc).from_buffer_copy(desc)
-    d.thread_count = cores
+    d.thread_count = threads
     st = C.c_int(-1)
     h = r.oref_create(C.byref(d), C.byref(st))
     fp = default_foam_params()
@@ -131,10 +162,42 @@ def cpu_baseline(desc, frames_budget_s: float = 12.0, threads: int | None = None
     return {"value": frames / el, "unit": "updates/s", "cores": cores, "kind": kind,
             "sample": f"{frames} frames of the same workload ({el:.1f} s): reference "
                       f"Ocean::update with velocity, FoamField::update and "
-                      f"{QUERIES_PER_FRAME} height_at calls per frame, thread_count={cores}"}
+                      f"{QUERIES_PER_FRAME} height_at calls per frame, thread_count={threads} "
+                      f"({cores} workers)"}
 
 
-def batched_leg(lib, frames: int, warmup: int, tune: bool = True):
+def cpu_query_baseline(make, batch_fn, xz, budget_s: float = 2.0):
+    """Queries/s of a reference query loop (oracle/_ref) over the points xz,
+    at 1 thread and at one thread per hardware thread."""
+    import numpy as np
+    r = ref_lib()
+    if r is None:
+        return None
+    h, destroy = make(r)
+    n = len(xz)
+    pts = np.ascontiguousarray(xz, np.float32).reshape(-1)
+    out = np.empty(n, np.float32)
+    fp = C.POINTER(C.c_float)
+    res = {}
+    for label, threads in (("1_thread", 1), ("all_threads", 0)):
+        fn = getattr(r, batch_fn)
+        fn(h, pts.ctypes.data_as(fp), min(n, 4096), out.ctypes.data_as(fp), threads)  # warm
+        calls = 0
+        t0 = time.perf_counter()
+        while True:
+            fn(h, pts.ctypes.data_as(fp), n, out.ctypes.data_as(fp), threads)
+            calls += 1
+            el = time.perf_counter() - t0
+            if el >= budget_s:
+                break
+        res[label] = round(calls * n / el, 1)
+    destroy(r, h)
+    res["threads_all"] = host_threads()
+    res["points_per_call"] = n
+    return res
+
+
+def batched_leg(lib, frames: int, warmup: int, tune: bool = True, precision: int = 0):
     """The batched uneven seas (256 seas, 1-3 levels of N 64..512 each) as one
     OceanBatch: autotuned, then `frames` frames timed between two
     synchronizes. Reports 512^2-equivalent patch updates per second (cells
@@ -151,6 +214,8 @@ def batched_leg(lib, frames: int, warmup: int, tune: bool = True):
         raise SystemExit(f"ocean_amd_batch_create failed: {st.value}")
     build_s = time.perf_counter() - t0
     log(f"bench: batch of {len(descs)} levels built in {build_s:.1f} s")
+    if precision:
+        assert lib.ocean_amd_batch_set_precision(b, precision) == 0
     if tune:
         lib.ocean_amd_batch_autotune(b, 5)
     groups = []
@@ -188,6 +253,7 @@ def batched_leg(lib, frames: int, warmup: int, tune: bool = True):
                                 for v in ("", "_item"))
                           for g in groups for p in ("row", "col")])},
         "groups": groups, "build_s": round(build_s, 2), "frames": frames,
+        "precision": ["exact", "fast", "fast_half"][precision],
     }
 
 
@@ -204,11 +270,11 @@ def batched_cpu_baseline(budget_s: float = 8.0):
     if r is None:
         return None
     from oceanlib_amd.workloads import config4_levels, desc_from
-    cores = max(1, min(16, os.cpu_count() or 1))
+    cores = host_threads()
     flat, _ = config4_levels(24)
     handles = []
     for kw in flat:
-        d = desc_from(dict(kw, thread_count=cores))
+        d = desc_from(dict(kw, thread_count=0))
         st = C.c_int(-1)
         handles.append((r.oref_create(C.byref(d), C.byref(st)), kw["size"] ** 2))
     cells = 0
@@ -226,7 +292,7 @@ def batched_cpu_baseline(budget_s: float = 8.0):
             "cores": cores, "kind": "reference",
             "sample": f"{frames} frames of the first 24 seas ({len(flat)} levels, "
                       f"{el:.1f} s): reference Ocean::update level by level, "
-                      f"thread_count={cores}"}
+                      f"thread_count=0 ({cores} workers)"}
 
 
 def learned_leg(lib, reps: int = 10):
@@ -299,6 +365,33 @@ def learned_leg(lib, reps: int = 10):
                      "max_err_mm": round(err * 1e3, 5),
                      "not_converged": int(np.count_nonzero(fln & 1)),
                      "fallbacks": int(np.count_nonzero(fln & 2))}
+    # The reference's own cascade query (CascadeStack::height_at: four
+    # fixed-point steps per level, src/cascade.cpp:118-130) on the same sea
+    # and points: the device form over the 3-level range, and the reference
+    # library on the host.
+    h4 = torch.empty(nq, dtype=torch.float32, device="cuda")
+
+    def call4():
+        assert lib.ocean_amd_batch_sample_device(b, 0, 3, C.c_void_p(xz.data_ptr()), nq, None,
+                                                 C.c_void_p(h4.data_ptr())) == 0
+    call4()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        call4()
+    e1.record(stream)
+    e1.synchronize()
+    ms4 = e0.elapsed_time(e1) / reps
+    out["cascade_height_at_4_steps"] = {"queries_per_s": round(nq / (ms4 * 1e-3), 1),
+                                        "ms_per_1M": round(ms4, 4)}
+    descs_cpu = (capi.OceanDesc * 3)(*[desc_from(kw) for kw in levels])
+
+    def make_cascade(r):
+        c = r.oref_cascade_create(descs_cpu, 3)
+        r.oref_cascade_update(c, 5.0)
+        return c, lambda rr, hh: rr.oref_cascade_destroy(hh)
+    out["cascade_height_at_4_steps"]["cpu_baseline"] = cpu_query_baseline(
+        make_cascade, "oref_cascade_height_batch_mt", xz[: 1 << 18].cpu().numpy())
     lib.ocean_amd_proposal_destroy(model)
     lib.ocean_amd_batch_destroy(b)
     out["speedup"] = round(out["learned"]["queries_per_s"] / out["plain"]["queries_per_s"], 4)
@@ -396,11 +489,11 @@ def scene_cpu_baseline(budget_s: float = 8.0):
     if r is None:
         return None
     lib = capi_mod().load()
-    cores = max(1, min(16, os.cpu_count() or 1))
+    cores = host_threads()
     levels, idesc, mask, recs = scene_inputs(lib, 1200)
     for d in levels:
-        d.thread_count = cores
-    idesc.thread_count = cores
+        d.thread_count = 0
+    idesc.thread_count = 0
     cas = r.oref_cascade_create(levels, len(levels))
     st = C.c_int(-1)
     field = r.oref_interaction_create(C.byref(idesc), C.byref(st))
@@ -425,7 +518,7 @@ def scene_cpu_baseline(budget_s: float = 8.0):
     return {"value": round(steps / el, 1), "unit": "steps/s", "cores": cores, "kind": "reference",
             "sample": f"{steps} steps of the same scene ({el:.1f} s): reference "
                       f"CascadeStack::update and InteractionField add + update, "
-                      f"thread_count={cores}"}
+                      f"thread_count=0 ({cores} workers)"}
 
 
 def handoff_leg(lib, sim, foam, n: int, reps: int = 200):
@@ -619,6 +712,35 @@ def main():
     lib.ocean_amd_synchronize(sim)
     separate_per_s = args.steps / (time.perf_counter() - t0)
 
+    # The opt-in fast modes on the same frame loop (not the headline: the
+    # headline is the bit-exact frame).
+    fast = {}
+    for mode, name in ((1, "fast"), (2, "fast_half")):
+        assert lib.ocean_amd_set_precision(sim, mode) == 0
+        fr_, fc_ = C.c_uint32(), C.c_uint32()
+        lib.ocean_amd_autotune_frame(sim, 20, C.byref(fr_), C.byref(fc_), None, None)
+        base = total + args.warmup + args.steps
+        for k in range(base, base + args.warmup):
+            frame(k)
+        lib.ocean_amd_synchronize(sim)
+        lib.ocean_amd_frame_timing(sim, TIMING_EVERY)
+        t0 = time.perf_counter()
+        for k in range(base + args.warmup, base + args.warmup + args.steps):
+            frame(k)
+        lib.ocean_amd_synchronize(sim)
+        fps_fast = args.steps / (time.perf_counter() - t0)
+        fr_ms, fc_ms, fs_ = C.c_float(), C.c_float(), C.c_uint32()
+        lib.ocean_amd_frame_timing_read(sim, C.byref(fr_ms), C.byref(fc_ms), C.byref(fs_))
+        lib.ocean_amd_frame_timing(sim, 0)
+        fast[name] = {"frames_per_s": round(fps_fast, 1),
+                      "update_device_ms": round(fr_ms.value + fc_ms.value, 5),
+                      "row_ms": round(fr_ms.value, 5), "col_ms": round(fc_ms.value, 5),
+                      "frame_shape": [fr_.value, fc_.value]}
+    assert lib.ocean_amd_set_precision(sim, 0) == 0
+    assert lib.ocean_amd_set_frame_shape(sim, shape_r.value, shape_c.value) == 0
+    if lib.ocean_amd_last_error(sim) != 0:
+        raise SystemExit(f"device error {lib.ocean_amd_last_error(sim)}")
+
     # The same kernels launched back to back with nothing between them.
     iso_r, iso_c = C.c_float(), C.c_float()
     assert lib.ocean_amd_profile_frame(sim, 100.0, 1.0 / 60.0, 200, C.byref(iso_r),
@@ -659,8 +781,26 @@ def main():
     lib.ocean_amd_synchronize(sim)
     qps = reps * nq / (time.perf_counter() - tq)
 
+    # The reference's Ocean::height_at on the host, same sea, 256K of the points.
+    lib.ocean_update(sim, 1000.0 / 60.0)
+    cpu_queries = None
+    if not args.no_cpu:
+        def make_ocean(r):
+            d = desc_from(CONFIG2)
+            st2 = C.c_int(-1)
+            h = r.oref_create(C.byref(d), C.byref(st2))
+            r.oref_update(h, 1000.0 / 60.0)
+            return h, lambda rr, hh: rr.oref_destroy(hh)
+        cpu_queries = cpu_query_baseline(make_ocean, "oref_height_batch_mt",
+                                         big[: 1 << 18].cpu().numpy())
+
     handoff = handoff_leg(lib, sim, foam, n)
     batched = batched_leg(lib, args.batch_frames, 5) if args.batch_frames > 0 else None
+    if batched is not None:
+        for mode, name in ((1, "fast"), (2, "fast_half")):
+            bf = batched_leg(lib, args.batch_frames, 5, precision=mode)
+            batched[name] = {"patch_updates_512_per_s": bf["patch_updates_512_per_s"],
+                             "ms_per_frame": bf["ms_per_frame"]}
     learned = learned_leg(lib) if not args.no_learned else None
     scene = scene_leg(lib, args.scene_frames, 30) if args.scene_frames > 0 else None
     if scene is not None and not args.no_cpu:
@@ -669,6 +809,10 @@ def main():
     if cpu is not None:
         one = cpu_baseline(desc, 4.0, threads=1)
         cpu["value_1_thread"] = one["value"] if one else None
+        sixteen = cpu_baseline(desc, 4.0, threads=16)
+        cpu["value_16_threads"] = sixteen["value"] if sixteen else None
+        cpu["cpu_model"] = cpu_model()
+        cpu["hardware_threads"] = host_threads()
     if batched is not None and not args.no_cpu:
         batched["cpu_baseline"] = batched_cpu_baseline()
 
@@ -701,6 +845,8 @@ def main():
         "frames_per_s_separate_calls": round(separate_per_s, 1),
         "frame_shape": {"row_variant": shape_r.value, "col_variant": shape_c.value},
         "height_at_queries_per_s": round(qps, 1),
+        "height_at_cpu_baseline": cpu_queries,
+        "fast_mode": fast,
         "rgba16_handoff": handoff,
         "batched": batched,
         "converged_height_at": learned,

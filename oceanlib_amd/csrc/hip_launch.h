@@ -189,7 +189,9 @@ hipError_t ocean_hip_level_checksums(const ocean_hip_level* levels, uint32_t cou
 #define OCEAN_HIP_MLP_OUT 16
 #define OCEAN_HIP_MLP_PARAMS 9360
 #define OCEAN_HIP_MLP_BF16 18432   /* W1^T, W2^T, W3^T, W1, W2, W3 in bf16 */
-#define OCEAN_HIP_MLP_PROP 8       /* floats of proposal scratch per query */
+#define OCEAN_HIP_MLP_PROP 8       /* floats of proposal scratch per query: the
+                                      start offsets, level-major float2 per
+                                      (level, query); 6 used at 3 levels */
 #define OCEAN_HIP_MLP_SPACING_PARTIALS 64  /* doubles of scratch per level */
 typedef struct ocean_hip_mlp {
     uint16_t* bf16;   /* the weights rounded to bf16, in the kernels' layouts */

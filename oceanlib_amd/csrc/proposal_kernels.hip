@@ -434,46 +434,146 @@ __global__ void spacing_final_kernel(const ocean_hip_level* __restrict__ levels,
     spacing[lv] = spacing_of(sum, levels[lv]);
 }
 
-// Proposals: 256 lanes = 4 waves of 64 queries; features -> MLP -> the
-// start offsets of each level in cells, prop[q * kPropStride + 2l (+1)].
-constexpr int kPropStride = 8;
+// Proposals. Each wavefront takes 64 queries at a time and strides through
+// the query list (a persistent grid: the network is staged into LDS once per
+// block): lane l gathers query l's features level by level into row l of the
+// wavefront's own LDS tile (bf16), then the network runs as W^T X^T on MFMA,
+// the weight matrices as the A operand (from LDS) and the activations as the
+// B operand: a layer's output tile lands in exactly the lanes and elements
+// the next layer's B operand needs, so the hidden activations never leave
+// registers and no wavefront waits for another (the block's one barrier is
+// after the staging). The products and their summation per 16-input block are
+// those of the row-major form (forward_kernel). Output: each level's start
+// offset in cells, level-major float2: prop[lv * count + q].
 
-__global__ __launch_bounds__(256) void proposal_kernel(const ocean_hip_level* __restrict__ levels,
-                                                       uint32_t level_count,
-                                                       const float2* __restrict__ xz, uint32_t count,
-                                                       const uint16_t* __restrict__ pb,
-                                                       const float* __restrict__ pf,
-                                                       const int32_t* __restrict__ spacing,
-                                                       float* __restrict__ prop)
+// relu(acc + bias) rounded to bf16, as the next layer's B fragment; lane
+// (i, g)'s D fragment of output tile `it` holds units 16 it + 4 g + e.
+__device__ __forceinline__ bf16x4 relu_bf16(f32x4 acc, const float* __restrict__ bias)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
-    InferLds& w = *reinterpret_cast<InferLds*>(smem);
-    uint16_t* tile = reinterpret_cast<uint16_t*>(smem + sizeof(InferLds)) + (threadIdx.x >> 6) * kWaveTile;
-    load_infer(w, pb, pf);
-    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
-    const bool live = q < count;
-    const float2 p = live ? xz[q] : make_float2(0.0f, 0.0f);
-    float f[kIn];
+    bf16x4 r;
 #pragma unroll
-    for (int k = 0; k < kIn; ++k) f[k] = 0.0f;
-#pragma unroll
-    for (int lv = 0; lv < kLevels; ++lv) {  // constant indices keep f in registers
-        if (live && uint32_t(lv) < level_count) {
-            const ocean_hip_level L = levels[lv];
-            level_features(L, base_cell(p.x, p.y, L), spacing[lv], f + lv * kFeatPerLevel);
-        }
+    for (int e = 0; e < 4; ++e) {
+        const float z = acc[e] + bias[e];
+        r[e] = short(to_bf16(z > 0.0f ? z : 0.0f));
     }
-    store_row(tile, f);
-    __syncthreads();
-    float y[kOut];
-    forward_wave(w, tile, y);
-    if (!live) return;
+    return r;
+}
+
+// LDS ordering inside one wavefront: its LDS operations execute in program
+// order, so only the compiler has to be kept from moving them.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Writes lane l's 20 features of one level as bf16 into row l of X at
+// column c0 (a multiple of 4).
+__device__ __forceinline__ void store_level_row(uint16_t* X, int c0, const float* f)
+{
+    const int l = threadIdx.x & 63;
 #pragma unroll
-    for (int lv = 0; lv < kLevels; ++lv) {
-        if (uint32_t(lv) < level_count) {
-            const float sp = float(spacing[lv]);
-            prop[size_t(q) * kPropStride + 2 * lv] = y[2 * lv] * sp;
-            prop[size_t(q) * kPropStride + 2 * lv + 1] = y[2 * lv + 1] * sp;
+    for (int c = 0; c < kFeatPerLevel / 4; ++c) {
+        const uint32_t lo = uint32_t(to_bf16(f[4 * c])) | (uint32_t(to_bf16(f[4 * c + 1])) << 16);
+        const uint32_t hi = uint32_t(to_bf16(f[4 * c + 2])) | (uint32_t(to_bf16(f[4 * c + 3])) << 16);
+        *reinterpret_cast<uint2*>(X + l * kPW + c0 + 4 * c) = make_uint2(lo, hi);
+    }
+}
+
+constexpr int kPropWaves = 4;  // wavefronts per block
+static_assert(kLevels * kFeatPerLevel <= kIn, "feature columns");
+
+__global__ __launch_bounds__(64 * kPropWaves) void proposal_kernel(
+    const ocean_hip_level* __restrict__ levels, uint32_t level_count, const float2* __restrict__ xz,
+    uint32_t count, const uint16_t* __restrict__ pb, const float* __restrict__ pf,
+    const int32_t* __restrict__ spacing, float2* __restrict__ prop)
+{
+    __shared__ InferLds w;
+    __shared__ __align__(16) uint16_t tiles[kPropWaves][64 * kPW];
+    const int l = threadIdx.x & 63;
+    const int i = l & 15, g = l >> 4;
+    uint16_t* X = tiles[threadIdx.x >> 6];
+    load_infer(w, pb, pf);
+    // feature columns past the levels in use stay zero
+    for (int c = level_count < uint32_t(kLevels) ? int(level_count) * kFeatPerLevel : kLevels * kFeatPerLevel;
+         c < kIn; ++c)
+        X[l * kPW + c] = 0;
+    __syncthreads();
+    float sp[kLevels];
+#pragma unroll
+    for (int lv = 0; lv < kLevels; ++lv) sp[lv] = uint32_t(lv) < level_count ? float(spacing[lv]) : 0.0f;
+    const uint32_t waves = gridDim.x * kPropWaves;
+    for (uint32_t base = (blockIdx.x * kPropWaves + (threadIdx.x >> 6)) * 64u; base < count;
+         base += waves * 64u) {
+        const uint32_t q = base + uint32_t(l);
+        const bool live = q < count;
+        const float2 p = live ? xz[q] : make_float2(0.0f, 0.0f);
+        wave_lds_sync();  // the previous round's reads of X are done
+#pragma unroll
+        for (int lv = 0; lv < kLevels; ++lv) {
+            if (uint32_t(lv) < level_count) {
+                float f[kFeatPerLevel];
+                if (live) {
+                    const ocean_hip_level L = levels[lv];
+                    level_features(L, base_cell(p.x, p.y, L), spacing[lv], f);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < kFeatPerLevel; ++k) f[k] = 0.0f;
+                }
+                store_level_row(X, lv * kFeatPerLevel, f);
+            }
+        }
+        wave_lds_sync();
+        // The weights' LDS addresses, re-derived every round through an empty
+        // asm: otherwise the compiler hoists all 36 loop-invariant A fragments
+        // out of the loop into 72 registers, which halves the wavefronts the
+        // feature gathers can overlap.
+        uint32_t wofs = uint32_t(i * kPW + 4 * g);
+        asm volatile("" : "+v"(wofs));
+        const uint16_t* W1 = w.wt + wofs;
+        const uint16_t* W2 = W1 + kHid * kPW;
+        const uint16_t* W3 = W1 + 2 * kHid * kPW;
+#pragma unroll 1
+        for (int qt = 0; qt < 4; ++qt) {
+            bf16x4 x[4];
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) x[kb] = ld4(X + (16 * qt + i) * kPW + 16 * kb + 4 * g);
+            bf16x4 h1[4], h2[4];
+#pragma unroll
+            for (int it = 0; it < 4; ++it) {
+                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int kb = 0; kb < 4; ++kb)
+                    acc = mfma16(ld4(W1 + 16 * it * kPW + 16 * kb), x[kb], acc);
+                h1[it] = relu_bf16(acc, w.b1 + 16 * it + 4 * g);
+            }
+#pragma unroll
+            for (int it = 0; it < 4; ++it) {
+                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int kb = 0; kb < 4; ++kb)
+                    acc = mfma16(ld4(W2 + 16 * it * kPW + 16 * kb), h1[kb], acc);
+                h2[it] = relu_bf16(acc, w.b2 + 16 * it + 4 * g);
+            }
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) acc = mfma16(ld4(W3 + 16 * kb), h2[kb], acc);
+            // lane (i, g) holds outputs 4 g .. 4 g + 3 of query base + 16 qt + i:
+            // g = 0 levels 0 and 1, g = 1 level 2 (outputs 4, 5)
+            const uint32_t qo = base + uint32_t(16 * qt + i);
+            if (qo < count && g < 2) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int lv = 2 * g + h;
+                    if (lv < kLevels && uint32_t(lv) < level_count) {
+                        const float s_lv = lv == 0 ? sp[0] : (lv == 1 ? sp[1] : sp[2]);
+                        const float du = (acc[2 * h] + w.b3[4 * g + 2 * h]) * s_lv;
+                        const float dv = (acc[2 * h + 1] + w.b3[4 * g + 2 * h + 1]) * s_lv;
+                        prop[size_t(lv) * count + qo] = make_float2(du, dv);
+                    }
+                }
+            }
         }
     }
 }
@@ -484,7 +584,7 @@ template <bool LEARNED>
 __global__ __launch_bounds__(256) void converged_kernel(
     const ocean_hip_level* __restrict__ levels, uint32_t level_count, uint32_t lv_begin,
     uint32_t lv_end, const float2* __restrict__ xz, uint32_t count, float tol,
-    const float* __restrict__ prop, float* __restrict__ heights, uint8_t* __restrict__ iters,
+    const float2* __restrict__ prop, float* __restrict__ heights, uint8_t* __restrict__ iters,
     uint8_t* __restrict__ flags)
 {
     const uint32_t q = blockIdx.x * 256u + threadIdx.x;
@@ -501,8 +601,9 @@ __global__ __launch_bounds__(256) void converged_kernel(
         const bool use = LEARNED && lv < uint32_t(kLevels);
         float du0 = 0.0f, dv0 = 0.0f;
         if (use) {
-            du0 = prop[size_t(q) * kPropStride + 2 * lv];
-            dv0 = prop[size_t(q) * kPropStride + 2 * lv + 1];
+            const float2 st = prop[size_t(lv) * count + q];
+            du0 = st.x;
+            dv0 = st.y;
             if (!(fabsf(du0) < 1.0e4f) || !(fabsf(dv0) < 1.0e4f)) {
                 du0 = dv0 = 0.0f;  // not finite or far off the patch: abandoned at once
                 fl |= 2u;
@@ -839,17 +940,30 @@ extern "C" hipError_t ocean_hip_height_converged(const ocean_hip_level* levels, 
     if (count == 0) return hipSuccess;
     const dim3 grid((count + 255u) / 256u);
     const float2* p = reinterpret_cast<const float2*>(xz);
+    float2* prop2 = reinterpret_cast<float2*>(prop);
     if (mlp != nullptr) {
         if (spacing == nullptr || prop == nullptr) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(proposal_kernel, grid, dim3(256), kInferSmem, s, levels, level_count, p,
-                           count, mlp->bf16, mlp->master, spacing, prop);
+        static int prop_grid = 0;  // resident proposal blocks on this device
+        if (prop_grid == 0) {
+            int dev = 0, cus = 0, per = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, proposal_kernel, 64 * kPropWaves, 0) !=
+                    hipSuccess)
+                return hipErrorInvalidValue;
+            prop_grid = cus * (per > 0 ? per : 1);
+        }
+        const uint32_t need = (count + 64u * kPropWaves - 1u) / (64u * kPropWaves);
+        const uint32_t pg = need < uint32_t(prop_grid) ? need : uint32_t(prop_grid);
+        hipLaunchKernelGGL(proposal_kernel, dim3(pg), dim3(64 * kPropWaves), 0, s, levels, level_count,
+                           p, count, mlp->bf16, mlp->master, spacing, prop2);
     }
     // One launch per level: all queries tap one level's 4 MB (at 512^2) at a
     // time, which an XCD's L2 holds, instead of every level's at once.
     for (uint32_t lv = 0; lv < level_count; ++lv) {
         if (mlp != nullptr)
             hipLaunchKernelGGL(converged_kernel<true>, grid, dim3(256), 0, s, levels, level_count, lv,
-                               lv + 1, p, count, tol, prop, heights, iters, flags);
+                               lv + 1, p, count, tol, prop2, heights, iters, flags);
         else
             hipLaunchKernelGGL(converged_kernel<false>, grid, dim3(256), 0, s, levels, level_count,
                                lv, lv + 1, p, count, tol, nullptr, heights, iters, flags);
@@ -902,9 +1016,6 @@ extern "C" int ocean_hip_mlp_configure()
 {
     static int done = 0;
     if (done) return 0;
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&proposal_kernel),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, int(kInferSmem)) != hipSuccess)
-        return -1;
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(&forward_kernel),
                             hipFuncAttributeMaxDynamicSharedMemorySize, int(kInferSmem)) != hipSuccess)
         return -1;

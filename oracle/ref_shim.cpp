@@ -24,8 +24,10 @@
 #include "ocean/ocean.h"
 #include "ocean/ocean.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <new>
+#include <thread>
 #include <vector>
 
 #define OREF extern "C" __attribute__((visibility("default")))
@@ -285,6 +287,46 @@ OREF void oref_height_batch(const ocean_sim* s, const float* xz, std::uint32_t c
 {
     const Ocean& o = *reinterpret_cast<const Ocean*>(s);
     for (std::uint32_t i = 0; i < count; ++i) heights[i] = o.height_at(xz[2 * i], xz[2 * i + 1]);
+}
+
+// The same query loops over `threads` host threads, each taking a contiguous
+// slice of the points (the reference's queries are const and may run
+// concurrently on one handle, include/ocean/ocean.hpp:17-31). threads == 0:
+// std::thread::hardware_concurrency(), as the reference's thread_count = 0.
+namespace {
+template <typename Fn>
+void split_threads(std::uint32_t count, std::uint32_t threads, Fn fn)
+{
+    if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    std::vector<std::thread> pool;
+    const std::uint32_t per = (count + threads - 1) / threads;
+    for (std::uint32_t t = 0; t < threads; ++t) {
+        const std::uint32_t a = t * per, b = std::min(count, a + per);
+        if (a >= b) break;
+        pool.emplace_back([=] { fn(a, b); });
+    }
+    for (std::thread& th : pool) th.join();
+}
+}  // namespace
+
+OREF std::uint32_t oref_hardware_concurrency() { return std::thread::hardware_concurrency(); }
+
+OREF void oref_height_batch_mt(const ocean_sim* s, const float* xz, std::uint32_t count,
+                               float* heights, std::uint32_t threads)
+{
+    const Ocean& o = *reinterpret_cast<const Ocean*>(s);
+    split_threads(count, threads, [&](std::uint32_t a, std::uint32_t b) {
+        for (std::uint32_t i = a; i < b; ++i) heights[i] = o.height_at(xz[2 * i], xz[2 * i + 1]);
+    });
+}
+
+OREF void oref_cascade_height_batch_mt(void* c, const float* xz, std::uint32_t count,
+                                       float* heights, std::uint32_t threads)
+{
+    const CascadeStack& st = *static_cast<const CascadeStack*>(c);
+    split_threads(count, threads, [&](std::uint32_t a, std::uint32_t b) {
+        for (std::uint32_t i = a; i < b; ++i) heights[i] = st.height_at(xz[2 * i], xz[2 * i + 1]);
+    });
 }
 
 // ---- the viewer's half conversion (examples/viewer/half.hpp, header-only;

@@ -313,6 +313,24 @@ int  ocean_amd_batch_height_converged(const ocean_amd_batch* batch, uint32_t fir
                                       uint32_t level_count, const float* xz_dev, uint32_t count,
                                       float tol, const ocean_amd_proposal* model,
                                       float* heights_dev, uint8_t* iters_dev, uint8_t* flags_dev);
+/* No counterpart in the reference. For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): ocean_amd_batch_height_converged with the same arguments and
+ * results, but the proposal's per-call scratch (the levels' stencil spacings
+ * and every query's start offsets) comes from a file-scope device array of
+ * the library instead of the model's own, so the call allocates nothing and
+ * the model's max_queries does not bound it. Limit: count <=
+ * OCEAN_AMD_STATIC_MAX_QUERIES (larger counts return OCEAN_ERROR_INVALID_ARG;
+ * split them into calls of at most that many). Each GPU holds one copy of the
+ * array (about 8.4 MB) until the process ends, shared by every call on that
+ * GPU: calls that overlap on one GPU (other streams, other threads) race on
+ * it. Without a model no scratch is used. */
+#define OCEAN_AMD_STATIC_MAX_QUERIES 262144u
+int  ocean_amd_batch_height_converged_static(const ocean_amd_batch* batch, uint32_t first,
+                                             uint32_t level_count, const float* xz_dev,
+                                             uint32_t count, float tol,
+                                             const ocean_amd_proposal* model, float* heights_dev,
+                                             uint8_t* iters_dev, uint8_t* flags_dev);
 int  ocean_amd_cascade_height_converged(const ocean_amd_cascade* stack, const float* xz_dev,
                                         uint32_t count, float tol, const ocean_amd_proposal* model,
                                         float* heights_dev, uint8_t* iters_dev,

@@ -92,6 +92,16 @@ bool height_converged(const OceanBatch& batch, std::size_t first, std::size_t le
 bool height_converged(const CascadeStack& stack, const float* xz_dev, std::uint32_t count, float tol,
                       const HeightProposal* model, float* heights_dev, std::uint8_t* iters_dev,
                       std::uint8_t* flags_dev) noexcept;
+// The batch form above with the proposal's per-call scratch taken from a
+// library-wide static device array instead of the model's own, so the call
+// allocates nothing: count <= 262144 (OCEAN_AMD_STATIC_MAX_QUERIES); calls
+// that overlap on one GPU share that array and race. *error receives the HIP
+// error of the array's one-time address lookup on this device, if any.
+bool height_converged_static(const OceanBatch& batch, std::size_t first, std::size_t level_count,
+                             const float* xz_dev, std::uint32_t count, float tol,
+                             const HeightProposal* model, float* heights_dev,
+                             std::uint8_t* iters_dev, std::uint8_t* flags_dev,
+                             int* error) noexcept;
 
 // Training pairs from batch levels [first, first + 3): per point the 60
 // features (count x inputs floats) and the plain solution's start offsets

@@ -260,6 +260,9 @@ AMD_SIGNATURES = {
     "ocean_amd_batch_height_converged": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_void_p,
                                                    C.c_uint32, C.c_float, _P, C.c_void_p,
                                                    C.c_void_p, C.c_void_p]),
+    "ocean_amd_batch_height_converged_static": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_void_p,
+                                                          C.c_uint32, C.c_float, _P, C.c_void_p,
+                                                          C.c_void_p, C.c_void_p]),
     "ocean_amd_cascade_height_converged": (C.c_int, [_P, C.c_void_p, C.c_uint32, C.c_float, _P,
                                                      C.c_void_p, C.c_void_p, C.c_void_p]),
     "ocean_amd_proposal_pairs": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_uint32, C.c_float,

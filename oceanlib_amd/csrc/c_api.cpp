@@ -1129,6 +1129,24 @@ int ocean_amd_batch_height_converged(const ocean_amd_batch* batch, uint32_t firs
                : OCEAN_ERROR_UNKNOWN;
 }
 
+int ocean_amd_batch_height_converged_static(const ocean_amd_batch* batch, uint32_t first,
+                                            uint32_t level_count, const float* xz_dev,
+                                            uint32_t count, float tol,
+                                            const ocean_amd_proposal* model, float* heights_dev,
+                                            uint8_t* iters_dev, uint8_t* flags_dev)
+{
+    if (!batch_range(batch, first, level_count) || !(tol > 0.0f)) return OCEAN_ERROR_INVALID_ARG;
+    if (count == 0) return OCEAN_OK;
+    if (xz_dev == nullptr || heights_dev == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    if (count > OCEAN_AMD_STATIC_MAX_QUERIES || (model != nullptr && level_count > 3u))
+        return OCEAN_ERROR_INVALID_ARG;
+    int error = 0;
+    return ocean::height_converged_static(*to_batch(batch), first, level_count, xz_dev, count, tol,
+                                          to_model(model), heights_dev, iters_dev, flags_dev, &error)
+               ? OCEAN_OK
+               : OCEAN_ERROR_UNKNOWN;
+}
+
 int ocean_amd_cascade_height_converged(const ocean_amd_cascade* stack, const float* xz_dev,
                                        uint32_t count, float tol, const ocean_amd_proposal* model,
                                        float* heights_dev, uint8_t* iters_dev, uint8_t* flags_dev)

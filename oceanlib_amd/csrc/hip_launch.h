@@ -217,6 +217,20 @@ hipError_t ocean_hip_height_converged(const ocean_hip_level* levels, uint32_t le
                                       const ocean_hip_mlp* mlp, const int32_t* spacing, float* prop,
                                       float* heights, uint8_t* iters, uint8_t* flags,
                                       hipStream_t s);
+/* The scratch of the allocation-free converged query: views into a file-scope
+ * device array of OCEAN_HIP_STATIC_SCRATCH_BYTES on the current device (its
+ * address looked up once per device), shared by every call on that device. */
+#define OCEAN_HIP_STATIC_MAX_QUERIES 262144u
+#define OCEAN_HIP_STATIC_SCRATCH_BYTES \
+    (256u + ((OCEAN_HIP_MLP_LEVELS * OCEAN_HIP_MLP_SPACING_PARTIALS * 8u + 255u) / 256u) * 256u + \
+     OCEAN_HIP_STATIC_MAX_QUERIES * OCEAN_HIP_MLP_PROP * 4u)
+typedef struct ocean_hip_static_scratch_view {
+    int32_t* spacing;          /* OCEAN_HIP_MLP_LEVELS                          */
+    double* spacing_partial;   /* OCEAN_HIP_MLP_LEVELS * SPACING_PARTIALS      */
+    float* prop;               /* max_queries * OCEAN_HIP_MLP_PROP             */
+    uint32_t max_queries;
+} ocean_hip_static_scratch_view;
+hipError_t ocean_hip_static_scratch(ocean_hip_static_scratch_view* out);
 /* Training pairs for the first MLP_LEVELS levels: features (count * IN),
  * targets (count * OUT) from a plain solve to `tol`, ok[q] = 1 if every
  * level converged. */

@@ -84,24 +84,37 @@ bool wait_for(const HeightProposal* model, hipStream_t s) noexcept
     return hipStreamWaitEvent(s, model->impl().done, 0) == hipSuccess;
 }
 
+// The proposal's scratch for one call: the model's own (sized at its
+// creation), or the library's static device array (the allocation-free form).
+struct ProposalScratch {
+    std::int32_t* spacing;
+    double* spacing_partial;
+    float* prop;
+    std::uint32_t max_queries;
+};
+
 bool converged(const ocean_hip_level* levels, std::size_t count_levels, hipStream_t s,
                const float* xz_dev, std::uint32_t count, float tol, const HeightProposal* model,
-               float* heights_dev, std::uint8_t* iters_dev, std::uint8_t* flags_dev) noexcept
+               float* heights_dev, std::uint8_t* iters_dev, std::uint8_t* flags_dev,
+               const ProposalScratch* use = nullptr) noexcept
 {
     if (count == 0) return true;
     if (xz_dev == nullptr || heights_dev == nullptr || count_levels == 0 || !(tol > 0.0f)) return false;
     if (model != nullptr) {
         HeightProposal::Impl& m = model->impl();
-        if (count_levels > ProposalShape::levels || count > m.max_queries) return false;
+        const ProposalScratch own{m.spacing, m.spacing_partial, m.prop, m.max_queries};
+        const ProposalScratch& sc = use != nullptr ? *use : own;
+        if (count_levels > ProposalShape::levels || count > sc.max_queries) return false;
         if (!wait_for(model, s)) return false;
-        if (ocean_hip_mlp_spacing(levels, static_cast<std::uint32_t>(count_levels), m.spacing,
-                                  m.spacing_partial, s) != hipSuccess)
+        if (ocean_hip_mlp_spacing(levels, static_cast<std::uint32_t>(count_levels), sc.spacing,
+                                  sc.spacing_partial, s) != hipSuccess)
             return false;
         const bool ok = ocean_hip_height_converged(levels, static_cast<std::uint32_t>(count_levels),
-                                                   xz_dev, count, tol, &m.mlp, m.spacing, m.prop,
+                                                   xz_dev, count, tol, &m.mlp, sc.spacing, sc.prop,
                                                    heights_dev, iters_dev, flags_dev,
                                                    s) == hipSuccess;
-        // The proposal's scratch is in use until this query ends.
+        // The proposal's parameters (and its own scratch) are in use until
+        // this query ends: later work on the model waits for it.
         return ok && hipEventRecord(m.done, s) == hipSuccess &&
                hipStreamWaitEvent(m.stream.s, m.done, 0) == hipSuccess;
     }
@@ -257,6 +270,32 @@ bool height_converged(const OceanBatch& batch, std::size_t first, std::size_t le
     const auto* levels = static_cast<const ocean_hip_level*>(batch.device_levels()) + first;
     return converged(levels, level_count, static_cast<hipStream_t>(batch.stream()), xz_dev, count, tol,
                      model, heights_dev, iters_dev, flags_dev);
+}
+
+bool height_converged_static(const OceanBatch& batch, std::size_t first, std::size_t level_count,
+                             const float* xz_dev, std::uint32_t count, float tol,
+                             const HeightProposal* model, float* heights_dev,
+                             std::uint8_t* iters_dev, std::uint8_t* flags_dev,
+                             int* error) noexcept
+{
+    *error = 0;
+    if (level_count == 0 || first > batch.level_count() || level_count > batch.level_count() - first)
+        return false;
+    if (count > OCEAN_HIP_STATIC_MAX_QUERIES) return false;
+    batch.level(0).device().make_current();
+    ProposalScratch sc{};
+    if (model != nullptr) {
+        ocean_hip_static_scratch_view v{};
+        const hipError_t e = ocean_hip_static_scratch(&v);
+        if (e != hipSuccess) {
+            *error = static_cast<int>(e);
+            return false;
+        }
+        sc = ProposalScratch{v.spacing, v.spacing_partial, v.prop, v.max_queries};
+    }
+    const auto* levels = static_cast<const ocean_hip_level*>(batch.device_levels()) + first;
+    return converged(levels, level_count, static_cast<hipStream_t>(batch.stream()), xz_dev, count, tol,
+                     model, heights_dev, iters_dev, flags_dev, model != nullptr ? &sc : nullptr);
 }
 
 bool height_converged(const CascadeStack& stack, const float* xz_dev, std::uint32_t count, float tol,

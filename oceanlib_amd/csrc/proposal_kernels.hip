@@ -21,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <stdint.h>
 
 namespace oceanamd {
@@ -1024,4 +1025,53 @@ extern "C" int ocean_hip_mlp_configure()
         return -1;
     done = 1;
     return 0;
+}
+
+// ---- the allocation-free converged query's scratch ------------------------
+// A file-scope device array holds what ocean_hip_height_converged needs
+// besides its inputs and outputs: the levels' stencil spacings, their partial
+// sums and the per-query start offsets, for up to OCEAN_HIP_STATIC_MAX_QUERIES
+// queries. Each GPU holds its own copy until the process ends; every call on
+// one GPU shares it, so calls that overlap on one GPU race. Host code never
+// passes the array's name to a kernel (in host code it is a stand-in, not a
+// device address): it uses the address hipGetSymbolAddress returns, looked
+// up once per device with that device current.
+namespace {
+constexpr size_t kStaticSpacingBytes = 256;
+constexpr size_t kStaticPartialBytes =
+    (size_t(OCEAN_HIP_MLP_LEVELS) * OCEAN_HIP_MLP_SPACING_PARTIALS * sizeof(double) + 255) / 256 * 256;
+constexpr size_t kStaticPropBytes = size_t(OCEAN_HIP_STATIC_MAX_QUERIES) * OCEAN_HIP_MLP_PROP * sizeof(float);
+constexpr size_t kStaticBytes = kStaticSpacingBytes + kStaticPartialBytes + kStaticPropBytes;
+static_assert(kStaticBytes == OCEAN_HIP_STATIC_SCRATCH_BYTES, "static scratch layout");
+constexpr int kStaticMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+extern "C" hipError_t ocean_hip_static_scratch(ocean_hip_static_scratch_view* out)
+{
+    static std::mutex mu;
+    static void* base[kStaticMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kStaticMaxDevices) return hipErrorInvalidDevice;
+    void* p = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if (base[dev] == nullptr) {
+            e = hipGetSymbolAddress(&base[dev], HIP_SYMBOL(scratch));
+            if (e != hipSuccess) {
+                base[dev] = nullptr;
+                return e;
+            }
+        }
+        p = base[dev];
+    }
+    char* b = static_cast<char*>(p);
+    out->spacing = reinterpret_cast<int32_t*>(b);
+    out->spacing_partial = reinterpret_cast<double*>(b + kStaticSpacingBytes);
+    out->prop = reinterpret_cast<float*>(b + kStaticSpacingBytes + kStaticPartialBytes);
+    out->max_queries = OCEAN_HIP_STATIC_MAX_QUERIES;
+    return hipSuccess;
 }

@@ -21,6 +21,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace oceanamd {
 
 // Streaming store of the row pass's fields: whole rows, read next by the
@@ -63,6 +65,23 @@ __device__ __forceinline__ void store_out(float4* p, float x, float y, float z, 
 struct BlockSync {
     __device__ void operator()() const { __syncthreads(); }
 };
+
+// A team whose lanes all lie in one wavefront needs no block barrier around
+// its LDS exchanges: a wavefront's LDS operations execute in program order,
+// so it is enough that the compiler keeps them in order. Row teams are T = N/8
+// consecutive lanes, inside one wavefront for N <= 512; column teams are
+// interleaved with the block's other columns (column-fastest lanes), inside
+// one wavefront only when the whole C * T block share is.
+struct WaveSync {
+    __device__ void operator()() const
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+};
+template <bool IN_WAVE>
+using TeamSync = typename std::conditional<IN_WAVE, WaveSync, BlockSync>::type;
 
 // Precision modes of the two passes (template argument MODE).
 //   kModeExact    : the reference's arithmetic, bit for bit (the default).
@@ -204,7 +223,8 @@ __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk
 #pragma unroll
     for (int f = 0; f < F; ++f) {
         uint32_t idx[8];
-        fft_team<N, BlockSync, MODE != kModeExact>(fr[f], fi[f], idx, J, x_team, tw, BlockSync{});
+        using Sync = TeamSync<(T <= 64)>;
+        fft_team<N, Sync, MODE != kModeExact>(fr[f], fi[f], idx, J, x_team, tw, Sync{});
         const size_t row0 = size_t(f) * CELLS + size_t(z) * N;
 #pragma unroll
         for (int a = 0; a < 8; ++a) store_field<N, MODE>(spec, row0 + idx[a], fr[f][a], fi[f][a]);
@@ -283,9 +303,11 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
 
     uint32_t idx[8];
 #pragma unroll
-    for (int f = 0; f < F; ++f)
-        fft_team<N, BlockSync, MODE != kModeExact>(fr[f], fi[f], idx, J, xch + c * team_stride<N>(),
-                                                   tw, BlockSync{});
+    for (int f = 0; f < F; ++f) {
+        using Sync = TeamSync<(C * T <= 64)>;
+        fft_team<N, Sync, MODE != kModeExact>(fr[f], fi[f], idx, J, xch + c * team_stride<N>(), tw,
+                                              Sync{});
+    }
 
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
@@ -397,10 +419,11 @@ __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32
     }
 
     float2* x_team = xch + tt * KX * team_stride<N>();
+    using RSync = TeamSync<(T <= 64)>;
     if constexpr (KX == 2) {
         uint32_t idx[8];
-        fft_team_k<N, 2, BlockSync, MODE != kModeExact>(fr, fi, idx, J, x_team, team_stride<N>(),
-                                                        tw, BlockSync{});
+        fft_team_k<N, 2, RSync, MODE != kModeExact>(fr, fi, idx, J, x_team, team_stride<N>(), tw,
+                                                    RSync{});
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
             const size_t row0 = size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
@@ -411,7 +434,7 @@ __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
             uint32_t idx[8];
-            fft_team<N, BlockSync, MODE != kModeExact>(fr[f], fi[f], idx, J, x_team, tw, BlockSync{});
+            fft_team<N, RSync, MODE != kModeExact>(fr[f], fi[f], idx, J, x_team, tw, RSync{});
             const size_t row0 = size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
 #pragma unroll
             for (int a = 0; a < 8; ++a) store_field<N, MODE>(spec, row0 + idx[a], fr[f][a], fi[f][a]);
@@ -461,16 +484,16 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
     }
 
     uint32_t idx[8];
+    using CSync = TeamSync<(C * T <= 64)>;
     if constexpr (KX == 2) {
-        fft_team_k<N, 2, BlockSync, MODE != kModeExact>(fr, fi, idx, J,
-                                                        xch + (g * C + c) * 2 * team_stride<N>(),
-                                                        team_stride<N>(), tw, BlockSync{});
+        fft_team_k<N, 2, CSync, MODE != kModeExact>(fr, fi, idx, J,
+                                                    xch + (g * C + c) * 2 * team_stride<N>(),
+                                                    team_stride<N>(), tw, CSync{});
     } else {
 #pragma unroll
         for (int f = 0; f < 2; ++f)
-            fft_team<N, BlockSync, MODE != kModeExact>(fr[f], fi[f], idx, J,
-                                                       xch + (g * C + c) * team_stride<N>(), tw,
-                                                       BlockSync{});
+            fft_team<N, CSync, MODE != kModeExact>(fr[f], fi[f], idx, J,
+                                                   xch + (g * C + c) * team_stride<N>(), tw, CSync{});
     }
 
     // Groups 0 and 1 share the finalize: group 0 packs the cells of registers

@@ -193,6 +193,8 @@ hipError_t ocean_hip_level_checksums(const ocean_hip_level* levels, uint32_t cou
                                       start offsets, level-major float2 per
                                       (level, query); 6 used at 3 levels */
 #define OCEAN_HIP_MLP_SPACING_PARTIALS 64  /* doubles of scratch per level */
+#define OCEAN_HIP_MLP_COARSE_SIDE 512      /* largest coarse grid side kept in scratch:
+                                              MLP_LEVELS * side^2 float2 */
 typedef struct ocean_hip_mlp {
     uint16_t* bf16;   /* the weights rounded to bf16, in the kernels' layouts */
     float* master;    /* fp32 parameters (biases are read from here)        */
@@ -209,13 +211,14 @@ hipError_t ocean_hip_mlp_spacing(const ocean_hip_level* levels, uint32_t level_c
 /* Heights at `count` points (device float2), each level inverted until the
  * fixed-point step is <= tol metres (at most 64 taps), summed over levels.
  * mlp != NULL: start from the proposal (needs `spacing` from
- * ocean_hip_mlp_spacing and `prop` scratch of count * MLP_PROP floats).
+ * ocean_hip_mlp_spacing, `coarse` scratch of MLP_LEVELS * COARSE_SIDE^2
+ * float2 and `prop` scratch of count * MLP_PROP floats).
  * iters (count * level_count bytes) and flags (count bytes: 1 = a level did
  * not converge, 2 = a proposal was abandoned) may be NULL. */
 hipError_t ocean_hip_height_converged(const ocean_hip_level* levels, uint32_t level_count,
                                       const float* xz, uint32_t count, float tol,
-                                      const ocean_hip_mlp* mlp, const int32_t* spacing, float* prop,
-                                      float* heights, uint8_t* iters, uint8_t* flags,
+                                      const ocean_hip_mlp* mlp, const int32_t* spacing, float* coarse,
+                                      float* prop, float* heights, uint8_t* iters, uint8_t* flags,
                                       hipStream_t s);
 /* The scratch of the allocation-free converged query: views into a file-scope
  * device array of OCEAN_HIP_STATIC_SCRATCH_BYTES on the current device (its
@@ -223,10 +226,12 @@ hipError_t ocean_hip_height_converged(const ocean_hip_level* levels, uint32_t le
 #define OCEAN_HIP_STATIC_MAX_QUERIES 262144u
 #define OCEAN_HIP_STATIC_SCRATCH_BYTES \
     (256u + ((OCEAN_HIP_MLP_LEVELS * OCEAN_HIP_MLP_SPACING_PARTIALS * 8u + 255u) / 256u) * 256u + \
+     OCEAN_HIP_MLP_LEVELS * OCEAN_HIP_MLP_COARSE_SIDE * OCEAN_HIP_MLP_COARSE_SIDE * 8u + \
      OCEAN_HIP_STATIC_MAX_QUERIES * OCEAN_HIP_MLP_PROP * 4u)
 typedef struct ocean_hip_static_scratch_view {
     int32_t* spacing;          /* OCEAN_HIP_MLP_LEVELS                          */
     double* spacing_partial;   /* OCEAN_HIP_MLP_LEVELS * SPACING_PARTIALS      */
+    float* coarse;             /* OCEAN_HIP_MLP_LEVELS * COARSE_SIDE^2 float2  */
     float* prop;               /* max_queries * OCEAN_HIP_MLP_PROP             */
     uint32_t max_queries;
 } ocean_hip_static_scratch_view;

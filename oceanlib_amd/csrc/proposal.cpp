@@ -30,6 +30,7 @@ struct HeightProposal::Impl {
     ocean_hip_mlp mlp{};
     std::int32_t* spacing = nullptr;   // 3 per call (queries)
     double* spacing_partial = nullptr; // 3 * 64
+    float* coarse = nullptr;           // 3 * COARSE_SIDE^2 float2
     float* prop = nullptr;             // max_queries * OCEAN_HIP_MLP_PROP
     float* partial = nullptr;          // (max_batch / 128) * params
     float* loss_partial = nullptr;
@@ -89,6 +90,7 @@ bool wait_for(const HeightProposal* model, hipStream_t s) noexcept
 struct ProposalScratch {
     std::int32_t* spacing;
     double* spacing_partial;
+    float* coarse;
     float* prop;
     std::uint32_t max_queries;
 };
@@ -102,7 +104,7 @@ bool converged(const ocean_hip_level* levels, std::size_t count_levels, hipStrea
     if (xz_dev == nullptr || heights_dev == nullptr || count_levels == 0 || !(tol > 0.0f)) return false;
     if (model != nullptr) {
         HeightProposal::Impl& m = model->impl();
-        const ProposalScratch own{m.spacing, m.spacing_partial, m.prop, m.max_queries};
+        const ProposalScratch own{m.spacing, m.spacing_partial, m.coarse, m.prop, m.max_queries};
         const ProposalScratch& sc = use != nullptr ? *use : own;
         if (count_levels > ProposalShape::levels || count > sc.max_queries) return false;
         if (!wait_for(model, s)) return false;
@@ -110,7 +112,8 @@ bool converged(const ocean_hip_level* levels, std::size_t count_levels, hipStrea
                                   sc.spacing_partial, s) != hipSuccess)
             return false;
         const bool ok = ocean_hip_height_converged(levels, static_cast<std::uint32_t>(count_levels),
-                                                   xz_dev, count, tol, &m.mlp, sc.spacing, sc.prop,
+                                                   xz_dev, count, tol, &m.mlp, sc.spacing, sc.coarse,
+                                                   sc.prop,
                                                    heights_dev, iters_dev, flags_dev,
                                                    s) == hipSuccess;
         // The proposal's parameters (and its own scratch) are in use until
@@ -119,7 +122,7 @@ bool converged(const ocean_hip_level* levels, std::size_t count_levels, hipStrea
                hipStreamWaitEvent(m.stream.s, m.done, 0) == hipSuccess;
     }
     return ocean_hip_height_converged(levels, static_cast<std::uint32_t>(count_levels), xz_dev, count,
-                                      tol, nullptr, nullptr, nullptr, heights_dev, iters_dev,
+                                      tol, nullptr, nullptr, nullptr, nullptr, heights_dev, iters_dev,
                                       flags_dev, s) == hipSuccess;
 }
 
@@ -141,6 +144,8 @@ HeightProposal::HeightProposal(const float* params, std::uint32_t max_queries)
         up256(P * 4), up256(P * 4), up256(P * 4), up256(P * 4),  // master, m1, m2, grad
         256,                                            // spacing
         up256(ProposalShape::levels * OCEAN_HIP_MLP_SPACING_PARTIALS * 8),
+        up256(std::size_t(ProposalShape::levels) * OCEAN_HIP_MLP_COARSE_SIDE *
+              OCEAN_HIP_MLP_COARSE_SIDE * 8),                   // coarse grids
         up256(std::size_t(max_queries) * OCEAN_HIP_MLP_PROP * 4),  // prop
         up256(blocks * P * 4),                          // partial
         up256(blocks * 4),                              // loss partial
@@ -162,6 +167,7 @@ HeightProposal::HeightProposal(const float* params, std::uint32_t max_queries)
     take(m.mlp.grad);
     take(m.spacing);
     take(m.spacing_partial);
+    take(m.coarse);
     take(m.prop);
     take(m.partial);
     take(m.loss_partial);
@@ -291,7 +297,7 @@ bool height_converged_static(const OceanBatch& batch, std::size_t first, std::si
             *error = static_cast<int>(e);
             return false;
         }
-        sc = ProposalScratch{v.spacing, v.spacing_partial, v.prop, v.max_queries};
+        sc = ProposalScratch{v.spacing, v.spacing_partial, v.coarse, v.prop, v.max_queries};
     }
     const auto* levels = static_cast<const ocean_hip_level*>(batch.device_levels()) + first;
     return converged(levels, level_count, static_cast<hipStream_t>(batch.stream()), xz_dev, count, tol,

@@ -177,29 +177,66 @@ __device__ __forceinline__ Solve solve_level(const ocean_hip_level& L, const Bas
     return s;
 }
 
-// Features of one level (20 floats) at spacing s.
+// Features of one level (20 floats) at spacing s (a power of two): the nodes
+// of the level's coarse grid, every s-th cell in x and z (NC = N / s per
+// side, periodic), nearest to the query and its 8 neighbours, each node's
+// (dx, dz) divided by s * cell; then the query's offset from the nearest node
+// in units of s cells, in [-0.5, 0.5). `coarse`, when not null, holds the
+// level's nodes already scaled (coarse_kernel); otherwise they are read from
+// the displacement itself (the same float values either way).
 __device__ __forceinline__ void level_features(const ocean_hip_level& L, const Base& b, int s,
-                                               float* f)
+                                               const float2* __restrict__ coarse, float* f)
 {
     const float4* __restrict__ disp = reinterpret_cast<const float4*>(L.disp);
-    const uint32_t n = L.n, mask = n - 1u;
-    const int up_x = b.fx >= 0.5f ? 1 : 0, up_z = b.fz >= 0.5f ? 1 : 0;
-    const int32_t cx = b.ix + up_x, cz = b.iz + up_z;
-    const float inv_h = float(L.n) / (L.patch_length * float(s));
+    const uint32_t n = L.n;
+    const uint32_t nc = n / uint32_t(s), cmask = nc - 1u;
+    const float fs = float(s);
+    const float ux = (float(b.ix) + b.fx) / fs;
+    const float uz = (float(b.iz) + b.fz) / fs;
+    const float rx = floorf(ux + 0.5f), rz = floorf(uz + 0.5f);
+    const int32_t cx = int32_t(rx), cz = int32_t(rz);
+    const float inv_h = float(L.n) / (L.patch_length * fs);
     int k = 0;
 #pragma unroll
     for (int bz = -1; bz <= 1; ++bz) {
+        const uint32_t zc = uint32_t(cz + bz) & cmask;
 #pragma unroll
         for (int ax = -1; ax <= 1; ++ax) {
-            const uint32_t x = uint32_t(cx + ax * s) & mask;
-            const uint32_t z = uint32_t(cz + bz * s) & mask;
-            const float4 v = disp[z * n + x];
-            f[k++] = v.x * inv_h;
-            f[k++] = v.z * inv_h;
+            const uint32_t xc = uint32_t(cx + ax) & cmask;
+            if (coarse != nullptr) {
+                const float2 v = coarse[zc * nc + xc];
+                f[k++] = v.x;
+                f[k++] = v.y;
+            } else {
+                const float4 v = disp[(zc * uint32_t(s)) * n + xc * uint32_t(s)];
+                f[k++] = v.x * inv_h;
+                f[k++] = v.z * inv_h;
+            }
         }
     }
-    f[k++] = (b.fx - float(up_x)) / float(s);
-    f[k++] = (b.fz - float(up_z)) / float(s);
+    f[k++] = ux - rx;
+    f[k++] = uz - rz;
+}
+
+// The levels' coarse grids, for levels whose grid has at most
+// OCEAN_HIP_MLP_COARSE_SIDE nodes a side: node (xc, zc) = the displacement's
+// (dx, dz) at cell (s xc, s zc) divided by s * cell, as level_features reads
+// it. grid: (OCEAN_HIP_MLP_COARSE_SIDE^2 / 256, levels).
+__global__ __launch_bounds__(256) void coarse_kernel(const ocean_hip_level* __restrict__ levels,
+                                                     const int32_t* __restrict__ spacing,
+                                                     float2* __restrict__ coarse)
+{
+    const ocean_hip_level L = levels[blockIdx.y];
+    const int s = spacing[blockIdx.y];
+    const uint32_t nc = L.n / uint32_t(s);
+    if (nc > OCEAN_HIP_MLP_COARSE_SIDE) return;
+    const uint32_t node = blockIdx.x * 256u + threadIdx.x;
+    if (node >= nc * nc) return;
+    const uint32_t xc = node % nc, zc = node / nc;
+    const float inv_h = float(L.n) / (L.patch_length * float(s));
+    const float4 v = reinterpret_cast<const float4*>(L.disp)[(zc * uint32_t(s)) * L.n + xc * uint32_t(s)];
+    coarse[size_t(blockIdx.y) * OCEAN_HIP_MLP_COARSE_SIDE * OCEAN_HIP_MLP_COARSE_SIDE + node] =
+        make_float2(v.x * inv_h, v.z * inv_h);
 }
 
 // ---- the MLP on one wavefront (64 rows) ---------------------------------------
@@ -375,11 +412,14 @@ constexpr int kSpacingBlocks = 64;
 
 __device__ __forceinline__ int32_t spacing_of(double sum, const ocean_hip_level& L)
 {
+    // s = the power of two nearest 2 rms / cell (in log scale), 1 .. 16
     const double cells = double(L.n) * double(L.n);
     const double rms = sqrt(sum / (2.0 * cells));
     const double cell = double(L.patch_length) / double(L.n);
-    const int s = int(rint(2.0 * rms / cell));
-    return s < 1 ? 1 : (s > 16 ? 16 : s);
+    const double r = 2.0 * rms / cell;
+    int32_t sp = 1;
+    while (sp < 16 && r >= double(sp) * 1.4142135623730951) sp *= 2;
+    return sp;
 }
 
 __device__ __forceinline__ double block_sum(double acc)
@@ -488,7 +528,7 @@ static_assert(kLevels * kFeatPerLevel <= kIn, "feature columns");
 __global__ __launch_bounds__(64 * kPropWaves) void proposal_kernel(
     const ocean_hip_level* __restrict__ levels, uint32_t level_count, const float2* __restrict__ xz,
     uint32_t count, const uint16_t* __restrict__ pb, const float* __restrict__ pf,
-    const int32_t* __restrict__ spacing, float2* __restrict__ prop)
+    const int32_t* __restrict__ spacing, const float2* __restrict__ coarse, float2* __restrict__ prop)
 {
     __shared__ InferLds w;
     __shared__ __align__(16) uint16_t tiles[kPropWaves][64 * kPW];
@@ -517,7 +557,12 @@ __global__ __launch_bounds__(64 * kPropWaves) void proposal_kernel(
                 float f[kFeatPerLevel];
                 if (live) {
                     const ocean_hip_level L = levels[lv];
-                    level_features(L, base_cell(p.x, p.y, L), spacing[lv], f);
+                    const int sl = spacing[lv];
+                    const float2* cg = L.n / uint32_t(sl) <= OCEAN_HIP_MLP_COARSE_SIDE
+                                           ? coarse + size_t(lv) * OCEAN_HIP_MLP_COARSE_SIDE *
+                                                          OCEAN_HIP_MLP_COARSE_SIDE
+                                           : nullptr;
+                    level_features(L, base_cell(p.x, p.y, L), sl, cg, f);
                 } else {
 #pragma unroll
                     for (int k = 0; k < kFeatPerLevel; ++k) f[k] = 0.0f;
@@ -645,7 +690,7 @@ __global__ __launch_bounds__(256) void pairs_kernel(const ocean_hip_level* __res
         const ocean_hip_level L = levels[lv];
         const Base b = base_cell(p.x, p.y, L);
         const int s = spacing[lv];
-        level_features(L, b, s, f + lv * kFeatPerLevel);
+        level_features(L, b, s, nullptr, f + lv * kFeatPerLevel);
         const float inv_cell = float(L.n) / L.patch_length;
         TapCache cache;
         const Solve r = solve_level(L, b, 0.0f, 0.0f, tol * inv_cell, 255, false, cache);
@@ -935,15 +980,20 @@ extern "C" hipError_t ocean_hip_mlp_spacing(const ocean_hip_level* levels, uint3
 extern "C" hipError_t ocean_hip_height_converged(const ocean_hip_level* levels, uint32_t level_count,
                                                  const float* xz, uint32_t count, float tol,
                                                  const ocean_hip_mlp* mlp, const int32_t* spacing,
-                                                 float* prop, float* heights, uint8_t* iters,
-                                                 uint8_t* flags, hipStream_t s)
+                                                 float* coarse, float* prop, float* heights,
+                                                 uint8_t* iters, uint8_t* flags, hipStream_t s)
 {
     if (count == 0) return hipSuccess;
     const dim3 grid((count + 255u) / 256u);
     const float2* p = reinterpret_cast<const float2*>(xz);
     float2* prop2 = reinterpret_cast<float2*>(prop);
     if (mlp != nullptr) {
-        if (spacing == nullptr || prop == nullptr) return hipErrorInvalidValue;
+        if (spacing == nullptr || prop == nullptr || coarse == nullptr) return hipErrorInvalidValue;
+        float2* coarse2 = reinterpret_cast<float2*>(coarse);
+        hipLaunchKernelGGL(coarse_kernel,
+                           dim3(OCEAN_HIP_MLP_COARSE_SIDE * OCEAN_HIP_MLP_COARSE_SIDE / 256u,
+                                level_count < uint32_t(kLevels) ? level_count : uint32_t(kLevels)),
+                           dim3(256), 0, s, levels, spacing, coarse2);
         static int prop_grid = 0;  // resident proposal blocks on this device
         if (prop_grid == 0) {
             int dev = 0, cus = 0, per = 0;
@@ -957,7 +1007,7 @@ extern "C" hipError_t ocean_hip_height_converged(const ocean_hip_level* levels, 
         const uint32_t need = (count + 64u * kPropWaves - 1u) / (64u * kPropWaves);
         const uint32_t pg = need < uint32_t(prop_grid) ? need : uint32_t(prop_grid);
         hipLaunchKernelGGL(proposal_kernel, dim3(pg), dim3(64 * kPropWaves), 0, s, levels, level_count,
-                           p, count, mlp->bf16, mlp->master, spacing, prop2);
+                           p, count, mlp->bf16, mlp->master, spacing, coarse2, prop2);
     }
     // One launch per level: all queries tap one level's 4 MB (at 512^2) at a
     // time, which an XCD's L2 holds, instead of every level's at once.
@@ -1030,7 +1080,8 @@ extern "C" int ocean_hip_mlp_configure()
 // ---- the allocation-free converged query's scratch ------------------------
 // A file-scope device array holds what ocean_hip_height_converged needs
 // besides its inputs and outputs: the levels' stencil spacings, their partial
-// sums and the per-query start offsets, for up to OCEAN_HIP_STATIC_MAX_QUERIES
+// sums, their coarse grids and the per-query start offsets, for up to
+// OCEAN_HIP_STATIC_MAX_QUERIES
 // queries. Each GPU holds its own copy until the process ends; every call on
 // one GPU shares it, so calls that overlap on one GPU race. Host code never
 // passes the array's name to a kernel (in host code it is a stand-in, not a
@@ -1040,8 +1091,10 @@ namespace {
 constexpr size_t kStaticSpacingBytes = 256;
 constexpr size_t kStaticPartialBytes =
     (size_t(OCEAN_HIP_MLP_LEVELS) * OCEAN_HIP_MLP_SPACING_PARTIALS * sizeof(double) + 255) / 256 * 256;
+constexpr size_t kStaticCoarseBytes =
+    size_t(OCEAN_HIP_MLP_LEVELS) * OCEAN_HIP_MLP_COARSE_SIDE * OCEAN_HIP_MLP_COARSE_SIDE * 2 * sizeof(float);
 constexpr size_t kStaticPropBytes = size_t(OCEAN_HIP_STATIC_MAX_QUERIES) * OCEAN_HIP_MLP_PROP * sizeof(float);
-constexpr size_t kStaticBytes = kStaticSpacingBytes + kStaticPartialBytes + kStaticPropBytes;
+constexpr size_t kStaticBytes = kStaticSpacingBytes + kStaticPartialBytes + kStaticCoarseBytes + kStaticPropBytes;
 static_assert(kStaticBytes == OCEAN_HIP_STATIC_SCRATCH_BYTES, "static scratch layout");
 constexpr int kStaticMaxDevices = 64;
 }  // namespace
@@ -1071,7 +1124,8 @@ extern "C" hipError_t ocean_hip_static_scratch(ocean_hip_static_scratch_view* ou
     char* b = static_cast<char*>(p);
     out->spacing = reinterpret_cast<int32_t*>(b);
     out->spacing_partial = reinterpret_cast<double*>(b + kStaticSpacingBytes);
-    out->prop = reinterpret_cast<float*>(b + kStaticSpacingBytes + kStaticPartialBytes);
+    out->coarse = reinterpret_cast<float*>(b + kStaticSpacingBytes + kStaticPartialBytes);
+    out->prop = reinterpret_cast<float*>(b + kStaticSpacingBytes + kStaticPartialBytes + kStaticCoarseBytes);
     out->max_queries = OCEAN_HIP_STATIC_MAX_QUERIES;
     return hipSuccess;
 }

@@ -3,8 +3,9 @@
     python -m oceanlib_amd.proposal_train [--out oceanlib_amd/data/height_proposal.npy]
 
 Training set: 64 seas (three 512^2 levels of 800 / 150 / 25 m, choppiness
-1.3, wind 5 + 20 s / 63 m/s for sea s, level k seeded 3 s + k, each at its own
-time), 156,250 points per sea uniform over a 2 km square: 10 M (query point ->
+1.3, wind 5 + 20 s / 63 m/s for sea s, seed s for its three levels (seeds
+0..63), each at its own time), 156,250 points per sea uniform over a 2 km
+square: 10 M (query point ->
 solved start offsets) pairs, solved by the plain fixed-point iteration to
 1e-6 m on the device (ocean_amd_proposal_pairs). The network is then trained
 with the library's own forward/backward/Adam kernels
@@ -56,7 +57,7 @@ def init_params(seed: int = 0) -> np.ndarray:
 def sea_levels(s: int, seas: int):
     wind = 5.0 + 20.0 * s / max(1, seas - 1)
     return [dict(size=512, patch_length=L, wind_speed=float(np.float32(wind)), choppiness=1.3,
-                 seed=3 * s + k) for k, L in enumerate(LEVEL_LENGTHS)]
+                 seed=s) for L in LEVEL_LENGTHS]
 
 
 def make_pairs(lib, seas: int, per_sea: int, tol: float, seed: int):
@@ -181,7 +182,7 @@ def main():
     meta = dict(seas=a.seas, per_sea=a.per_sea, pairs=int(ok.size), converged=int(ok.sum()),
                 tol_m=a.tol, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed,
                 levels=[dict(patch_length=L, size=512, choppiness=1.3) for L in LEVEL_LENGTHS],
-                winds="5 + 20 s / (seas - 1) m/s", level_seeds="3 s + k",
+                winds="5 + 20 s / (seas - 1) m/s", level_seeds="s (seeds 0 .. seas - 1)",
                 spacings=spacings, pair_seconds=round(t_pairs, 2), history=history, **info)
     with open(os.path.splitext(a.out)[0] + ".json", "w") as f:
         json.dump(meta, f, indent=1)

@@ -79,30 +79,41 @@ def height_converged(levels, X, Z, tol=1e-9):
 
 
 def spacing(disp, n, L):
-    """Stencil spacing of a level: clamp(rint(2 rms / cell), 1, 16)."""
+    """Stencil spacing of a level: the power of two nearest 2 rms / cell in
+    log scale (s doubles while 2 rms / cell >= s sqrt 2), 1 .. 16."""
     d = np.asarray(disp, np.float32).reshape(n * n, 4).astype(np.float64)
     rms = np.sqrt(np.sum(d[:, 0] ** 2 + d[:, 2] ** 2) / (2.0 * n * n))
     cell = np.float64(np.float32(L)) / n
-    return int(np.clip(np.rint(2.0 * rms / cell), 1, 16))
+    r = 2.0 * rms / cell
+    s = 1
+    while s < 16 and r >= s * 1.4142135623730951:
+        s *= 2
+    return s
 
 
 def level_features(disp, n, L, s, X, Z):
-    """The 20 float32 features of one level (rows x 20)."""
+    """The 20 float32 features of one level (rows x 20): the (dx, dz) of the
+    3 x 3 nodes of the level's coarse grid (every s-th cell) around the node
+    nearest the query, over s * cell, then the query's offset from that node
+    in units of s cells."""
     d = np.asarray(disp, np.float32).reshape(n * n, 4)
     ix, iz, fx, fz = base_cell(X, Z, n, L)
-    fx32, fz32 = fx.astype(np.float32), fz.astype(np.float32)
-    up_x = (fx32 >= np.float32(0.5)).astype(np.int64)
-    up_z = (fz32 >= np.float32(0.5)).astype(np.int64)
-    cx, cz = ix + up_x, iz + up_z
-    inv_h = np.float32(n) / (np.float32(L) * np.float32(s))
+    fs = np.float32(s)
+    ux = (ix.astype(np.float32) + fx.astype(np.float32)) / fs
+    uz = (iz.astype(np.float32) + fz.astype(np.float32)) / fs
+    rx = np.floor(ux + np.float32(0.5)).astype(np.float32)
+    rz = np.floor(uz + np.float32(0.5)).astype(np.float32)
+    cx, cz = rx.astype(np.int64), rz.astype(np.int64)
+    nc = n // s
+    inv_h = np.float32(n) / (np.float32(L) * fs)
     f = []
     for bz in (-1, 0, 1):
         for ax in (-1, 0, 1):
-            x = (cx + ax * s) & (n - 1)
-            z = (cz + bz * s) & (n - 1)
-            v = d[z * n + x]
+            xc = (cx + ax) & (nc - 1)
+            zc = (cz + bz) & (nc - 1)
+            v = d[(zc * s) * n + xc * s]
             f.append(v[:, 0] * inv_h)
             f.append(v[:, 2] * inv_h)
-    f.append((fx32 - up_x.astype(np.float32)) / np.float32(s))
-    f.append((fz32 - up_z.astype(np.float32)) / np.float32(s))
+    f.append((ux - rx).astype(np.float32))
+    f.append((uz - rz).astype(np.float32))
     return np.stack(f, 1).astype(np.float32)

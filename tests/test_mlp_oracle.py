@@ -116,18 +116,31 @@ def test_inversion_matches_reference_four_steps_on_oracle_buffers(steps):
 
 
 def test_features_layout_and_spacing():
+    """Spacing: the power of two nearest 2 rms / cell (log scale), closed form
+    from the RMS; features: the coarse grid's nodes at multiples of s cells,
+    the centre node nearest the query, offsets in units of s cells."""
     n, L = 32, 64.0
     rng = np.random.default_rng(3)
     disp = rng.normal(0, 0.5, (n * n, 4)).astype(np.float32)
     s = inv.spacing(disp, n, L)
     rms = np.sqrt(np.mean(disp[:, 0].astype(np.float64) ** 2 + disp[:, 2] ** 2) / 2)
-    assert s == int(np.clip(np.rint(2 * rms / (L / n)), 1, 16))
+    r = 2 * rms / (L / n)
+    assert s == int(np.clip(2 ** np.rint(np.log2(r)), 1, 16))
+    for scale, want in ((0.01, 1), (1.0, 1), (1.5, 2), (2.7, 2), (3.0, 4), (40.0, 16)):
+        d = np.zeros((n * n, 4), np.float32)
+        d[:, 0] = scale * (L / n) / 2 * np.sqrt(2.0)  # rms = scale * cell / 2 -> r = scale
+        assert inv.spacing(d, n, L) == want, (scale, want)
     X = np.array([10.3], np.float32)
     Z = np.array([50.9], np.float32)
-    f = inv.level_features(disp, n, L, 1, X, Z)
-    assert f.shape == (1, 20)
-    # the centre node is the node nearest the query point; its (dx, dz) are features 8, 9
-    cx, cz = int(np.floor(10.3 / 2 + 0.5)) % n, int(np.floor(50.9 / 2 + 0.5)) % n
-    assert f[0, 8] == np.float32(disp[cz * n + cx, 0] * np.float32(n / L))
-    assert f[0, 9] == np.float32(disp[cz * n + cx, 2] * np.float32(n / L))
-    assert abs(f[0, 18] - (10.3 / 2 - round(10.3 / 2))) < 1e-5
+    for sp in (1, 4):
+        f = inv.level_features(disp, n, L, sp, X, Z)
+        assert f.shape == (1, 20)
+        # centre node: the coarse node nearest the query (cell = 2 m, node = sp cells)
+        ux, uz = 10.3 / 2 / sp, 50.9 / 2 / sp
+        cx, cz = (int(np.floor(ux + 0.5)) % (n // sp)) * sp, (int(np.floor(uz + 0.5)) % (n // sp)) * sp
+        inv_h = np.float32(n) / (np.float32(L) * np.float32(sp))
+        assert f[0, 8] == np.float32(disp[cz * n + cx, 0] * inv_h)
+        assert f[0, 9] == np.float32(disp[cz * n + cx, 2] * inv_h)
+        # the +x neighbour is s cells further along the row (periodic)
+        assert f[0, 10] == np.float32(disp[cz * n + (cx + sp) % n, 0] * inv_h)
+        assert abs(f[0, 18] - (ux - round(ux))) < 1e-5 and abs(f[0, 19] - (uz - round(uz))) < 1e-5

@@ -140,6 +140,24 @@ __device__ __forceinline__ ocean_hip_surface sample_level(const float4* __restri
     return s;
 }
 
+// The height alone of sample_level (the same taps and blend; no normal or
+// velocity loads): Ocean::height_at on one level.
+__device__ __forceinline__ float height_level(const float4* __restrict__ disp, uint32_t n,
+                                              float inv_cell, float wx, float wz)
+{
+    float u = wx, v = wz;
+    CellCache cache;
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        const Tap t = make_tap(u, v, n, inv_cell);
+        const float4 d = fetch4_cached(disp, t, cache);
+        u = wx - d.x;
+        v = wz - d.z;
+    }
+    const Tap t = make_tap(u, v, n, inv_cell);
+    return fetch4_cached(disp, t, cache).y;
+}
+
 __device__ __forceinline__ float sample_wrap(const float* __restrict__ f, float u, float v, int n)
 {
     const float fu = floorf(u);

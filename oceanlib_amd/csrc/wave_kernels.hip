@@ -56,6 +56,25 @@ __global__ __launch_bounds__(kBlock) void cascade_sample_kernel(
     if (out != nullptr) out[q] = o;
 }
 
+// CascadeStack::height_at for large batches, one launch per level: every
+// query taps one level's displacement (4 MB at 512^2, which an XCD's L2
+// holds) at a time instead of all levels' at once. The running sum is kept in
+// `heights` between the launches, with the reference's float additions in
+// level order (the first level's launch adds to 0.0f, as the reference's sum
+// starts, src/cascade.cpp:94,118-120).
+__global__ __launch_bounds__(kBlock) void cascade_height_level_kernel(
+    const ocean_hip_level* __restrict__ level, bool first, const float2* __restrict__ xz,
+    uint32_t count, float* __restrict__ heights)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    const ocean_hip_level L = *level;
+    const float2 p = xz[q];
+    const float h = height_level(reinterpret_cast<const float4*>(L.disp), L.n,
+                                 static_cast<float>(L.n) / L.patch_length, p.x, p.y);
+    heights[q] = (first ? 0.0f : heights[q]) + h;
+}
+
 // reference src/foam.cpp:89-139
 __device__ __forceinline__ void foam_step_cell(uint32_t i, const float4* __restrict__ disp,
                                                const float4* __restrict__ vel,
@@ -278,6 +297,14 @@ extern "C" hipError_t ocean_hip_cascade_sample(const ocean_hip_level* levels,
                                                float* heights, hipStream_t s)
 {
     if (count == 0) return hipSuccess;
+    if (out == nullptr && heights != nullptr && count > 65536u) {
+        // heights alone, many points: level by level
+        const dim3 g((count + kBlock - 1) / kBlock), b(kBlock);
+        for (uint32_t l = 0; l < level_count; ++l)
+            hipLaunchKernelGGL(cascade_height_level_kernel, g, b, 0, s, levels + l, l == 0,
+                               reinterpret_cast<const float2*>(xz), count, heights);
+        return hipGetLastError();
+    }
     const dim3 b = query_block(count), g((count + b.x - 1) / b.x);
     hipLaunchKernelGGL(cascade_sample_kernel, g, b, 0, s, levels, level_count,
                        reinterpret_cast<const float2*>(xz), count, out, heights);

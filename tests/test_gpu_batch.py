@@ -279,3 +279,40 @@ def test_foam_on_batch_level_vs_oracle(lib):
         for of in ofoams:
             olib.orc_foam_destroy(of)
         b.close()
+
+
+def test_cascade_heights_level_by_level_equal_the_full_query(lib):
+    """Cascade heights alone for more than 64K points are computed one level
+    per launch (the running sum kept in the output between launches); they
+    must equal, bit for bit, the heights of the full cascade query (one
+    launch over all levels) and the oracle's CascadeStack::sample_at heights
+    (reference src/cascade.cpp:82-153)."""
+    from oceanlib_amd.hipmem import DeviceArray
+    kws = [dict(size=256, patch_length=L, wind_speed=15.0, seed=42) for L in (800.0, 150.0, 25.0)]
+    descs = [desc_from(kw) for kw in kws]
+    b = Batch(lib, descs)
+    orc = [OracleOcean(d) for d in descs]
+    try:
+        b.update(7.5)
+        for o in orc:
+            o.update(7.5)
+        n = 100_000
+        rng = np.random.default_rng(6)
+        xz = rng.uniform(-1500, 1500, (n, 2)).astype(np.float32)
+        dxz = DeviceArray.from_numpy(xz)
+        h = DeviceArray.from_numpy(np.full(n, np.nan, np.float32))
+        h_full = DeviceArray.from_numpy(np.full(n, np.nan, np.float32))
+        full = DeviceArray((n, 10), np.float32)
+        assert lib.ocean_amd_batch_sample_device(b.h, 0, 3, dxz.ptr, n, None, h.ptr) == 0
+        assert lib.ocean_amd_batch_sample_device(b.h, 0, 3, dxz.ptr, n, full.ptr, h_full.ptr) == 0
+        assert lib.ocean_amd_batch_synchronize(b.h) == 0
+        got, want_full = h.numpy(), h_full.numpy()
+        assert_bitexact(got, want_full, "per-level heights vs full query")
+        assert_bitexact(got, full.numpy()[:, 0].copy(), "per-level heights vs full surfaces")
+        handles = (C.c_void_p * 3)(*[o.h for o in orc])
+        wc = capi.OceanSurfaceEx()
+        for k in range(0, n, 997):
+            oracle().orc_cascade_sample(handles, 3, float(xz[k, 0]), float(xz[k, 1]), C.byref(wc))
+            assert np.float32(got[k]).tobytes() == np.float32(wc.height).tobytes(), k
+    finally:
+        b.close()

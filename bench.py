@@ -249,7 +249,7 @@ def batched_leg(lib, frames: int, warmup: int, tune: bool = True, precision: int
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5),
                      "traffic": traffic_from_profiles(
-                         [tuple(f"batch_{p}{v}_kernel<{g['size']}, {6 if g['velocity'] else 4}>"
+                         [tuple(f"batch_{p}{v}_kernel<{g['size']}, {6 if g['velocity'] else 4}, 0>"
                                 for v in ("", "_item"))
                           for g in groups for p in ("row", "col")])},
         "groups": groups, "build_s": round(build_s, 2), "frames": frames,

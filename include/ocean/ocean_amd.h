@@ -316,13 +316,13 @@ int  ocean_amd_batch_height_converged(const ocean_amd_batch* batch, uint32_t fir
 /* No counterpart in the reference. For callers that cannot allocate (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc): ocean_amd_batch_height_converged with the same arguments and
- * results, but the proposal's per-call scratch (the levels' stencil spacings
- * and every query's start offsets) comes from a file-scope device array of
+ * results, but the proposal's per-call scratch (the levels' stencil spacings,
+ * their coarse grids and every query's start offsets) comes from a file-scope device array of
  * the library instead of the model's own, so the call allocates nothing and
  * the model's max_queries does not bound it. Limit: count <=
  * OCEAN_AMD_STATIC_MAX_QUERIES (larger counts return OCEAN_ERROR_INVALID_ARG;
  * split them into calls of at most that many). Each GPU holds one copy of the
- * array (about 8.4 MB) until the process ends, shared by every call on that
+ * array (about 14.7 MB) until the process ends, shared by every call on that
  * GPU: calls that overlap on one GPU (other streams, other threads) race on
  * it. Without a model no scratch is used. */
 #define OCEAN_AMD_STATIC_MAX_QUERIES 262144u

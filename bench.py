@@ -521,6 +521,67 @@ def scene_cpu_baseline(budget_s: float = 8.0):
                       f"thread_count=0 ({cores} workers)"}
 
 
+def interaction_setup_leg(lib, with_cpu: bool):
+    """The interaction field's set-up work: creation (the least-squares
+    stencil fit and its symbol scan, on the device) at the scene's 256^2 /
+    radius 6 and at 1024^2 / radius 12, and set_obstruction of a moving hull
+    (the scene's hull shifted one cell per call, so the mask tables are
+    rebuilt on every call, as for a hull that moves each frame), per call
+    including the device work. The reference library's times for the same
+    calls on this host beside them."""
+    import numpy as np
+    from oceanlib_amd import hipmem
+    from oceanlib_amd.workloads import config3_hull_mask
+    cm = capi_mod()
+    r = ref_lib() if with_cpu else None
+    out = {}
+    for n, radius in ((256, 6), (1024, 12)):
+        d = cm.InteractionDesc()
+        lib.ocean_interaction_desc_init(C.byref(d))
+        d.size, d.extent, d.kernel_radius = n, 0.1 * n, radius
+        st = C.c_int(-1)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            f = lib.ocean_interaction_create(C.byref(d), C.byref(st))
+            times.append(time.perf_counter() - t0)
+            if not f:
+                raise SystemExit(f"interaction create failed: {st.value}")
+            if _ < 2:
+                lib.ocean_interaction_destroy(f)
+        hull = config3_hull_mask(n)
+        masks = [np.ascontiguousarray(np.roll(hull, k, axis=1)) for k in range(16)]
+        ptrs = [m.ctypes.data_as(C.POINTER(C.c_uint8)) for m in masks]
+        calls = 200
+        for k in range(20):
+            lib.ocean_interaction_set_obstruction(f, ptrs[k % 16])
+        hipmem.synchronize()
+        t0 = time.perf_counter()
+        for k in range(calls):
+            lib.ocean_interaction_set_obstruction(f, ptrs[k % 16])
+        hipmem.synchronize()
+        obst_ms = (time.perf_counter() - t0) * 1000.0 / calls
+        lib.ocean_interaction_destroy(f)
+        row = {"create_ms": round(min(times) * 1000.0, 2),
+               "set_obstruction_ms": round(obst_ms, 4),
+               "solid_cells": int(hull.sum())}
+        if r is not None:
+            d.thread_count = 0
+            t0 = time.perf_counter()
+            rf = r.oref_interaction_create(C.byref(d), C.byref(st))
+            row["reference_create_ms"] = round((time.perf_counter() - t0) * 1000.0, 2)
+            if rf:
+                rcalls = 50
+                t0 = time.perf_counter()
+                for k in range(rcalls):
+                    r.oref_interaction_set_obstruction(rf, ptrs[k % 16])
+                row["reference_set_obstruction_ms"] = round(
+                    (time.perf_counter() - t0) * 1000.0 / rcalls, 4)
+                r.oref_interaction_destroy(rf)
+        out[f"{n}^2 radius {radius}"] = row
+    return out
+
+
 def handoff_leg(lib, sim, foam, n: int, reps: int = 200):
     """The renderer handoff of the same patch: the displacement (alpha = the
     persistent foam) and the normal as RGBA16F, into device arrays and into
@@ -602,7 +663,8 @@ def main():
         print(json.dumps({"converged_height_at": learned_leg(lib)}), flush=True)
         return
     if args.scene_only:
-        print(json.dumps({"scene": scene_leg(lib, max(args.scene_frames, 1), 30)}), flush=True)
+        print(json.dumps({"scene": scene_leg(lib, max(args.scene_frames, 1), 30),
+                          "setup": interaction_setup_leg(lib, not args.no_cpu)}), flush=True)
         return
     desc = desc_from(CONFIG2)
     n = desc.size
@@ -805,6 +867,8 @@ def main():
     scene = scene_leg(lib, args.scene_frames, 30) if args.scene_frames > 0 else None
     if scene is not None and not args.no_cpu:
         scene["cpu_baseline"] = scene_cpu_baseline()
+    if scene is not None:
+        scene["setup"] = interaction_setup_leg(lib, not args.no_cpu)
     cpu = None if args.no_cpu else cpu_baseline(desc)
     if cpu is not None:
         one = cpu_baseline(desc, 4.0, threads=1)

@@ -122,6 +122,13 @@ public:
     // (8-byte aligned), a device array enqueued on the field's stream, or a
     // host array filled before returning.
     bool pack_rgba16(std::uint16_t* out, bool on_device) const noexcept;
+    // The obstruction tables the field applies (reference
+    // src/interaction.cpp:255-292): the solid cells in index order and each
+    // one's nearest fluid cell (0xFFFFFFFF: none). Copies at most `capacity`
+    // of each into host arrays and returns the number of solid cells (0
+    // without a mask).
+    std::uint32_t obstruction_tables(std::uint32_t* solid, std::uint32_t* source,
+                                     std::uint32_t capacity) const noexcept;
     void synchronize() const noexcept;
     [[nodiscard]] int last_error() const noexcept;
     [[nodiscard]] detail::InteractionDevice& device() const noexcept;

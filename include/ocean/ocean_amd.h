@@ -150,10 +150,15 @@ int ocean_amd_get_tables(const ocean_sim* sim, float* h0, float* omega, uint32_t
 int ocean_amd_build_spectrum(const ocean_desc* desc, float* h0, float* omega);
 int ocean_amd_build_twiddles(uint32_t n, float* tw);
 /* The interaction field's (2P+1)^2 convolution taps as the library fits them
- * (reference src/core/iwave_kernel.cpp:177-437, least-squares method) and the
- * largest realised symbol; returns the tap count or -1. */
+ * (reference src/core/iwave_kernel.cpp:177-437, least-squares method: the
+ * normal equations and the realised-symbol scan on the current device) and
+ * the largest realised symbol; returns the tap count or -1. */
 int ocean_amd_build_iwave_taps(uint32_t radius, double cell, double depth, float* taps,
                                double* max_symbol);
+/* The same fit with the scan's summary: sym[0..3] = max_symbol, min_symbol,
+ * band_low, peak_relative_error (the reference's IWaveKernel fields). */
+int ocean_amd_build_iwave_stencil(uint32_t radius, double cell, double depth, float* taps,
+                                  double* sym);
 
 /* ---- persistent advected foam ------------------------------------------
  * C access to the reference's C++ FoamField (include/ocean/foam.hpp:38-142,
@@ -373,6 +378,14 @@ int ocean_amd_interaction_pack_rgba16_device(const ocean_interaction* field, uin
  * the scheme's discrete energy, computed on the device in the reference's
  * summation order. NaN for a NULL field. */
 double ocean_amd_interaction_energy(const ocean_interaction* field);
+/* The obstruction tables the field applies (reference
+ * src/interaction.cpp:255-292, which exports no accessor for them): the solid
+ * cells in index order and each one's nearest fluid cell (0xFFFFFFFF: none),
+ * built on the device. Copies at most `capacity` of each into the host arrays
+ * (either may be NULL) and returns the number of solid cells; 0 for a NULL
+ * field or one without a mask. */
+uint32_t ocean_amd_interaction_obstruction_tables(const ocean_interaction* field, uint32_t* solid,
+                                                  uint32_t* source, uint32_t capacity);
 
 #if defined(__cplusplus)
 }

@@ -205,6 +205,8 @@ AMD_SIGNATURES = {
     "ocean_amd_build_twiddles": (C.c_int, [C.c_uint32, _F]),
     "ocean_amd_build_iwave_taps": (C.c_int, [C.c_uint32, C.c_double, C.c_double, _F,
                                              C.POINTER(C.c_double)]),
+    "ocean_amd_build_iwave_stencil": (C.c_int, [C.c_uint32, C.c_double, C.c_double, _F,
+                                                C.POINTER(C.c_double)]),
     "ocean_amd_foam_desc_init": (None, [C.POINTER(FoamDesc)]),
     "ocean_amd_foam_create": (_P, [_P, C.POINTER(FoamDesc), C.POINTER(C.c_int)]),
     "ocean_amd_foam_destroy": (None, [_P]),
@@ -220,6 +222,8 @@ AMD_SIGNATURES = {
     "ocean_amd_interaction_pack_rgba16": (C.c_int, [_P, _P]),
     "ocean_amd_interaction_pack_rgba16_device": (C.c_int, [_P, _P]),
     "ocean_amd_interaction_energy": (C.c_double, [_P]),
+    "ocean_amd_interaction_obstruction_tables": (C.c_uint32, [_P, C.POINTER(C.c_uint32),
+                                                              C.POINTER(C.c_uint32), C.c_uint32]),
     "ocean_amd_foam_last_substeps": (C.c_uint32, [_P]),
     "ocean_amd_foam_at": (C.c_float, [_P, C.c_float, C.c_float]),
     "ocean_amd_foam_coverage": (C.c_double, [_P]),

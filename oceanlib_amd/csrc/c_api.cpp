@@ -1001,15 +1001,42 @@ void ocean_sample_combined(const ocean_sim* sim, const ocean_interaction* field,
 
 #include "iwave_kernel_host.hpp"
 
+namespace {
+
+int build_stencil(uint32_t radius, double cell, double depth, float* taps, double* sym)
+{
+    if (taps == nullptr || radius < 1 || radius > 12 || !(cell > 0.0)) return -1;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return -1;
+    oceanamd::IWaveStencil k;
+    const oceanamd::StencilStatus st = oceanamd::build_iwave_stencil(radius, cell, depth, s, k);
+    (void)hipStreamDestroy(s);
+    if (st != oceanamd::StencilStatus::Ok) return -1;
+    std::memcpy(taps, k.taps.data(), k.taps.size() * sizeof(float));
+    if (sym != nullptr) {
+        sym[0] = k.max_symbol;
+        sym[1] = k.min_symbol;
+        sym[2] = k.band_low;
+        sym[3] = k.peak_relative_error;
+    }
+    return static_cast<int>(k.taps.size());
+}
+
+}  // namespace
+
 extern "C" int ocean_amd_build_iwave_taps(uint32_t radius, double cell, double depth, float* taps,
                                           double* max_symbol)
 {
-    if (taps == nullptr || radius < 1 || radius > 12 || !(cell > 0.0)) return -1;
-    oceanamd::IWaveStencil k;
-    if (!oceanamd::build_iwave_stencil(radius, cell, depth, k)) return -1;
-    std::memcpy(taps, k.taps.data(), k.taps.size() * sizeof(float));
-    if (max_symbol != nullptr) *max_symbol = k.max_symbol;
-    return static_cast<int>(k.taps.size());
+    double sym[4];
+    const int r = build_stencil(radius, cell, depth, taps, sym);
+    if (r > 0 && max_symbol != nullptr) *max_symbol = sym[0];
+    return r;
+}
+
+extern "C" int ocean_amd_build_iwave_stencil(uint32_t radius, double cell, double depth, float* taps,
+                                             double* sym)
+{
+    return build_stencil(radius, cell, depth, taps, sym);
 }
 
 // ---- ocean_amd.h: converged queries and the learned proposal ---------------
@@ -1229,6 +1256,13 @@ double ocean_amd_interaction_energy(const ocean_interaction* field)
 {
     if (field == nullptr) return std::numeric_limits<double>::quiet_NaN();
     return to_cfield(field)->energy();
+}
+
+uint32_t ocean_amd_interaction_obstruction_tables(const ocean_interaction* field, uint32_t* solid,
+                                                  uint32_t* source, uint32_t capacity)
+{
+    if (field == nullptr) return 0;
+    return to_cfield(field)->obstruction_tables(solid, source, capacity);
 }
 
 int ocean_amd_interaction_pack_rgba16_device(const ocean_interaction* field, uint16_t* out_dev)

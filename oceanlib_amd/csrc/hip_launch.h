@@ -306,11 +306,42 @@ hipError_t ocean_hip_iwave_inject_args(const ocean_hip_iwave* f, const ocean_hip
                                        uint32_t count, float* means, hipStream_t s);
 /* Solid cells: Dirichlet zero (dirichlet != 0) or copy from the nearest
  * fluid cell. `solid[k]` is the k-th solid cell's index and `source[k]` its
- * nearest fluid cell (0xFFFFFFFF: none, the cell is left as it is); both are
- * device arrays of `count` entries. */
+ * nearest fluid cell (0xFFFFFFFF: none, the cell is left as it is), for k <
+ * *count (device memory, as written by ocean_hip_iwave_mask_tables); at most
+ * `max_count` entries. */
 hipError_t ocean_hip_iwave_obstruct(const ocean_hip_iwave* f, const uint32_t* solid,
-                                    const uint32_t* source, uint32_t count, int dirichlet,
-                                    hipStream_t s);
+                                    const uint32_t* source, const uint32_t* count,
+                                    uint32_t max_count, int dirichlet, hipStream_t s);
+/* The obstruction tables of an n x n mask (n a power of two in 32..1024;
+ * mask != 0 is solid), reference src/interaction.cpp:255-292: near[i] = the
+ * nearest fluid cell of cell i as the reference's BFS assigns it (the fluid
+ * cell at the smallest L1 distance, the smallest index among equals;
+ * 0xFFFFFFFF when there is no fluid cell), solid[] = the solid cells in index
+ * order, source[k] = near[solid[k]], *count = their number. The mask is read
+ * once from `src` (device memory or mapped host memory, 16-byte aligned) and
+ * copied to `keep` (device memory) unless `keep` is NULL, in which case `src`
+ * must be device memory. `work` is device scratch of n*n + n uint32; near,
+ * solid, source hold n*n each. */
+hipError_t ocean_hip_iwave_mask_tables(const uint8_t* src, uint8_t* keep, uint32_t n,
+                                       uint32_t* work, uint32_t* near, uint32_t* solid,
+                                       uint32_t* source, uint32_t* count, hipStream_t s);
+/* dst <- src shifted by whole cells (dst(x, z) = src(x + dx, z + dz), 0 outside). */
+hipError_t ocean_hip_iwave_mask_shift(const uint8_t* src, uint8_t* dst, uint32_t n, int32_t dx,
+                                      int32_t dz, hipStream_t s);
+/* The least-squares stencil fit's normal equations (reference
+ * src/core/iwave_kernel.cpp:286-336): ata_lower[r (r+1)/2 + c] (c <= r) and
+ * atb[r], each summed over the `samples` rows of psi (sample-major, mm values
+ * each) in order, with per-sample weight w and symbol sym. mm <= MAX_BASIS. */
+#define OCEAN_HIP_IWAVE_MAX_BASIS 90
+hipError_t ocean_hip_iwave_fit_normal(const double* psi, const double* w, const double* sym,
+                                      uint32_t samples, uint32_t mm, double* ata_lower,
+                                      double* atb, hipStream_t s);
+/* The realised symbol of (2 radius + 1)^2 `taps` with cell `dx` at every
+ * point (zi, xi) of the (scan+1)^2 grid over [0, k_max]^2, zi-major, with
+ * the device's double cosine: out[2 pt] = realised symbol, out[2 pt + 1] =
+ * the true symbol k tanh(k depth) (k when depth <= 0) at |k|. */
+hipError_t ocean_hip_iwave_symbol_scan(const float* taps, uint32_t radius, double dx, double k_max,
+                                       double depth, uint32_t scan, double* out, hipStream_t s);
 /* One leapfrog step: old <- c1 (2 cur - c2 old - g dt^2 (G * cur)). */
 hipError_t ocean_hip_iwave_step(const ocean_hip_iwave* f, hipStream_t s);
 /* energy() of the reference: `scratch` holds 2*N*N doubles; result in *out. */

@@ -2,11 +2,12 @@
 //
 // The host keeps exactly the reference's bookkeeping (src/interaction.cpp):
 // descriptor validation, the fixed-step accumulator and substep clamp, the
-// source queue, the integer grid origin, and the nearest-fluid table rebuilt
-// on set_obstruction / recenter (a breadth-first search, not per frame).
-// Everything per cell and per substep runs on the device (iwave_kernels.hip):
-// stamps, obstruction, the convolution step, the RGBA finalize, the recentre
-// shift and the queries.
+// source queue and the integer grid origin. Everything per cell runs on the
+// device: stamps, obstruction, the convolution step, the RGBA finalize, the
+// recentre shift and the queries (iwave_kernels.hip), and the set-up work
+// (iwave_init_kernels.hip): the stencil fit's normal equations and symbol
+// scan, and the obstruction mask with its nearest-fluid table, rebuilt on
+// set_obstruction / recenter.
 #include "ocean/interaction.hpp"
 
 #include "iwave_kernel_host.hpp"
@@ -91,8 +92,13 @@ struct InteractionField::Impl {
     float* c1 = nullptr;
     float* c2 = nullptr;
     float* out = nullptr;
-    std::uint32_t* d_solid = nullptr;
+    std::uint32_t* d_solid = nullptr;   // the solid cells in index order
     std::uint32_t* d_source = nullptr;  // per solid cell: its nearest fluid cell
+    std::uint32_t* d_nearest = nullptr; // per cell: its nearest fluid cell
+    std::uint32_t* d_mask_work = nullptr;
+    std::uint32_t* d_solid_count = nullptr;
+    std::uint8_t* d_mask[2] = {nullptr, nullptr};  // the mask, and recenter's shift target
+    int mask_i = 0;
     ocean_hip_stamp* d_stamps = nullptr;
     detail::MappedQueries mapped;  // host queries of a few points: no staged copies
     std::mutex query_mu;           // concurrent const queries share the staging
@@ -118,15 +124,15 @@ struct InteractionField::Impl {
     bool mirror_stale = true;
 
     // host bookkeeping (reference Impl members)
-    std::vector<std::uint8_t> mask;
-    std::vector<std::uint8_t> mask_shift;  // recenter's copy of the mask (sized once)
-    std::vector<std::uint32_t> nearest, bfs, frontier;
-    std::vector<std::uint8_t> frontier_mark;
-    std::uint32_t* h_solid = nullptr;   // pinned: solid cell indices
-    std::uint32_t* h_source = nullptr;  // pinned: their nearest fluid cells
-    hipEvent_t mask_uploaded = nullptr;
-    bool mask_upload_pending = false;
-    std::uint32_t solid_count = 0;
+    // set_obstruction's uploads: two pinned buffers used in turn, mapped into
+    // the device's address space when the runtime allows (the row kernel then
+    // reads the mask straight from them), each reused only once the work that
+    // reads it has completed.
+    std::uint8_t* h_mask_stage[2] = {nullptr, nullptr};
+    const std::uint8_t* d_mask_stage[2] = {nullptr, nullptr};  // null: not mapped
+    hipEvent_t mask_read[2] = {nullptr, nullptr};
+    bool mask_read_pending[2] = {false, false};
+    int mask_turn = 0;
     bool has_mask = false;
     std::vector<Disturbance> queue;
     std::uint32_t queue_count = 0;
@@ -149,9 +155,10 @@ struct InteractionField::Impl {
         if (h_qout) hipHostFree(h_qout);
         for (hipEvent_t e : stamps_free)
             if (e) hipEventDestroy(e);
-        if (h_solid) hipHostFree(h_solid);
-        if (h_source) hipHostFree(h_source);
-        if (mask_uploaded) hipEventDestroy(mask_uploaded);
+        for (std::uint8_t* b : h_mask_stage)
+            if (b) hipHostFree(b);
+        for (hipEvent_t e : mask_read)
+            if (e) hipEventDestroy(e);
         if (dev.stream) hipStreamDestroy(dev.stream);
     }
 
@@ -206,75 +213,14 @@ struct InteractionField::Impl {
         }
     }
 
-    // reference src/interaction.cpp:255-292: a multi-source BFS from every
-    // fluid cell in index order, each dequeued cell claiming its unclaimed
-    // 4-neighbours (+x, -x, +z, -z) for its nearest fluid cell. A fluid cell
-    // with no solid neighbour claims nothing, so the same table comes from
-    // visiting only the fluid cells next to a solid cell, in index order, then
-    // the claimed solid cells in the order they were claimed. Only the solid
-    // cells' entries are used, and only they are uploaded (with the solid
-    // list, from pinned staging; the copy is ordered before later kernels on
-    // the stream, and the staging is rewritten only once it has completed).
+    // reference src/interaction.cpp:255-292: the solid cells and their
+    // nearest fluid cells, from the device mask (iwave_init_kernels.hip gives
+    // the closed form of the reference's BFS table that the kernels compute).
     void rebuild_mask_tables() noexcept
     {
-        solid_count = 0;
         if (!has_mask) return;
-        constexpr std::uint32_t kUnset = 0xFFFFFFFFu;
-        if (mask_upload_pending) {
-            dev.note(hipEventSynchronize(mask_uploaded));
-            mask_upload_pending = false;
-        }
-        for (std::size_t i = 0; i < cells; ++i) {
-            if (mask[i] != 0) {
-                nearest[i] = kUnset;
-                h_solid[solid_count++] = static_cast<std::uint32_t>(i);
-            }
-        }
-        const int ni = static_cast<int>(n);
-        const int ddx[4] = {1, -1, 0, 0};
-        const int ddz[4] = {0, 0, 1, -1};
-        // the fluid cells next to a solid cell, in index order
-        std::uint32_t edge = 0;
-        for (std::uint32_t k = 0; k < solid_count; ++k) {
-            const std::uint32_t i = h_solid[k];
-            const int x = static_cast<int>(i % n), z = static_cast<int>(i / n);
-            for (int e = 0; e < 4; ++e) {
-                const int nx = x + ddx[e], nz = z + ddz[e];
-                if (nx < 0 || nz < 0 || nx >= ni || nz >= ni) continue;
-                const std::uint32_t j = static_cast<std::uint32_t>(nz) * n + static_cast<std::uint32_t>(nx);
-                if (mask[j] == 0 && !frontier_mark[j]) {
-                    frontier_mark[j] = 1;
-                    frontier[edge++] = j;
-                }
-            }
-        }
-        std::sort(frontier.begin(), frontier.begin() + edge);
-        std::uint32_t tail = 0;
-        auto claim = [&](std::uint32_t i, std::uint32_t src) {
-            const int x = static_cast<int>(i % n), z = static_cast<int>(i / n);
-            for (int e = 0; e < 4; ++e) {
-                const int nx = x + ddx[e], nz = z + ddz[e];
-                if (nx < 0 || nz < 0 || nx >= ni || nz >= ni) continue;
-                const std::uint32_t j = static_cast<std::uint32_t>(nz) * n + static_cast<std::uint32_t>(nx);
-                if (mask[j] != 0 && nearest[j] == kUnset) {
-                    nearest[j] = src;
-                    bfs[tail++] = j;
-                }
-            }
-        };
-        for (std::uint32_t e = 0; e < edge; ++e) {
-            frontier_mark[frontier[e]] = 0;
-            claim(frontier[e], frontier[e]);
-        }
-        for (std::uint32_t head = 0; head < tail; ++head) claim(bfs[head], nearest[bfs[head]]);
-        for (std::uint32_t k = 0; k < solid_count; ++k) h_source[k] = nearest[h_solid[k]];
-        if (solid_count == 0) return;
-        dev.note(hipMemcpyAsync(d_solid, h_solid, solid_count * sizeof(std::uint32_t), hipMemcpyHostToDevice,
-                                dev.stream));
-        dev.note(hipMemcpyAsync(d_source, h_source, solid_count * sizeof(std::uint32_t),
-                                hipMemcpyHostToDevice, dev.stream));
-        dev.note(hipEventRecord(mask_uploaded, dev.stream));
-        mask_upload_pending = true;
+        dev.note(ocean_hip_iwave_mask_tables(d_mask[mask_i], nullptr, n, d_mask_work, d_nearest,
+                                             d_solid, d_source, d_solid_count, dev.stream));
     }
 
     // The scalar part of inject() (reference src/interaction.cpp:322-348);
@@ -330,9 +276,19 @@ InteractionField::InteractionField(const InteractionDesc& d)
     m.stride = align16(static_cast<std::size_t>(m.n) + 2 * p);
     m.plane_floats = (static_cast<std::size_t>(m.n) + 2 * p) * m.stride;
 
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        throw std::runtime_error("ocean: no HIP device available");
+    detail::hip_check(hipGetDevice(&m.dev.device), "hipGetDevice");
+    detail::hip_check(hipStreamCreateWithFlags(&m.dev.stream, hipStreamNonBlocking), "hipStreamCreate");
+
     // reference src/interaction.cpp:158-193
-    if (!oceanamd::build_iwave_stencil(d.kernel_radius, m.cell, d.depth, m.kernel))
-        throw std::invalid_argument("ocean: iWave kernel fit is singular (degenerate basis)");
+    switch (oceanamd::build_iwave_stencil(d.kernel_radius, m.cell, d.depth, m.dev.stream, m.kernel)) {
+        case oceanamd::StencilStatus::Ok: break;
+        case oceanamd::StencilStatus::Singular:
+            throw std::invalid_argument("ocean: iWave kernel fit is singular (degenerate basis)");
+        default: throw std::runtime_error("ocean: iWave kernel fit failed on the device");
+    }
     if (!oceanamd::stencil_is_stable(m.kernel))
         throw std::invalid_argument("ocean: interaction kernel_radius " +
                                     std::to_string(d.kernel_radius) +
@@ -342,12 +298,6 @@ InteractionField::InteractionField(const InteractionDesc& d)
     if (d.fixed_dt > 0.0f && d.fixed_dt > m.dt_limit)
         throw std::invalid_argument("ocean: interaction fixed_dt exceeds the stability limit of " +
                                     std::to_string(m.dt_limit) + " s for this configuration");
-
-    int count = 0;
-    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
-        throw std::runtime_error("ocean: no HIP device available");
-    detail::hip_check(hipGetDevice(&m.dev.device), "hipGetDevice");
-    detail::hip_check(hipStreamCreateWithFlags(&m.dev.stream, hipStreamNonBlocking), "hipStreamCreate");
     for (hipEvent_t& e : m.stamps_free)
         detail::hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     m.first.assign(std::size_t(d.max_substeps) + 1, 0u);
@@ -363,7 +313,12 @@ InteractionField::InteractionField(const InteractionDesc& d)
     const std::size_t o_c2 = off;      off = up(off + m.cells * 4);
     const std::size_t o_out = off;     off = up(off + m.cells * 16);
     const std::size_t o_solid = off;   off = up(off + m.cells * 4);
+    const std::size_t o_source = off;  off = up(off + m.cells * 4);
     const std::size_t o_near = off;    off = up(off + m.cells * 4);
+    const std::size_t o_mwork = off;   off = up(off + (m.cells + 2 * std::size_t(m.n)) * 4);
+    const std::size_t o_scount = off;  off = up(off + 4);
+    std::size_t o_mask[2];
+    for (auto& o : o_mask) { o = off; off = up(off + m.cells); }
     const std::size_t o_stamps = off;  off = up(off + m.stamp_cap * sizeof(ocean_hip_stamp));
     const std::size_t o_means = off;   off = up(off + m.stamp_cap * 4);
     const std::size_t o_qxz = off;     off = up(off + std::size_t(detail::kQueryStage) * 8);
@@ -377,7 +332,11 @@ InteractionField::InteractionField(const InteractionDesc& d)
     m.c2 = reinterpret_cast<float*>(base + o_c2);
     m.out = reinterpret_cast<float*>(base + o_out);
     m.d_solid = reinterpret_cast<std::uint32_t*>(base + o_solid);
-    m.d_source = reinterpret_cast<std::uint32_t*>(base + o_near);
+    m.d_source = reinterpret_cast<std::uint32_t*>(base + o_source);
+    m.d_nearest = reinterpret_cast<std::uint32_t*>(base + o_near);
+    m.d_mask_work = reinterpret_cast<std::uint32_t*>(base + o_mwork);
+    m.d_solid_count = reinterpret_cast<std::uint32_t*>(base + o_scount);
+    for (int i = 0; i < 2; ++i) m.d_mask[i] = reinterpret_cast<std::uint8_t*>(base + o_mask[i]);
     m.d_stamps = reinterpret_cast<ocean_hip_stamp*>(base + o_stamps);
     m.d_means = reinterpret_cast<float*>(base + o_means);
     m.d_qxz = reinterpret_cast<float*>(base + o_qxz);
@@ -391,21 +350,29 @@ InteractionField::InteractionField(const InteractionDesc& d)
     detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m.h_qxz), std::size_t(detail::kQueryStage) * 8, hipHostMallocDefault), "hipHostMalloc");
     detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m.h_qout), std::size_t(detail::kQueryStage) * sizeof(ocean_hip_surface), hipHostMallocDefault), "hipHostMalloc");
 
-    m.mask.assign(m.cells, 0);
-    m.mask_shift.assign(m.cells, 0);
-    m.nearest.assign(m.cells, 0);
-    m.bfs.assign(m.cells, 0);
-    m.frontier.assign(m.cells, 0);
-    m.frontier_mark.assign(m.cells, 0);
-    for (std::uint32_t** b : {&m.h_solid, &m.h_source})
-        detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(b), m.cells * 4, hipHostMallocDefault), "hipHostMalloc");
-    detail::hip_check(hipEventCreateWithFlags(&m.mask_uploaded, hipEventDisableTiming), "hipEventCreate");
+    for (int i = 0; i < 2; ++i) {
+        if (hipHostMalloc(reinterpret_cast<void**>(&m.h_mask_stage[i]), m.cells, hipHostMallocMapped) ==
+            hipSuccess) {
+            void* dp = nullptr;
+            if (hipHostGetDevicePointer(&dp, m.h_mask_stage[i], 0) == hipSuccess)
+                m.d_mask_stage[i] = static_cast<const std::uint8_t*>(dp);
+        } else {
+            m.h_mask_stage[i] = nullptr;
+        }
+        (void)hipGetLastError();  // a refused mapping leaves the copied upload; clear it
+        if (m.h_mask_stage[i] == nullptr)
+            detail::hip_check(hipHostMalloc(reinterpret_cast<void**>(&m.h_mask_stage[i]), m.cells,
+                                            hipHostMallocDefault), "hipHostMalloc");
+        detail::hip_check(hipEventCreateWithFlags(&m.mask_read[i], hipEventDisableTiming), "hipEventCreate");
+    }
     m.queue.resize(d.max_sources);
 
     std::vector<float> h1(m.cells), h2(m.cells);
     m.build_damping(h1, h2);
     hipStream_t s = m.dev.stream;
     detail::hip_check(hipMemsetAsync(m.plane[0], 0, 3 * (o_plane[1] - o_plane[0]), s), "clear planes");
+    detail::hip_check(hipMemsetAsync(m.d_mask[0], 0, m.cells, s), "clear mask");
+    detail::hip_check(hipMemsetAsync(m.d_solid_count, 0, 4, s), "clear solid count");
     detail::hip_check(hipMemcpyAsync(m.taps, m.kernel.taps.data(), w * w * 4, hipMemcpyHostToDevice, s), "upload taps");
     detail::hip_check(hipMemcpyAsync(m.c1, h1.data(), m.cells * 4, hipMemcpyHostToDevice, s), "upload c1");
     detail::hip_check(hipMemcpyAsync(m.c2, h2.data(), m.cells * 4, hipMemcpyHostToDevice, s), "upload c2");
@@ -506,9 +473,9 @@ void InteractionField::update(float dt) noexcept
         else if (c > 0)
             m.dev.note(ocean_hip_iwave_inject(&f, m.d_stamps + first[sidx], c,
                                               m.d_means + first[sidx], s));
-        if (m.has_mask && m.solid_count > 0)
-            m.dev.note(ocean_hip_iwave_obstruct(&f, m.d_solid, m.d_source, m.solid_count,
-                                                dirichlet, s));
+        if (m.has_mask)
+            m.dev.note(ocean_hip_iwave_obstruct(&f, m.d_solid, m.d_source, m.d_solid_count,
+                                                static_cast<std::uint32_t>(m.cells), dirichlet, s));
         m.dev.note(ocean_hip_iwave_step(&f, s));
         std::swap(m.cur_i, m.old_i);
     }
@@ -542,21 +509,16 @@ void InteractionField::recenter(float world_x, float world_z) noexcept
     if (std::abs(dx) >= n || std::abs(dz) >= n) {
         for (int i = 0; i < 3; ++i)
             m.dev.note(hipMemsetAsync(m.plane[i], 0, m.plane_floats * 4, m.dev.stream));
-        std::fill(m.mask.begin(), m.mask.end(), std::uint8_t(0));
+        m.dev.note(hipMemsetAsync(m.d_mask[m.mask_i], 0, m.cells, m.dev.stream));
         m.rebuild_mask_tables();
         m.finalize();
         return;
     }
     m.shift_planes(dx, dz);
     if (m.has_mask) {
-        std::copy(m.mask.begin(), m.mask.end(), m.mask_shift.begin());
-        const std::vector<std::uint8_t>& src = m.mask_shift;
-        for (std::int32_t z = 0; z < n; ++z)
-            for (std::int32_t x = 0; x < n; ++x) {
-                const std::int32_t sx = x + dx, sz = z + dz;
-                m.mask[static_cast<std::size_t>(z) * n + x] =
-                    (sx >= 0 && sx < n && sz >= 0 && sz < n) ? src[static_cast<std::size_t>(sz) * n + sx] : 0;
-            }
+        m.dev.note(ocean_hip_iwave_mask_shift(m.d_mask[m.mask_i], m.d_mask[1 - m.mask_i], m.n, dx, dz,
+                                              m.dev.stream));
+        m.mask_i = 1 - m.mask_i;
         m.rebuild_mask_tables();
     }
     m.finalize();
@@ -572,14 +534,30 @@ void InteractionField::set_obstruction(const std::uint8_t* m8) noexcept
     Impl& m = *impl_;
     m.dev.make_current();
     if (m8 == nullptr) {
-        std::fill(m.mask.begin(), m.mask.end(), std::uint8_t(0));
+        m.dev.note(hipMemsetAsync(m.d_mask[m.mask_i], 0, m.cells, m.dev.stream));
         m.has_mask = false;
-        m.solid_count = 0;
         return;
     }
-    std::memcpy(m.mask.data(), m8, m.cells);
+    const int t = m.mask_turn;
+    m.mask_turn = 1 - t;
+    if (m.mask_read_pending[t]) {
+        m.dev.note(hipEventSynchronize(m.mask_read[t]));
+        m.mask_read_pending[t] = false;
+    }
+    std::memcpy(m.h_mask_stage[t], m8, m.cells);
     m.has_mask = true;
-    m.rebuild_mask_tables();
+    if (m.d_mask_stage[t] != nullptr) {
+        // the row kernel reads the staging and keeps a device copy of the mask
+        m.dev.note(ocean_hip_iwave_mask_tables(m.d_mask_stage[t], m.d_mask[m.mask_i], m.n,
+                                               m.d_mask_work, m.d_nearest, m.d_solid, m.d_source,
+                                               m.d_solid_count, m.dev.stream));
+    } else {
+        m.dev.note(hipMemcpyAsync(m.d_mask[m.mask_i], m.h_mask_stage[t], m.cells,
+                                  hipMemcpyHostToDevice, m.dev.stream));
+        m.rebuild_mask_tables();
+    }
+    m.dev.note(hipEventRecord(m.mask_read[t], m.dev.stream));
+    m.mask_read_pending[t] = true;
 }
 
 bool InteractionField::pack_rgba16(std::uint16_t* out, bool on_device) const noexcept
@@ -689,6 +667,25 @@ double InteractionField::energy() const noexcept
     m.dev.note(hipMemcpyAsync(&e, res, 8, hipMemcpyDeviceToHost, m.dev.stream));
     m.dev.note(hipStreamSynchronize(m.dev.stream));
     return e;
+}
+
+std::uint32_t InteractionField::obstruction_tables(std::uint32_t* solid, std::uint32_t* source,
+                                                   std::uint32_t capacity) const noexcept
+{
+    Impl& m = *impl_;
+    if (!m.has_mask) return 0;
+    std::lock_guard<std::mutex> lock(m.query_mu);
+    m.dev.make_current();
+    std::uint32_t count = 0;
+    m.dev.note(hipMemcpyAsync(&count, m.d_solid_count, 4, hipMemcpyDeviceToHost, m.dev.stream));
+    m.dev.note(hipStreamSynchronize(m.dev.stream));
+    const std::uint32_t k = std::min(count, capacity);
+    if (k > 0 && solid != nullptr)
+        m.dev.note(hipMemcpyAsync(solid, m.d_solid, std::size_t(k) * 4, hipMemcpyDeviceToHost, m.dev.stream));
+    if (k > 0 && source != nullptr)
+        m.dev.note(hipMemcpyAsync(source, m.d_source, std::size_t(k) * 4, hipMemcpyDeviceToHost, m.dev.stream));
+    m.dev.note(hipStreamSynchronize(m.dev.stream));
+    return count;
 }
 
 bool InteractionField::sample_combined_batch(const void* device_levels, std::uint32_t level_count,

@@ -1,5 +1,7 @@
-// iWave stencil construction on the host (see iwave_kernel_host.cpp).
+// iWave stencil construction (see iwave_kernel_host.cpp).
 #pragma once
+
+#include "hip_launch.h"
 
 #include <cstdint>
 #include <vector>
@@ -17,9 +19,14 @@ struct IWaveStencil {
     double peak_relative_error = 0.0;
 };
 
-// Least-squares fit of radius `radius` for `cell` metres and `depth`
-// (<= 0 deep). False if the normal equations are singular.
-bool build_iwave_stencil(std::uint32_t radius, double cell, double depth, IWaveStencil& out);
+enum class StencilStatus { Ok, Singular, DeviceError };
+
+// Least-squares fit of radius `radius` (1..12) for `cell` metres and `depth`
+// (<= 0 deep), with its normal equations and symbol scan on the current
+// device, enqueued on `stream` (waited for before returning). Singular: the
+// normal equations are not positive definite.
+StencilStatus build_iwave_stencil(std::uint32_t radius, double cell, double depth,
+                                  hipStream_t stream, IWaveStencil& out);
 
 // The reference's stability test (src/core/iwave_kernel.hpp:166-169).
 inline bool stencil_is_stable(const IWaveStencil& k)

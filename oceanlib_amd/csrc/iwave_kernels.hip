@@ -18,6 +18,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 namespace oceanamd {
 namespace {
 
@@ -165,21 +167,25 @@ __global__ __launch_bounds__(kBlock) void stamp_apply_kernel(ocean_hip_iwave f, 
     }
 }
 
+// The solid list and its count come from the device (the mask tables), so
+// the grid strides over an upper bound.
 __global__ __launch_bounds__(kBlock) void obstruct_kernel(ocean_hip_iwave f,
                                                           const uint32_t* __restrict__ solid,
                                                           const uint32_t* __restrict__ source,
-                                                          uint32_t count, int dirichlet)
+                                                          const uint32_t* __restrict__ count_p,
+                                                          int dirichlet)
 {
-    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= count) return;
-    const uint32_t i = solid[k];
-    const size_t dst = size_t(i / f.n) * f.stride + (i % f.n);
-    if (dirichlet) {
-        f.cur[dst] = 0.0f;
-    } else {
-        const uint32_t src = source[k];
-        if (src == 0xFFFFFFFFu) return;
-        f.cur[dst] = f.cur[size_t(src / f.n) * f.stride + (src % f.n)];
+    const uint32_t count = *count_p;
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < count; k += gridDim.x * kBlock) {
+        const uint32_t i = solid[k];
+        const size_t dst = size_t(i / f.n) * f.stride + (i % f.n);
+        if (dirichlet) {
+            f.cur[dst] = 0.0f;
+        } else {
+            const uint32_t src = source[k];
+            if (src == 0xFFFFFFFFu) continue;
+            f.cur[dst] = f.cur[size_t(src / f.n) * f.stride + (src % f.n)];
+        }
     }
 }
 
@@ -399,12 +405,13 @@ extern "C" hipError_t ocean_hip_iwave_inject_args(const ocean_hip_iwave* f,
 }
 
 extern "C" hipError_t ocean_hip_iwave_obstruct(const ocean_hip_iwave* f, const uint32_t* solid,
-                                               const uint32_t* source, uint32_t count,
-                                               int dirichlet, hipStream_t s)
+                                               const uint32_t* source, const uint32_t* count,
+                                               uint32_t max_count, int dirichlet, hipStream_t s)
 {
-    if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(obstruct_kernel, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                       *f, solid, source, count, dirichlet);
+    if (max_count == 0) return hipSuccess;
+    const uint32_t blocks = std::min<uint32_t>((max_count + kBlock - 1) / kBlock, 512u);
+    hipLaunchKernelGGL(obstruct_kernel, dim3(blocks), dim3(kBlock), 0, s, *f, solid, source, count,
+                       dirichlet);
     return hipGetLastError();
 }
 

@@ -218,6 +218,41 @@ EXPORT int orc_iwave_taps(uint32_t radius, double cell, double depth, float* tap
     return w * w;
 }
 
+/* The fit's taps and its scan summary: sym[0..3] = max_symbol, min_symbol,
+ * band_low, peak_relative_error (iwave_kernel.cpp:400-436). */
+EXPORT int orc_iwave_stencil(uint32_t radius, double cell, double depth, float* taps, double* sym)
+{
+    kernel_t k;
+    if (!build_kernel((int)radius, cell, depth, &k)) return -1;
+    const int w = 2 * (int)radius + 1;
+    memcpy(taps, k.taps, sizeof(float) * (size_t)w * w);
+    sym[0] = k.max_symbol;
+    sym[1] = k.min_symbol;
+    sym[2] = k.band_low;
+    sym[3] = k.peak_relative_error;
+    free(k.taps);
+    return w * w;
+}
+
+/* The realised symbol of `taps` (radius, float-stored cell) at every point
+ * (zi, xi) of the (scan+1)^2 grid over [0, pi/cell]^2, zi-major, as the
+ * scan of iwave_kernel.cpp:408-432 evaluates it. */
+EXPORT void orc_iwave_symbol_scan(const float* taps, uint32_t radius, float cell, double k_max,
+                                  int scan, double* out)
+{
+    kernel_t k;
+    k.radius = (int)radius;
+    k.width = 2 * (int)radius + 1;
+    k.cell = cell;
+    k.taps = (float*)taps;
+    for (int zi = 0; zi <= scan; ++zi)
+        for (int xi = 0; xi <= scan; ++xi) {
+            const double kx = k_max * xi / scan;
+            const double kz = k_max * zi / scan;
+            out[(size_t)zi * (scan + 1) + xi] = realised_symbol(&k, kx, kz);
+        }
+}
+
 /* ---------------- the field: interaction.cpp:71-908 ---------------------- */
 
 typedef struct {
@@ -280,24 +315,23 @@ static void build_damping(orc_iw* m)
     }
 }
 
-/* interaction.cpp:255-292 */
-static void rebuild_mask_tables(orc_iw* m)
+/* interaction.cpp:255-292: a multi-source BFS from every fluid cell
+ * (mask == 0) in index order; nearest[i] = the fluid cell that claims cell i
+ * (0xFFFFFFFF when the mask has no fluid cell). `queue` holds n*n entries. */
+static void bfs_nearest(const uint8_t* mask, uint32_t n, uint32_t* nearest, uint32_t* queue)
 {
-    m->solid_count = 0;
-    if (!m->has_mask) return;
-    const uint32_t n = m->n;
+    const size_t cells = (size_t)n * n;
     uint32_t head = 0, tail = 0;
-    for (size_t i = 0; i < m->cells; ++i) {
-        if (m->mask[i] == 0) {
-            m->nearest[i] = (uint32_t)i;
-            m->bfs[tail++] = (uint32_t)i;
+    for (size_t i = 0; i < cells; ++i) {
+        if (mask[i] == 0) {
+            nearest[i] = (uint32_t)i;
+            queue[tail++] = (uint32_t)i;
         } else {
-            m->nearest[i] = 0xFFFFFFFFu;
-            m->solid[m->solid_count++] = (uint32_t)i;
+            nearest[i] = 0xFFFFFFFFu;
         }
     }
     while (head < tail) {
-        const uint32_t i = m->bfs[head++];
+        const uint32_t i = queue[head++];
         const uint32_t x = i % n, z = i / n;
         const int ddx[4] = {1, -1, 0, 0};
         const int ddz[4] = {0, 0, 1, -1};
@@ -305,12 +339,29 @@ static void rebuild_mask_tables(orc_iw* m)
             const int nx = (int)x + ddx[e], nz = (int)z + ddz[e];
             if (nx < 0 || nz < 0 || nx >= (int)n || nz >= (int)n) continue;
             const uint32_t j = (uint32_t)nz * n + (uint32_t)nx;
-            if (m->nearest[j] == 0xFFFFFFFFu) {
-                m->nearest[j] = m->nearest[i];
-                m->bfs[tail++] = j;
+            if (nearest[j] == 0xFFFFFFFFu) {
+                nearest[j] = nearest[i];
+                queue[tail++] = j;
             }
         }
     }
+}
+
+static void rebuild_mask_tables(orc_iw* m)
+{
+    m->solid_count = 0;
+    if (!m->has_mask) return;
+    for (size_t i = 0; i < m->cells; ++i)
+        if (m->mask[i] != 0) m->solid[m->solid_count++] = (uint32_t)i;
+    bfs_nearest(m->mask, m->n, m->nearest, m->bfs);
+}
+
+/* The reference's nearest-fluid table of a bare n x n mask. */
+EXPORT void orc_nearest_fluid(const uint8_t* mask, uint32_t n, uint32_t* nearest)
+{
+    uint32_t* queue = malloc(sizeof(uint32_t) * (size_t)n * n);
+    bfs_nearest(mask, n, nearest, queue);
+    free(queue);
 }
 
 /* interaction.cpp:295-318 */
@@ -697,6 +748,15 @@ EXPORT double orc_iw_energy(const orc_iw* m)
 }
 
 EXPORT const float* orc_iw_field(const orc_iw* m) { return m->out; }
+/* The solid cells in index order and their nearest fluid cells; returns the count. */
+EXPORT uint32_t orc_iw_mask_tables(const orc_iw* m, uint32_t* solid, uint32_t* source)
+{
+    for (uint32_t k = 0; k < m->solid_count; ++k) {
+        solid[k] = m->solid[k];
+        source[k] = m->nearest[m->solid[k]];
+    }
+    return m->solid_count;
+}
 EXPORT float orc_iw_fixed_dt(const orc_iw* m) { return m->dt; }
 EXPORT float orc_iw_dt_limit(const orc_iw* m) { return m->dt_limit; }
 EXPORT float orc_iw_origin_x(const orc_iw* m) { return (float)m->origin_x * m->cell; }

@@ -266,6 +266,26 @@ OREF int oref_iwave_taps(std::uint32_t radius, double cell, double depth, float*
     }
 }
 
+// The same fit with its scan summary: sym = max_symbol, min_symbol, band_low,
+// peak_relative_error.
+OREF int oref_iwave_stencil(std::uint32_t radius, double cell, double depth, float* taps,
+                            double* sym)
+{
+    try {
+        IWaveKernel k;
+        build_iwave_kernel(radius, cell, depth, KernelMethod::LeastSquares, k);
+        const std::size_t w = 2u * radius + 1u;
+        for (std::size_t i = 0; i < w * w; ++i) taps[i] = k.taps[i];
+        sym[0] = k.max_symbol;
+        sym[1] = k.min_symbol;
+        sym[2] = k.band_low;
+        sym[3] = k.peak_relative_error;
+        return static_cast<int>(w * w);
+    } catch (...) {
+        return -1;
+    }
+}
+
 // A caller's loop of sample_at over an array of points, kept in C++ so the
 // CPU baseline times the reference's query, not the FFI call.
 OREF void oref_sample_batch(const ocean_sim* s, const float* xz, std::uint32_t count,

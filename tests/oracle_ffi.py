@@ -23,6 +23,7 @@ _P = C.c_void_p
 _F = C.POINTER(C.c_float)
 _U32 = C.POINTER(C.c_uint32)
 _D = C.POINTER(OceanDesc)
+_DBL = C.POINTER(C.c_double)
 
 
 class FoamParams(C.Structure):
@@ -70,6 +71,10 @@ _ORACLE_ONLY = {
     "orc_foam_coverage": (C.c_double, [_P]),
     "orc_iwave_taps": (C.c_int, [C.c_uint32, C.c_double, C.c_double, _F,
                                  C.POINTER(C.c_double)]),
+    "orc_iwave_stencil": (C.c_int, [C.c_uint32, C.c_double, C.c_double, _F, _DBL]),
+    "orc_iwave_symbol_scan": (None, [_F, C.c_uint32, C.c_float, C.c_double, C.c_int, _DBL]),
+    "orc_nearest_fluid": (None, [C.POINTER(C.c_uint8), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "orc_iw_mask_tables": (C.c_uint32, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "orc_iw_create": (_P, [C.POINTER(InteractionDesc)]),
     "orc_iw_destroy": (None, [_P]),
     "orc_iw_add": (None, [_P, C.POINTER(Disturbance)]),
@@ -126,6 +131,7 @@ _REF_ONLY = {
     "oref_interaction_energy": (C.c_double, [_P]),
     "oref_sample_combined": (None, [_P, _P, C.c_float, C.c_float, C.POINTER(OceanSurfaceEx)]),
     "oref_iwave_taps": (C.c_int, [C.c_uint32, C.c_double, C.c_double, _F]),
+    "oref_iwave_stencil": (C.c_int, [C.c_uint32, C.c_double, C.c_double, _F, _DBL]),
 }
 
 _cache: dict = {}

@@ -8,6 +8,8 @@ energy() at each checked frame, recentring, impulses kept over zero-length
 frames, the substep clamp and queue overflow. The
 expf the Ricker stamp depends on is the host libm's in both.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -110,21 +112,75 @@ def test_desc_validation_matches_reference():
         assert not oracle().orc_iw_create(C.byref(d)), kw
 
 
-@pytest.mark.parametrize("radius,cell,depth", [(6, 0.1, 0.0), (6, 0.25, 0.0), (3, 0.1, 1.5),
-                                               (12, 0.05, 0.0)])
-def test_library_stencil_equals_oracle(radius, cell, depth):
-    """The taps the library uploads (host C++ fit) equal the oracle's."""
-    from oceanlib_amd import capi
-    import ctypes as C
-    lib = capi.load()
+STENCIL_CASES = [(6, 0.1, 0.0), (1, 0.25, 0.0), (4, 0.25, 2.0), (12, 0.05, 0.0), (8, 0.5, 0.3),
+                 (2, 1.0, 0.0), (3, 0.1, 1.5), (10, 0.02, 5.0), (5, 2.0, 0.5), (7, 0.125, 0.0)]
+
+
+@needs_ref
+@pytest.mark.parametrize("radius,cell,depth", STENCIL_CASES)
+def test_stencil_summary_vs_reference(radius, cell, depth):
+    """The fit's taps and its scan summary (max / min realised symbol, band
+    edge, peak relative error: reference src/core/iwave_kernel.cpp:400-436),
+    oracle against the reference build, bit for bit."""
     w = 2 * radius + 1
-    a = np.zeros(w * w, np.float32)
-    b = np.zeros(w * w, np.float32)
-    ma, mb = C.c_double(), C.c_double()
-    assert lib.ocean_amd_build_iwave_taps(radius, cell, depth, fptr(a), C.byref(ma)) == w * w
-    assert oracle().orc_iwave_taps(radius, cell, depth, fptr(b), C.byref(mb)) == w * w
+    a, b = np.zeros(w * w, np.float32), np.zeros(w * w, np.float32)
+    sa, sb = np.zeros(4), np.zeros(4)
+    dp = lambda v: v.ctypes.data_as(C.POINTER(C.c_double))
+    assert oracle().orc_iwave_stencil(radius, cell, depth, fptr(a), dp(sa)) == w * w
+    assert ref().oref_iwave_stencil(radius, cell, depth, fptr(b), dp(sb)) == w * w
     assert_bitexact(a, b, "taps")
-    assert ma.value == mb.value
+    assert sa.tobytes() == sb.tobytes(), (sa, sb)
+    assert sa[0] > 0 and sa[1] <= 0 and sa[3] > 0
+
+
+def nearest_closed_form(mask):
+    """The fluid cell at the smallest L1 distance, the smallest index among
+    equals (brute force over all fluid cells); 0xFFFFFFFF without fluid."""
+    n = mask.shape[0]
+    fz, fx = np.nonzero(mask == 0)
+    out = np.full(n * n, 0xFFFFFFFF, np.uint32)
+    if fz.size == 0:
+        return out
+    fidx = fz.astype(np.int64) * n + fx
+    for z in range(n):
+        for x in range(n):
+            d = np.abs(fx - x) + np.abs(fz - z)
+            best = d.min()
+            out[z * n + x] = fidx[d == best].min()
+    return out
+
+
+def test_nearest_fluid_bfs_equals_closed_form():
+    """The reference's nearest-fluid BFS (src/interaction.cpp:255-292, restated
+    as orc_nearest_fluid) assigns every cell the fluid cell at the smallest L1
+    distance with the smallest index among equals: the closed form the device
+    computes (iwave_init_kernels.hip). Checked by brute force on random masks
+    of every density, sizes 1..40, blobs, lone fluid cells, no fluid."""
+    rng = np.random.default_rng(11)
+    cases = []
+    for t in range(160):
+        n = int(rng.integers(1, 41))
+        p = float(rng.random())
+        cases.append((rng.random((n, n)) < p).astype(np.uint8))
+    for n in (7, 16, 33):
+        m = np.ones((n, n), np.uint8)
+        cases.append(m.copy())                       # no fluid at all
+        m[n // 2, n // 3] = 0
+        cases.append(m.copy())                       # one fluid cell
+        m[n - 1, 0] = 0
+        m[0, n - 1] = 0
+        cases.append(m.copy())                       # equidistant fluid cells
+        zz, xx = np.mgrid[0:n, 0:n]
+        cases.append(((xx - n / 2) ** 2 + (zz - n / 2) ** 2 < (n / 3) ** 2).astype(np.uint8))
+        cases.append(((xx + zz) % 3 == 0).astype(np.uint8))
+    for mask in cases:
+        n = mask.shape[0]
+        got = np.zeros(n * n, np.uint32)
+        mm = np.ascontiguousarray(mask)
+        oracle().orc_nearest_fluid(mm.ctypes.data_as(C.POINTER(C.c_uint8)), n,
+                                   got.ctypes.data_as(C.POINTER(C.c_uint32)))
+        want = nearest_closed_form(mask)
+        assert np.array_equal(got, want), (n, int(mask.sum()))
 
 
 @needs_ref

@@ -179,6 +179,8 @@ OceanBatch::OceanBatch(std::span<const OceanDesc> descs)
         gr.levels = m.d_levels;
         gr.items = m.d_items + first[g];
         gr.n_items = static_cast<std::uint32_t>(end - first[g]);
+        gr.radix_row = 8;
+        gr.radix_col = 8;
         // Until autotune() runs: as many blocks as stay resident.
         const int occ_r = std::max(1, ocean_hip_batch_occupancy(gr.n, gr.fields, 1));
         const int occ_c = std::max(1, ocean_hip_batch_occupancy(gr.n, gr.fields, 0));
@@ -239,6 +241,38 @@ bool OceanBatch::set_precision(std::uint32_t mode) noexcept
 }
 
 std::uint32_t OceanBatch::precision() const noexcept { return impl_->mode; }
+
+bool OceanBatch::set_group_launch(std::size_t g, std::uint32_t radix_row, std::uint32_t grid_row,
+                                  std::uint32_t radix_col, std::uint32_t grid_col) noexcept
+{
+    Impl& m = *impl_;
+    if (g >= m.groups.size()) return false;
+    ocean_hip_batch_group& gr = m.groups[g];
+    auto ok = [&](std::uint32_t radix, std::uint32_t grid) {
+        if (radix != 8u && radix != 4u) return false;
+        if (!ocean_hip_batch_has_radix(gr.n, gr.fields, gr.mode, radix)) return false;
+        const std::uint32_t units = radix == 4u ? 2u * gr.n_items : gr.n_items;
+        return grid >= 1u && grid <= units;
+    };
+    if (!ok(radix_row, grid_row) || !ok(radix_col, grid_col)) return false;
+    m.levels.front().device().make_current();
+    m.note(hipStreamSynchronize(m.stream.s));  // the graph may still be running
+    gr.radix_row = radix_row;
+    gr.grid_row = grid_row;
+    gr.radix_col = radix_col;
+    gr.grid_col = grid_col;
+    m.info[g].radix_row = radix_row;
+    m.info[g].grid_row = grid_row;
+    m.info[g].radix_col = radix_col;
+    m.info[g].grid_col = grid_col;
+    try {
+        m.rebuild_graph();
+    } catch (...) {
+        m.note(hipErrorUnknown);
+        return false;
+    }
+    return m.status == 0;
+}
 
 double OceanBatch::time() const noexcept { return impl_->time; }
 std::size_t OceanBatch::level_count() const noexcept { return impl_->levels.size(); }
@@ -331,42 +365,57 @@ void OceanBatch::autotune(std::uint32_t reps) noexcept
         return;
     }
     const double t2 = m.time * kInvTwoPiD;
+    // Per group and pass: the FFT's radix grouping (8 points per lane, or 4
+    // with each item in two blocks) and the persistent grid size, timed on
+    // the device; every choice computes the same bits.
     for (std::size_t g = 0; g < m.groups.size(); ++g) {
         ocean_hip_batch_group& gr = m.groups[g];
         for (int pass = 0; pass < 2; ++pass) {
-            std::vector<std::uint32_t> cand;
-            for (int k : {1, 2, 3, 4, 6, 8, 12, 16}) {
-                const std::uint32_t c = whole_xcds(std::uint32_t(m.cus) * std::uint32_t(k));
-                if (c < gr.n_items) cand.push_back(c);
-            }
-            cand.push_back(gr.n_items);  // one block per item
             std::uint32_t best = pass == 0 ? gr.grid_row : gr.grid_col;
+            std::uint32_t best_radix = pass == 0 ? gr.radix_row : gr.radix_col;
             float best_ms = 1e30f;
-            for (std::uint32_t c : cand) {
-                ocean_hip_batch_group t = gr;
-                t.grid_row = c;
-                t.grid_col = c;
-                m.note(ocean_hip_batch_group_frame(&t, t2, 1u << pass, m.stream.s));  // warm
-                hipEventRecord(e0, m.stream.s);
-                for (std::uint32_t r = 0; r < reps; ++r)
-                    m.note(ocean_hip_batch_group_frame(&t, t2, 1u << pass, m.stream.s));
-                hipEventRecord(e1, m.stream.s);
-                hipEventSynchronize(e1);
-                float ms = 0.0f;
-                hipEventElapsedTime(&ms, e0, e1);
-                ms /= static_cast<float>(reps);
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best = c;
+            for (std::uint32_t radix : {8u, 4u}) {
+                if (!ocean_hip_batch_has_radix(gr.n, gr.fields, gr.mode, radix)) continue;
+                const std::uint32_t units = radix == 4u ? 2u * gr.n_items : gr.n_items;
+                std::vector<std::uint32_t> cand;
+                for (int k : {1, 2, 3, 4, 6, 8, 12, 16}) {
+                    const std::uint32_t c = whole_xcds(std::uint32_t(m.cus) * std::uint32_t(k));
+                    if (c < units) cand.push_back(c);
+                }
+                cand.push_back(units);  // one block per unit
+                for (std::uint32_t c : cand) {
+                    ocean_hip_batch_group t = gr;
+                    t.grid_row = c;
+                    t.grid_col = c;
+                    t.radix_row = radix;
+                    t.radix_col = radix;
+                    m.note(ocean_hip_batch_group_frame(&t, t2, 1u << pass, m.stream.s));  // warm
+                    hipEventRecord(e0, m.stream.s);
+                    for (std::uint32_t r = 0; r < reps; ++r)
+                        m.note(ocean_hip_batch_group_frame(&t, t2, 1u << pass, m.stream.s));
+                    hipEventRecord(e1, m.stream.s);
+                    hipEventSynchronize(e1);
+                    float ms = 0.0f;
+                    hipEventElapsedTime(&ms, e0, e1);
+                    ms /= static_cast<float>(reps);
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = c;
+                        best_radix = radix;
+                    }
                 }
             }
             if (pass == 0) {
                 gr.grid_row = best;
+                gr.radix_row = best_radix;
                 m.info[g].grid_row = best;
+                m.info[g].radix_row = best_radix;
                 m.info[g].row_ms = best_ms;
             } else {
                 gr.grid_col = best;
+                gr.radix_col = best_radix;
                 m.info[g].grid_col = best;
+                m.info[g].radix_col = best_radix;
                 m.info[g].col_ms = best_ms;
             }
         }

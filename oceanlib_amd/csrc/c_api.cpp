@@ -773,7 +773,17 @@ int ocean_amd_batch_group_info(const ocean_amd_batch* batch, uint32_t group,
     out->grid_col = g.grid_col;
     out->row_ms = g.row_ms;
     out->col_ms = g.col_ms;
+    out->radix_row = g.radix_row;
+    out->radix_col = g.radix_col;
     return OCEAN_OK;
+}
+int ocean_amd_batch_set_group_launch(ocean_amd_batch* batch, uint32_t group, uint32_t radix_row,
+                                     uint32_t grid_row, uint32_t radix_col, uint32_t grid_col)
+{
+    if (batch == nullptr || group >= to_batch(batch)->group_count()) return OCEAN_ERROR_INVALID_ARG;
+    return to_batch(batch)->set_group_launch(group, radix_row, grid_row, radix_col, grid_col)
+               ? OCEAN_OK
+               : OCEAN_ERROR_INVALID_ARG;
 }
 int ocean_amd_batch_checksums(const ocean_amd_batch* batch, uint32_t first, uint32_t count,
                               uint64_t* out)

@@ -10,8 +10,9 @@
 //               (src/core/evolve.cpp:186-327) from registers into the RGBA32F
 //               displacement / normal (/ velocity) buffers.
 //
-// The block must have exactly TEAMS*N/8 (resp. C*N/8) threads; all of them
-// reach every barrier. Work on `tw` and `xch` (LDS) starts with a barrier, so
+// The block must have exactly TEAMS*N/R (resp. C*N/R) threads, R the points
+// per lane of the FFT (fft_team.hpp; 8 unless a shape picks 4 or 16); all of
+// them reach every barrier. Work on `tw` and `xch` (LDS) starts with a barrier, so
 // a persistent block can call these back to back for different levels.
 #pragma once
 
@@ -141,12 +142,12 @@ __device__ __forceinline__ uint32_t row_of_team(uint32_t g)
     return q == 0u ? uint32_t(N / 2) : uint32_t(N) - q;
 }
 
-template <int N, int F, int TEAMS, int MODE = kModeExact>
+template <int N, int F, int TEAMS, int MODE = kModeExact, int R = 8>
 __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk,
                                           double time_over_two_pi, float2* __restrict__ tw,
                                           float2* __restrict__ xch, uint32_t tid = threadIdx.x)
 {
-    constexpr int T = FftShape<N>::T;
+    constexpr int T = FftShape<N, R>::T;
     constexpr int BLOCK = TEAMS * T;
     constexpr uint32_t CELLS = uint32_t(N) * N;
     const float2* __restrict__ h0 = reinterpret_cast<const float2*>(L.h0);
@@ -166,11 +167,11 @@ __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk
     const uint32_t z = row_of_team<N>(blk * TEAMS + team);
     const uint32_t mz = (uint32_t(N) - z) & uint32_t(N - 1);
 
-    float fr[F][8], fi[F][8];
+    float fr[F][R], fi[F][R];
     const float kz = wave_component(z, N, patch_length);
 
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
+    for (int a = 0; a < R; ++a) {
         const uint32_t x = J + uint32_t(a) * T;
         const uint32_t mx = (uint32_t(N) - x) & uint32_t(N - 1);
         const uint32_t i = z * N + x;
@@ -222,12 +223,12 @@ __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk
     float2* x_team = xch + team * team_stride<N>();
 #pragma unroll
     for (int f = 0; f < F; ++f) {
-        uint32_t idx[8];
+        uint32_t idx[R];
         using Sync = TeamSync<(T <= 64)>;
-        fft_team<N, Sync, MODE != kModeExact>(fr[f], fi[f], idx, J, x_team, tw, Sync{});
+        fft_team<N, Sync, MODE != kModeExact, R>(fr[f], fi[f], idx, J, x_team, tw, Sync{});
         const size_t row0 = size_t(f) * CELLS + size_t(z) * N;
 #pragma unroll
-        for (int a = 0; a < 8; ++a) store_field<N, MODE>(spec, row0 + idx[a], fr[f][a], fi[f][a]);
+        for (int a = 0; a < R; ++a) store_field<N, MODE>(spec, row0 + idx[a], fr[f][a], fi[f][a]);
     }
 }
 
@@ -265,12 +266,12 @@ __device__ __forceinline__ void pack_cell(float h, float d_x, float d_z, float h
     store_out<STREAM_NORMAL>(normal + o, nx, ny, nz, jac);
 }
 
-template <int N, int F, int C, bool STREAM = false, int MODE = kModeExact>
+template <int N, int F, int C, bool STREAM = false, int MODE = kModeExact, int R = 8>
 __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk,
                                           float2* __restrict__ tw, float2* __restrict__ xch,
                                           uint32_t tid = threadIdx.x)
 {
-    constexpr int T = FftShape<N>::T;
+    constexpr int T = FftShape<N, R>::T;
     constexpr int BLOCK = C * T;
     constexpr uint32_t CELLS = uint32_t(N) * N;
     const void* __restrict__ spec = L.spec;
@@ -290,27 +291,27 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
     const uint32_t J = tid / C;
     const uint32_t x = blk * C + c;
 
-    float fr[F][8], fi[F][8];
+    float fr[F][R], fi[F][R];
 #pragma unroll
     for (int f = 0; f < F; ++f) {
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
+        for (int a = 0; a < R; ++a) {
             const float2 v = load_field<MODE>(spec, size_t(f) * CELLS + size_t(J + a * T) * N + x);
             fr[f][a] = v.x;
             fi[f][a] = v.y;
         }
     }
 
-    uint32_t idx[8];
+    uint32_t idx[R];
 #pragma unroll
     for (int f = 0; f < F; ++f) {
         using Sync = TeamSync<(C * T <= 64)>;
-        fft_team<N, Sync, MODE != kModeExact>(fr[f], fi[f], idx, J, xch + c * team_stride<N>(), tw,
+        fft_team<N, Sync, MODE != kModeExact, R>(fr[f], fi[f], idx, J, xch + c * team_stride<N>(), tw,
                                               Sync{});
     }
 
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
+    for (int a = 0; a < R; ++a) {
         const size_t o = size_t(idx[a]) * N + x;
         pack_cell<(STREAM && C * 16 >= 128), STREAM, MODE>(fr[0][a], fi[0][a], fr[1][a], fi[1][a],
                                                      fr[2][a], fi[2][a], fr[3][a], fi[3][a], lambda,
@@ -330,16 +331,16 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
 // The column finalize needs fields 0-3 of a cell together: groups 0 and 1
 // swap half of their fields through LDS (`xfer`) and each packs half of the
 // cells; group 2 (velocity) writes its output itself. Requires T = N/8 >= 64, so a team is whole wavefronts and the
-// group branches are uniform. KX = 2 transforms a team's two fields together
+// group branches are uniform (T = N/R). KX = 2 transforms a team's two fields together
 // (fft_team_k: shared index arithmetic and twiddle reads, one pair of
 // barriers per exchange for both), KX = 1 one after the other.
 
-template <int N, int F, int TEAMS, int KX, int MODE = kModeExact>
+template <int N, int F, int TEAMS, int KX, int MODE = kModeExact, int R = 8>
 __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32_t blk,
                                                 double time_over_two_pi, float2* __restrict__ tw,
                                                 float2* __restrict__ xch)
 {
-    constexpr int T = FftShape<N>::T;
+    constexpr int T = FftShape<N, R>::T;
     constexpr int G = F / 2;
     constexpr int BLOCK = TEAMS * G * T;
     constexpr uint32_t CELLS = uint32_t(N) * N;
@@ -363,11 +364,11 @@ __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32
     const uint32_t z = row_of_team<N>(blk * TEAMS + rteam);
     const uint32_t mz = (uint32_t(N) - z) & uint32_t(N - 1);
 
-    float fr[2][8], fi[2][8];
+    float fr[2][R], fi[2][R];
     const float kz = wave_component(z, N, patch_length);
 
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
+    for (int a = 0; a < R; ++a) {
         const uint32_t x = J + uint32_t(a) * T;
         const uint32_t mx = (uint32_t(N) - x) & uint32_t(N - 1);
         const uint32_t i = z * N + x;
@@ -421,33 +422,33 @@ __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32
     float2* x_team = xch + tt * KX * team_stride<N>();
     using RSync = TeamSync<(T <= 64)>;
     if constexpr (KX == 2) {
-        uint32_t idx[8];
-        fft_team_k<N, 2, RSync, MODE != kModeExact>(fr, fi, idx, J, x_team, team_stride<N>(), tw,
+        uint32_t idx[R];
+        fft_team_k<N, 2, RSync, MODE != kModeExact, R>(fr, fi, idx, J, x_team, team_stride<N>(), tw,
                                                     RSync{});
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
             const size_t row0 = size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
 #pragma unroll
-            for (int a = 0; a < 8; ++a) store_field<N, MODE>(spec, row0 + idx[a], fr[f][a], fi[f][a]);
+            for (int a = 0; a < R; ++a) store_field<N, MODE>(spec, row0 + idx[a], fr[f][a], fi[f][a]);
         }
     } else {
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
-            uint32_t idx[8];
-            fft_team<N, RSync, MODE != kModeExact>(fr[f], fi[f], idx, J, x_team, tw, RSync{});
+            uint32_t idx[R];
+            fft_team<N, RSync, MODE != kModeExact, R>(fr[f], fi[f], idx, J, x_team, tw, RSync{});
             const size_t row0 = size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
 #pragma unroll
-            for (int a = 0; a < 8; ++a) store_field<N, MODE>(spec, row0 + idx[a], fr[f][a], fi[f][a]);
+            for (int a = 0; a < R; ++a) store_field<N, MODE>(spec, row0 + idx[a], fr[f][a], fi[f][a]);
         }
     }
 }
 
-template <int N, int F, int C, int KX, int MODE = kModeExact>
+template <int N, int F, int C, int KX, int MODE = kModeExact, int R = 8>
 __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32_t blk,
                                                 float2* __restrict__ tw, float2* __restrict__ xch,
                                                 float2* __restrict__ xfer)
 {
-    constexpr int T = FftShape<N>::T;
+    constexpr int T = FftShape<N, R>::T;
     constexpr int G = F / 2;
     constexpr int BLOCK = C * T * G;
     constexpr uint32_t CELLS = uint32_t(N) * N;
@@ -471,11 +472,11 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
     const uint32_t J = l / C;
     const uint32_t x = blk * C + c;
 
-    float fr[2][8], fi[2][8];
+    float fr[2][R], fi[2][R];
 #pragma unroll
     for (int f = 0; f < 2; ++f) {
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
+        for (int a = 0; a < R; ++a) {
             const float2 v =
                 load_field<MODE>(spec, size_t(2u * g + uint32_t(f)) * CELLS + size_t(J + a * T) * N + x);
             fr[f][a] = v.x;
@@ -483,35 +484,35 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
         }
     }
 
-    uint32_t idx[8];
+    uint32_t idx[R];
     using CSync = TeamSync<(C * T <= 64)>;
     if constexpr (KX == 2) {
-        fft_team_k<N, 2, CSync, MODE != kModeExact>(fr, fi, idx, J,
+        fft_team_k<N, 2, CSync, MODE != kModeExact, R>(fr, fi, idx, J,
                                                     xch + (g * C + c) * 2 * team_stride<N>(),
                                                     team_stride<N>(), tw, CSync{});
     } else {
 #pragma unroll
         for (int f = 0; f < 2; ++f)
-            fft_team<N, CSync, MODE != kModeExact>(fr[f], fi[f], idx, J,
+            fft_team<N, CSync, MODE != kModeExact, R>(fr[f], fi[f], idx, J,
                                                    xch + (g * C + c) * team_stride<N>(), tw, CSync{});
     }
 
     // Groups 0 and 1 share the finalize: group 0 packs the cells of registers
-    // a = 0..3, group 1 those of a = 4..7. Lanes with the same (c, J) hold the
-    // same indices in both groups, and the cells of a = 0..3 and of a = 4..7
-    // are disjoint, so one LDS area per column carries both hand-overs:
-    // group 1's fields 2, 3 of a = 0..3 and group 0's fields 0, 1 of a = 4..7.
+    // a < R/2, group 1 those of a >= R/2. Lanes with the same (c, J) hold the
+    // same indices in both groups, and the two halves' cells are disjoint, so
+    // one LDS area per column carries both hand-overs: group 1's fields 2, 3
+    // of a < R/2 and group 0's fields 0, 1 of a >= R/2.
     float2* xf0 = xfer + (c * 2u + 0u) * N;
     float2* xf1 = xfer + (c * 2u + 1u) * N;
     if (g == 0u) {
 #pragma unroll
-        for (int a = 4; a < 8; ++a) {
+        for (int a = R / 2; a < R; ++a) {
             xf0[idx[a]] = make_float2(fr[0][a], fi[0][a]);
             xf1[idx[a]] = make_float2(fr[1][a], fi[1][a]);
         }
     } else if (g == 1u) {
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
+        for (int a = 0; a < R / 2; ++a) {
             xf0[idx[a]] = make_float2(fr[0][a], fi[0][a]);
             xf1[idx[a]] = make_float2(fr[1][a], fi[1][a]);
         }
@@ -520,14 +521,14 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
         if (g == 2u) {
             // reference src/core/evolve.cpp:186-203
 #pragma unroll
-            for (int a = 0; a < 8; ++a)
+            for (int a = 0; a < R; ++a)
                 store_out<false>(vel + size_t(idx[a]) * N + x, fr[0][a], fi[0][a], fr[1][a], 0.0f);
         }
     }
     __syncthreads();
     if (g == 0u) {
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
+        for (int a = 0; a < R / 2; ++a) {
             const float2 v2 = xf0[idx[a]];
             const float2 v3 = xf1[idx[a]];
             pack_cell<false, false, MODE>(fr[0][a], fi[0][a], fr[1][a], fi[1][a], v2.x, v2.y, v3.x,
@@ -536,7 +537,7 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
         }
     } else if (g == 1u) {
 #pragma unroll
-        for (int a = 4; a < 8; ++a) {
+        for (int a = R / 2; a < R; ++a) {
             const float2 v0 = xf0[idx[a]];
             const float2 v1 = xf1[idx[a]];
             pack_cell<false, false, MODE>(v0.x, v0.y, v1.x, v1.y, fr[0][a], fi[0][a], fr[1][a],

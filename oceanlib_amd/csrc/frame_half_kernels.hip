@@ -17,7 +17,8 @@ extern "C" int ocean_hip_frame_shapes_fast_2(uint32_t n, uint32_t fields, int* r
     return frame_shapes_mode<kModeFastHalf>(n, int(fields), *row_variants, *col_variants);
 }
 
-extern "C" void* ocean_hip_batch_kernel_fast_2(uint32_t n, int fields, int row, int per_item)
+extern "C" void* ocean_hip_batch_kernel_fast_2(uint32_t n, int fields, int row, int per_item,
+                                               int radix)
 {
-    return batch_kernel_mode<kModeFastHalf>(n, fields, row != 0, per_item != 0);
+    return batch_kernel_mode<kModeFastHalf>(n, fields, row != 0, per_item != 0, radix);
 }

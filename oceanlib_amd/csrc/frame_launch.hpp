@@ -47,77 +47,96 @@ __device__ __forceinline__ uint32_t xcd_group(uint32_t b, uint32_t g)
     return (b % 8u) * (g / 8u) + b / 8u;
 }
 
-template <int N, int F, int TEAMS, int MODE>
-__global__ __launch_bounds__(TEAMS * FftShape<N>::T) void row_pass_kernel(ocean_hip_level L,
-                                                                          double time_over_two_pi)
+// R: points per lane of the FFT (its radix grouping, fft_team.hpp).
+template <int N, int F, int TEAMS, int MODE, int R>
+__global__ __launch_bounds__((TEAMS * FftShape<N, R>::T)) void row_pass_kernel(ocean_hip_level L,
+                                                                             double time_over_two_pi)
 {
     __shared__ float2 tw[N / 2];
     __shared__ float2 xch[TEAMS * team_stride<N>()];
-    row_block<N, F, TEAMS, MODE>(L, xcd_group(blockIdx.x, gridDim.x), time_over_two_pi, tw, xch);
+    row_block<N, F, TEAMS, MODE, R>(L, xcd_group(blockIdx.x, gridDim.x), time_over_two_pi, tw,
+                                    xch);
 }
 
-template <int N, int F, int C, int MODE>
-__global__ __launch_bounds__(C * FftShape<N>::T) void col_pass_kernel(ocean_hip_level L)
+template <int N, int F, int C, int MODE, int R>
+__global__ __launch_bounds__((C * FftShape<N, R>::T)) void col_pass_kernel(ocean_hip_level L)
 {
     __shared__ float2 tw[N / 2];
     __shared__ float2 xch[C * team_stride<N>()];
-    col_block<N, F, C, false, MODE>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch);
+    col_block<N, F, C, false, MODE, R>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch);
 }
 
-template <int N, int F, int TEAMS, int KX, int MODE>
-__global__ __launch_bounds__(TEAMS * (F / 2) * FftShape<N>::T) void row_split_kernel(
+template <int N, int F, int TEAMS, int KX, int MODE, int R>
+__global__ __launch_bounds__((TEAMS * (F / 2) * FftShape<N, R>::T)) void row_split_kernel(
     ocean_hip_level L, double time_over_two_pi)
 {
     __shared__ float2 tw[N / 2];
     __shared__ float2 xch[TEAMS * (F / 2) * KX * team_stride<N>()];
-    row_block_split<N, F, TEAMS, KX, MODE>(L, xcd_group(blockIdx.x, gridDim.x), time_over_two_pi,
-                                           tw, xch);
+    row_block_split<N, F, TEAMS, KX, MODE, R>(L, xcd_group(blockIdx.x, gridDim.x),
+                                              time_over_two_pi, tw, xch);
 }
 
-template <int N, int F, int C, int KX, int MODE>
-__global__ __launch_bounds__(C * (F / 2) * FftShape<N>::T) void col_split_kernel(ocean_hip_level L)
+template <int N, int F, int C, int KX, int MODE, int R>
+__global__ __launch_bounds__((C * (F / 2) * FftShape<N, R>::T)) void col_split_kernel(ocean_hip_level L)
 {
     __shared__ float2 tw[N / 2];
     __shared__ float2 xch[C * (F / 2) * KX * team_stride<N>()];
     __shared__ float2 xfer[C * 2 * N];
-    col_block_split<N, F, C, KX, MODE>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch, xfer);
+    col_block_split<N, F, C, KX, MODE, R>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch, xfer);
 }
 
 // Launch shapes of the per-level passes. Variant 0 is the default shape;
-// the others (rows per block TEAMS, columns per block C, each 1..32) are what
-// ocean_hip_frame_shapes lists for the autotuner. Every shape runs the same
-// row_block / col_block code on the same cells, so results do not depend on
-// the shape.
-template <int N, int TEAMS>
+// the others (rows per block TEAMS, columns per block C, each 1..32, the
+// field split, and the FFT's radix grouping R = 4 / 8 / 16 points per lane)
+// are what ocean_hip_frame_shapes lists for the autotuner. Every shape runs
+// the same butterflies in the same order on the same cells (only where the
+// values live between stages changes), so results do not depend on the shape.
+template <int N, int TEAMS, int R = 8>
 constexpr bool row_ok()
 {
-    constexpr int lanes = TEAMS * FftShape<N>::T;
-    return TEAMS <= N && lanes <= 512 && lanes >= 64 && lanes % 64 == 0;  // 1024 lanes spill at F = 6
+    if constexpr (N < 2 * R) {
+        return false;
+    } else {
+        constexpr int lanes = TEAMS * FftShape<N, R>::T;
+        return TEAMS <= N && lanes <= 512 && lanes >= 64 && lanes % 64 == 0;  // 1024 lanes spill at F = 6
+    }
 }
-template <int N, int C>
+template <int N, int C, int R = 8>
 constexpr bool col_ok()
 {
-    constexpr int lanes = C * FftShape<N>::T;
-    return C <= N && lanes <= 512 && lanes >= 64 && lanes % 64 == 0 &&
-           C * team_stride<N>() * 8 + N * 4 <= 65536;
+    if constexpr (N < 2 * R) {
+        return false;
+    } else {
+        constexpr int lanes = C * FftShape<N, R>::T;
+        return C <= N && lanes <= 512 && lanes >= 64 && lanes % 64 == 0 &&
+               C * team_stride<N>() * 8 + N * 4 <= 65536;
+    }
 }
 
 // Field-split shapes (row_split_kernel / col_split_kernel): whole-wavefront
 // teams, at most 512 lanes, static LDS within 64 KB. KX = fields a team
 // exchanges between the same barriers (1: one field after the other, 2: both).
-template <int N, int F, int TEAMS, int KX>
+template <int N, int F, int TEAMS, int KX, int R = 8>
 constexpr bool row_split_ok()
 {
-    constexpr int lanes = TEAMS * (F / 2) * FftShape<N>::T;
-    return FftShape<N>::T >= 64 && TEAMS <= N && lanes <= 512 &&
-           (N / 2 + TEAMS * (F / 2) * KX * team_stride<N>()) * 8 <= 65536;
+    if constexpr (N < 64 * R) {
+        return false;
+    } else {
+        constexpr int lanes = TEAMS * (F / 2) * FftShape<N, R>::T;
+        return TEAMS <= N && lanes <= 512 &&
+               (N / 2 + TEAMS * (F / 2) * KX * team_stride<N>()) * 8 <= 65536;
+    }
 }
-template <int N, int F, int C, int KX>
+template <int N, int F, int C, int KX, int R = 8>
 constexpr bool col_split_ok()
 {
-    constexpr int lanes = C * (F / 2) * FftShape<N>::T;
-    return FftShape<N>::T >= 64 && C <= N && lanes <= 512 &&
-           (N / 2 + C * (F / 2) * KX * team_stride<N>() + C * 2 * N) * 8 <= 65536;
+    if constexpr (N < 64 * R) {
+        return false;
+    } else {
+        constexpr int lanes = C * (F / 2) * FftShape<N, R>::T;
+        return C <= N && lanes <= 512 &&
+               (N / 2 + C * (F / 2) * KX * team_stride<N>() + C * 2 * N) * 8 <= 65536;
+    }
 }
 
 struct Pick {
@@ -141,30 +160,30 @@ void launch_on(const Launch& k, Kernel kernel, dim3 grid, dim3 block, Args... ar
         hipLaunchKernelGGL(kernel, grid, block, 0, k.s, args...);
 }
 
-template <int N, int F, int TEAMS, int KX, int MODE>
+template <int N, int F, int TEAMS, int KX, int MODE, int R = 8>
 bool try_row_split(Pick& p, const ocean_hip_level& L, double t2, const Launch& k)
 {
-    if constexpr (!row_split_ok<N, F, TEAMS, KX>()) {
+    if constexpr (!row_split_ok<N, F, TEAMS, KX, R>()) {
         return false;
     } else {
         if (p.seen++ != p.want) return false;
         if (k.s != nullptr || L.n != 0)
-            launch_on(k, row_split_kernel<N, F, TEAMS, KX, MODE>, dim3(N / TEAMS),
-                      dim3(TEAMS * (F / 2) * FftShape<N>::T), L, t2);
+            launch_on(k, row_split_kernel<N, F, TEAMS, KX, MODE, R>, dim3(N / TEAMS),
+                      dim3(TEAMS * (F / 2) * FftShape<N, R>::T), L, t2);
         return true;
     }
 }
 
-template <int N, int F, int C, int KX, int MODE>
+template <int N, int F, int C, int KX, int MODE, int R = 8>
 bool try_col_split(Pick& p, const ocean_hip_level& L, const Launch& k)
 {
-    if constexpr (!col_split_ok<N, F, C, KX>()) {
+    if constexpr (!col_split_ok<N, F, C, KX, R>()) {
         return false;
     } else {
         if (p.seen++ != p.want) return false;
         if (k.s != nullptr || L.n != 0)
-            launch_on(k, col_split_kernel<N, F, C, KX, MODE>, dim3(N / C), dim3(C * (F / 2) * FftShape<N>::T),
-                      L);
+            launch_on(k, col_split_kernel<N, F, C, KX, MODE, R>, dim3(N / C),
+                      dim3(C * (F / 2) * FftShape<N, R>::T), L);
         return true;
     }
 }
@@ -180,8 +199,8 @@ bool try_row(Pick& p, const ocean_hip_level& L, double t2, const Launch& k, bool
         if (!is_default && TEAMS == RowShape<N>::TEAMS) return false;  // listed first already
         if (p.seen++ != p.want) return false;
         if (k.s != nullptr || L.n != 0)
-            launch_on(k, row_pass_kernel<N, F, TEAMS, MODE>, dim3(N / TEAMS), dim3(TEAMS * FftShape<N>::T),
-                      L, t2);
+            launch_on(k, row_pass_kernel<N, F, TEAMS, MODE, 8>, dim3(N / TEAMS),
+                      dim3(TEAMS * FftShape<N>::T), L, t2);
         return true;
     }
 }
@@ -195,7 +214,36 @@ bool try_col(Pick& p, const ocean_hip_level& L, const Launch& k, bool is_default
         if (!is_default && C == ColShape<N>::C) return false;
         if (p.seen++ != p.want) return false;
         if (k.s != nullptr || L.n != 0)
-            launch_on(k, col_pass_kernel<N, F, C, MODE>, dim3(N / C), dim3(C * FftShape<N>::T), L);
+            launch_on(k, col_pass_kernel<N, F, C, MODE, 8>, dim3(N / C), dim3(C * FftShape<N>::T), L);
+        return true;
+    }
+}
+
+// Other radix groupings of the unsplit passes (bit-exact mode).
+template <int N, int F, int TEAMS, int MODE, int R>
+bool try_row_r(Pick& p, const ocean_hip_level& L, double t2, const Launch& k)
+{
+    if constexpr (!row_ok<N, TEAMS, R>() || MODE != kModeExact) {
+        return false;
+    } else {
+        if (p.seen++ != p.want) return false;
+        if (k.s != nullptr || L.n != 0)
+            launch_on(k, row_pass_kernel<N, F, TEAMS, MODE, R>, dim3(N / TEAMS),
+                      dim3(TEAMS * FftShape<N, R>::T), L, t2);
+        return true;
+    }
+}
+
+template <int N, int F, int C, int MODE, int R>
+bool try_col_r(Pick& p, const ocean_hip_level& L, const Launch& k)
+{
+    if constexpr (!col_ok<N, C, R>() || MODE != kModeExact) {
+        return false;
+    } else {
+        if (p.seen++ != p.want) return false;
+        if (k.s != nullptr || L.n != 0)
+            launch_on(k, col_pass_kernel<N, F, C, MODE, R>, dim3(N / C), dim3(C * FftShape<N, R>::T),
+                      L);
         return true;
     }
 }
@@ -210,7 +258,15 @@ bool row_variant(int v, const ocean_hip_level& L, double t2, const Launch& k)
            try_row<N, F, 16, M>(p, L, t2, k, false) || try_row<N, F, 32, M>(p, L, t2, k, false) ||
            try_row_split<N, F, 1, 1, M>(p, L, t2, k) || try_row_split<N, F, 2, 1, M>(p, L, t2, k) ||
            try_row_split<N, F, 4, 1, M>(p, L, t2, k) || try_row_split<N, F, 1, 2, M>(p, L, t2, k) ||
-           try_row_split<N, F, 2, 2, M>(p, L, t2, k);
+           try_row_split<N, F, 2, 2, M>(p, L, t2, k) ||
+           // 4 points per lane: twice the lanes, stages in pairs
+           try_row_r<N, F, 2, M, 4>(p, L, t2, k) || try_row_r<N, F, 4, M, 4>(p, L, t2, k) ||
+           try_row_split<N, F, 1, 1, M, 4>(p, L, t2, k) ||
+           try_row_split<N, F, 2, 1, M, 4>(p, L, t2, k) ||
+           try_row_split<N, F, 1, 2, M, 4>(p, L, t2, k) ||
+           // 16 points per lane: half the lanes, stages in fours
+           try_row_split<N, F, 1, 1, M, 16>(p, L, t2, k) ||
+           try_row_split<N, F, 2, 1, M, 16>(p, L, t2, k);
 }
 
 template <int N, int F, int M>
@@ -222,7 +278,11 @@ bool col_variant(int v, const ocean_hip_level& L, const Launch& k)
            try_col<N, F, 8, M>(p, L, k, false) || try_col<N, F, 16, M>(p, L, k, false) ||
            try_col<N, F, 32, M>(p, L, k, false) || try_col_split<N, F, 1, 1, M>(p, L, k) ||
            try_col_split<N, F, 2, 1, M>(p, L, k) || try_col_split<N, F, 4, 1, M>(p, L, k) ||
-           try_col_split<N, F, 1, 2, M>(p, L, k) || try_col_split<N, F, 2, 2, M>(p, L, k);
+           try_col_split<N, F, 1, 2, M>(p, L, k) || try_col_split<N, F, 2, 2, M>(p, L, k) ||
+           try_col_r<N, F, 2, M, 4>(p, L, k) || try_col_r<N, F, 4, M, 4>(p, L, k) ||
+           try_col_split<N, F, 1, 1, M, 4>(p, L, k) || try_col_split<N, F, 2, 1, M, 4>(p, L, k) ||
+           try_col_split<N, F, 1, 2, M, 4>(p, L, k) ||
+           try_col_split<N, F, 1, 1, M, 16>(p, L, k) || try_col_split<N, F, 2, 1, M, 16>(p, L, k);
 }
 
 // Number of variants of each pass (counted without launching: a null level
@@ -318,10 +378,10 @@ __device__ __forceinline__ uint32_t item_lane()
     return t;
 }
 
-template <int N>
+template <int N, int R = 8>
 struct BatchShape {
-    static constexpr int PER = 2048 / N;  // rows (row pass) or columns (column pass) per item
-    static_assert(PER * FftShape<N>::T == kBlock, "batched blocks are 256 lanes");
+    static constexpr int PER = kBlock * R / N;  // rows (row pass) or columns (column pass) per block
+    static_assert(PER * FftShape<N, R>::T == kBlock, "batched blocks are 256 lanes");
 };
 
 // Field-split items (two teams per row or column, as in the single-level
@@ -385,9 +445,90 @@ __global__ __launch_bounds__(kBlock) void batch_col_item_kernel(
                                                  tw, xch);
 }
 
+// 4 FFT points per lane (the radix grouping the batch autotuner may pick per
+// pass): a work item's PER = 2048/N rows or columns are done by two blocks of
+// PER/2, unit u of 2 n_items taking half u / n_items of item u % n_items, so
+// with an item count that is a multiple of 8 (every group of N >= 128) the
+// two halves of an item run on the XCD the item's place in the list gives it.
+// Every row and column goes through the same butterflies as with 8 points per
+// lane.
 template <int N, int F, int M>
-void* batch_fn_f(bool row, bool per_item)
+__global__ __launch_bounds__(kBlock) void batch_row_r4_kernel(
+    const ocean_hip_level* __restrict__ levels, const uint32_t* __restrict__ items,
+    uint32_t n_items, double t2)
 {
+    __shared__ float2 tw[N / 2];
+    __shared__ float2 xch[BatchShape<N, 4>::PER * team_stride<N>()];
+    const uint32_t units = 2u * n_items;
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const uint32_t it = items[u % n_items];
+        const uint32_t blk = 2u * (it & kItemBlockMask) + u / n_items;
+        row_block<N, F, BatchShape<N, 4>::PER, M, 4>(levels[it >> kItemBlockBits], blk, t2, tw, xch,
+                                                     item_lane());
+    }
+}
+
+template <int N, int F, int M>
+__global__ __launch_bounds__(kBlock) void batch_col_r4_kernel(
+    const ocean_hip_level* __restrict__ levels, const uint32_t* __restrict__ items,
+    uint32_t n_items)
+{
+    __shared__ float2 tw[N / 2];
+    __shared__ float2 xch[BatchShape<N, 4>::PER * team_stride<N>()];
+    const uint32_t units = 2u * n_items;
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const uint32_t it = items[u % n_items];
+        const uint32_t blk = 2u * (it & kItemBlockMask) + u / n_items;
+        col_block<N, F, BatchShape<N, 4>::PER, true, M, 4>(levels[it >> kItemBlockBits], blk, tw,
+                                                           xch, item_lane());
+    }
+}
+
+template <int N, int F, int M>
+__global__ __launch_bounds__(kBlock) void batch_row_r4_item_kernel(
+    const ocean_hip_level* __restrict__ levels, const uint32_t* __restrict__ items,
+    uint32_t n_items, double t2)
+{
+    __shared__ float2 tw[N / 2];
+    __shared__ float2 xch[BatchShape<N, 4>::PER * team_stride<N>()];
+    const uint32_t u = blockIdx.x;
+    if (u >= 2u * n_items) return;
+    const uint32_t it = items[u % n_items];
+    row_block<N, F, BatchShape<N, 4>::PER, M, 4>(levels[it >> kItemBlockBits],
+                                                 2u * (it & kItemBlockMask) + u / n_items, t2, tw,
+                                                 xch);
+}
+
+template <int N, int F, int M>
+__global__ __launch_bounds__(kBlock) void batch_col_r4_item_kernel(
+    const ocean_hip_level* __restrict__ levels, const uint32_t* __restrict__ items,
+    uint32_t n_items)
+{
+    __shared__ float2 tw[N / 2];
+    __shared__ float2 xch[BatchShape<N, 4>::PER * team_stride<N>()];
+    const uint32_t u = blockIdx.x;
+    if (u >= 2u * n_items) return;
+    const uint32_t it = items[u % n_items];
+    col_block<N, F, BatchShape<N, 4>::PER, true, M, 4>(levels[it >> kItemBlockBits],
+                                                       2u * (it & kItemBlockMask) + u / n_items, tw,
+                                                       xch);
+}
+
+template <int N, int F, int M>
+void* batch_fn_f(bool row, bool per_item, int radix)
+{
+    if (radix == 4) {
+        if constexpr (N > 1024) {
+            return nullptr;  // 4 points per lane: N <= 1024 (one row per 256-lane block)
+        } else {
+            if (per_item)
+                return row ? reinterpret_cast<void*>(batch_row_r4_item_kernel<N, F, M>)
+                           : reinterpret_cast<void*>(batch_col_r4_item_kernel<N, F, M>);
+            return row ? reinterpret_cast<void*>(batch_row_r4_kernel<N, F, M>)
+                       : reinterpret_cast<void*>(batch_col_r4_kernel<N, F, M>);
+        }
+    }
+    if (radix != 8) return nullptr;
     if (per_item)
         return row ? reinterpret_cast<void*>(batch_row_item_kernel<N, F, M>)
                    : reinterpret_cast<void*>(batch_col_item_kernel<N, F, M>);
@@ -396,23 +537,25 @@ void* batch_fn_f(bool row, bool per_item)
 }
 
 template <int N, int M>
-void* batch_fn(int fields, bool row, bool per_item)
+void* batch_fn(int fields, bool row, bool per_item, int radix)
 {
-    return fields == 6 ? batch_fn_f<N, 6, M>(row, per_item) : batch_fn_f<N, 4, M>(row, per_item);
+    return fields == 6 ? batch_fn_f<N, 6, M>(row, per_item, radix)
+                       : batch_fn_f<N, 4, M>(row, per_item, radix);
 }
 
 // The batched kernel of mode M; per_item: the grid has a block for every
-// item (the loop-free kernels).
+// unit (the loop-free kernels); radix: FFT points per lane, 8 or 4 (NULL if
+// not available for this N).
 template <int M>
-void* batch_kernel_mode(uint32_t n, int fields, bool row, bool per_item)
+void* batch_kernel_mode(uint32_t n, int fields, bool row, bool per_item, int radix = 8)
 {
     switch (n) {
-        case 64: return batch_fn<64, M>(fields, row, per_item);
-        case 128: return batch_fn<128, M>(fields, row, per_item);
-        case 256: return batch_fn<256, M>(fields, row, per_item);
-        case 512: return batch_fn<512, M>(fields, row, per_item);
-        case 1024: return batch_fn<1024, M>(fields, row, per_item);
-        case 2048: return batch_fn<2048, M>(fields, row, per_item);
+        case 64: return batch_fn<64, M>(fields, row, per_item, radix);
+        case 128: return batch_fn<128, M>(fields, row, per_item, radix);
+        case 256: return batch_fn<256, M>(fields, row, per_item, radix);
+        case 512: return batch_fn<512, M>(fields, row, per_item, radix);
+        case 1024: return batch_fn<1024, M>(fields, row, per_item, radix);
+        case 2048: return batch_fn<2048, M>(fields, row, per_item, radix);
         default: return nullptr;
     }
 }

@@ -144,6 +144,8 @@ typedef struct ocean_hip_batch_group {
     uint32_t grid_row;             /* persistent grid sizes (autotuned)        */
     uint32_t grid_col;
     uint32_t mode;                 /* precision mode of the group's kernels    */
+    uint32_t radix_row;            /* FFT points per lane of each pass: 8 (or  */
+    uint32_t radix_col;            /* 0) or 4 (autotuned; each item in halves) */
 } ocean_hip_batch_group;
 /* The opt-in fast modes' kernels (mode 1: frame_fast_kernels.hip, mode 2:
  * frame_half_kernels.hip): the per-level passes of a level whose shape
@@ -155,8 +157,8 @@ hipError_t ocean_hip_frame_fast_2(const ocean_hip_level* level, double time_over
                                   hipStream_t s, unsigned passes, hipEvent_t* events);
 int ocean_hip_frame_shapes_fast_1(uint32_t n, uint32_t fields, int* row_variants, int* col_variants);
 int ocean_hip_frame_shapes_fast_2(uint32_t n, uint32_t fields, int* row_variants, int* col_variants);
-void* ocean_hip_batch_kernel_fast_1(uint32_t n, int fields, int row, int per_item);
-void* ocean_hip_batch_kernel_fast_2(uint32_t n, int fields, int row, int per_item);
+void* ocean_hip_batch_kernel_fast_1(uint32_t n, int fields, int row, int per_item, int radix);
+void* ocean_hip_batch_kernel_fast_2(uint32_t n, int fields, int row, int per_item, int radix);
 /* Variant counts of the per-level passes of one precision mode. */
 int ocean_hip_frame_shapes_mode(uint32_t n, uint32_t fields, uint32_t mode, int* row_variants,
                                 int* col_variants);
@@ -172,6 +174,9 @@ hipError_t ocean_hip_batch_graph_launch(ocean_hip_batch_graph* g, double time_ov
 void ocean_hip_batch_graph_destroy(ocean_hip_batch_graph* g);
 /* Resident blocks per CU of a group's batched kernel (occupancy). */
 int ocean_hip_batch_occupancy(uint32_t n, uint32_t fields, int row);
+/* Whether the batched kernels of (n, fields, mode) exist with `radix` FFT
+ * points per lane (8 always; 4 for N <= 1024). */
+int ocean_hip_batch_has_radix(uint32_t n, uint32_t fields, uint32_t mode, uint32_t radix);
 
 /* Per level of a device array of `count` levels: the checksum of its output
  * buffers (displacement, normal, velocity words w_i; sum of w_i * K_i mod

@@ -33,6 +33,8 @@ def emul(tmp_path_factory):
     lib.emul_fft_team.restype = C.c_int
     lib.emul_fft_team_k.argtypes = [C.c_uint32, C.c_uint32] + [C.POINTER(C.c_float)] * 4
     lib.emul_fft_team_k.restype = C.c_int
+    lib.emul_fft_team_r.argtypes = [C.c_uint32, C.c_uint32] + [C.POINTER(C.c_float)] * 4
+    lib.emul_fft_team_r.restype = C.c_int
     return lib
 
 
@@ -79,3 +81,27 @@ def test_team_fft_of_several_sequences_equals_oracle(emul, n, k):
     assert emul.emul_fft_team_k(n, k, fptr(re), fptr(im), fptr(twr), fptr(twi)) == 0
     assert_bitexact(re, want_re, f"re n={n} k={k}")
     assert_bitexact(im, want_im, f"im n={n} k={k}")
+
+
+@pytest.mark.parametrize("n", [32, 64, 128, 256, 512, 1024, 2048])
+@pytest.mark.parametrize("r", [4, 16])
+def test_team_fft_other_radix_groupings_equal_oracle(emul, n, r):
+    """fft_team with 4 or 16 points per lane (stages grouped in pairs or fours
+    between the exchanges): the same butterflies, the oracle's bits."""
+    twr = np.zeros(n // 2, np.float32)
+    twi = np.zeros(n // 2, np.float32)
+    oracle().orc_twiddles(n, fptr(twr), fptr(twi))
+    rng = np.random.default_rng(7 * n + r)
+    for trial in range(2):
+        re = rng.standard_normal(n).astype(np.float32)
+        im = rng.standard_normal(n).astype(np.float32)
+        if trial == 1:
+            re[:] = 0.0
+            im[:] = -0.0
+            im[n // 2 + 1] = 1.0
+        a, b = re.copy(), im.copy()
+        oracle().orc_fft_inverse(n, fptr(a), fptr(b))
+        c, d = re.copy(), im.copy()
+        assert emul.emul_fft_team_r(n, r, fptr(c), fptr(d), fptr(twr), fptr(twi)) == 0
+        assert_bitexact(c, a, f"re n={n} r={r}")
+        assert_bitexact(d, b, f"im n={n} r={r}")

@@ -154,6 +154,57 @@ def test_mixed_batch_bitexact(lib):
         b.close()
 
 
+def test_every_batch_launch_choice_bitexact(lib):
+    """Each group's launch choices set by hand - 8 or 4 FFT points per lane
+    for either pass (4: stages in pairs, every item in two blocks), a small
+    persistent grid or one block per unit - give every level's frame the
+    oracle's bits (checksums over the frame's output words)."""
+    descs = [desc_from(kw) for kw in MIXED]
+    b = Batch(lib, descs)
+    orc = [OracleOcean(d) for d in descs]
+    try:
+        t = 21.75
+        want = []
+        for i, o in enumerate(orc):
+            o.update(t)
+            words = [o.displacement(), o.normal()] + ([o.velocity()] if descs[i].compute_velocity
+                                                       else [])
+            want.append(checksum_words(np.concatenate(words)))
+        groups = b.groups()
+        sums = (C.c_uint64 * len(descs))()
+        tried = 0
+        for g, info in enumerate(groups):
+            radices = (8, 4) if info.size <= 1024 else (8,)
+            for rr in radices:
+                for rc in radices:
+                    for small in (True, False):
+                        ur = info.items * (2 if rr == 4 else 1)
+                        uc = info.items * (2 if rc == 4 else 1)
+                        gr, gc = (min(8, ur), min(8, uc)) if small else (ur, uc)
+                        assert lib.ocean_amd_batch_set_group_launch(b.h, g, rr, gr, rc, gc) == 0
+                        got = b.groups()[g]
+                        assert (got.radix_row, got.grid_row, got.radix_col, got.grid_col) == \
+                            (rr, gr, rc, gc)
+                        b.update(t + 1.0)  # a different frame first
+                        b.update(t)
+                        assert lib.ocean_amd_batch_checksums(b.h, 0, len(descs), sums) == 0
+                        assert list(sums) == want, (info.size, rr, rc, small)
+                        tried += 1
+            # choices that do not exist are refused
+            assert lib.ocean_amd_batch_set_group_launch(b.h, g, 16, 1, 8, 1) != 0
+            assert lib.ocean_amd_batch_set_group_launch(b.h, g, 8, 0, 8, 1) != 0
+            assert lib.ocean_amd_batch_set_group_launch(b.h, g, 8, info.items + 1, 8, 1) != 0
+        assert tried == 4 * 2 * len(groups)
+        lib.ocean_amd_batch_autotune(b.h, 2)
+        b.update(t)
+        assert lib.ocean_amd_batch_checksums(b.h, 0, len(descs), sums) == 0
+        assert list(sums) == want
+        print("tuned:", [(g.size, g.radix_row, g.grid_row, g.radix_col, g.grid_col)
+                         for g in b.groups()])
+    finally:
+        b.close()
+
+
 def test_batch_queries(lib):
     """Batch queries: one level answers as that ocean's sample_at, a range of
     levels as a cascade of those levels (reference src/cascade.cpp:82-153),

@@ -202,10 +202,11 @@ def test_queries_bitexact(lib):
         g.close()
 
 
-@pytest.mark.parametrize("n", [32, 64, 256, 512, 1024, 2048, 4096])
+@pytest.mark.parametrize("n", [32, 64, 128, 256, 512, 1024, 2048, 4096])
 def test_every_launch_shape_is_bitexact(lib, n):
     """Each launch-shape variant of both passes (rows / columns per
-    workgroup) gives the oracle's bits; the autotuner's pick does too."""
+    workgroup, field split, 4 / 8 / 16 FFT points per lane) gives the
+    oracle's bits; the autotuner's pick does too."""
     vel = 1 if n in (256, 512, 1024) else 0
     d = capi.default_desc(lib, size=n, patch_length=float(n) * 0.7, wind_speed=17.0, seed=n + 3,
                           choppiness=1.2, compute_velocity=vel, water_depth=30.0 if vel else 0.0)

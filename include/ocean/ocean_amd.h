@@ -100,6 +100,15 @@ int ocean_amd_frame_shapes(const ocean_sim* sim, uint32_t* row_variants, uint32_
 int ocean_amd_set_frame_shape(ocean_sim* sim, uint32_t row_variant, uint32_t col_variant);
 int ocean_amd_autotune_frame(ocean_sim* sim, uint32_t reps, uint32_t* row_variant,
                              uint32_t* col_variant, float* row_ms, float* col_ms);
+/* The layout of the complex fields the row pass hands to the column pass:
+ * 0 = row-major (the default), 1 / 2 / 3 = tiles of 2 / 4 / 8 columns x N
+ * rows, so a column pass's loads of adjacent columns over consecutive rows
+ * are contiguous runs (bit-exact mode, N >= 256). Setting a layout selects its default shapes; the shape counts and
+ * indices above refer to the current layout, and the autotuner tunes every
+ * available layout and keeps the fastest. Results do not depend on it.
+ * OCEAN_ERROR_INVALID_ARG for a layout the ocean cannot use. */
+int ocean_amd_set_frame_layout(ocean_sim* sim, uint32_t layout);
+uint32_t ocean_amd_get_frame_layout(const ocean_sim* sim);
 
 /* ---- opt-in fast mode ----------------------------------------------------
  * No reference counterpart: the reference's frame is bit-exact only, and it

@@ -191,6 +191,8 @@ AMD_SIGNATURES = {
     "ocean_amd_profile_frame": (C.c_int, [_P, C.c_double, C.c_double, C.c_uint32, _F, _F]),
     "ocean_amd_get_tables": (C.c_int, [_P, _F, _F, C.POINTER(C.c_uint32)]),
     "ocean_amd_frame_shapes": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "ocean_amd_set_frame_layout": (C.c_int, [_P, C.c_uint32]),
+    "ocean_amd_get_frame_layout": (C.c_uint32, [_P]),
     "ocean_amd_set_frame_shape": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
     "ocean_amd_set_precision": (C.c_int, [_P, C.c_uint32]),
     "ocean_amd_get_precision": (C.c_uint32, [_P]),

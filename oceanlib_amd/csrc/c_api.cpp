@@ -295,17 +295,54 @@ int ocean_amd_get_tables(const ocean_sim* sim, float* h0, float* omega, uint32_t
     return dev.status == 0 ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
 }
 
+namespace {
+
+// A level's shape word: row variant (bits 0-7), column variant (8-15),
+// precision mode (16-23), layout of the fields between the passes (24-31).
+uint32_t shape_mode(uint32_t shape) { return (shape >> 16) & 0xffu; }
+uint32_t shape_layout(uint32_t shape) { return (shape >> 24) & 0xffu; }
+
+int shape_counts(const ocean::detail::OceanDevice& dev, uint32_t layout, int& r, int& c)
+{
+    return ocean_hip_frame_shapes_layout(dev.level.n, uint32_t(dev.fields), shape_mode(dev.level.shape),
+                                         layout, &r, &c);
+}
+
+void upload_shape(ocean::detail::OceanDevice& dev, uint32_t shape)
+{
+    dev.make_current();
+    dev.level.shape = shape;
+    dev.note(hipMemcpyAsync(dev.d_level, &dev.level, sizeof(dev.level), hipMemcpyHostToDevice, dev.stream));
+    dev.note(hipStreamSynchronize(dev.stream));
+}
+
+}  // namespace
+
 int ocean_amd_frame_shapes(const ocean_sim* sim, uint32_t* row_variants, uint32_t* col_variants)
 {
     if (sim == nullptr) return OCEAN_ERROR_INVALID_ARG;
     const auto& dev = to_cpp(sim)->device();
     int r = 0, c = 0;
-    if (ocean_hip_frame_shapes_mode(dev.level.n, uint32_t(dev.fields), (dev.level.shape >> 16) & 0xffu,
-                                    &r, &c) != 0)
-        return OCEAN_ERROR_INVALID_ARG;
+    if (shape_counts(dev, shape_layout(dev.level.shape), r, c) != 0) return OCEAN_ERROR_INVALID_ARG;
     if (row_variants) *row_variants = static_cast<uint32_t>(r);
     if (col_variants) *col_variants = static_cast<uint32_t>(c);
     return OCEAN_OK;
+}
+
+int ocean_amd_set_frame_layout(ocean_sim* sim, uint32_t layout)
+{
+    if (sim == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    auto& dev = to_cpp(sim)->device();
+    int r = 0, c = 0;
+    if (shape_counts(dev, layout, r, c) != 0) return OCEAN_ERROR_INVALID_ARG;
+    // the default shapes of the new layout
+    upload_shape(dev, (shape_mode(dev.level.shape) << 16) | (layout << 24));
+    return dev.status == 0 ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
+}
+
+uint32_t ocean_amd_get_frame_layout(const ocean_sim* sim)
+{
+    return sim == nullptr ? 0u : shape_layout(to_cpp(sim)->device().level.shape);
 }
 
 int ocean_amd_set_frame_shape(ocean_sim* sim, uint32_t row_variant, uint32_t col_variant)
@@ -315,8 +352,7 @@ int ocean_amd_set_frame_shape(ocean_sim* sim, uint32_t row_variant, uint32_t col
     auto& dev = o.device();
     int r = 0, c = 0;
     const uint32_t mode_bits = dev.level.shape & 0xffff0000u;
-    if (ocean_hip_frame_shapes_mode(dev.level.n, uint32_t(dev.fields), mode_bits >> 16, &r, &c) != 0 ||
-        row_variant >= uint32_t(r) ||
+    if (shape_counts(dev, shape_layout(dev.level.shape), r, c) != 0 || row_variant >= uint32_t(r) ||
         col_variant >= uint32_t(c))
         return OCEAN_ERROR_INVALID_ARG;
     dev.make_current();
@@ -332,15 +368,13 @@ int ocean_amd_autotune_frame(ocean_sim* sim, uint32_t reps, uint32_t* row_varian
     if (sim == nullptr || reps == 0 || reps > 256) return OCEAN_ERROR_INVALID_ARG;
     ocean::Ocean& o = *to_cpp(sim);
     auto& dev = o.device();
-    int nr = 0, nc = 0;
-    const uint32_t mode_bits = dev.level.shape & 0xffff0000u;
-    if (ocean_hip_frame_shapes_mode(dev.level.n, uint32_t(dev.fields), mode_bits >> 16, &nr, &nc) != 0)
-        return OCEAN_ERROR_INVALID_ARG;
     dev.make_current();
     // Each candidate is timed inside whole frames (row pass then column pass,
     // the other pass at its current pick), by the dispatches' own events, so
     // it sees the caches a frame leaves: the column pass reads fields the
     // row pass just wrote on other XCDs, the row pass follows a column pass.
+    // Every layout of the fields between the passes is tuned this way, and
+    // the layout with the smallest row + column time is kept.
     std::vector<hipEvent_t> ev(std::size_t(reps) * 4, nullptr);
     bool ok = true;
     for (auto& e : ev) ok = ok && hipEventCreate(&e) == hipSuccess;
@@ -354,29 +388,47 @@ int ocean_amd_autotune_frame(ocean_sim* sim, uint32_t reps, uint32_t* row_varian
     }
     const double t2 = o.time() * 0.159154943091895335769;
     const uint32_t saved = dev.level.shape;
-    float best[2] = {1e30f, 1e30f};
-    uint32_t pick[2] = {saved & 0xffu, (saved >> 8) & 0xffu};
-    for (unsigned pass = 0; pass < 2 && ok; ++pass) {
-        const int count = pass == 0 ? nr : nc;
-        for (int v = 0; v < count && ok; ++v) {
-            dev.level.shape =
-                mode_bits | (pass == 0 ? (uint32_t(v) | (pick[1] << 8)) : (pick[0] | (uint32_t(v) << 8)));
-            dev.note(ocean_hip_frame(&dev.level, t2, dev.stream));  // warm
-            for (uint32_t i = 0; i < reps; ++i)
-                dev.note(ocean_hip_frame_timed(&dev.level, t2, dev.stream, &ev[std::size_t(i) * 4]));
-            ok = hipStreamSynchronize(dev.stream) == hipSuccess;
-            double sum = 0.0;
-            for (uint32_t i = 0; i < reps && ok; ++i) {
-                float ms = 0.0f;
-                hipEvent_t* e = &ev[std::size_t(i) * 4 + 2 * pass];
-                ok = hipEventElapsedTime(&ms, e[0], e[1]) == hipSuccess;
-                sum += ms;
+    const uint32_t mode = shape_mode(saved);
+    float best_total = 1e30f, best_out[2] = {0.0f, 0.0f};
+    uint32_t best_shape = saved;
+    for (uint32_t layout = 0; layout < 4u && ok; ++layout) {
+        int nr = 0, nc = 0;
+        if (shape_counts(dev, layout, nr, nc) != 0) continue;
+        const uint32_t base = (mode << 16) | (layout << 24);
+        float best[2] = {1e30f, 1e30f};
+        uint32_t pick[2] = {0u, 0u};
+        if (layout == shape_layout(saved)) {
+            pick[0] = saved & 0xffu;
+            pick[1] = (saved >> 8) & 0xffu;
+        }
+        for (unsigned pass = 0; pass < 2 && ok; ++pass) {
+            const int count = pass == 0 ? nr : nc;
+            for (int v = 0; v < count && ok; ++v) {
+                dev.level.shape =
+                    base | (pass == 0 ? (uint32_t(v) | (pick[1] << 8)) : (pick[0] | (uint32_t(v) << 8)));
+                dev.note(ocean_hip_frame(&dev.level, t2, dev.stream));  // warm
+                for (uint32_t i = 0; i < reps; ++i)
+                    dev.note(ocean_hip_frame_timed(&dev.level, t2, dev.stream, &ev[std::size_t(i) * 4]));
+                ok = hipStreamSynchronize(dev.stream) == hipSuccess;
+                double sum = 0.0;
+                for (uint32_t i = 0; i < reps && ok; ++i) {
+                    float ms = 0.0f;
+                    hipEvent_t* e = &ev[std::size_t(i) * 4 + 2 * pass];
+                    ok = hipEventElapsedTime(&ms, e[0], e[1]) == hipSuccess;
+                    sum += ms;
+                }
+                const float ms = float(sum / reps);
+                if (ok && ms < best[pass]) {
+                    best[pass] = ms;
+                    pick[pass] = uint32_t(v);
+                }
             }
-            const float ms = float(sum / reps);
-            if (ok && ms < best[pass]) {
-                best[pass] = ms;
-                pick[pass] = uint32_t(v);
-            }
+        }
+        if (ok && best[0] + best[1] < best_total) {
+            best_total = best[0] + best[1];
+            best_out[0] = best[0];
+            best_out[1] = best[1];
+            best_shape = base | pick[0] | (pick[1] << 8);
         }
     }
     release();
@@ -384,14 +436,14 @@ int ocean_amd_autotune_frame(ocean_sim* sim, uint32_t reps, uint32_t* row_varian
         dev.level.shape = saved;
         return OCEAN_ERROR_UNKNOWN;
     }
-    dev.level.shape = mode_bits | pick[0] | (pick[1] << 8);
+    dev.level.shape = best_shape;
     dev.note(hipMemcpyAsync(dev.d_level, &dev.level, sizeof(dev.level), hipMemcpyHostToDevice, dev.stream));
     dev.note(ocean_hip_frame(&dev.level, t2, dev.stream));  // a complete frame at the current time
     dev.note(hipStreamSynchronize(dev.stream));
-    if (row_variant) *row_variant = pick[0];
-    if (col_variant) *col_variant = pick[1];
-    if (row_ms) *row_ms = best[0];
-    if (col_ms) *col_ms = best[1];
+    if (row_variant) *row_variant = best_shape & 0xffu;
+    if (col_variant) *col_variant = (best_shape >> 8) & 0xffu;
+    if (row_ms) *row_ms = best_out[0];
+    if (col_ms) *col_ms = best_out[1];
     return dev.status == 0 ? OCEAN_OK : OCEAN_ERROR_UNKNOWN;
 }
 

@@ -97,13 +97,31 @@ constexpr int kModeFastHalf = 2;
 
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 
+// Element (z, x) of field f in the level's `spec` scratch, in the layout W:
+// rows of N elements (W = 0), or tiles of W columns x N rows whose rows are W
+// elements long (W > 0), so that a column team's loads of W adjacent columns
+// over consecutive rows are one contiguous run instead of one piece per row.
+template <int N, int W>
+__device__ __forceinline__ size_t spec_at(uint32_t f, uint32_t z, uint32_t x)
+{
+    constexpr size_t CELLS = size_t(N) * N;
+    if constexpr (W == 0)
+        return size_t(f) * CELLS + size_t(z) * N + x;
+    else
+        return size_t(f) * CELLS + size_t(x / W) * (size_t(N) * W) + size_t(z) * W + (x % W);
+}
+
 // One complex value of field f at cell `cell` (element offset within the
-// level's `spec`), in the mode's storage: float2, or fp16 pairs.
-template <int N, int MODE>
+// level's `spec`), in the mode's storage: float2, or fp16 pairs. Row-major
+// rows are streamed out (whole lines); tiled rows are pieces of W elements,
+// left to the L2 to merge.
+template <int N, int MODE, int W = 0>
 __device__ __forceinline__ void store_field(void* spec, size_t at, float x, float y)
 {
     if constexpr (MODE == kModeFastHalf) {
         reinterpret_cast<f16x2*>(spec)[at] = f16x2{static_cast<_Float16>(x), static_cast<_Float16>(y)};
+    } else if constexpr (W != 0) {
+        reinterpret_cast<float2*>(spec)[at] = make_float2(x, y);
     } else {
         store_stream<N>(reinterpret_cast<float2*>(spec) + at, x, y);
     }
@@ -142,14 +160,13 @@ __device__ __forceinline__ uint32_t row_of_team(uint32_t g)
     return q == 0u ? uint32_t(N / 2) : uint32_t(N) - q;
 }
 
-template <int N, int F, int TEAMS, int MODE = kModeExact, int R = 8>
+template <int N, int F, int TEAMS, int MODE = kModeExact, int R = 8, int W = 0>
 __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk,
                                           double time_over_two_pi, float2* __restrict__ tw,
                                           float2* __restrict__ xch, uint32_t tid = threadIdx.x)
 {
     constexpr int T = FftShape<N, R>::T;
     constexpr int BLOCK = TEAMS * T;
-    constexpr uint32_t CELLS = uint32_t(N) * N;
     const float2* __restrict__ h0 = reinterpret_cast<const float2*>(L.h0);
     const float* __restrict__ omega = L.omega;
     const float* __restrict__ kinv = L.kinv;
@@ -226,9 +243,9 @@ __device__ __forceinline__ void row_block(const ocean_hip_level& L, uint32_t blk
         uint32_t idx[R];
         using Sync = TeamSync<(T <= 64)>;
         fft_team<N, Sync, MODE != kModeExact, R>(fr[f], fi[f], idx, J, x_team, tw, Sync{});
-        const size_t row0 = size_t(f) * CELLS + size_t(z) * N;
 #pragma unroll
-        for (int a = 0; a < R; ++a) store_field<N, MODE>(spec, row0 + idx[a], fr[f][a], fi[f][a]);
+        for (int a = 0; a < R; ++a)
+            store_field<N, MODE, W>(spec, spec_at<N, W>(uint32_t(f), z, idx[a]), fr[f][a], fi[f][a]);
     }
 }
 
@@ -266,14 +283,13 @@ __device__ __forceinline__ void pack_cell(float h, float d_x, float d_z, float h
     store_out<STREAM_NORMAL>(normal + o, nx, ny, nz, jac);
 }
 
-template <int N, int F, int C, bool STREAM = false, int MODE = kModeExact, int R = 8>
+template <int N, int F, int C, bool STREAM = false, int MODE = kModeExact, int R = 8, int W = 0>
 __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk,
                                           float2* __restrict__ tw, float2* __restrict__ xch,
                                           uint32_t tid = threadIdx.x)
 {
     constexpr int T = FftShape<N, R>::T;
     constexpr int BLOCK = C * T;
-    constexpr uint32_t CELLS = uint32_t(N) * N;
     const void* __restrict__ spec = L.spec;
     const float2* __restrict__ twg = reinterpret_cast<const float2*>(L.twiddles);
     float4* __restrict__ disp = reinterpret_cast<float4*>(L.disp);
@@ -296,7 +312,7 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
     for (int f = 0; f < F; ++f) {
 #pragma unroll
         for (int a = 0; a < R; ++a) {
-            const float2 v = load_field<MODE>(spec, size_t(f) * CELLS + size_t(J + a * T) * N + x);
+            const float2 v = load_field<MODE>(spec, spec_at<N, W>(uint32_t(f), J + uint32_t(a * T), x));
             fr[f][a] = v.x;
             fi[f][a] = v.y;
         }
@@ -335,7 +351,7 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
 // (fft_team_k: shared index arithmetic and twiddle reads, one pair of
 // barriers per exchange for both), KX = 1 one after the other.
 
-template <int N, int F, int TEAMS, int KX, int MODE = kModeExact, int R = 8>
+template <int N, int F, int TEAMS, int KX, int MODE = kModeExact, int R = 8, int W = 0>
 __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32_t blk,
                                                 double time_over_two_pi, float2* __restrict__ tw,
                                                 float2* __restrict__ xch)
@@ -343,7 +359,6 @@ __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32
     constexpr int T = FftShape<N, R>::T;
     constexpr int G = F / 2;
     constexpr int BLOCK = TEAMS * G * T;
-    constexpr uint32_t CELLS = uint32_t(N) * N;
     static_assert(T >= 64 && F % 2 == 0, "field split needs whole-wavefront teams");
     const float2* __restrict__ h0 = reinterpret_cast<const float2*>(L.h0);
     const float* __restrict__ omega = L.omega;
@@ -427,23 +442,25 @@ __device__ __forceinline__ void row_block_split(const ocean_hip_level& L, uint32
                                                     RSync{});
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
-            const size_t row0 = size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
 #pragma unroll
-            for (int a = 0; a < R; ++a) store_field<N, MODE>(spec, row0 + idx[a], fr[f][a], fi[f][a]);
+            for (int a = 0; a < R; ++a)
+                store_field<N, MODE, W>(spec, spec_at<N, W>(2u * g + uint32_t(f), z, idx[a]), fr[f][a],
+                                        fi[f][a]);
         }
     } else {
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
             uint32_t idx[R];
             fft_team<N, RSync, MODE != kModeExact, R>(fr[f], fi[f], idx, J, x_team, tw, RSync{});
-            const size_t row0 = size_t(2u * g + uint32_t(f)) * CELLS + size_t(z) * N;
 #pragma unroll
-            for (int a = 0; a < R; ++a) store_field<N, MODE>(spec, row0 + idx[a], fr[f][a], fi[f][a]);
+            for (int a = 0; a < R; ++a)
+                store_field<N, MODE, W>(spec, spec_at<N, W>(2u * g + uint32_t(f), z, idx[a]), fr[f][a],
+                                        fi[f][a]);
         }
     }
 }
 
-template <int N, int F, int C, int KX, int MODE = kModeExact, int R = 8>
+template <int N, int F, int C, int KX, int MODE = kModeExact, int R = 8, int W = 0>
 __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32_t blk,
                                                 float2* __restrict__ tw, float2* __restrict__ xch,
                                                 float2* __restrict__ xfer)
@@ -451,7 +468,6 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
     constexpr int T = FftShape<N, R>::T;
     constexpr int G = F / 2;
     constexpr int BLOCK = C * T * G;
-    constexpr uint32_t CELLS = uint32_t(N) * N;
     static_assert(T >= 64 && F % 2 == 0, "field split needs whole-wavefront teams");
     const void* __restrict__ spec = L.spec;
     const float2* __restrict__ twg = reinterpret_cast<const float2*>(L.twiddles);
@@ -478,7 +494,7 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
 #pragma unroll
         for (int a = 0; a < R; ++a) {
             const float2 v =
-                load_field<MODE>(spec, size_t(2u * g + uint32_t(f)) * CELLS + size_t(J + a * T) * N + x);
+                load_field<MODE>(spec, spec_at<N, W>(2u * g + uint32_t(f), J + uint32_t(a * T), x));
             fr[f][a] = v.x;
             fi[f][a] = v.y;
         }

@@ -135,6 +135,18 @@ extern "C" int ocean_hip_frame_shapes_mode(uint32_t n, uint32_t fields, uint32_t
     return 0;
 }
 
+extern "C" int ocean_hip_frame_shapes_layout(uint32_t n, uint32_t fields, uint32_t mode,
+                                             uint32_t layout, int* row_variants, int* col_variants)
+{
+    if (layout == 0u) return ocean_hip_frame_shapes_mode(n, fields, mode, row_variants, col_variants);
+    if (mode != 0u) return -1;
+    int r = 0, c = 0;
+    if (frame_shapes_mode<kModeExact>(n, int(fields), r, c, int(layout)) != 0) return -1;
+    if (row_variants) *row_variants = r;
+    if (col_variants) *col_variants = c;
+    return 0;
+}
+
 extern "C" int ocean_hip_batchable(uint32_t n) { return batch_kernel(n, 4, true) != nullptr; }
 
 extern "C" uint32_t ocean_hip_batch_items(uint32_t n) { return n * n / 2048u; }

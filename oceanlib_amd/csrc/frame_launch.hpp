@@ -48,41 +48,43 @@ __device__ __forceinline__ uint32_t xcd_group(uint32_t b, uint32_t g)
 }
 
 // R: points per lane of the FFT (its radix grouping, fft_team.hpp).
-template <int N, int F, int TEAMS, int MODE, int R>
+// R: points per lane of the FFT (its radix grouping, fft_team.hpp); W: the
+// layout of the fields between the passes (spec_at, frame_device.hpp).
+template <int N, int F, int TEAMS, int MODE, int R, int W>
 __global__ __launch_bounds__((TEAMS * FftShape<N, R>::T)) void row_pass_kernel(ocean_hip_level L,
                                                                              double time_over_two_pi)
 {
     __shared__ float2 tw[N / 2];
     __shared__ float2 xch[TEAMS * team_stride<N>()];
-    row_block<N, F, TEAMS, MODE, R>(L, xcd_group(blockIdx.x, gridDim.x), time_over_two_pi, tw,
-                                    xch);
+    row_block<N, F, TEAMS, MODE, R, W>(L, xcd_group(blockIdx.x, gridDim.x), time_over_two_pi, tw,
+                                       xch);
 }
 
-template <int N, int F, int C, int MODE, int R>
+template <int N, int F, int C, int MODE, int R, int W>
 __global__ __launch_bounds__((C * FftShape<N, R>::T)) void col_pass_kernel(ocean_hip_level L)
 {
     __shared__ float2 tw[N / 2];
     __shared__ float2 xch[C * team_stride<N>()];
-    col_block<N, F, C, false, MODE, R>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch);
+    col_block<N, F, C, false, MODE, R, W>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch);
 }
 
-template <int N, int F, int TEAMS, int KX, int MODE, int R>
+template <int N, int F, int TEAMS, int KX, int MODE, int R, int W>
 __global__ __launch_bounds__((TEAMS * (F / 2) * FftShape<N, R>::T)) void row_split_kernel(
     ocean_hip_level L, double time_over_two_pi)
 {
     __shared__ float2 tw[N / 2];
     __shared__ float2 xch[TEAMS * (F / 2) * KX * team_stride<N>()];
-    row_block_split<N, F, TEAMS, KX, MODE, R>(L, xcd_group(blockIdx.x, gridDim.x),
-                                              time_over_two_pi, tw, xch);
+    row_block_split<N, F, TEAMS, KX, MODE, R, W>(L, xcd_group(blockIdx.x, gridDim.x),
+                                                 time_over_two_pi, tw, xch);
 }
 
-template <int N, int F, int C, int KX, int MODE, int R>
+template <int N, int F, int C, int KX, int MODE, int R, int W>
 __global__ __launch_bounds__((C * (F / 2) * FftShape<N, R>::T)) void col_split_kernel(ocean_hip_level L)
 {
     __shared__ float2 tw[N / 2];
     __shared__ float2 xch[C * (F / 2) * KX * team_stride<N>()];
     __shared__ float2 xfer[C * 2 * N];
-    col_block_split<N, F, C, KX, MODE, R>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch, xfer);
+    col_block_split<N, F, C, KX, MODE, R, W>(L, xcd_group(blockIdx.x, gridDim.x), tw, xch, xfer);
 }
 
 // Launch shapes of the per-level passes. Variant 0 is the default shape;
@@ -160,7 +162,7 @@ void launch_on(const Launch& k, Kernel kernel, dim3 grid, dim3 block, Args... ar
         hipLaunchKernelGGL(kernel, grid, block, 0, k.s, args...);
 }
 
-template <int N, int F, int TEAMS, int KX, int MODE, int R = 8>
+template <int N, int F, int TEAMS, int KX, int MODE, int R = 8, int W = 0>
 bool try_row_split(Pick& p, const ocean_hip_level& L, double t2, const Launch& k)
 {
     if constexpr (!row_split_ok<N, F, TEAMS, KX, R>()) {
@@ -168,13 +170,13 @@ bool try_row_split(Pick& p, const ocean_hip_level& L, double t2, const Launch& k
     } else {
         if (p.seen++ != p.want) return false;
         if (k.s != nullptr || L.n != 0)
-            launch_on(k, row_split_kernel<N, F, TEAMS, KX, MODE, R>, dim3(N / TEAMS),
+            launch_on(k, row_split_kernel<N, F, TEAMS, KX, MODE, R, W>, dim3(N / TEAMS),
                       dim3(TEAMS * (F / 2) * FftShape<N, R>::T), L, t2);
         return true;
     }
 }
 
-template <int N, int F, int C, int KX, int MODE, int R = 8>
+template <int N, int F, int C, int KX, int MODE, int R = 8, int W = 0>
 bool try_col_split(Pick& p, const ocean_hip_level& L, const Launch& k)
 {
     if constexpr (!col_split_ok<N, F, C, KX, R>()) {
@@ -182,7 +184,7 @@ bool try_col_split(Pick& p, const ocean_hip_level& L, const Launch& k)
     } else {
         if (p.seen++ != p.want) return false;
         if (k.s != nullptr || L.n != 0)
-            launch_on(k, col_split_kernel<N, F, C, KX, MODE, R>, dim3(N / C),
+            launch_on(k, col_split_kernel<N, F, C, KX, MODE, R, W>, dim3(N / C),
                       dim3(C * (F / 2) * FftShape<N, R>::T), L);
         return true;
     }
@@ -190,7 +192,7 @@ bool try_col_split(Pick& p, const ocean_hip_level& L, const Launch& k)
 
 // The fast modes list only the default shape and the field-split shapes (the
 // ones the autotuner picks at N >= 512), which keeps their instantiations few.
-template <int N, int F, int TEAMS, int MODE>
+template <int N, int F, int TEAMS, int MODE, int W = 0>
 bool try_row(Pick& p, const ocean_hip_level& L, double t2, const Launch& k, bool is_default)
 {
     if constexpr ((!row_ok<N, TEAMS>() || MODE != kModeExact) && TEAMS != RowShape<N>::TEAMS) {
@@ -199,13 +201,13 @@ bool try_row(Pick& p, const ocean_hip_level& L, double t2, const Launch& k, bool
         if (!is_default && TEAMS == RowShape<N>::TEAMS) return false;  // listed first already
         if (p.seen++ != p.want) return false;
         if (k.s != nullptr || L.n != 0)
-            launch_on(k, row_pass_kernel<N, F, TEAMS, MODE, 8>, dim3(N / TEAMS),
+            launch_on(k, row_pass_kernel<N, F, TEAMS, MODE, 8, W>, dim3(N / TEAMS),
                       dim3(TEAMS * FftShape<N>::T), L, t2);
         return true;
     }
 }
 
-template <int N, int F, int C, int MODE>
+template <int N, int F, int C, int MODE, int W = 0>
 bool try_col(Pick& p, const ocean_hip_level& L, const Launch& k, bool is_default)
 {
     if constexpr ((!col_ok<N, C>() || MODE != kModeExact) && C != ColShape<N>::C) {
@@ -214,13 +216,13 @@ bool try_col(Pick& p, const ocean_hip_level& L, const Launch& k, bool is_default
         if (!is_default && C == ColShape<N>::C) return false;
         if (p.seen++ != p.want) return false;
         if (k.s != nullptr || L.n != 0)
-            launch_on(k, col_pass_kernel<N, F, C, MODE, 8>, dim3(N / C), dim3(C * FftShape<N>::T), L);
+            launch_on(k, col_pass_kernel<N, F, C, MODE, 8, W>, dim3(N / C), dim3(C * FftShape<N>::T), L);
         return true;
     }
 }
 
 // Other radix groupings of the unsplit passes (bit-exact mode).
-template <int N, int F, int TEAMS, int MODE, int R>
+template <int N, int F, int TEAMS, int MODE, int R, int W = 0>
 bool try_row_r(Pick& p, const ocean_hip_level& L, double t2, const Launch& k)
 {
     if constexpr (!row_ok<N, TEAMS, R>() || MODE != kModeExact) {
@@ -228,13 +230,13 @@ bool try_row_r(Pick& p, const ocean_hip_level& L, double t2, const Launch& k)
     } else {
         if (p.seen++ != p.want) return false;
         if (k.s != nullptr || L.n != 0)
-            launch_on(k, row_pass_kernel<N, F, TEAMS, MODE, R>, dim3(N / TEAMS),
+            launch_on(k, row_pass_kernel<N, F, TEAMS, MODE, R, W>, dim3(N / TEAMS),
                       dim3(TEAMS * FftShape<N, R>::T), L, t2);
         return true;
     }
 }
 
-template <int N, int F, int C, int MODE, int R>
+template <int N, int F, int C, int MODE, int R, int W = 0>
 bool try_col_r(Pick& p, const ocean_hip_level& L, const Launch& k)
 {
     if constexpr (!col_ok<N, C, R>() || MODE != kModeExact) {
@@ -242,68 +244,118 @@ bool try_col_r(Pick& p, const ocean_hip_level& L, const Launch& k)
     } else {
         if (p.seen++ != p.want) return false;
         if (k.s != nullptr || L.n != 0)
-            launch_on(k, col_pass_kernel<N, F, C, MODE, R>, dim3(N / C), dim3(C * FftShape<N, R>::T),
+            launch_on(k, col_pass_kernel<N, F, C, MODE, R, W>, dim3(N / C), dim3(C * FftShape<N, R>::T),
                       L);
         return true;
     }
 }
 
-template <int N, int F, int M>
+template <int N, int F, int M, int W = 0>
 bool row_variant(int v, const ocean_hip_level& L, double t2, const Launch& k)
 {
     Pick p{v, 0};
-    return try_row<N, F, RowShape<N>::TEAMS, M>(p, L, t2, k, true) ||
-           try_row<N, F, 1, M>(p, L, t2, k, false) || try_row<N, F, 2, M>(p, L, t2, k, false) ||
-           try_row<N, F, 4, M>(p, L, t2, k, false) || try_row<N, F, 8, M>(p, L, t2, k, false) ||
-           try_row<N, F, 16, M>(p, L, t2, k, false) || try_row<N, F, 32, M>(p, L, t2, k, false) ||
-           try_row_split<N, F, 1, 1, M>(p, L, t2, k) || try_row_split<N, F, 2, 1, M>(p, L, t2, k) ||
-           try_row_split<N, F, 4, 1, M>(p, L, t2, k) || try_row_split<N, F, 1, 2, M>(p, L, t2, k) ||
-           try_row_split<N, F, 2, 2, M>(p, L, t2, k) ||
+    return try_row<N, F, RowShape<N>::TEAMS, M, W>(p, L, t2, k, true) ||
+           try_row<N, F, 1, M, W>(p, L, t2, k, false) || try_row<N, F, 2, M, W>(p, L, t2, k, false) ||
+           try_row<N, F, 4, M, W>(p, L, t2, k, false) || try_row<N, F, 8, M, W>(p, L, t2, k, false) ||
+           try_row<N, F, 16, M, W>(p, L, t2, k, false) || try_row<N, F, 32, M, W>(p, L, t2, k, false) ||
+           try_row_split<N, F, 1, 1, M, 8, W>(p, L, t2, k) ||
+           try_row_split<N, F, 2, 1, M, 8, W>(p, L, t2, k) ||
+           try_row_split<N, F, 4, 1, M, 8, W>(p, L, t2, k) ||
+           try_row_split<N, F, 1, 2, M, 8, W>(p, L, t2, k) ||
+           try_row_split<N, F, 2, 2, M, 8, W>(p, L, t2, k) ||
            // 4 points per lane: twice the lanes, stages in pairs
-           try_row_r<N, F, 2, M, 4>(p, L, t2, k) || try_row_r<N, F, 4, M, 4>(p, L, t2, k) ||
-           try_row_split<N, F, 1, 1, M, 4>(p, L, t2, k) ||
-           try_row_split<N, F, 2, 1, M, 4>(p, L, t2, k) ||
-           try_row_split<N, F, 1, 2, M, 4>(p, L, t2, k) ||
+           try_row_r<N, F, 2, M, 4, W>(p, L, t2, k) || try_row_r<N, F, 4, M, 4, W>(p, L, t2, k) ||
+           try_row_split<N, F, 1, 1, M, 4, W>(p, L, t2, k) ||
+           try_row_split<N, F, 2, 1, M, 4, W>(p, L, t2, k) ||
+           try_row_split<N, F, 1, 2, M, 4, W>(p, L, t2, k) ||
            // 16 points per lane: half the lanes, stages in fours
-           try_row_split<N, F, 1, 1, M, 16>(p, L, t2, k) ||
-           try_row_split<N, F, 2, 1, M, 16>(p, L, t2, k);
+           try_row_split<N, F, 1, 1, M, 16, W>(p, L, t2, k) ||
+           try_row_split<N, F, 2, 1, M, 16, W>(p, L, t2, k);
 }
 
-template <int N, int F, int M>
+template <int N, int F, int M, int W = 0>
 bool col_variant(int v, const ocean_hip_level& L, const Launch& k)
 {
     Pick p{v, 0};
-    return try_col<N, F, ColShape<N>::C, M>(p, L, k, true) || try_col<N, F, 1, M>(p, L, k, false) ||
-           try_col<N, F, 2, M>(p, L, k, false) || try_col<N, F, 4, M>(p, L, k, false) ||
-           try_col<N, F, 8, M>(p, L, k, false) || try_col<N, F, 16, M>(p, L, k, false) ||
-           try_col<N, F, 32, M>(p, L, k, false) || try_col_split<N, F, 1, 1, M>(p, L, k) ||
-           try_col_split<N, F, 2, 1, M>(p, L, k) || try_col_split<N, F, 4, 1, M>(p, L, k) ||
-           try_col_split<N, F, 1, 2, M>(p, L, k) || try_col_split<N, F, 2, 2, M>(p, L, k) ||
-           try_col_r<N, F, 2, M, 4>(p, L, k) || try_col_r<N, F, 4, M, 4>(p, L, k) ||
-           try_col_split<N, F, 1, 1, M, 4>(p, L, k) || try_col_split<N, F, 2, 1, M, 4>(p, L, k) ||
-           try_col_split<N, F, 1, 2, M, 4>(p, L, k) ||
-           try_col_split<N, F, 1, 1, M, 16>(p, L, k) || try_col_split<N, F, 2, 1, M, 16>(p, L, k);
+    return try_col<N, F, ColShape<N>::C, M, W>(p, L, k, true) ||
+           try_col<N, F, 1, M, W>(p, L, k, false) || try_col<N, F, 2, M, W>(p, L, k, false) ||
+           try_col<N, F, 4, M, W>(p, L, k, false) || try_col<N, F, 8, M, W>(p, L, k, false) ||
+           try_col<N, F, 16, M, W>(p, L, k, false) || try_col<N, F, 32, M, W>(p, L, k, false) ||
+           try_col_split<N, F, 1, 1, M, 8, W>(p, L, k) || try_col_split<N, F, 2, 1, M, 8, W>(p, L, k) ||
+           try_col_split<N, F, 4, 1, M, 8, W>(p, L, k) || try_col_split<N, F, 1, 2, M, 8, W>(p, L, k) ||
+           try_col_split<N, F, 2, 2, M, 8, W>(p, L, k) ||
+           try_col_r<N, F, 2, M, 4, W>(p, L, k) || try_col_r<N, F, 4, M, 4, W>(p, L, k) ||
+           try_col_split<N, F, 1, 1, M, 4, W>(p, L, k) || try_col_split<N, F, 2, 1, M, 4, W>(p, L, k) ||
+           try_col_split<N, F, 1, 2, M, 4, W>(p, L, k) ||
+           try_col_split<N, F, 1, 1, M, 16, W>(p, L, k) ||
+           try_col_split<N, F, 2, 1, M, 16, W>(p, L, k);
+}
+
+// Layouts of the fields between the passes a level may use (bits 24-31 of
+// its shape): 0 row-major; 1, 2, 3 tiles of 2, 4, 8 columns (bit-exact mode,
+// N >= 256).
+constexpr int kLayouts = 4;
+constexpr int tile_width(int layout) { return layout == 0 ? 0 : (1 << layout); }
+template <int N, int M>
+constexpr bool tiled_ok()
+{
+    return M == kModeExact && N >= 256;
 }
 
 // Number of variants of each pass (counted without launching: a null level
 // view with n = 0 and no stream).
-template <int N, int F, int M>
+template <int N, int F, int M, int W>
 void count_variants_f(int& rows, int& cols)
 {
     ocean_hip_level none{};
     rows = 0;
-    while (row_variant<N, F, M>(rows, none, 0.0, Launch{})) ++rows;
+    while (row_variant<N, F, M, W>(rows, none, 0.0, Launch{})) ++rows;
     cols = 0;
-    while (col_variant<N, F, M>(cols, none, Launch{})) ++cols;
+    while (col_variant<N, F, M, W>(cols, none, Launch{})) ++cols;
+}
+
+// false: the layout is not available for (N, M).
+template <int N, int M, int W>
+void count_variants_w(int fields, int& rows, int& cols)
+{
+    if (fields == 6)
+        count_variants_f<N, 6, M, W>(rows, cols);
+    else
+        count_variants_f<N, 4, M, W>(rows, cols);
 }
 
 template <int N, int M>
-void count_variants(int fields, int& rows, int& cols)
+bool count_variants(int fields, int& rows, int& cols, int layout = 0)
 {
-    if (fields == 6)
-        count_variants_f<N, 6, M>(rows, cols);
-    else
-        count_variants_f<N, 4, M>(rows, cols);
+    if (layout == 0) {
+        count_variants_w<N, M, 0>(fields, rows, cols);
+        return true;
+    }
+    if constexpr (!tiled_ok<N, M>()) {
+        return false;
+    } else {
+        switch (layout) {
+            case 1: count_variants_w<N, M, tile_width(1)>(fields, rows, cols); return true;
+            case 2: count_variants_w<N, M, tile_width(2)>(fields, rows, cols); return true;
+            case 3: count_variants_w<N, M, tile_width(3)>(fields, rows, cols); return true;
+            default: return false;
+        }
+    }
+}
+
+template <int N, int M, int W>
+bool launch_passes(const ocean_hip_level* L, int rv, int cv, double t2, unsigned passes,
+                   const Launch& row, const Launch& col)
+{
+    bool ok = true;
+    if (L->vel != nullptr) {
+        if (passes & 1u) ok = ok && row_variant<N, 6, M, W>(rv, *L, t2, row);
+        if (passes & 2u) ok = ok && col_variant<N, 6, M, W>(cv, *L, col);
+    } else {
+        if (passes & 1u) ok = ok && row_variant<N, 4, M, W>(rv, *L, t2, row);
+        if (passes & 2u) ok = ok && col_variant<N, 4, M, W>(cv, *L, col);
+    }
+    return ok;
 }
 
 template <int N, int M>
@@ -313,13 +365,17 @@ hipError_t launch_frame_n(const ocean_hip_level* L, double time_over_two_pi, hip
     const int rv = int(L->shape & 0xffu), cv = int((L->shape >> 8) & 0xffu);
     const Launch row{s, events ? events[0] : nullptr, events ? events[1] : nullptr};
     const Launch col{s, events ? events[2] : nullptr, events ? events[3] : nullptr};
-    bool ok = true;
-    if (L->vel != nullptr) {
-        if (passes & 1u) ok = ok && row_variant<N, 6, M>(rv, *L, time_over_two_pi, row);
-        if (passes & 2u) ok = ok && col_variant<N, 6, M>(cv, *L, col);
-    } else {
-        if (passes & 1u) ok = ok && row_variant<N, 4, M>(rv, *L, time_over_two_pi, row);
-        if (passes & 2u) ok = ok && col_variant<N, 4, M>(cv, *L, col);
+    const int layout = int((L->shape >> 24) & 0xffu);
+    bool ok = false;
+    if (layout == 0) {
+        ok = launch_passes<N, M, 0>(L, rv, cv, time_over_two_pi, passes, row, col);
+    } else if constexpr (tiled_ok<N, M>()) {
+        switch (layout) {
+            case 1: ok = launch_passes<N, M, tile_width(1)>(L, rv, cv, time_over_two_pi, passes, row, col); break;
+            case 2: ok = launch_passes<N, M, tile_width(2)>(L, rv, cv, time_over_two_pi, passes, row, col); break;
+            case 3: ok = launch_passes<N, M, tile_width(3)>(L, rv, cv, time_over_two_pi, passes, row, col); break;
+            default: break;
+        }
     }
     if (!ok) return hipErrorInvalidValue;
     return hipGetLastError();
@@ -345,21 +401,22 @@ hipError_t launch_frame_mode(const ocean_hip_level* L, double t2, hipStream_t s,
 }
 
 template <int M>
-int frame_shapes_mode(uint32_t n, int f, int& r, int& c)
+int frame_shapes_mode(uint32_t n, int f, int& r, int& c, int layout = 0)
 {
+    bool ok = false;
     switch (n) {
-        case 16: count_variants<16, M>(f, r, c); break;
-        case 32: count_variants<32, M>(f, r, c); break;
-        case 64: count_variants<64, M>(f, r, c); break;
-        case 128: count_variants<128, M>(f, r, c); break;
-        case 256: count_variants<256, M>(f, r, c); break;
-        case 512: count_variants<512, M>(f, r, c); break;
-        case 1024: count_variants<1024, M>(f, r, c); break;
-        case 2048: count_variants<2048, M>(f, r, c); break;
-        case 4096: count_variants<4096, M>(f, r, c); break;
+        case 16: ok = count_variants<16, M>(f, r, c, layout); break;
+        case 32: ok = count_variants<32, M>(f, r, c, layout); break;
+        case 64: ok = count_variants<64, M>(f, r, c, layout); break;
+        case 128: ok = count_variants<128, M>(f, r, c, layout); break;
+        case 256: ok = count_variants<256, M>(f, r, c, layout); break;
+        case 512: ok = count_variants<512, M>(f, r, c, layout); break;
+        case 1024: ok = count_variants<1024, M>(f, r, c, layout); break;
+        case 2048: ok = count_variants<2048, M>(f, r, c, layout); break;
+        case 4096: ok = count_variants<4096, M>(f, r, c, layout); break;
         default: return -1;
     }
-    return 0;
+    return ok ? 0 : -1;
 }
 
 // ---- batched persistent passes (N in [64, 2048], 256-lane blocks) --------

@@ -162,6 +162,11 @@ void* ocean_hip_batch_kernel_fast_2(uint32_t n, int fields, int row, int per_ite
 /* Variant counts of the per-level passes of one precision mode. */
 int ocean_hip_frame_shapes_mode(uint32_t n, uint32_t fields, uint32_t mode, int* row_variants,
                                 int* col_variants);
+/* The same for a layout of the fields between the passes (a level's shape
+ * bits 24-31): 0 row-major, 1 / 2 / 3 tiles of 2 / 4 / 8 columns x N rows
+ * (bit-exact mode, N >= 256); -1 where the layout is not available. */
+int ocean_hip_frame_shapes_layout(uint32_t n, uint32_t fields, uint32_t mode, uint32_t layout,
+                                  int* row_variants, int* col_variants);
 /* Direct launches of one group (passes: 1 row, 2 column, 3 both), for tuning. */
 hipError_t ocean_hip_batch_group_frame(const ocean_hip_batch_group* group,
                                        double time_over_two_pi, unsigned passes, hipStream_t s);

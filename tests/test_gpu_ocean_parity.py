@@ -205,34 +205,44 @@ def test_queries_bitexact(lib):
 @pytest.mark.parametrize("n", [32, 64, 128, 256, 512, 1024, 2048, 4096])
 def test_every_launch_shape_is_bitexact(lib, n):
     """Each launch-shape variant of both passes (rows / columns per
-    workgroup, field split, 4 / 8 / 16 FFT points per lane) gives the
-    oracle's bits; the autotuner's pick does too."""
+    workgroup, field split, 4 / 8 / 16 FFT points per lane), in each layout
+    of the fields between the passes (row-major; tiles of 2, 4, 8 columns
+    from N = 256), gives the oracle's bits; the autotuner's pick does too."""
     vel = 1 if n in (256, 512, 1024) else 0
     d = capi.default_desc(lib, size=n, patch_length=float(n) * 0.7, wind_speed=17.0, seed=n + 3,
                           choppiness=1.2, compute_velocity=vel, water_depth=30.0 if vel else 0.0)
     g, o = GpuOcean(lib, d), OracleOcean(d)
     rows, cols = C.c_uint32(), C.c_uint32()
-    assert lib.ocean_amd_frame_shapes(g.h, C.byref(rows), C.byref(cols)) == 0
-    assert rows.value >= 1 and cols.value >= 1
     try:
         t = 4.25
         o.update(t)
         want = (o.displacement(), o.normal(), o.velocity())
-        shapes = [(r, 0) for r in range(rows.value)] + [(0, c) for c in range(1, cols.value)]
-        for r, c in shapes:
-            assert lib.ocean_amd_set_frame_shape(g.h, r, c) == 0
-            g.update(t + 1.0)  # a different frame first, so stale buffers cannot pass
-            g.update(t)
-            dd, nr, v = g.buffers()
-            assert_bitexact(dd, want[0], f"n={n} shape=({r},{c}) displacement")
-            assert_bitexact(nr, want[1], f"n={n} shape=({r},{c}) normal")
-            if vel:
-                assert_bitexact(v, want[2], f"n={n} shape=({r},{c}) velocity")
-        assert lib.ocean_amd_set_frame_shape(g.h, rows.value, 0) != 0
+        layouts = [0] + ([1, 2, 3] if n >= 256 else [])
+        assert lib.ocean_amd_set_frame_layout(g.h, 1) == (0 if n >= 256 else capi.OCEAN_ERROR_INVALID_ARG)
+        assert lib.ocean_amd_set_frame_layout(g.h, 4) == capi.OCEAN_ERROR_INVALID_ARG
+        for layout in layouts:
+            assert lib.ocean_amd_set_frame_layout(g.h, layout) == 0
+            assert lib.ocean_amd_get_frame_layout(g.h) == layout
+            assert lib.ocean_amd_frame_shapes(g.h, C.byref(rows), C.byref(cols)) == 0
+            assert rows.value >= 1 and cols.value >= 1
+            shapes = [(r, 0) for r in range(rows.value)] + [(0, c) for c in range(1, cols.value)]
+            for r, c in shapes:
+                assert lib.ocean_amd_set_frame_shape(g.h, r, c) == 0
+                g.update(t + 1.0)  # a different frame first, so stale buffers cannot pass
+                g.update(t)
+                dd, nr, v = g.buffers()
+                what = f"n={n} layout={layout} shape=({r},{c})"
+                assert_bitexact(dd, want[0], f"{what} displacement")
+                assert_bitexact(nr, want[1], f"{what} normal")
+                if vel:
+                    assert_bitexact(v, want[2], f"{what} velocity")
+            assert lib.ocean_amd_set_frame_shape(g.h, rows.value, 0) != 0
         rv, cv = C.c_uint32(), C.c_uint32()
         rms, cms = C.c_float(), C.c_float()
         assert lib.ocean_amd_autotune_frame(g.h, 3, C.byref(rv), C.byref(cv), C.byref(rms),
                                             C.byref(cms)) == 0
+        assert lib.ocean_amd_get_frame_layout(g.h) in layouts
+        assert lib.ocean_amd_frame_shapes(g.h, C.byref(rows), C.byref(cols)) == 0
         assert rv.value < rows.value and cv.value < cols.value and rms.value > 0 and cms.value > 0
         g.update(t)
         dd, nr, _ = g.buffers()

@@ -633,9 +633,9 @@ def main():
                     help="skip the converged / learned height query leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched leg: keep the default grid sizes")
-    ap.add_argument("--frame-shape", default=None, metavar="ROW,COL",
-                    help="fix the frame kernels' launch shapes instead of autotuning "
-                         "(counter runs, where timing is perturbed)")
+    ap.add_argument("--frame-shape", default=None, metavar="[LAYOUT:]ROW,COL",
+                    help="fix the frame kernels' field layout and launch shapes instead of "
+                         "autotuning (counter runs, where timing is perturbed)")
     ap.add_argument("--queries-only", action="store_true",
                     help="run only the converged / learned height query leg (for rocprofv3)")
     ap.add_argument("--scene-frames", type=int, default=600,
@@ -683,17 +683,24 @@ def main():
     shape_r, shape_c = C.c_uint32(), C.c_uint32()
     tune_r, tune_c = C.c_float(), C.c_float()
     if args.frame_shape:
-        shape_r.value, shape_c.value = (int(v) for v in args.frame_shape.split(","))
-        if lib.ocean_amd_set_frame_shape(sim, shape_r.value, shape_c.value) != 0:
+        spec = args.frame_shape
+        layout = 0
+        if ":" in spec:
+            layout, spec = int(spec.split(":")[0]), spec.split(":")[1]
+        shape_r.value, shape_c.value = (int(v) for v in spec.split(","))
+        if (lib.ocean_amd_set_frame_layout(sim, layout) != 0 or
+                lib.ocean_amd_set_frame_shape(sim, shape_r.value, shape_c.value) != 0):
             raise SystemExit(f"bad --frame-shape {args.frame_shape}")
-        log(f"bench: frame shapes row {shape_r.value} col {shape_c.value} (fixed)")
+        log(f"bench: frame layout {layout}, shapes row {shape_r.value} col {shape_c.value} (fixed)")
     else:
         if lib.ocean_amd_autotune_frame(sim, 20, C.byref(shape_r), C.byref(shape_c),
                                         C.byref(tune_r), C.byref(tune_c)) != 0:
             raise SystemExit("frame autotune failed")
-        log(f"bench: frame shapes row {shape_r.value} col {shape_c.value} "
+        log(f"bench: frame layout {lib.ocean_amd_get_frame_layout(sim)}, "
+            f"shapes row {shape_r.value} col {shape_c.value} "
             f"({tune_r.value * 1e3:.1f} / {tune_c.value * 1e3:.1f} us)")
 
+    frame_layout = int(lib.ocean_amd_get_frame_layout(sim))
     total = args.warmup + args.steps
     frames_xz = 64
     xz_host = query_points(frames_xz, QUERIES_PER_FRAME, desc.patch_length)
@@ -799,6 +806,7 @@ def main():
                       "row_ms": round(fr_ms.value, 5), "col_ms": round(fc_ms.value, 5),
                       "frame_shape": [fr_.value, fc_.value]}
     assert lib.ocean_amd_set_precision(sim, 0) == 0
+    assert lib.ocean_amd_set_frame_layout(sim, frame_layout) == 0
     assert lib.ocean_amd_set_frame_shape(sim, shape_r.value, shape_c.value) == 0
     if lib.ocean_amd_last_error(sim) != 0:
         raise SystemExit(f"device error {lib.ocean_amd_last_error(sim)}")
@@ -907,7 +915,8 @@ def main():
         "kernels": kernels,
         "update_device_ms": round(row_ms.value + col_ms.value, 5),
         "frames_per_s_separate_calls": round(separate_per_s, 1),
-        "frame_shape": {"row_variant": shape_r.value, "col_variant": shape_c.value},
+        "frame_shape": {"layout": frame_layout, "row_variant": shape_r.value,
+                        "col_variant": shape_c.value},
         "height_at_queries_per_s": round(qps, 1),
         "height_at_cpu_baseline": cpu_queries,
         "fast_mode": fast,

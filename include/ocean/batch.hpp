@@ -29,6 +29,8 @@ struct BatchGroupInfo {
     float col_ms = 0.0f;
     std::uint32_t radix_row = 8;    // FFT points per lane of each pass (8, or 4:
     std::uint32_t radix_col = 8;    // stages in pairs over twice the lanes)
+    std::uint32_t layout = 0;       // fields between the passes: 0 row-major,
+                                    // 1 tiles as wide as an item's columns
 };
 
 class OceanBatch {
@@ -68,12 +70,16 @@ public:
     [[nodiscard]] std::uint32_t precision() const noexcept;
     [[nodiscard]] std::size_t group_count() const noexcept;
     [[nodiscard]] BatchGroupInfo group(std::size_t g) const noexcept;
-    // Sets group g's launch choices by hand (what autotune() picks): the FFT
-    // points per lane of each pass (8, or 4 for N <= 1024) and each pass's
-    // grid, 1 .. its block units (the group's items, twice that with 4 points
-    // per lane). False, and nothing changed, for a choice that does not exist.
-    bool set_group_launch(std::size_t g, std::uint32_t radix_row, std::uint32_t grid_row,
-                          std::uint32_t radix_col, std::uint32_t grid_col) noexcept;
+    // Sets group g's launch choices by hand (what autotune() picks): the
+    // layout of the fields between the passes (0 row-major; 1 tiles as wide
+    // as a work item's 2048/N columns, bit-exact mode, N = 256 .. 1024), the
+    // FFT points per lane of each pass (8, or 4 for N <= 1024) and each
+    // pass's grid, 1 .. its block units (the group's items, twice that with 4
+    // points per lane). False, and nothing changed, for a choice that does
+    // not exist.
+    bool set_group_launch(std::size_t g, std::uint32_t layout, std::uint32_t radix_row,
+                          std::uint32_t grid_row, std::uint32_t radix_col,
+                          std::uint32_t grid_col) noexcept;
 
     // Checksums of levels [first, first+count) as they stand after the work
     // queued so far (blocks): per level, its buffers (displacement, normal,

@@ -243,6 +243,8 @@ typedef struct ocean_amd_batch_group {
     float    col_ms;
     uint32_t radix_row;   /* FFT points per lane of each pass, as tuned:  */
     uint32_t radix_col;   /* 8, or 4 (stages in pairs, twice the lanes)   */
+    uint32_t layout;      /* fields between the passes: 0 row-major, 1    */
+                          /* tiles as wide as a work item's columns       */
 } ocean_amd_batch_group;
 
 ocean_amd_batch* ocean_amd_batch_create(const ocean_desc* levels, uint32_t count,
@@ -274,13 +276,16 @@ uint32_t      ocean_amd_batch_group_count(const ocean_amd_batch* batch);
 int           ocean_amd_batch_group_info(const ocean_amd_batch* batch, uint32_t group,
                                          ocean_amd_batch_group* out);
 /* Group `group`'s launch choices by hand (what ocean_amd_batch_autotune
- * picks): FFT points per lane of each pass (8, or 4 for N <= 1024) and each
- * pass's grid, 1 .. the group's items (twice that with 4 points per lane).
+ * picks): the layout of the fields between the passes (0 row-major; 1 tiles
+ * as wide as a work item's 2048/N columns, bit-exact mode, N = 256 .. 1024),
+ * FFT points per lane of each pass (8, or 4 for N <= 1024) and each pass's
+ * grid, 1 .. the group's items (twice that with 4 points per lane).
  * OCEAN_ERROR_INVALID_ARG, nothing changed, for a choice that does not exist.
  * Results do not depend on these choices. */
 int           ocean_amd_batch_set_group_launch(ocean_amd_batch* batch, uint32_t group,
-                                               uint32_t radix_row, uint32_t grid_row,
-                                               uint32_t radix_col, uint32_t grid_col);
+                                               uint32_t layout, uint32_t radix_row,
+                                               uint32_t grid_row, uint32_t radix_col,
+                                               uint32_t grid_col);
 /* Checksums of levels [first, first + count) after the work queued so far
  * (blocks until done): per level its buffers (displacement, normal, then
  * velocity when on) as uint32 words w_i, i counted across the buffers, summed

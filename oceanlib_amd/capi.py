@@ -136,7 +136,7 @@ class BatchGroup(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in
                 ("size", "velocity", "levels", "items", "grid_row", "grid_col")] + \
                [("row_ms", C.c_float), ("col_ms", C.c_float), ("radix_row", C.c_uint32),
-                ("radix_col", C.c_uint32)]
+                ("radix_col", C.c_uint32), ("layout", C.c_uint32)]
 
 
 PROPOSAL_PARAMS = 9360
@@ -251,7 +251,7 @@ AMD_SIGNATURES = {
     "ocean_amd_batch_set_precision": (C.c_int, [_P, C.c_uint32]),
     "ocean_amd_batch_group_count": (C.c_uint32, [_P]),
     "ocean_amd_batch_group_info": (C.c_int, [_P, C.c_uint32, C.POINTER(BatchGroup)]),
-    "ocean_amd_batch_set_group_launch": (C.c_int, [_P] + [C.c_uint32] * 5),
+    "ocean_amd_batch_set_group_launch": (C.c_int, [_P] + [C.c_uint32] * 6),
     "ocean_amd_batch_checksums": (C.c_int, [_P, C.c_uint32, C.c_uint32,
                                             C.POINTER(C.c_uint64)]),
     "ocean_amd_batch_synchronize": (C.c_int, [_P]),

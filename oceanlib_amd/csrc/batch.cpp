@@ -228,7 +228,16 @@ bool OceanBatch::set_precision(std::uint32_t mode) noexcept
     if (mode > 2u) return false;
     m.levels.front().device().make_current();
     m.note(hipStreamSynchronize(m.stream.s));  // the graph may still be running
-    for (ocean_hip_batch_group& g : m.groups) g.mode = mode;
+    for (std::size_t i = 0; i < m.groups.size(); ++i) {
+        ocean_hip_batch_group& g = m.groups[i];
+        g.mode = mode;
+        // a layout the mode's kernels lack falls back to row-major
+        if (!ocean_hip_batch_has_launch(g.n, g.fields, mode, g.radix_row, g.layout) ||
+            !ocean_hip_batch_has_launch(g.n, g.fields, mode, g.radix_col, g.layout)) {
+            g.layout = 0;
+            m.info[i].layout = 0;
+        }
+    }
     for (std::size_t i : m.loose) m.levels[i].device().level.shape = mode << 16;
     try {
         m.rebuild_graph();
@@ -242,21 +251,24 @@ bool OceanBatch::set_precision(std::uint32_t mode) noexcept
 
 std::uint32_t OceanBatch::precision() const noexcept { return impl_->mode; }
 
-bool OceanBatch::set_group_launch(std::size_t g, std::uint32_t radix_row, std::uint32_t grid_row,
-                                  std::uint32_t radix_col, std::uint32_t grid_col) noexcept
+bool OceanBatch::set_group_launch(std::size_t g, std::uint32_t layout, std::uint32_t radix_row,
+                                  std::uint32_t grid_row, std::uint32_t radix_col,
+                                  std::uint32_t grid_col) noexcept
 {
     Impl& m = *impl_;
     if (g >= m.groups.size()) return false;
     ocean_hip_batch_group& gr = m.groups[g];
     auto ok = [&](std::uint32_t radix, std::uint32_t grid) {
         if (radix != 8u && radix != 4u) return false;
-        if (!ocean_hip_batch_has_radix(gr.n, gr.fields, gr.mode, radix)) return false;
+        if (!ocean_hip_batch_has_launch(gr.n, gr.fields, gr.mode, radix, layout)) return false;
         const std::uint32_t units = radix == 4u ? 2u * gr.n_items : gr.n_items;
         return grid >= 1u && grid <= units;
     };
     if (!ok(radix_row, grid_row) || !ok(radix_col, grid_col)) return false;
     m.levels.front().device().make_current();
     m.note(hipStreamSynchronize(m.stream.s));  // the graph may still be running
+    gr.layout = layout;
+    m.info[g].layout = layout;
     gr.radix_row = radix_row;
     gr.grid_row = grid_row;
     gr.radix_col = radix_col;
@@ -365,60 +377,83 @@ void OceanBatch::autotune(std::uint32_t reps) noexcept
         return;
     }
     const double t2 = m.time * kInvTwoPiD;
-    // Per group and pass: the FFT's radix grouping (8 points per lane, or 4
-    // with each item in two blocks) and the persistent grid size, timed on
-    // the device; every choice computes the same bits.
+    // Per group: for each layout of the fields between the passes, per pass
+    // the FFT's radix grouping (8 points per lane, or 4 with each item in
+    // two blocks) and the persistent grid size, timed on the device; the
+    // layout with the smallest row + column time is kept. Every choice
+    // computes the same bits.
     for (std::size_t g = 0; g < m.groups.size(); ++g) {
         ocean_hip_batch_group& gr = m.groups[g];
-        for (int pass = 0; pass < 2; ++pass) {
-            std::uint32_t best = pass == 0 ? gr.grid_row : gr.grid_col;
-            std::uint32_t best_radix = pass == 0 ? gr.radix_row : gr.radix_col;
-            float best_ms = 1e30f;
-            for (std::uint32_t radix : {8u, 4u}) {
-                if (!ocean_hip_batch_has_radix(gr.n, gr.fields, gr.mode, radix)) continue;
-                const std::uint32_t units = radix == 4u ? 2u * gr.n_items : gr.n_items;
-                std::vector<std::uint32_t> cand;
-                for (int k : {1, 2, 3, 4, 6, 8, 12, 16}) {
-                    const std::uint32_t c = whole_xcds(std::uint32_t(m.cus) * std::uint32_t(k));
-                    if (c < units) cand.push_back(c);
-                }
-                cand.push_back(units);  // one block per unit
-                for (std::uint32_t c : cand) {
-                    ocean_hip_batch_group t = gr;
-                    t.grid_row = c;
-                    t.grid_col = c;
-                    t.radix_row = radix;
-                    t.radix_col = radix;
-                    m.note(ocean_hip_batch_group_frame(&t, t2, 1u << pass, m.stream.s));  // warm
-                    hipEventRecord(e0, m.stream.s);
-                    for (std::uint32_t r = 0; r < reps; ++r)
-                        m.note(ocean_hip_batch_group_frame(&t, t2, 1u << pass, m.stream.s));
-                    hipEventRecord(e1, m.stream.s);
-                    hipEventSynchronize(e1);
-                    float ms = 0.0f;
-                    hipEventElapsedTime(&ms, e0, e1);
-                    ms /= static_cast<float>(reps);
-                    if (ms < best_ms) {
-                        best_ms = ms;
-                        best = c;
-                        best_radix = radix;
+        float best_total = 1e30f;
+        ocean_hip_batch_group chosen = gr;
+        float chosen_ms[2] = {0.0f, 0.0f};
+        for (std::uint32_t layout : {0u, 1u}) {
+            if (!ocean_hip_batch_has_launch(gr.n, gr.fields, gr.mode, 8u, layout)) continue;
+            ocean_hip_batch_group tuned = gr;
+            tuned.layout = layout;
+            float pass_ms[2] = {1e30f, 1e30f};
+            for (int pass = 0; pass < 2; ++pass) {
+                std::uint32_t best = pass == 0 ? gr.grid_row : gr.grid_col;
+                std::uint32_t best_radix = 8u;
+                for (std::uint32_t radix : {8u, 4u}) {
+                    if (!ocean_hip_batch_has_launch(gr.n, gr.fields, gr.mode, radix, layout)) continue;
+                    const std::uint32_t units = radix == 4u ? 2u * gr.n_items : gr.n_items;
+                    std::vector<std::uint32_t> cand;
+                    for (int k : {1, 2, 3, 4, 6, 8, 12, 16}) {
+                        const std::uint32_t c = whole_xcds(std::uint32_t(m.cus) * std::uint32_t(k));
+                        if (c < units) cand.push_back(c);
+                    }
+                    cand.push_back(units);  // one block per unit
+                    for (std::uint32_t c : cand) {
+                        ocean_hip_batch_group t = tuned;
+                        t.grid_row = c;
+                        t.grid_col = c;
+                        t.radix_row = radix;
+                        t.radix_col = radix;
+                        // the row pass writes the layout the column pass reads
+                        if (pass == 1) m.note(ocean_hip_batch_group_frame(&tuned, t2, 1u, m.stream.s));
+                        m.note(ocean_hip_batch_group_frame(&t, t2, 1u << pass, m.stream.s));  // warm
+                        hipEventRecord(e0, m.stream.s);
+                        for (std::uint32_t r = 0; r < reps; ++r)
+                            m.note(ocean_hip_batch_group_frame(&t, t2, 1u << pass, m.stream.s));
+                        hipEventRecord(e1, m.stream.s);
+                        hipEventSynchronize(e1);
+                        float ms = 0.0f;
+                        hipEventElapsedTime(&ms, e0, e1);
+                        ms /= static_cast<float>(reps);
+                        if (ms < pass_ms[pass]) {
+                            pass_ms[pass] = ms;
+                            best = c;
+                            best_radix = radix;
+                        }
                     }
                 }
+                if (pass == 0) {
+                    tuned.grid_row = best;
+                    tuned.radix_row = best_radix;
+                } else {
+                    tuned.grid_col = best;
+                    tuned.radix_col = best_radix;
+                }
             }
-            if (pass == 0) {
-                gr.grid_row = best;
-                gr.radix_row = best_radix;
-                m.info[g].grid_row = best;
-                m.info[g].radix_row = best_radix;
-                m.info[g].row_ms = best_ms;
-            } else {
-                gr.grid_col = best;
-                gr.radix_col = best_radix;
-                m.info[g].grid_col = best;
-                m.info[g].radix_col = best_radix;
-                m.info[g].col_ms = best_ms;
+            if (pass_ms[0] + pass_ms[1] < best_total) {
+                best_total = pass_ms[0] + pass_ms[1];
+                chosen = tuned;
+                chosen_ms[0] = pass_ms[0];
+                chosen_ms[1] = pass_ms[1];
             }
         }
+        gr = chosen;
+        BatchGroupInfo& bi = m.info[g];
+        bi.layout = gr.layout;
+        bi.grid_row = gr.grid_row;
+        bi.radix_row = gr.radix_row;
+        bi.row_ms = chosen_ms[0];
+        bi.grid_col = gr.grid_col;
+        bi.radix_col = gr.radix_col;
+        bi.col_ms = chosen_ms[1];
+        // the levels' fields and outputs at the batch's time, in the chosen layout
+        m.note(ocean_hip_batch_group_frame(&gr, t2, 3u, m.stream.s));
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);

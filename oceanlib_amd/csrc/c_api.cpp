@@ -827,13 +827,15 @@ int ocean_amd_batch_group_info(const ocean_amd_batch* batch, uint32_t group,
     out->col_ms = g.col_ms;
     out->radix_row = g.radix_row;
     out->radix_col = g.radix_col;
+    out->layout = g.layout;
     return OCEAN_OK;
 }
-int ocean_amd_batch_set_group_launch(ocean_amd_batch* batch, uint32_t group, uint32_t radix_row,
-                                     uint32_t grid_row, uint32_t radix_col, uint32_t grid_col)
+int ocean_amd_batch_set_group_launch(ocean_amd_batch* batch, uint32_t group, uint32_t layout,
+                                     uint32_t radix_row, uint32_t grid_row, uint32_t radix_col,
+                                     uint32_t grid_col)
 {
     if (batch == nullptr || group >= to_batch(batch)->group_count()) return OCEAN_ERROR_INVALID_ARG;
-    return to_batch(batch)->set_group_launch(group, radix_row, grid_row, radix_col, grid_col)
+    return to_batch(batch)->set_group_launch(group, layout, radix_row, grid_row, radix_col, grid_col)
                ? OCEAN_OK
                : OCEAN_ERROR_INVALID_ARG;
 }

@@ -21,15 +21,16 @@ namespace {
 // The batched kernel of a group's mode (the fast modes' kernels live in
 // frame_fast_kernels.hip).
 void* batch_kernel(uint32_t n, int fields, bool row, bool per_item = false, uint32_t mode = 0,
-                   uint32_t radix = 8)
+                   uint32_t radix = 8, uint32_t layout = 0)
 {
     const int r = radix == 0u ? 8 : int(radix);
-    if (mode == 1u)
-        return ocean_hip_batch_kernel_fast_1(n, fields, row ? 1 : 0, per_item ? 1 : 0, r);
-    if (mode == 2u)
-        return ocean_hip_batch_kernel_fast_2(n, fields, row ? 1 : 0, per_item ? 1 : 0, r);
+    if (mode == 1u || mode == 2u) {
+        if (layout != 0u) return nullptr;
+        return mode == 1u ? ocean_hip_batch_kernel_fast_1(n, fields, row ? 1 : 0, per_item ? 1 : 0, r)
+                          : ocean_hip_batch_kernel_fast_2(n, fields, row ? 1 : 0, per_item ? 1 : 0, r);
+    }
     if (mode != 0u) return nullptr;
-    return batch_kernel_mode<kModeExact>(n, fields, row, per_item, r);
+    return batch_kernel_mode<kModeExact>(n, fields, row, per_item, r, int(layout));
 }
 
 // Blocks a pass's work takes: an item per block, two with 4 points per lane.
@@ -48,7 +49,7 @@ hipKernelNodeParams group_params(ocean_hip_batch_group& g, bool row, double* t2,
     hipKernelNodeParams p{};
     const uint32_t grid = row ? g.grid_row : g.grid_col;
     p.func = batch_kernel(g.n, static_cast<int>(g.fields), row, grid >= pass_units(g, row), g.mode,
-                          row ? g.radix_row : g.radix_col);
+                          row ? g.radix_row : g.radix_col, g.layout);
     p.gridDim = dim3(grid);
     p.blockDim = dim3(kBlock);
     p.sharedMemBytes = 0;
@@ -248,10 +249,11 @@ extern "C" int ocean_hip_batch_occupancy(uint32_t n, uint32_t fields, int row)
     return blocks;
 }
 
-extern "C" int ocean_hip_batch_has_radix(uint32_t n, uint32_t fields, uint32_t mode, uint32_t radix)
+extern "C" int ocean_hip_batch_has_launch(uint32_t n, uint32_t fields, uint32_t mode, uint32_t radix,
+                                          uint32_t layout)
 {
-    return batch_kernel(n, static_cast<int>(fields), true, false, mode, radix) != nullptr &&
-           batch_kernel(n, static_cast<int>(fields), false, false, mode, radix) != nullptr;
+    return batch_kernel(n, static_cast<int>(fields), true, false, mode, radix, layout) != nullptr &&
+           batch_kernel(n, static_cast<int>(fields), false, false, mode, radix, layout) != nullptr;
 }
 
 // ---- buffer checksums -----------------------------------------------------

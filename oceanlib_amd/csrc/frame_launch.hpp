@@ -444,7 +444,7 @@ struct BatchShape {
 // Field-split items (two teams per row or column, as in the single-level
 // split kernels) were measured slower here: the batch is throughput-bound,
 // and the split repeats the per-cell phase and sincos in both teams.
-template <int N, int F, int M>
+template <int N, int F, int M, int W>
 __global__ __launch_bounds__(kBlock) void batch_row_kernel(const ocean_hip_level* __restrict__ levels,
                                                            const uint32_t* __restrict__ items,
                                                            uint32_t n_items, double t2)
@@ -453,12 +453,12 @@ __global__ __launch_bounds__(kBlock) void batch_row_kernel(const ocean_hip_level
     __shared__ float2 xch[BatchShape<N>::PER * team_stride<N>()];
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const uint32_t it = items[i];
-        row_block<N, F, BatchShape<N>::PER, M>(levels[it >> kItemBlockBits], it & kItemBlockMask,
+        row_block<N, F, BatchShape<N>::PER, M, 8, W>(levels[it >> kItemBlockBits], it & kItemBlockMask,
                                                t2, tw, xch, item_lane());
     }
 }
 
-template <int N, int F, int M>
+template <int N, int F, int M, int W>
 __global__ __launch_bounds__(kBlock) void batch_col_kernel(const ocean_hip_level* __restrict__ levels,
                                                            const uint32_t* __restrict__ items,
                                                            uint32_t n_items)
@@ -467,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void batch_col_kernel(const ocean_hip_level
     __shared__ float2 xch[BatchShape<N>::PER * team_stride<N>()];
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const uint32_t it = items[i];
-        col_block<N, F, BatchShape<N>::PER, true, M>(levels[it >> kItemBlockBits],
+        col_block<N, F, BatchShape<N>::PER, true, M, 8, W>(levels[it >> kItemBlockBits],
                                                   it & kItemBlockMask, tw, xch, item_lane());
     }
 }
@@ -476,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void batch_col_kernel(const ocean_hip_level
 // (what the autotuner picks for the large groups): without the item loop the
 // compiler needs fewer registers (512^2 column pass 157 instead of 187 VGPRs,
 // 3 waves per SIMD instead of 2).
-template <int N, int F, int M>
+template <int N, int F, int M, int W>
 __global__ __launch_bounds__(kBlock) void batch_row_item_kernel(
     const ocean_hip_level* __restrict__ levels, const uint32_t* __restrict__ items,
     uint32_t n_items, double t2)
@@ -485,11 +485,11 @@ __global__ __launch_bounds__(kBlock) void batch_row_item_kernel(
     __shared__ float2 xch[BatchShape<N>::PER * team_stride<N>()];
     if (blockIdx.x >= n_items) return;
     const uint32_t it = items[blockIdx.x];
-    row_block<N, F, BatchShape<N>::PER, M>(levels[it >> kItemBlockBits], it & kItemBlockMask, t2,
+    row_block<N, F, BatchShape<N>::PER, M, 8, W>(levels[it >> kItemBlockBits], it & kItemBlockMask, t2,
                                            tw, xch);
 }
 
-template <int N, int F, int M>
+template <int N, int F, int M, int W>
 __global__ __launch_bounds__(kBlock) void batch_col_item_kernel(
     const ocean_hip_level* __restrict__ levels, const uint32_t* __restrict__ items,
     uint32_t n_items)
@@ -498,7 +498,7 @@ __global__ __launch_bounds__(kBlock) void batch_col_item_kernel(
     __shared__ float2 xch[BatchShape<N>::PER * team_stride<N>()];
     if (blockIdx.x >= n_items) return;
     const uint32_t it = items[blockIdx.x];
-    col_block<N, F, BatchShape<N>::PER, true, M>(levels[it >> kItemBlockBits], it & kItemBlockMask,
+    col_block<N, F, BatchShape<N>::PER, true, M, 8, W>(levels[it >> kItemBlockBits], it & kItemBlockMask,
                                                  tw, xch);
 }
 
@@ -509,7 +509,7 @@ __global__ __launch_bounds__(kBlock) void batch_col_item_kernel(
 // two halves of an item run on the XCD the item's place in the list gives it.
 // Every row and column goes through the same butterflies as with 8 points per
 // lane.
-template <int N, int F, int M>
+template <int N, int F, int M, int W>
 __global__ __launch_bounds__(kBlock) void batch_row_r4_kernel(
     const ocean_hip_level* __restrict__ levels, const uint32_t* __restrict__ items,
     uint32_t n_items, double t2)
@@ -520,12 +520,12 @@ __global__ __launch_bounds__(kBlock) void batch_row_r4_kernel(
     for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
         const uint32_t it = items[u % n_items];
         const uint32_t blk = 2u * (it & kItemBlockMask) + u / n_items;
-        row_block<N, F, BatchShape<N, 4>::PER, M, 4>(levels[it >> kItemBlockBits], blk, t2, tw, xch,
+        row_block<N, F, BatchShape<N, 4>::PER, M, 4, W>(levels[it >> kItemBlockBits], blk, t2, tw, xch,
                                                      item_lane());
     }
 }
 
-template <int N, int F, int M>
+template <int N, int F, int M, int W>
 __global__ __launch_bounds__(kBlock) void batch_col_r4_kernel(
     const ocean_hip_level* __restrict__ levels, const uint32_t* __restrict__ items,
     uint32_t n_items)
@@ -536,12 +536,12 @@ __global__ __launch_bounds__(kBlock) void batch_col_r4_kernel(
     for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
         const uint32_t it = items[u % n_items];
         const uint32_t blk = 2u * (it & kItemBlockMask) + u / n_items;
-        col_block<N, F, BatchShape<N, 4>::PER, true, M, 4>(levels[it >> kItemBlockBits], blk, tw,
+        col_block<N, F, BatchShape<N, 4>::PER, true, M, 4, W>(levels[it >> kItemBlockBits], blk, tw,
                                                            xch, item_lane());
     }
 }
 
-template <int N, int F, int M>
+template <int N, int F, int M, int W>
 __global__ __launch_bounds__(kBlock) void batch_row_r4_item_kernel(
     const ocean_hip_level* __restrict__ levels, const uint32_t* __restrict__ items,
     uint32_t n_items, double t2)
@@ -551,12 +551,12 @@ __global__ __launch_bounds__(kBlock) void batch_row_r4_item_kernel(
     const uint32_t u = blockIdx.x;
     if (u >= 2u * n_items) return;
     const uint32_t it = items[u % n_items];
-    row_block<N, F, BatchShape<N, 4>::PER, M, 4>(levels[it >> kItemBlockBits],
+    row_block<N, F, BatchShape<N, 4>::PER, M, 4, W>(levels[it >> kItemBlockBits],
                                                  2u * (it & kItemBlockMask) + u / n_items, t2, tw,
                                                  xch);
 }
 
-template <int N, int F, int M>
+template <int N, int F, int M, int W>
 __global__ __launch_bounds__(kBlock) void batch_col_r4_item_kernel(
     const ocean_hip_level* __restrict__ levels, const uint32_t* __restrict__ items,
     uint32_t n_items)
@@ -566,53 +566,73 @@ __global__ __launch_bounds__(kBlock) void batch_col_r4_item_kernel(
     const uint32_t u = blockIdx.x;
     if (u >= 2u * n_items) return;
     const uint32_t it = items[u % n_items];
-    col_block<N, F, BatchShape<N, 4>::PER, true, M, 4>(levels[it >> kItemBlockBits],
+    col_block<N, F, BatchShape<N, 4>::PER, true, M, 4, W>(levels[it >> kItemBlockBits],
                                                        2u * (it & kItemBlockMask) + u / n_items, tw,
                                                        xch);
 }
 
-template <int N, int F, int M>
-void* batch_fn_f(bool row, bool per_item, int radix)
+template <int N, int F, int M, int W>
+void* batch_fn_w(bool row, bool per_item, int radix)
 {
     if (radix == 4) {
         if constexpr (N > 1024) {
             return nullptr;  // 4 points per lane: N <= 1024 (one row per 256-lane block)
         } else {
             if (per_item)
-                return row ? reinterpret_cast<void*>(batch_row_r4_item_kernel<N, F, M>)
-                           : reinterpret_cast<void*>(batch_col_r4_item_kernel<N, F, M>);
-            return row ? reinterpret_cast<void*>(batch_row_r4_kernel<N, F, M>)
-                       : reinterpret_cast<void*>(batch_col_r4_kernel<N, F, M>);
+                return row ? reinterpret_cast<void*>(batch_row_r4_item_kernel<N, F, M, W>)
+                           : reinterpret_cast<void*>(batch_col_r4_item_kernel<N, F, M, W>);
+            return row ? reinterpret_cast<void*>(batch_row_r4_kernel<N, F, M, W>)
+                       : reinterpret_cast<void*>(batch_col_r4_kernel<N, F, M, W>);
         }
     }
     if (radix != 8) return nullptr;
     if (per_item)
-        return row ? reinterpret_cast<void*>(batch_row_item_kernel<N, F, M>)
-                   : reinterpret_cast<void*>(batch_col_item_kernel<N, F, M>);
-    return row ? reinterpret_cast<void*>(batch_row_kernel<N, F, M>)
-               : reinterpret_cast<void*>(batch_col_kernel<N, F, M>);
+        return row ? reinterpret_cast<void*>(batch_row_item_kernel<N, F, M, W>)
+                   : reinterpret_cast<void*>(batch_col_item_kernel<N, F, M, W>);
+    return row ? reinterpret_cast<void*>(batch_row_kernel<N, F, M, W>)
+               : reinterpret_cast<void*>(batch_col_kernel<N, F, M, W>);
+}
+
+// Layouts of a group's fields between the passes: 0 row-major; 1 tiles as
+// wide as a work item's columns (PER = 2048/N), so an item's column loads
+// are one contiguous run (bit-exact mode, N = 256 .. 1024).
+template <int N, int M>
+constexpr bool batch_tiled_ok()
+{
+    return M == kModeExact && N >= 256 && N <= 1024;
+}
+
+template <int N, int F, int M>
+void* batch_fn_f(bool row, bool per_item, int radix, int layout)
+{
+    if (layout == 0) return batch_fn_w<N, F, M, 0>(row, per_item, radix);
+    if constexpr (batch_tiled_ok<N, M>()) {
+        if (layout == 1) return batch_fn_w<N, F, M, BatchShape<N>::PER>(row, per_item, radix);
+    }
+    return nullptr;
 }
 
 template <int N, int M>
-void* batch_fn(int fields, bool row, bool per_item, int radix)
+void* batch_fn(int fields, bool row, bool per_item, int radix, int layout)
 {
-    return fields == 6 ? batch_fn_f<N, 6, M>(row, per_item, radix)
-                       : batch_fn_f<N, 4, M>(row, per_item, radix);
+    return fields == 6 ? batch_fn_f<N, 6, M>(row, per_item, radix, layout)
+                       : batch_fn_f<N, 4, M>(row, per_item, radix, layout);
 }
 
 // The batched kernel of mode M; per_item: the grid has a block for every
-// unit (the loop-free kernels); radix: FFT points per lane, 8 or 4 (NULL if
-// not available for this N).
+// unit (the loop-free kernels); radix: FFT points per lane, 8 or 4; layout of
+// the fields between the passes (NULL if not available for this N and mode).
 template <int M>
-void* batch_kernel_mode(uint32_t n, int fields, bool row, bool per_item, int radix = 8)
+void* batch_kernel_mode(uint32_t n, int fields, bool row, bool per_item, int radix = 8,
+                        int layout = 0)
 {
     switch (n) {
-        case 64: return batch_fn<64, M>(fields, row, per_item, radix);
-        case 128: return batch_fn<128, M>(fields, row, per_item, radix);
-        case 256: return batch_fn<256, M>(fields, row, per_item, radix);
-        case 512: return batch_fn<512, M>(fields, row, per_item, radix);
-        case 1024: return batch_fn<1024, M>(fields, row, per_item, radix);
-        case 2048: return batch_fn<2048, M>(fields, row, per_item, radix);
+        case 64: return batch_fn<64, M>(fields, row, per_item, radix, layout);
+        case 128: return batch_fn<128, M>(fields, row, per_item, radix, layout);
+        case 256: return batch_fn<256, M>(fields, row, per_item, radix, layout);
+        case 512: return batch_fn<512, M>(fields, row, per_item, radix, layout);
+        case 1024: return batch_fn<1024, M>(fields, row, per_item, radix, layout);
+        case 2048: return batch_fn<2048, M>(fields, row, per_item, radix, layout);
         default: return nullptr;
     }
 }

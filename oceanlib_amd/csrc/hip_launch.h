@@ -146,6 +146,8 @@ typedef struct ocean_hip_batch_group {
     uint32_t mode;                 /* precision mode of the group's kernels    */
     uint32_t radix_row;            /* FFT points per lane of each pass: 8 (or  */
     uint32_t radix_col;            /* 0) or 4 (autotuned; each item in halves) */
+    uint32_t layout;               /* fields between the passes: 0 row-major, */
+                                   /* 1 tiles of an item's 2048/N columns     */
 } ocean_hip_batch_group;
 /* The opt-in fast modes' kernels (mode 1: frame_fast_kernels.hip, mode 2:
  * frame_half_kernels.hip): the per-level passes of a level whose shape
@@ -180,8 +182,10 @@ void ocean_hip_batch_graph_destroy(ocean_hip_batch_graph* g);
 /* Resident blocks per CU of a group's batched kernel (occupancy). */
 int ocean_hip_batch_occupancy(uint32_t n, uint32_t fields, int row);
 /* Whether the batched kernels of (n, fields, mode) exist with `radix` FFT
- * points per lane (8 always; 4 for N <= 1024). */
-int ocean_hip_batch_has_radix(uint32_t n, uint32_t fields, uint32_t mode, uint32_t radix);
+ * points per lane (8 always; 4 for N <= 1024) and `layout` (0 always; 1 in
+ * bit-exact mode for N = 256 .. 1024). */
+int ocean_hip_batch_has_launch(uint32_t n, uint32_t fields, uint32_t mode, uint32_t radix,
+                               uint32_t layout);
 
 /* Per level of a device array of `count` levels: the checksum of its output
  * buffers (displacement, normal, velocity words w_i; sum of w_i * K_i mod

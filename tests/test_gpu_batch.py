@@ -155,10 +155,12 @@ def test_mixed_batch_bitexact(lib):
 
 
 def test_every_batch_launch_choice_bitexact(lib):
-    """Each group's launch choices set by hand - 8 or 4 FFT points per lane
-    for either pass (4: stages in pairs, every item in two blocks), a small
-    persistent grid or one block per unit - give every level's frame the
-    oracle's bits (checksums over the frame's output words)."""
+    """Each group's launch choices set by hand - the layout of the fields
+    between the passes (row-major, or tiles as wide as an item's columns for
+    N = 256 .. 1024), 8 or 4 FFT points per lane for either pass (4: stages in
+    pairs, every item in two blocks), a small persistent grid or one block per
+    unit - give every level's frame the oracle's bits (checksums over the
+    frame's output words)."""
     descs = [desc_from(kw) for kw in MIXED]
     b = Batch(lib, descs)
     orc = [OracleOcean(d) for d in descs]
@@ -175,32 +177,45 @@ def test_every_batch_launch_choice_bitexact(lib):
         tried = 0
         for g, info in enumerate(groups):
             radices = (8, 4) if info.size <= 1024 else (8,)
-            for rr in radices:
-                for rc in radices:
-                    for small in (True, False):
-                        ur = info.items * (2 if rr == 4 else 1)
-                        uc = info.items * (2 if rc == 4 else 1)
-                        gr, gc = (min(8, ur), min(8, uc)) if small else (ur, uc)
-                        assert lib.ocean_amd_batch_set_group_launch(b.h, g, rr, gr, rc, gc) == 0
-                        got = b.groups()[g]
-                        assert (got.radix_row, got.grid_row, got.radix_col, got.grid_col) == \
-                            (rr, gr, rc, gc)
-                        b.update(t + 1.0)  # a different frame first
-                        b.update(t)
-                        assert lib.ocean_amd_batch_checksums(b.h, 0, len(descs), sums) == 0
-                        assert list(sums) == want, (info.size, rr, rc, small)
-                        tried += 1
+            layouts = (0, 1) if 256 <= info.size <= 1024 else (0,)
+            for layout in layouts:
+                for rr in radices:
+                    for rc in radices:
+                        for small in (True, False):
+                            ur = info.items * (2 if rr == 4 else 1)
+                            uc = info.items * (2 if rc == 4 else 1)
+                            gr, gc = (min(8, ur), min(8, uc)) if small else (ur, uc)
+                            assert lib.ocean_amd_batch_set_group_launch(b.h, g, layout, rr, gr, rc,
+                                                                        gc) == 0
+                            got = b.groups()[g]
+                            assert (got.layout, got.radix_row, got.grid_row, got.radix_col,
+                                    got.grid_col) == (layout, rr, gr, rc, gc)
+                            b.update(t + 1.0)  # a different frame first
+                            b.update(t)
+                            assert lib.ocean_amd_batch_checksums(b.h, 0, len(descs), sums) == 0
+                            assert list(sums) == want, (info.size, layout, rr, rc, small)
+                            tried += 1
             # choices that do not exist are refused
-            assert lib.ocean_amd_batch_set_group_launch(b.h, g, 16, 1, 8, 1) != 0
-            assert lib.ocean_amd_batch_set_group_launch(b.h, g, 8, 0, 8, 1) != 0
-            assert lib.ocean_amd_batch_set_group_launch(b.h, g, 8, info.items + 1, 8, 1) != 0
-        assert tried == 4 * 2 * len(groups)
+            assert lib.ocean_amd_batch_set_group_launch(b.h, g, 0, 16, 1, 8, 1) != 0
+            assert lib.ocean_amd_batch_set_group_launch(b.h, g, 0, 8, 0, 8, 1) != 0
+            assert lib.ocean_amd_batch_set_group_launch(b.h, g, 0, 8, info.items + 1, 8, 1) != 0
+            assert lib.ocean_amd_batch_set_group_launch(b.h, g, 2, 8, 1, 8, 1) != 0
+            if not 256 <= info.size <= 1024:
+                assert lib.ocean_amd_batch_set_group_launch(b.h, g, 1, 8, 1, 8, 1) != 0
+        assert tried == sum(8 * (2 if 256 <= g.size <= 1024 else 1) for g in groups)
         lib.ocean_amd_batch_autotune(b.h, 2)
         b.update(t)
         assert lib.ocean_amd_batch_checksums(b.h, 0, len(descs), sums) == 0
         assert list(sums) == want
-        print("tuned:", [(g.size, g.radix_row, g.grid_row, g.radix_col, g.grid_col)
+        print("tuned:", [(g.size, g.layout, g.radix_row, g.grid_row, g.radix_col, g.grid_col)
                          for g in b.groups()])
+        # a fast mode has no tiled kernels: its groups fall back to row-major
+        assert lib.ocean_amd_batch_set_precision(b.h, 1) == 0
+        assert all(g.layout == 0 for g in b.groups())
+        assert lib.ocean_amd_batch_set_precision(b.h, 0) == 0
+        b.update(t)
+        assert lib.ocean_amd_batch_checksums(b.h, 0, len(descs), sums) == 0
+        assert list(sums) == want
     finally:
         b.close()
 

@@ -374,7 +374,8 @@ int ocean_amd_autotune_frame(ocean_sim* sim, uint32_t reps, uint32_t* row_varian
     // it sees the caches a frame leaves: the column pass reads fields the
     // row pass just wrote on other XCDs, the row pass follows a column pass.
     // Every layout of the fields between the passes is tuned this way, and
-    // the layout with the smallest row + column time is kept.
+    // the layout with the smallest row + column time is kept (row-major
+    // unless a tiled layout is at least 5% faster).
     std::vector<hipEvent_t> ev(std::size_t(reps) * 4, nullptr);
     bool ok = true;
     for (auto& e : ev) ok = ok && hipEventCreate(&e) == hipSuccess;
@@ -424,7 +425,12 @@ int ocean_amd_autotune_frame(ocean_sim* sim, uint32_t reps, uint32_t* row_varian
                 }
             }
         }
-        if (ok && best[0] + best[1] < best_total) {
+        // a tiled layout must beat the row-major one by 5%: these back-to-back
+        // frames flatter the tiled row pass, whose pieced stores cost more
+        // after other kernels' traffic (at 512^2 with velocity: 2.8% ahead
+        // here, even in a frame loop with the foam step and queries)
+        const float margin = layout == 0 ? 1.0f : 0.95f;
+        if (ok && best[0] + best[1] < best_total * margin) {
             best_total = best[0] + best[1];
             best_out[0] = best[0];
             best_out[1] = best[1];

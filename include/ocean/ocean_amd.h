@@ -364,6 +364,14 @@ int  ocean_amd_cascade_height_converged(const ocean_amd_cascade* stack, const fl
                                         uint32_t count, float tol, const ocean_amd_proposal* model,
                                         float* heights_dev, uint8_t* iters_dev,
                                         uint8_t* flags_dev);
+/* The network's proposal alone, for batch levels [first, first + 3): each
+ * level's proposed start offset from the query point in cells, level-major
+ * (starts_dev: 3 * count float2, starts_dev[2 * (lv * count + q)] = du,
+ * [+1] = dv), and the stencil spacings used (spacing_dev: 3 int32, may be
+ * NULL), all device. count <= the model's max_queries. */
+int  ocean_amd_proposal_starts(const ocean_amd_batch* batch, uint32_t first, const float* xz_dev,
+                               uint32_t count, const ocean_amd_proposal* model, float* starts_dev,
+                               int32_t* spacing_dev);
 /* Training pairs from batch levels [first, first + 3): features (count x 64),
  * targets (count x 16: each level's solved start offset in stencil units,
  * plain iteration to tol), ok (count bytes), spacing (3 int32), all device. */

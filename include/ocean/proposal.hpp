@@ -103,6 +103,15 @@ bool height_converged_static(const OceanBatch& batch, std::size_t first, std::si
                              std::uint8_t* iters_dev, std::uint8_t* flags_dev,
                              int* error) noexcept;
 
+// The network's proposal alone for batch levels [first, first + 3): each
+// level's proposed start offset from the query point, in cells, level-major
+// (starts_dev[2 * (lv * count + q)] = du, [+1] = dv), and the stencil
+// spacings used (spacing_dev, 3 int32, may be null). count <= the model's
+// max_queries. Enqueued on the batch's stream.
+bool proposal_starts(const OceanBatch& batch, std::size_t first, const float* xz_dev,
+                     std::uint32_t count, const HeightProposal& model, float* starts_dev,
+                     std::int32_t* spacing_dev) noexcept;
+
 // Training pairs from batch levels [first, first + 3): per point the 60
 // features (count x inputs floats) and the plain solution's start offsets
 // to `tol` (count x outputs floats); ok_dev[q] = 1 when all three converged.

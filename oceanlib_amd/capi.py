@@ -273,6 +273,8 @@ AMD_SIGNATURES = {
                                                           C.c_void_p, C.c_void_p]),
     "ocean_amd_cascade_height_converged": (C.c_int, [_P, C.c_void_p, C.c_uint32, C.c_float, _P,
                                                      C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ocean_amd_proposal_starts": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_uint32, _P, C.c_void_p,
+                                            C.c_void_p]),
     "ocean_amd_proposal_pairs": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_uint32, C.c_float,
                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ocean_amd_batch_last_error": (C.c_int, [_P]),

@@ -1260,6 +1260,20 @@ int ocean_amd_cascade_height_converged(const ocean_amd_cascade* stack, const flo
                : OCEAN_ERROR_UNKNOWN;
 }
 
+int ocean_amd_proposal_starts(const ocean_amd_batch* batch, uint32_t first, const float* xz_dev,
+                              uint32_t count, const ocean_amd_proposal* model, float* starts_dev,
+                              int32_t* spacing_dev)
+{
+    if (!batch_range(batch, first, 3u) || model == nullptr) return OCEAN_ERROR_INVALID_ARG;
+    if (count == 0) return OCEAN_OK;
+    if (xz_dev == nullptr || starts_dev == nullptr || count > to_model(model)->max_queries())
+        return OCEAN_ERROR_INVALID_ARG;
+    return ocean::proposal_starts(*to_batch(batch), first, xz_dev, count, *to_model(model), starts_dev,
+                                  spacing_dev)
+               ? OCEAN_OK
+               : OCEAN_ERROR_UNKNOWN;
+}
+
 int ocean_amd_proposal_pairs(const ocean_amd_batch* batch, uint32_t first, const float* xz_dev,
                              uint32_t count, float tol, float* feats_dev, float* targets_dev,
                              uint8_t* ok_dev, int32_t* spacing_dev)

@@ -234,6 +234,12 @@ hipError_t ocean_hip_height_converged(const ocean_hip_level* levels, uint32_t le
                                       const ocean_hip_mlp* mlp, const int32_t* spacing, float* coarse,
                                       float* prop, float* heights, uint8_t* iters, uint8_t* flags,
                                       hipStream_t s);
+/* The proposal alone: each of the first min(level_count, MLP_LEVELS) levels'
+ * proposed start offset in cells, level-major float2 (prop[lv * count + q]),
+ * with `spacing` and `coarse` as for ocean_hip_height_converged. */
+hipError_t ocean_hip_mlp_propose(const ocean_hip_level* levels, uint32_t level_count, const float* xz,
+                                 uint32_t count, const ocean_hip_mlp* mlp, const int32_t* spacing,
+                                 float* coarse, float* prop, hipStream_t s);
 /* The scratch of the allocation-free converged query: views into a file-scope
  * device array of OCEAN_HIP_STATIC_SCRATCH_BYTES on the current device (its
  * address looked up once per device), shared by every call on that device. */

@@ -314,6 +314,29 @@ bool height_converged(const CascadeStack& stack, const float* xz_dev, std::uint3
                      count, tol, model, heights_dev, iters_dev, flags_dev);
 }
 
+bool proposal_starts(const OceanBatch& batch, std::size_t first, const float* xz_dev,
+                     std::uint32_t count, const HeightProposal& model, float* starts_dev,
+                     std::int32_t* spacing_dev) noexcept
+{
+    if (first > batch.level_count() || ProposalShape::levels > batch.level_count() - first) return false;
+    if (count == 0) return true;
+    HeightProposal::Impl& m = model.impl();
+    if (xz_dev == nullptr || starts_dev == nullptr || count > m.max_queries) return false;
+    batch.level(0).device().make_current();
+    const auto* levels = static_cast<const ocean_hip_level*>(batch.device_levels()) + first;
+    const auto s = static_cast<hipStream_t>(batch.stream());
+    if (!wait_for(&model, s)) return false;
+    if (ocean_hip_mlp_spacing(levels, ProposalShape::levels, m.spacing, m.spacing_partial, s) != hipSuccess ||
+        ocean_hip_mlp_propose(levels, ProposalShape::levels, xz_dev, count, &m.mlp, m.spacing, m.coarse,
+                              starts_dev, s) != hipSuccess)
+        return false;
+    if (spacing_dev != nullptr &&
+        hipMemcpyAsync(spacing_dev, m.spacing, ProposalShape::levels * sizeof(std::int32_t),
+                       hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return false;
+    return hipEventRecord(m.done, s) == hipSuccess && hipStreamWaitEvent(m.stream.s, m.done, 0) == hipSuccess;
+}
+
 bool proposal_pairs(const OceanBatch& batch, std::size_t first, const float* xz_dev,
                     std::uint32_t count, float tol, float* feats_dev, float* targets_dev,
                     std::uint8_t* ok_dev, std::int32_t* spacing_dev) noexcept

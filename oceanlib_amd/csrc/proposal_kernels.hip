@@ -45,15 +45,17 @@ constexpr int kMaxIter = 64;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2v __attribute__((ext_vector_type(2)));
 
-// fp32 -> bf16, round to nearest even (NaN stays NaN).
-__device__ __forceinline__ uint16_t to_bf16(float f)
+// Two fp32 -> a packed pair of bf16 (`lo` in the low half), round to nearest
+// even, NaN stays NaN: one v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi)
 {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return uint16_t(u >> 16);
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2v{lo, hi}, bf16x2v));
 }
+__device__ __forceinline__ uint16_t to_bf16(float f) { return uint16_t(pack_bf16(f, 0.0f)); }
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
 __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
@@ -84,18 +86,29 @@ struct Base {
     float fx, fz;    // offset inside that cell, [0, 1)
 };
 
-__device__ __forceinline__ Base base_cell(float X, float Z, const ocean_hip_level& L)
+// g = n / patch_length, cells per metre: a kernel whose lanes all query one
+// level computes it once (base_cell_g) instead of per query.
+__device__ __forceinline__ double cells_per_metre(const ocean_hip_level& L)
 {
-    const double g = double(L.n) / double(L.patch_length);
+    return double(L.n) / double(L.patch_length);
+}
+
+__device__ __forceinline__ Base base_cell_g(float X, float Z, double g, uint32_t n)
+{
     const double gx = double(X) * g, gz = double(Z) * g;
     const double flx = floor(gx), flz = floor(gz);
-    const int64_t m = int64_t(L.n) - 1;
+    const int64_t m = int64_t(n) - 1;
     Base b;
     b.ix = int32_t(int64_t(flx) & m);
     b.iz = int32_t(int64_t(flz) & m);
     b.fx = float(gx - flx);
     b.fz = float(gz - flz);
     return b;
+}
+
+__device__ __forceinline__ Base base_cell(float X, float Z, const ocean_hip_level& L)
+{
+    return base_cell_g(X, Z, cells_per_metre(L), L.n);
 }
 
 // Bilinear (dx, dz[, h]) at cell position (ix + px, iz + pz), periodic. The
@@ -191,8 +204,9 @@ __device__ __forceinline__ void level_features(const ocean_hip_level& L, const B
     const uint32_t n = L.n;
     const uint32_t nc = n / uint32_t(s), cmask = nc - 1u;
     const float fs = float(s);
-    const float ux = (float(b.ix) + b.fx) / fs;
-    const float uz = (float(b.iz) + b.fz) / fs;
+    const float inv_s = 1.0f / fs;  // s is a power of two: exact, and x * inv_s == x / s
+    const float ux = (float(b.ix) + b.fx) * inv_s;
+    const float uz = (float(b.iz) + b.fz) * inv_s;
     const float rx = floorf(ux + 0.5f), rz = floorf(uz + 0.5f);
     const int32_t cx = int32_t(rx), cz = int32_t(rz);
     const float inv_h = float(L.n) / (L.patch_length * fs);
@@ -334,9 +348,8 @@ __device__ __forceinline__ void store_row(uint16_t* X, const float* f)
     const int l = threadIdx.x & 63;
 #pragma unroll
     for (int c = 0; c < kIn / 4; ++c) {
-        const uint32_t lo = uint32_t(to_bf16(f[4 * c])) | (uint32_t(to_bf16(f[4 * c + 1])) << 16);
-        const uint32_t hi = uint32_t(to_bf16(f[4 * c + 2])) | (uint32_t(to_bf16(f[4 * c + 3])) << 16);
-        *reinterpret_cast<uint2*>(X + l * kPW + 4 * c) = make_uint2(lo, hi);
+        *reinterpret_cast<uint2*>(X + l * kPW + 4 * c) =
+            make_uint2(pack_bf16(f[4 * c], f[4 * c + 1]), pack_bf16(f[4 * c + 2], f[4 * c + 3]));
     }
 }
 
@@ -489,15 +502,13 @@ __global__ void spacing_final_kernel(const ocean_hip_level* __restrict__ levels,
 
 // relu(acc + bias) rounded to bf16, as the next layer's B fragment; lane
 // (i, g)'s D fragment of output tile `it` holds units 16 it + 4 g + e.
+// (max(z, 0) and z > 0 ? z : 0 differ only in the sign of a zero, which no
+// later sum sees: every accumulation starts from +0.)
 __device__ __forceinline__ bf16x4 relu_bf16(f32x4 acc, const float* __restrict__ bias)
 {
-    bf16x4 r;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const float z = acc[e] + bias[e];
-        r[e] = short(to_bf16(z > 0.0f ? z : 0.0f));
-    }
-    return r;
+    const uint32_t lo = pack_bf16(fmaxf(acc[0] + bias[0], 0.0f), fmaxf(acc[1] + bias[1], 0.0f));
+    const uint32_t hi = pack_bf16(fmaxf(acc[2] + bias[2], 0.0f), fmaxf(acc[3] + bias[3], 0.0f));
+    return __builtin_bit_cast(bf16x4, u32x2v{lo, hi});
 }
 
 // LDS ordering inside one wavefront: its LDS operations execute in program
@@ -509,41 +520,75 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Writes lane l's 20 features of one level as bf16 into row l of X at
-// column c0 (a multiple of 4).
-__device__ __forceinline__ void store_level_row(uint16_t* X, int c0, const float* f)
+// The 20 features of one level (level_features' values and order) as 10
+// bf16 pairs, read from the level's coarse grid: node k's (dx, dz) in pair k,
+// the query's offset from the nearest node in pair 9.
+__device__ __forceinline__ void coarse_features_bf16(const Base& b, float inv_s, uint32_t nc,
+                                                     const float2* __restrict__ coarse, uint32_t* pk)
 {
-    const int l = threadIdx.x & 63;
+    const uint32_t cmask = nc - 1u;
+    const float ux = (float(b.ix) + b.fx) * inv_s;
+    const float uz = (float(b.iz) + b.fz) * inv_s;
+    const float rx = floorf(ux + 0.5f), rz = floorf(uz + 0.5f);
+    const int32_t cx = int32_t(rx), cz = int32_t(rz);
+    int k = 0;
 #pragma unroll
-    for (int c = 0; c < kFeatPerLevel / 4; ++c) {
-        const uint32_t lo = uint32_t(to_bf16(f[4 * c])) | (uint32_t(to_bf16(f[4 * c + 1])) << 16);
-        const uint32_t hi = uint32_t(to_bf16(f[4 * c + 2])) | (uint32_t(to_bf16(f[4 * c + 3])) << 16);
-        *reinterpret_cast<uint2*>(X + l * kPW + c0 + 4 * c) = make_uint2(lo, hi);
+    for (int bz = -1; bz <= 1; ++bz) {
+        const uint32_t row = (uint32_t(cz + bz) & cmask) * nc;
+#pragma unroll
+        for (int ax = -1; ax <= 1; ++ax) {
+            const float2 v = coarse[row + (uint32_t(cx + ax) & cmask)];
+            pk[k++] = pack_bf16(v.x, v.y);
+        }
     }
+    pk[k] = pack_bf16(ux - rx, uz - rz);
 }
 
-constexpr int kPropWaves = 4;  // wavefronts per block
-static_assert(kLevels * kFeatPerLevel <= kIn, "feature columns");
+constexpr int kPropWaves = 4;                 // wavefronts per block
+constexpr int kXP = kLevels * kFeatPerLevel;  // X row pitch, bf16: the 60 features
+static_assert(kXP == 60 && kIn == 64, "X holds features 0..59; inputs 60..63 are zero");
+// A row pitch of 30 dwords puts the 16 rows an operand load touches on 16
+// distinct bank pairs, with no padding: 64 x 60 bf16 = 7.5 KB a wavefront,
+// so three 4-wave blocks (weights included) fit a CU's LDS.
 
-__global__ __launch_bounds__(64 * kPropWaves) void proposal_kernel(
+__global__ __launch_bounds__(64 * kPropWaves) __attribute__((amdgpu_waves_per_eu(3))) void proposal_kernel(
     const ocean_hip_level* __restrict__ levels, uint32_t level_count, const float2* __restrict__ xz,
     uint32_t count, const uint16_t* __restrict__ pb, const float* __restrict__ pf,
     const int32_t* __restrict__ spacing, const float2* __restrict__ coarse, float2* __restrict__ prop)
 {
     __shared__ InferLds w;
-    __shared__ __align__(16) uint16_t tiles[kPropWaves][64 * kPW];
+    __shared__ __align__(16) uint32_t tiles[kPropWaves][64 * kXP / 2];
     const int l = threadIdx.x & 63;
     const int i = l & 15, g = l >> 4;
-    uint16_t* X = tiles[threadIdx.x >> 6];
+    uint32_t* X = tiles[threadIdx.x >> 6];
     load_infer(w, pb, pf);
-    // feature columns past the levels in use stay zero
-    for (int c = level_count < uint32_t(kLevels) ? int(level_count) * kFeatPerLevel : kLevels * kFeatPerLevel;
-         c < kIn; ++c)
-        X[l * kPW + c] = 0;
-    __syncthreads();
-    float sp[kLevels];
+    const uint32_t used = level_count < uint32_t(kLevels) ? level_count : uint32_t(kLevels);
+    // feature columns of levels not in use stay zero
+    for (uint32_t c = used * (kFeatPerLevel / 2); c < uint32_t(kXP / 2); ++c) X[l * (kXP / 2) + c] = 0u;
+    // each level's constants, once per block
+    double gl[kLevels];
+    float inv_s[kLevels], sp[kLevels];
+    uint32_t nl[kLevels], ncl[kLevels];
+    const float2* cgl[kLevels];
 #pragma unroll
-    for (int lv = 0; lv < kLevels; ++lv) sp[lv] = uint32_t(lv) < level_count ? float(spacing[lv]) : 0.0f;
+    for (int lv = 0; lv < kLevels; ++lv) {
+        gl[lv] = 0.0;
+        inv_s[lv] = sp[lv] = 0.0f;
+        nl[lv] = ncl[lv] = 0u;
+        cgl[lv] = nullptr;
+        if (uint32_t(lv) < used) {
+            const ocean_hip_level L = levels[lv];
+            const int32_t s = spacing[lv];
+            gl[lv] = cells_per_metre(L);
+            sp[lv] = float(s);
+            inv_s[lv] = 1.0f / float(s);  // s is a power of two: exact
+            nl[lv] = L.n;
+            ncl[lv] = L.n / uint32_t(s);
+            if (ncl[lv] <= OCEAN_HIP_MLP_COARSE_SIDE)
+                cgl[lv] = coarse + size_t(lv) * OCEAN_HIP_MLP_COARSE_SIDE * OCEAN_HIP_MLP_COARSE_SIDE;
+        }
+    }
+    __syncthreads();
     const uint32_t waves = gridDim.x * kPropWaves;
     for (uint32_t base = (blockIdx.x * kPropWaves + (threadIdx.x >> 6)) * 64u; base < count;
          base += waves * 64u) {
@@ -553,21 +598,25 @@ __global__ __launch_bounds__(64 * kPropWaves) void proposal_kernel(
         wave_lds_sync();  // the previous round's reads of X are done
 #pragma unroll
         for (int lv = 0; lv < kLevels; ++lv) {
-            if (uint32_t(lv) < level_count) {
-                float f[kFeatPerLevel];
+            if (uint32_t(lv) < used) {
+                uint32_t pk[kFeatPerLevel / 2];
                 if (live) {
-                    const ocean_hip_level L = levels[lv];
-                    const int sl = spacing[lv];
-                    const float2* cg = L.n / uint32_t(sl) <= OCEAN_HIP_MLP_COARSE_SIDE
-                                           ? coarse + size_t(lv) * OCEAN_HIP_MLP_COARSE_SIDE *
-                                                          OCEAN_HIP_MLP_COARSE_SIDE
-                                           : nullptr;
-                    level_features(L, base_cell(p.x, p.y, L), sl, cg, f);
+                    const Base b = base_cell_g(p.x, p.y, gl[lv], nl[lv]);
+                    if (cgl[lv] != nullptr) {
+                        coarse_features_bf16(b, inv_s[lv], ncl[lv], cgl[lv], pk);
+                    } else {
+                        float f[kFeatPerLevel];
+                        level_features(levels[lv], b, int(sp[lv]), nullptr, f);
+#pragma unroll
+                        for (int k = 0; k < kFeatPerLevel / 2; ++k) pk[k] = pack_bf16(f[2 * k], f[2 * k + 1]);
+                    }
                 } else {
 #pragma unroll
-                    for (int k = 0; k < kFeatPerLevel; ++k) f[k] = 0.0f;
+                    for (int k = 0; k < kFeatPerLevel / 2; ++k) pk[k] = 0u;
                 }
-                store_level_row(X, lv * kFeatPerLevel, f);
+                uint2* dst = reinterpret_cast<uint2*>(X + l * (kXP / 2) + lv * (kFeatPerLevel / 2));
+#pragma unroll
+                for (int c = 0; c < kFeatPerLevel / 4; ++c) dst[c] = make_uint2(pk[2 * c], pk[2 * c + 1]);
             }
         }
         wave_lds_sync();
@@ -580,11 +629,18 @@ __global__ __launch_bounds__(64 * kPropWaves) void proposal_kernel(
         const uint16_t* W1 = w.wt + wofs;
         const uint16_t* W2 = W1 + kHid * kPW;
         const uint16_t* W3 = W1 + 2 * kHid * kPW;
+        const uint16_t* Xh = reinterpret_cast<const uint16_t*>(X);
 #pragma unroll 1
         for (int qt = 0; qt < 4; ++qt) {
+            // B fragments of X^T: inputs 16 kb + 4 g .. + 3 of query 16 qt + i;
+            // inputs 60..63 (kb = 3, g = 3) are the zero features
+            const uint16_t* xr = Xh + (16 * qt + i) * kXP + 4 * g;
             bf16x4 x[4];
-#pragma unroll
-            for (int kb = 0; kb < 4; ++kb) x[kb] = ld4(X + (16 * qt + i) * kPW + 16 * kb + 4 * g);
+            x[0] = ld4(xr);
+            x[1] = ld4(xr + 16);
+            x[2] = ld4(xr + 32);
+            const bf16x4 x3 = ld4(g < 3 ? xr + 48 : xr);
+            x[3] = g < 3 ? x3 : bf16x4{0, 0, 0, 0};
             bf16x4 h1[4], h2[4];
 #pragma unroll
             for (int it = 0; it < 4; ++it) {
@@ -612,7 +668,7 @@ __global__ __launch_bounds__(64 * kPropWaves) void proposal_kernel(
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     const int lv = 2 * g + h;
-                    if (lv < kLevels && uint32_t(lv) < level_count) {
+                    if (lv < kLevels && uint32_t(lv) < used) {
                         const float s_lv = lv == 0 ? sp[0] : (lv == 1 ? sp[1] : sp[2]);
                         const float du = (acc[2 * h] + w.b3[4 * g + 2 * h]) * s_lv;
                         const float dv = (acc[2 * h + 1] + w.b3[4 * g + 2 * h + 1]) * s_lv;
@@ -977,6 +1033,37 @@ extern "C" hipError_t ocean_hip_mlp_spacing(const ocean_hip_level* levels, uint3
     return hipGetLastError();
 }
 
+extern "C" hipError_t ocean_hip_mlp_propose(const ocean_hip_level* levels, uint32_t level_count,
+                                            const float* xz, uint32_t count, const ocean_hip_mlp* mlp,
+                                            const int32_t* spacing, float* coarse, float* prop,
+                                            hipStream_t s)
+{
+    if (count == 0) return hipSuccess;
+    if (mlp == nullptr || spacing == nullptr || prop == nullptr || coarse == nullptr)
+        return hipErrorInvalidValue;
+    float2* coarse2 = reinterpret_cast<float2*>(coarse);
+    hipLaunchKernelGGL(coarse_kernel,
+                       dim3(OCEAN_HIP_MLP_COARSE_SIDE * OCEAN_HIP_MLP_COARSE_SIDE / 256u,
+                            level_count < uint32_t(kLevels) ? level_count : uint32_t(kLevels)),
+                       dim3(256), 0, s, levels, spacing, coarse2);
+    static int prop_grid = 0;  // resident proposal blocks on this device
+    if (prop_grid == 0) {
+        int dev = 0, cus = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, proposal_kernel, 64 * kPropWaves, 0) !=
+                hipSuccess)
+            return hipErrorInvalidValue;
+        prop_grid = cus * (per > 0 ? per : 1);
+    }
+    const uint32_t need = (count + 64u * kPropWaves - 1u) / (64u * kPropWaves);
+    const uint32_t pg = need < uint32_t(prop_grid) ? need : uint32_t(prop_grid);
+    hipLaunchKernelGGL(proposal_kernel, dim3(pg), dim3(64 * kPropWaves), 0, s, levels, level_count,
+                       reinterpret_cast<const float2*>(xz), count, mlp->bf16, mlp->master, spacing,
+                       coarse2, reinterpret_cast<float2*>(prop));
+    return hipGetLastError();
+}
+
 extern "C" hipError_t ocean_hip_height_converged(const ocean_hip_level* levels, uint32_t level_count,
                                                  const float* xz, uint32_t count, float tol,
                                                  const ocean_hip_mlp* mlp, const int32_t* spacing,
@@ -988,26 +1075,9 @@ extern "C" hipError_t ocean_hip_height_converged(const ocean_hip_level* levels, 
     const float2* p = reinterpret_cast<const float2*>(xz);
     float2* prop2 = reinterpret_cast<float2*>(prop);
     if (mlp != nullptr) {
-        if (spacing == nullptr || prop == nullptr || coarse == nullptr) return hipErrorInvalidValue;
-        float2* coarse2 = reinterpret_cast<float2*>(coarse);
-        hipLaunchKernelGGL(coarse_kernel,
-                           dim3(OCEAN_HIP_MLP_COARSE_SIDE * OCEAN_HIP_MLP_COARSE_SIDE / 256u,
-                                level_count < uint32_t(kLevels) ? level_count : uint32_t(kLevels)),
-                           dim3(256), 0, s, levels, spacing, coarse2);
-        static int prop_grid = 0;  // resident proposal blocks on this device
-        if (prop_grid == 0) {
-            int dev = 0, cus = 0, per = 0;
-            if (hipGetDevice(&dev) != hipSuccess ||
-                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, proposal_kernel, 64 * kPropWaves, 0) !=
-                    hipSuccess)
-                return hipErrorInvalidValue;
-            prop_grid = cus * (per > 0 ? per : 1);
-        }
-        const uint32_t need = (count + 64u * kPropWaves - 1u) / (64u * kPropWaves);
-        const uint32_t pg = need < uint32_t(prop_grid) ? need : uint32_t(prop_grid);
-        hipLaunchKernelGGL(proposal_kernel, dim3(pg), dim3(64 * kPropWaves), 0, s, levels, level_count,
-                           p, count, mlp->bf16, mlp->master, spacing, coarse2, prop2);
+        const hipError_t e = ocean_hip_mlp_propose(levels, level_count, xz, count, mlp, spacing, coarse,
+                                                   prop, s);
+        if (e != hipSuccess) return e;
     }
     // One launch per level: all queries tap one level's 4 MB (at 512^2) at a
     // time, which an XCD's L2 holds, instead of every level's at once.

@@ -30,7 +30,7 @@ PARAMS = IN * HID + HID + HID * HID + HID + HID * OUT + OUT
 
 
 def bf16(a) -> np.ndarray:
-    """fp32 -> bf16 -> fp32, round to nearest even (as the kernel's to_bf16)."""
+    """fp32 -> bf16 -> fp32, round to nearest even (as the kernels' v_cvt_pk_bf16_f32)."""
     u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
     nan = (u & 0x7F800000) == 0x7F800000
     r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16

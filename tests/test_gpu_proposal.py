@@ -213,6 +213,45 @@ def test_pairs_match_oracle(lib, sea):
     assert np.all(f[:, 60:] == 0) and np.all(t[:, 6:] == 0)
 
 
+def test_proposal_starts_match_oracle_network(lib, sea):
+    """The proposal kernel (features read from the levels' coarse grids, the
+    network as bf16 MFMA tiles) against the float64 oracle network on the
+    same points' features from the pairs kernel: every level's proposed start
+    offset, in cells, within the bf16 tolerance of the forward test. The
+    point count is not a multiple of a round's 64 queries."""
+    b, _ = sea
+    n = 4096 + 37
+    xz = points(n, 12)
+    params = np.load(PARAMS_FILE, allow_pickle=False) if os.path.exists(PARAMS_FILE) \
+        else random_params(5)
+    m = model(lib, params, max_queries=n)
+    try:
+        dxz = DeviceArray.from_numpy(xz)
+        f = DeviceArray((n, mo.IN), np.float32)
+        t = DeviceArray((n, mo.OUT), np.float32)
+        ok = DeviceArray((n,), np.uint8)
+        sp = DeviceArray((3,), np.int32)
+        assert lib.ocean_amd_proposal_pairs(b, 0, dxz.ptr, n, 1e-6, f.ptr, t.ptr, ok.ptr, sp.ptr) == 0
+        st = DeviceArray((3, n, 2), np.float32)
+        sp2 = DeviceArray((3,), np.int32)
+        assert lib.ocean_amd_proposal_starts(b, 0, dxz.ptr, n, m, st.ptr, sp2.ptr) == 0
+        assert lib.ocean_amd_batch_synchronize(b) == 0
+        # more queries than the model's scratch, and no model, are refused
+        assert lib.ocean_amd_proposal_starts(b, 0, dxz.ptr, n + 1, m, st.ptr, None) != 0
+        assert lib.ocean_amd_proposal_starts(b, 0, dxz.ptr, n, None, st.ptr, None) != 0
+    finally:
+        lib.ocean_amd_proposal_destroy(m)
+    f, sp, sp2, got = f.numpy(), sp.numpy(), sp2.numpy(), st.numpy()
+    assert np.array_equal(sp, sp2)
+    want, _ = mo.forward(params, f)
+    for lv in range(3):
+        w = want[:, 2 * lv:2 * lv + 2] * sp[lv]
+        err = np.abs(got[lv] - w)
+        scale = np.abs(w).max()
+        assert np.median(err) <= 1e-5 * scale, (lv, np.median(err), scale)
+        assert err.max() <= 2e-2 * scale, (lv, err.max(), scale)
+
+
 def test_plain_converged_heights_within_0_01_mm(lib, sea):
     b, bufs = sea
     xz = points(16384, 8)

@@ -222,24 +222,31 @@ typedef struct ocean_hip_mlp {
  * spread over 64 blocks per level, else one block per level. */
 hipError_t ocean_hip_mlp_spacing(const ocean_hip_level* levels, uint32_t level_count,
                                  int32_t* spacing, double* partial, hipStream_t s);
+/* The per-block partial sums of the levels' displacement RMS alone
+ * (count * SPACING_PARTIALS doubles, count <= 256); ocean_hip_mlp_propose
+ * adds them up. */
+hipError_t ocean_hip_mlp_spacing_partial(const ocean_hip_level* levels, uint32_t level_count,
+                                         double* partial, hipStream_t s);
 /* Heights at `count` points (device float2), each level inverted until the
  * fixed-point step is <= tol metres (at most 64 taps), summed over levels.
- * mlp != NULL: start from the proposal (needs `spacing` from
- * ocean_hip_mlp_spacing, `coarse` scratch of MLP_LEVELS * COARSE_SIDE^2
- * float2 and `prop` scratch of count * MLP_PROP floats).
+ * mlp != NULL: start from the proposal (needs `spacing_partial` from
+ * ocean_hip_mlp_spacing_partial; writes `spacing`, MLP_LEVELS int32; uses
+ * `coarse` scratch of MLP_LEVELS * COARSE_SIDE^2 float2 and `prop` scratch of
+ * count * MLP_PROP floats).
  * iters (count * level_count bytes) and flags (count bytes: 1 = a level did
  * not converge, 2 = a proposal was abandoned) may be NULL. */
 hipError_t ocean_hip_height_converged(const ocean_hip_level* levels, uint32_t level_count,
                                       const float* xz, uint32_t count, float tol,
-                                      const ocean_hip_mlp* mlp, const int32_t* spacing, float* coarse,
-                                      float* prop, float* heights, uint8_t* iters, uint8_t* flags,
-                                      hipStream_t s);
+                                      const ocean_hip_mlp* mlp, const double* spacing_partial,
+                                      int32_t* spacing, float* coarse, float* prop, float* heights,
+                                      uint8_t* iters, uint8_t* flags, hipStream_t s);
 /* The proposal alone: each of the first min(level_count, MLP_LEVELS) levels'
  * proposed start offset in cells, level-major float2 (prop[lv * count + q]),
- * with `spacing` and `coarse` as for ocean_hip_height_converged. */
+ * with `spacing_partial`, `spacing` and `coarse` as for
+ * ocean_hip_height_converged. */
 hipError_t ocean_hip_mlp_propose(const ocean_hip_level* levels, uint32_t level_count, const float* xz,
-                                 uint32_t count, const ocean_hip_mlp* mlp, const int32_t* spacing,
-                                 float* coarse, float* prop, hipStream_t s);
+                                 uint32_t count, const ocean_hip_mlp* mlp, const double* spacing_partial,
+                                 int32_t* spacing, float* coarse, float* prop, hipStream_t s);
 /* The scratch of the allocation-free converged query: views into a file-scope
  * device array of OCEAN_HIP_STATIC_SCRATCH_BYTES on the current device (its
  * address looked up once per device), shared by every call on that device. */

@@ -108,22 +108,21 @@ bool converged(const ocean_hip_level* levels, std::size_t count_levels, hipStrea
         const ProposalScratch& sc = use != nullptr ? *use : own;
         if (count_levels > ProposalShape::levels || count > sc.max_queries) return false;
         if (!wait_for(model, s)) return false;
-        if (ocean_hip_mlp_spacing(levels, static_cast<std::uint32_t>(count_levels), sc.spacing,
-                                  sc.spacing_partial, s) != hipSuccess)
+        if (ocean_hip_mlp_spacing_partial(levels, static_cast<std::uint32_t>(count_levels),
+                                          sc.spacing_partial, s) != hipSuccess)
             return false;
         const bool ok = ocean_hip_height_converged(levels, static_cast<std::uint32_t>(count_levels),
-                                                   xz_dev, count, tol, &m.mlp, sc.spacing, sc.coarse,
-                                                   sc.prop,
-                                                   heights_dev, iters_dev, flags_dev,
-                                                   s) == hipSuccess;
+                                                   xz_dev, count, tol, &m.mlp, sc.spacing_partial,
+                                                   sc.spacing, sc.coarse, sc.prop, heights_dev,
+                                                   iters_dev, flags_dev, s) == hipSuccess;
         // The proposal's parameters (and its own scratch) are in use until
         // this query ends: later work on the model waits for it.
         return ok && hipEventRecord(m.done, s) == hipSuccess &&
                hipStreamWaitEvent(m.stream.s, m.done, 0) == hipSuccess;
     }
     return ocean_hip_height_converged(levels, static_cast<std::uint32_t>(count_levels), xz_dev, count,
-                                      tol, nullptr, nullptr, nullptr, nullptr, heights_dev, iters_dev,
-                                      flags_dev, s) == hipSuccess;
+                                      tol, nullptr, nullptr, nullptr, nullptr, nullptr, heights_dev,
+                                      iters_dev, flags_dev, s) == hipSuccess;
 }
 
 }  // namespace
@@ -326,9 +325,9 @@ bool proposal_starts(const OceanBatch& batch, std::size_t first, const float* xz
     const auto* levels = static_cast<const ocean_hip_level*>(batch.device_levels()) + first;
     const auto s = static_cast<hipStream_t>(batch.stream());
     if (!wait_for(&model, s)) return false;
-    if (ocean_hip_mlp_spacing(levels, ProposalShape::levels, m.spacing, m.spacing_partial, s) != hipSuccess ||
-        ocean_hip_mlp_propose(levels, ProposalShape::levels, xz_dev, count, &m.mlp, m.spacing, m.coarse,
-                              starts_dev, s) != hipSuccess)
+    if (ocean_hip_mlp_spacing_partial(levels, ProposalShape::levels, m.spacing_partial, s) != hipSuccess ||
+        ocean_hip_mlp_propose(levels, ProposalShape::levels, xz_dev, count, &m.mlp, m.spacing_partial,
+                              m.spacing, m.coarse, starts_dev, s) != hipSuccess)
         return false;
     if (spacing_dev != nullptr &&
         hipMemcpyAsync(spacing_dev, m.spacing, ProposalShape::levels * sizeof(std::int32_t),

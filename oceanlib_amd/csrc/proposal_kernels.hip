@@ -10,7 +10,8 @@
 // query point if the step does not fall.
 //
 // The MLP (per query, 3 levels): 60 features -> 64 -> 64 -> 6, ReLU, bf16
-// weights and activations on MFMA (v_mfma_f32_16x16x16_bf16), fp32
+// weights and activations on MFMA (v_mfma_f32_16x16x32_bf16 for inference,
+// v_mfma_f32_16x16x16_bf16 for training), fp32
 // accumulate, fp32 biases. Features of level l: the (dx, dz) of the 3x3
 // nodes at spacing s_l cells around the node nearest the query, divided by
 // s_l * cell_l, and the query's offset from that node divided by s_l. Outputs:
@@ -63,6 +64,21 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// v_mfma_f32_16x16x32_bf16 on the concatenated fragments of two 16-input
+// blocks: A[i][k] from lane i + 16 * (k / 8), element k % 8 (so a lane's
+// elements 0..3 are its 16x16x16 fragment of the first block and 4..7 of the
+// second), the same for B, D as for 16x16x16. One instruction for what two
+// chained mfma16 compute.
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f32x4 mfma32(bf16x4 a0, bf16x4 a1, bf16x4 b0, bf16x4 b1, f32x4 c)
+{
+    const s16x8 a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+    const s16x8 b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                   c, 0, 0, 0);
+}
+
 // 4 consecutive bf16 from LDS (8-byte aligned).
 __device__ __forceinline__ bf16x4 ld4(const uint16_t* p)
 {
@@ -111,16 +127,18 @@ __device__ __forceinline__ Base base_cell(float X, float Z, const ocean_hip_leve
     return base_cell_g(X, Z, cells_per_metre(L), L.n);
 }
 
-// Bilinear (dx, dz[, h]) at cell position (ix + px, iz + pz), periodic. The
-// four texels of the last cell are kept in `c`: once the iteration settles
-// into its final cell, later taps blend registers (the same expression on the
-// same values) instead of loading again.
+// Bilinear (dx, dz[, h]) at cell position (ix + px, iz + pz), periodic, as
+// two lerps along x and one along z with fused multiply-adds (the converged
+// queries are judged by tolerance, not bit for bit). The cell's texels are
+// kept in `k` as the (x0, z0) texel and its difference to (x1, z0), and the
+// (x0, z1) texel and its difference to (x1, z1): once the iteration settles
+// into its final cell, later taps blend registers instead of loading again.
 struct TapCache {
     uint32_t key = 0xffffffffu;  // z0 * n + x0 of the cached cell
     // Zero-initialised: the first tap always loads (key < n * n), but an
     // indeterminate member would let the compiler drop the components that
     // only the final tap reads (the height) from the loop's cached cell.
-    float4 a{}, c{}, d{}, e{};
+    float3 a{}, ca{}, d{}, ed{};
 };
 
 template <bool HEIGHT>
@@ -135,17 +153,22 @@ __device__ __forceinline__ void tap(const float4* __restrict__ disp, uint32_t n,
     const uint32_t key = z0 * n + x0;
     if (key != k.key) {
         const uint32_t x1 = (x0 + 1u) & mask, z1 = (z0 + 1u) & mask;
-        k.a = disp[key];
-        k.c = disp[z0 * n + x1];
-        k.d = disp[z1 * n + x0];
-        k.e = disp[z1 * n + x1];
+        const float4 a = disp[key], c = disp[z0 * n + x1];
+        const float4 d = disp[z1 * n + x0], e = disp[z1 * n + x1];
+        k.a = make_float3(a.x, a.y, a.z);
+        k.ca = make_float3(c.x - a.x, c.y - a.y, c.z - a.z);
+        k.d = make_float3(d.x, d.y, d.z);
+        k.ed = make_float3(e.x - d.x, e.y - d.y, e.z - d.z);
         k.key = key;
     }
-    const float w00 = (1.0f - tu) * (1.0f - tv), w10 = tu * (1.0f - tv);
-    const float w01 = (1.0f - tu) * tv, w11 = tu * tv;
-    dx = k.a.x * w00 + k.c.x * w10 + k.d.x * w01 + k.e.x * w11;
-    dz = k.a.z * w00 + k.c.z * w10 + k.d.z * w01 + k.e.z * w11;
-    if (HEIGHT) h = k.a.y * w00 + k.c.y * w10 + k.d.y * w01 + k.e.y * w11;
+    const float x_0 = __builtin_fmaf(tu, k.ca.x, k.a.x), x_1 = __builtin_fmaf(tu, k.ed.x, k.d.x);
+    const float z_0 = __builtin_fmaf(tu, k.ca.z, k.a.z), z_1 = __builtin_fmaf(tu, k.ed.z, k.d.z);
+    dx = __builtin_fmaf(tv, x_1 - x_0, x_0);
+    dz = __builtin_fmaf(tv, z_1 - z_0, z_0);
+    if (HEIGHT) {
+        const float y_0 = __builtin_fmaf(tu, k.ca.y, k.a.y), y_1 = __builtin_fmaf(tu, k.ed.y, k.d.y);
+        h = __builtin_fmaf(tv, y_1 - y_0, y_0);
+    }
 }
 
 struct Solve {
@@ -232,27 +255,6 @@ __device__ __forceinline__ void level_features(const ocean_hip_level& L, const B
     f[k++] = uz - rz;
 }
 
-// The levels' coarse grids, for levels whose grid has at most
-// OCEAN_HIP_MLP_COARSE_SIDE nodes a side: node (xc, zc) = the displacement's
-// (dx, dz) at cell (s xc, s zc) divided by s * cell, as level_features reads
-// it. grid: (OCEAN_HIP_MLP_COARSE_SIDE^2 / 256, levels).
-__global__ __launch_bounds__(256) void coarse_kernel(const ocean_hip_level* __restrict__ levels,
-                                                     const int32_t* __restrict__ spacing,
-                                                     float2* __restrict__ coarse)
-{
-    const ocean_hip_level L = levels[blockIdx.y];
-    const int s = spacing[blockIdx.y];
-    const uint32_t nc = L.n / uint32_t(s);
-    if (nc > OCEAN_HIP_MLP_COARSE_SIDE) return;
-    const uint32_t node = blockIdx.x * 256u + threadIdx.x;
-    if (node >= nc * nc) return;
-    const uint32_t xc = node % nc, zc = node / nc;
-    const float inv_h = float(L.n) / (L.patch_length * float(s));
-    const float4 v = reinterpret_cast<const float4*>(L.disp)[(zc * uint32_t(s)) * L.n + xc * uint32_t(s)];
-    coarse[size_t(blockIdx.y) * OCEAN_HIP_MLP_COARSE_SIDE * OCEAN_HIP_MLP_COARSE_SIDE + node] =
-        make_float2(v.x * inv_h, v.z * inv_h);
-}
-
 // ---- the MLP on one wavefront (64 rows) ---------------------------------------
 //
 // v_mfma_f32_16x16x16_bf16 lane layout: A[i][k] from lane i + 16 * (k / 4),
@@ -290,10 +292,11 @@ __device__ __forceinline__ void bf16_slots(int p, int& t, int& r)
 // Inference LDS: the transposed weights at a padded row pitch, so the
 // 16 rows an MFMA operand load touches fall in distinct banks.
 constexpr int kPW = 68;
-struct InferLds {
+struct alignas(16) InferLds {
     uint16_t wt[(kHid + kHid + kOut) * kPW];  // W1^T rows, then W2^T, then W3^T
-    float b1[kHid], b2[kHid], b3[kOut];
+    float b1[kHid], b2[kHid], b3[kOut];       // 16-byte aligned: read as 4-float fragments
 };
+static_assert(sizeof(uint16_t) * (kHid + kHid + kOut) * kPW % 16 == 0, "bias alignment");
 constexpr int kWaveTile = 64 * kPW + 2 * 16 * kPW;  // X (64 rows) + two 16-row buffers
 
 __device__ void load_infer(InferLds& w, const uint16_t* __restrict__ pb, const float* __restrict__ pf)
@@ -488,6 +491,42 @@ __global__ void spacing_final_kernel(const ocean_hip_level* __restrict__ levels,
     spacing[lv] = spacing_of(sum, levels[lv]);
 }
 
+// The levels' stencil spacings and coarse grids, for levels whose grid has at
+// most OCEAN_HIP_MLP_COARSE_SIDE nodes a side: node (xc, zc) = the
+// displacement's (dx, dz) at cell (s xc, s zc) divided by s * cell, as
+// level_features reads it. Every block adds its level's kSpacingBlocks
+// partial sums (spacing_partial_kernel) in block order, as
+// spacing_final_kernel does, and block 0 writes the spacing. Each block
+// strides over the level's nodes. grid: (kCoarseBlocks, levels).
+constexpr uint32_t kCoarseBlocks = 128;
+__global__ __launch_bounds__(256) void coarse_kernel(const ocean_hip_level* __restrict__ levels,
+                                                     const double* __restrict__ partial,
+                                                     int32_t* __restrict__ spacing,
+                                                     float2* __restrict__ coarse)
+{
+    __shared__ int32_t s_level;
+    const ocean_hip_level L = levels[blockIdx.y];
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+#pragma unroll 8
+        for (int b = 0; b < kSpacingBlocks; ++b) sum += partial[blockIdx.y * kSpacingBlocks + b];
+        s_level = spacing_of(sum, L);
+        if (blockIdx.x == 0) spacing[blockIdx.y] = s_level;
+    }
+    __syncthreads();
+    const int s = s_level;
+    const uint32_t nc = L.n / uint32_t(s);
+    if (nc > OCEAN_HIP_MLP_COARSE_SIDE) return;
+    const float inv_h = float(L.n) / (L.patch_length * float(s));
+    const float4* __restrict__ disp = reinterpret_cast<const float4*>(L.disp);
+    float2* __restrict__ out = coarse + size_t(blockIdx.y) * OCEAN_HIP_MLP_COARSE_SIDE * OCEAN_HIP_MLP_COARSE_SIDE;
+    for (uint32_t node = blockIdx.x * 256u + threadIdx.x; node < nc * nc; node += kCoarseBlocks * 256u) {
+        const uint32_t xc = node % nc, zc = node / nc;
+        const float4 v = disp[(zc * uint32_t(s)) * L.n + xc * uint32_t(s)];
+        out[node] = make_float2(v.x * inv_h, v.z * inv_h);
+    }
+}
+
 // Proposals. Each wavefront takes 64 queries at a time and strides through
 // the query list (a persistent grid: the network is staged into LDS once per
 // block): lane l gathers query l's features level by level into row l of the
@@ -497,19 +536,23 @@ __global__ void spacing_final_kernel(const ocean_hip_level* __restrict__ levels,
 // the next layer's B operand needs, so the hidden activations never leave
 // registers and no wavefront waits for another (the block's one barrier is
 // after the staging). The products and their summation per 16-input block are
-// those of the row-major form (forward_kernel). Output: each level's start
-// offset in cells, level-major float2: prop[lv * count + q].
+// those of the row-major form (forward_kernel); the accumulators start from
+// the biases instead of adding them last. Output: each level's start offset
+// in cells, level-major float2: prop[lv * count + q].
 
 // relu(acc + bias) rounded to bf16, as the next layer's B fragment; lane
 // (i, g)'s D fragment of output tile `it` holds units 16 it + 4 g + e.
-// (max(z, 0) and z > 0 ? z : 0 differ only in the sign of a zero, which no
-// later sum sees: every accumulation starts from +0.)
-__device__ __forceinline__ bf16x4 relu_bf16(f32x4 acc, const float* __restrict__ bias)
+// The proposal kernel's accumulators start from the bias (z = b + sum), so
+// here only max(z, 0) remains; max and z > 0 ? z : 0 differ only in the sign
+// of a zero, which leaves every later sum unchanged.
+__device__ __forceinline__ bf16x4 relu_bf16(f32x4 z)
 {
-    const uint32_t lo = pack_bf16(fmaxf(acc[0] + bias[0], 0.0f), fmaxf(acc[1] + bias[1], 0.0f));
-    const uint32_t hi = pack_bf16(fmaxf(acc[2] + bias[2], 0.0f), fmaxf(acc[3] + bias[3], 0.0f));
+    const uint32_t lo = pack_bf16(fmaxf(z[0], 0.0f), fmaxf(z[1], 0.0f));
+    const uint32_t hi = pack_bf16(fmaxf(z[2], 0.0f), fmaxf(z[3], 0.0f));
     return __builtin_bit_cast(bf16x4, u32x2v{lo, hi});
 }
+// Four consecutive fp32 from LDS (16-byte aligned): a bias fragment.
+__device__ __forceinline__ f32x4 ld_bias(const float* p) { return *reinterpret_cast<const f32x4*>(p); }
 
 // LDS ordering inside one wavefront: its LDS operations execute in program
 // order, so only the compiler has to be kept from moving them.
@@ -590,11 +633,16 @@ __global__ __launch_bounds__(64 * kPropWaves) __attribute__((amdgpu_waves_per_eu
     }
     __syncthreads();
     const uint32_t waves = gridDim.x * kPropWaves;
-    for (uint32_t base = (blockIdx.x * kPropWaves + (threadIdx.x >> 6)) * 64u; base < count;
-         base += waves * 64u) {
+    const uint32_t first = (blockIdx.x * kPropWaves + (threadIdx.x >> 6)) * 64u;
+    // Each round's query point is loaded one round ahead, so that load's
+    // latency overlaps the previous round's network.
+    float2 p_next = first + uint32_t(l) < count ? xz[first + uint32_t(l)] : make_float2(0.0f, 0.0f);
+    for (uint32_t base = first; base < count; base += waves * 64u) {
         const uint32_t q = base + uint32_t(l);
         const bool live = q < count;
-        const float2 p = live ? xz[q] : make_float2(0.0f, 0.0f);
+        const float2 p = p_next;
+        const uint32_t qn = q + waves * 64u;
+        if (qn < count) p_next = xz[qn];
         wave_lds_sync();  // the previous round's reads of X are done
 #pragma unroll
         for (int lv = 0; lv < kLevels; ++lv) {
@@ -644,23 +692,26 @@ __global__ __launch_bounds__(64 * kPropWaves) __attribute__((amdgpu_waves_per_eu
             bf16x4 h1[4], h2[4];
 #pragma unroll
             for (int it = 0; it < 4; ++it) {
-                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                f32x4 acc = ld_bias(w.b1 + 16 * it + 4 * g);
 #pragma unroll
-                for (int kb = 0; kb < 4; ++kb)
-                    acc = mfma16(ld4(W1 + 16 * it * kPW + 16 * kb), x[kb], acc);
-                h1[it] = relu_bf16(acc, w.b1 + 16 * it + 4 * g);
+                for (int kb = 0; kb < 4; kb += 2)
+                    acc = mfma32(ld4(W1 + 16 * it * kPW + 16 * kb), ld4(W1 + 16 * it * kPW + 16 * kb + 16),
+                                 x[kb], x[kb + 1], acc);
+                h1[it] = relu_bf16(acc);
             }
 #pragma unroll
             for (int it = 0; it < 4; ++it) {
-                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                f32x4 acc = ld_bias(w.b2 + 16 * it + 4 * g);
 #pragma unroll
-                for (int kb = 0; kb < 4; ++kb)
-                    acc = mfma16(ld4(W2 + 16 * it * kPW + 16 * kb), h1[kb], acc);
-                h2[it] = relu_bf16(acc, w.b2 + 16 * it + 4 * g);
+                for (int kb = 0; kb < 4; kb += 2)
+                    acc = mfma32(ld4(W2 + 16 * it * kPW + 16 * kb), ld4(W2 + 16 * it * kPW + 16 * kb + 16),
+                                 h1[kb], h1[kb + 1], acc);
+                h2[it] = relu_bf16(acc);
             }
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            f32x4 acc = ld_bias(w.b3 + 4 * g);
 #pragma unroll
-            for (int kb = 0; kb < 4; ++kb) acc = mfma16(ld4(W3 + 16 * kb), h2[kb], acc);
+            for (int kb = 0; kb < 4; kb += 2)
+                acc = mfma32(ld4(W3 + 16 * kb), ld4(W3 + 16 * kb + 16), h2[kb], h2[kb + 1], acc);
             // lane (i, g) holds outputs 4 g .. 4 g + 3 of query base + 16 qt + i:
             // g = 0 levels 0 and 1, g = 1 level 2 (outputs 4, 5)
             const uint32_t qo = base + uint32_t(16 * qt + i);
@@ -670,8 +721,8 @@ __global__ __launch_bounds__(64 * kPropWaves) __attribute__((amdgpu_waves_per_eu
                     const int lv = 2 * g + h;
                     if (lv < kLevels && uint32_t(lv) < used) {
                         const float s_lv = lv == 0 ? sp[0] : (lv == 1 ? sp[1] : sp[2]);
-                        const float du = (acc[2 * h] + w.b3[4 * g + 2 * h]) * s_lv;
-                        const float dv = (acc[2 * h + 1] + w.b3[4 * g + 2 * h + 1]) * s_lv;
+                        const float du = acc[2 * h] * s_lv;
+                        const float dv = acc[2 * h + 1] * s_lv;
                         prop[size_t(lv) * count + qo] = make_float2(du, dv);
                     }
                 }
@@ -1017,6 +1068,16 @@ using namespace oceanamd;
 
 extern "C" uint32_t ocean_hip_mlp_train_blocks(uint32_t batch) { return batch / uint32_t(kTrainRows); }
 
+extern "C" hipError_t ocean_hip_mlp_spacing_partial(const ocean_hip_level* levels, uint32_t level_count,
+                                                    double* partial, hipStream_t s)
+{
+    if (level_count == 0) return hipSuccess;
+    if (partial == nullptr || level_count > 256u) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(spacing_partial_kernel, dim3(kSpacingBlocks, level_count), dim3(256), 0, s, levels,
+                       partial);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t ocean_hip_mlp_spacing(const ocean_hip_level* levels, uint32_t level_count,
                                             int32_t* spacing, double* partial, hipStream_t s)
 {
@@ -1035,17 +1096,17 @@ extern "C" hipError_t ocean_hip_mlp_spacing(const ocean_hip_level* levels, uint3
 
 extern "C" hipError_t ocean_hip_mlp_propose(const ocean_hip_level* levels, uint32_t level_count,
                                             const float* xz, uint32_t count, const ocean_hip_mlp* mlp,
-                                            const int32_t* spacing, float* coarse, float* prop,
-                                            hipStream_t s)
+                                            const double* spacing_partial, int32_t* spacing,
+                                            float* coarse, float* prop, hipStream_t s)
 {
     if (count == 0) return hipSuccess;
-    if (mlp == nullptr || spacing == nullptr || prop == nullptr || coarse == nullptr)
+    if (mlp == nullptr || spacing_partial == nullptr || spacing == nullptr || prop == nullptr ||
+        coarse == nullptr)
         return hipErrorInvalidValue;
     float2* coarse2 = reinterpret_cast<float2*>(coarse);
     hipLaunchKernelGGL(coarse_kernel,
-                       dim3(OCEAN_HIP_MLP_COARSE_SIDE * OCEAN_HIP_MLP_COARSE_SIDE / 256u,
-                            level_count < uint32_t(kLevels) ? level_count : uint32_t(kLevels)),
-                       dim3(256), 0, s, levels, spacing, coarse2);
+                       dim3(kCoarseBlocks, level_count < uint32_t(kLevels) ? level_count : uint32_t(kLevels)),
+                       dim3(256), 0, s, levels, spacing_partial, spacing, coarse2);
     static int prop_grid = 0;  // resident proposal blocks on this device
     if (prop_grid == 0) {
         int dev = 0, cus = 0, per = 0;
@@ -1066,17 +1127,18 @@ extern "C" hipError_t ocean_hip_mlp_propose(const ocean_hip_level* levels, uint3
 
 extern "C" hipError_t ocean_hip_height_converged(const ocean_hip_level* levels, uint32_t level_count,
                                                  const float* xz, uint32_t count, float tol,
-                                                 const ocean_hip_mlp* mlp, const int32_t* spacing,
-                                                 float* coarse, float* prop, float* heights,
-                                                 uint8_t* iters, uint8_t* flags, hipStream_t s)
+                                                 const ocean_hip_mlp* mlp, const double* spacing_partial,
+                                                 int32_t* spacing, float* coarse, float* prop,
+                                                 float* heights, uint8_t* iters, uint8_t* flags,
+                                                 hipStream_t s)
 {
     if (count == 0) return hipSuccess;
     const dim3 grid((count + 255u) / 256u);
     const float2* p = reinterpret_cast<const float2*>(xz);
     float2* prop2 = reinterpret_cast<float2*>(prop);
     if (mlp != nullptr) {
-        const hipError_t e = ocean_hip_mlp_propose(levels, level_count, xz, count, mlp, spacing, coarse,
-                                                   prop, s);
+        const hipError_t e = ocean_hip_mlp_propose(levels, level_count, xz, count, mlp, spacing_partial,
+                                                   spacing, coarse, prop, s);
         if (e != hipSuccess) return e;
     }
     // One launch per level: all queries tap one level's 4 MB (at 512^2) at a

@@ -127,12 +127,22 @@ __device__ __forceinline__ void store_field(void* spec, size_t at, float x, floa
     }
 }
 
-template <int MODE>
+// A single ocean's column pass reads each field element once, written by the
+// row pass on other XCDs: with NT a streaming load, so the fields do not
+// displace the row pass's constant tables (h0, omega, 1/|k|: 4 MB at 512^2,
+// an eighth of it per XCD) from the XCD's L2 between frames (in the frame
+// loop the row pass then runs at its back-to-back time: 10.9 -> 8.8 us at
+// 512^2). The batched passes load normally: there streaming loads cost the
+// whole batch 23% (164k -> 126k patch-updates/s, A/B on one box).
+template <int MODE, bool NT = false>
 __device__ __forceinline__ float2 load_field(const void* spec, size_t at)
 {
     if constexpr (MODE == kModeFastHalf) {
         const f16x2 h = reinterpret_cast<const f16x2*>(spec)[at];
         return make_float2(static_cast<float>(h.x), static_cast<float>(h.y));
+    } else if constexpr (NT) {
+        const nt_f32x2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f32x2*>(spec) + at);
+        return make_float2(v.x, v.y);
     } else {
         return reinterpret_cast<const float2*>(spec)[at];
     }
@@ -312,7 +322,7 @@ __device__ __forceinline__ void col_block(const ocean_hip_level& L, uint32_t blk
     for (int f = 0; f < F; ++f) {
 #pragma unroll
         for (int a = 0; a < R; ++a) {
-            const float2 v = load_field<MODE>(spec, spec_at<N, W>(uint32_t(f), J + uint32_t(a * T), x));
+            const float2 v = load_field<MODE, !STREAM>(spec, spec_at<N, W>(uint32_t(f), J + uint32_t(a * T), x));
             fr[f][a] = v.x;
             fi[f][a] = v.y;
         }
@@ -494,7 +504,7 @@ __device__ __forceinline__ void col_block_split(const ocean_hip_level& L, uint32
 #pragma unroll
         for (int a = 0; a < R; ++a) {
             const float2 v =
-                load_field<MODE>(spec, spec_at<N, W>(2u * g + uint32_t(f), J + uint32_t(a * T), x));
+                load_field<MODE, true>(spec, spec_at<N, W>(2u * g + uint32_t(f), J + uint32_t(a * T), x));
             fr[f][a] = v.x;
             fi[f][a] = v.y;
         }

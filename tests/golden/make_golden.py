@@ -7,7 +7,8 @@ only data: raw small buffers (.npz, no pickles) and 64-bit digests of larger
 ones (JSON). The fixtures pin the oracle (tests/test_oracle_*.py) and, on a
 GPU, the HIP path (tests/test_gpu_*.py) to the reference's own output.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py                 (all fixtures)
+    python tests/golden/make_golden.py config3_frames  (config3_frames.json only)
 """
 from __future__ import annotations
 
@@ -179,6 +180,40 @@ def run_config3_interaction(r, frames=1200, every=50):
     return out
 
 
+def run_config3_every_frame(r, frames=1200):
+    """The three-cascade scene at every one of its 1200 frames: each level's
+    displacement and normal digests and the 512 cascade queries' digest, and
+    the interaction field's digest after every substep."""
+    levels = [desc_from(kw) for kw in CONFIG3_LEVELS]
+    arr = (type(levels[0]) * len(levels))(*levels)
+    h = r.oref_cascade_create(arr, len(levels))
+    assert h
+    surf = OceanSurfaceEx()
+    pts = query_points(1, 512, 800.0)[0]
+    cascade = []
+    for k in range(frames):
+        r.oref_cascade_update(h, k / 60.0)
+        row = []
+        for li, d in enumerate(levels):
+            n = d.size
+            row.append(digest(np.ctypeslib.as_array(r.oref_cascade_displacement(h, li), shape=(4 * n * n,))))
+            row.append(digest(np.ctypeslib.as_array(r.oref_cascade_normal(h, li), shape=(4 * n * n,))))
+        q = np.zeros((len(pts), 10), np.float32)
+        for i, (x, z) in enumerate(pts):
+            r.oref_cascade_sample(h, float(x), float(z), C.byref(surf))
+            q[i] = np.frombuffer(bytes(surf), np.float32)
+        row.append(digest(q))
+        cascade.append(row)
+    r.oref_cascade_destroy(h)
+    every = run_config3_interaction(r, frames=frames, every=1)
+    return {"cascade": cascade, "interaction": [every[str(k)] for k in range(frames)]}
+
+
+def write_config3_frames(r):
+    with open(os.path.join(HERE, "config3_frames.json"), "w") as f:
+        json.dump(run_config3_every_frame(r), f, indent=0)
+
+
 def write_config2_final(r):
     with open(os.path.join(HERE, "config2_final.json"), "w") as f:
         json.dump(run_config2_final(r), f)
@@ -217,5 +252,8 @@ def main():
 if __name__ == "__main__":
     if sys.argv[1:] == ["config2_final"]:
         write_config2_final(ref())
+    elif sys.argv[1:] == ["config3_frames"]:
+        write_config3_frames(ref())
     else:
         main()
+        write_config3_frames(ref())

@@ -157,6 +157,14 @@ def test_foam_substep_clamp_and_whole_cell_shift(lib):
 
 
 def test_config3_cascade_vs_golden_and_oracle(lib, golden):
+    """All 1200 frames of the three cascades against the reference's digests
+    (every level's displacement and normal and the 512 cascade queries, every
+    frame: tests/golden/config3_frames.json), and the single-point cascade
+    query against the oracle at the frames of the short fixture."""
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                           "config3_frames.json")) as f:
+        every = json.load(f)["cascade"]
+    assert len(every) == 1200
     levels = [desc_from(kw) for kw in CONFIG3_LEVELS]
     arr = (capi.OceanDesc * 3)(*levels)
     st = C.c_int(-1)
@@ -168,21 +176,27 @@ def test_config3_cascade_vs_golden_and_oracle(lib, golden):
     harr = (C.c_void_p * 3)(*[o.h for o in orcs])
     pts = query_points(1, 512, 800.0)[0]
     try:
-        for k in ("0", "1", "2", "60", "600", "1199"):
-            t = int(k) / 60.0
+        flat = np.ascontiguousarray(pts.reshape(-1))
+        for kf in range(1200):
+            k = str(kf)
+            t = kf / 60.0
             lib.ocean_amd_cascade_update(h, t)
-            g = golden["config3_cascade"][k]
+            want = every[kf]
+            if k in golden["config3_cascade"]:
+                g = golden["config3_cascade"][k]
+                assert [want[0:2], want[2:4], want[4:6]] == g["levels"] and want[6] == g["queries"]
             for li, d in enumerate(levels):
                 b = lib.ocean_amd_cascade_get_buffers(h, li)
                 nn = 4 * d.size * d.size
                 disp = np.ctypeslib.as_array(b.displacement, shape=(nn,))
                 nrm = np.ctypeslib.as_array(b.normal, shape=(nn,))
-                assert [digest(disp), digest(nrm)] == g["levels"][li], (k, li)
+                assert [digest(disp), digest(nrm)] == want[2 * li:2 * li + 2], (k, li)
             out = (capi.OceanSurfaceEx * len(pts))()
-            flat = np.ascontiguousarray(pts.reshape(-1))
             assert lib.ocean_amd_cascade_sample_batch(h, fptr(flat), len(pts), out) == 0
             q = np.frombuffer(bytes(out), np.float32).reshape(len(pts), 10)
-            assert digest(q) == g["queries"], k
+            assert digest(q) == want[6], k
+            if k not in golden["config3_cascade"]:
+                continue
             # single-point form and the oracle
             for o in orcs:
                 o.update(t)

@@ -33,8 +33,15 @@ def lib():
 
 
 def test_config3_interaction_scene_vs_golden_and_oracle(lib):
+    """The scene's field after every one of its 1200 substeps against the
+    reference's digests (tests/golden/config3_frames.json), and bit for bit
+    against the oracle, energy included, every 200th."""
     with open(os.path.join(GOLDEN, "digests.json")) as f:
-        want = json.load(f)["config3_interaction"]
+        sparse = json.load(f)["config3_interaction"]
+    with open(os.path.join(GOLDEN, "config3_frames.json")) as f:
+        every = json.load(f)["interaction"]
+    assert len(every) == 1200 and all(every[int(k)] == v for k, v in sparse.items())
+    want = {str(k): v for k, v in enumerate(every)}
     d = idesc(lib.ocean_interaction_desc_init, **CONFIG3_INTERACTION)
     g = Field("gpu", lib, d)
     o = Field("orc", oracle(), d)

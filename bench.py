@@ -83,6 +83,73 @@ def traffic_from_profiles(kernels):
         return None
 
 
+def batch_kernel_names(g, p):
+    """The names (as the profiler prints them, arguments dropped) a batch
+    group's row or column pass dispatches under: the looped or loop-free
+    kernel of its radix, mode 0 (bit-exact) and field layout."""
+    n, f = g["size"], (6 if g["velocity"] else 4)
+    radix = g["radix_row"] if p == "row" else g["radix_col"]
+    r4 = "_r4" if radix == 4 else ""
+    w = 0 if g["layout"] == 0 else 2048 // n
+    return tuple(f"batch_{p}{r4}{v}_kernel<{n}, {f}, 0, {w}>" for v in ("", "_item"))
+
+
+def kernel_key(name: str) -> str:
+    """A profiler kernel name without namespaces and arguments:
+    'void ns::(anonymous namespace)::k<1, 2>(args)' -> 'k<1, 2>'."""
+    head = name.split("(ocean")[0].split("(unsigned")[0].split("(HIP")[0].split("(float")[0]
+    head = head.replace("(anonymous namespace)::", "")
+    return head.split("::")[-1].strip()
+
+
+def traffic_from_pmc(fetch_csv: str, write_csv: str, out_json: str):
+    """profiles/traffic.json from two rocprofv3 --pmc runs (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch) of `bench.py --profile-only`: each kernel's
+    mean per dispatch, each counter scaled by what it reads for the run's
+    256 MiB copy, and the single patch's frame kernels (the N = 512, F = 6
+    row and column kernels of the fixed frame shape) also as row_pass /
+    col_pass. Prints the table the traffic summary in profiles/ holds."""
+    import csv
+    from collections import defaultdict
+
+    def load(path, counter):
+        vals = defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter:
+                vals[kernel_key(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+        return vals
+    fetch, write = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    copy_kib = 262144.0
+    cf = [v for v in fetch.get("__amd_rocclr_copyBuffer", []) if v > copy_kib / 4]
+    cw = [v for v in write.get("__amd_rocclr_copyBuffer", []) if v > copy_kib / 2]
+    f_fac = copy_kib / (sum(cf) / len(cf))
+    w_fac = copy_kib / (sum(cw) / len(cw))
+    rec = {"copy_factor_fetch": f_fac, "copy_factor_write": w_fac, "copy_kib": copy_kib,
+           "copy_fetch_kib_measured": sum(cf) / len(cf), "copy_write_kib_measured": sum(cw) / len(cw),
+           "kernels": {}}
+    row_b, col_b = algorithmic_bytes(512, 6, True)
+    for k in sorted(set(fetch) & set(write)):
+        if not ("pass" in k or "split" in k or k.startswith("batch_")):
+            continue
+        fk = sum(fetch[k]) / len(fetch[k]) * f_fac
+        wk = sum(write[k]) / len(write[k]) * w_fac
+        e = {"fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
+             "traffic_bytes": round((fk + wk) * 1024), "dispatches": len(fetch[k])}
+        rec["kernels"][k] = e
+        # the frame loop's shape: the N = 512, F = 6 kernel with the most dispatches
+        for alias, prefix, alg in (("row_pass", ("row_pass_kernel<512, 6,", "row_split_kernel<512, 6,"), row_b),
+                                   ("col_pass", ("col_pass_kernel<512, 6,", "col_split_kernel<512, 6,"), col_b)):
+            if k.startswith(prefix) and e["dispatches"] > rec["kernels"].get(alias, {}).get("dispatches", 0):
+                rec["kernels"][alias] = dict(e, kernel=k, alg_bytes=alg)
+    with open(out_json, "w") as f:
+        json.dump(rec, f, indent=1)
+    print(f"copy factors: fetch x{f_fac:.4f}, write x{w_fac:.4f}")
+    print("| kernel | dispatches | fetch KiB | write KiB |")
+    print("|---|---|---|---|")
+    for k, e in rec["kernels"].items():
+        print(f"| `{k}` | {e['dispatches']} | {e['fetch_kib']:,.0f} | {e['write_kib']:,.0f} |")
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -249,9 +316,7 @@ def batched_leg(lib, frames: int, warmup: int, tune: bool = True, precision: int
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5),
                      "traffic": traffic_from_profiles(
-                         [tuple(f"batch_{p}{v}_kernel<{g['size']}, {6 if g['velocity'] else 4}, 0>"
-                                for v in ("", "_item"))
-                          for g in groups for p in ("row", "col")])},
+                         [batch_kernel_names(g, p) for g in groups for p in ("row", "col")])},
         "groups": groups, "build_s": round(build_s, 2), "frames": frames,
         "precision": ["exact", "fast", "fast_half"][precision],
     }
@@ -644,7 +709,14 @@ def main():
                     help="run only the three-cascade + interaction scene leg (for rocprofv3)")
     ap.add_argument("--profile-only", action="store_true",
                     help="run only warmup + timed frames (for rocprofv3 runs)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/traffic.json from two rocprofv3 --pmc counter_collection "
+                         "CSVs (FETCH_SIZE, WRITE_SIZE) of a --profile-only run, and exit")
     args = ap.parse_args()
+    if args.traffic_from:
+        traffic_from_pmc(args.traffic_from[0], args.traffic_from[1],
+                         os.path.join(ROOT, "profiles", "traffic.json"))
+        return
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only; --gpus {args.gpus} is not supported "
               "(independent oceans would shard by ocean across GPUs with no communication)",

@@ -141,8 +141,24 @@ struct TapCache {
     float3 a{}, ca{}, d{}, ed{};
 };
 
+// A level's displacement as a global-memory pointer (the levels' pointers
+// arrive through a struct, so the compiler cannot infer it): texel loads then
+// take a scalar base plus a 32-bit byte offset (global_load with saddr)
+// instead of flat loads with 64-bit address arithmetic per texel.
+typedef const __attribute__((address_space(1))) float4 gfloat4;
+__device__ __forceinline__ gfloat4* global_texels(const void* p) { return (gfloat4*)(p); }
+// (dx, height, dz) of texel i: the foam channel is not read.
+typedef float f32x3 __attribute__((ext_vector_type(3)));
+__device__ __forceinline__ float3 texel(gfloat4* __restrict__ disp, uint32_t i)
+{
+    typedef const __attribute__((address_space(1))) char gchar;
+    typedef const __attribute__((address_space(1))) f32x3 gf32x3;
+    const f32x3 v = *(gf32x3*)((gchar*)disp + (i << 4));  // i < n^2 <= 4096^2: the offset fits 32 bits
+    return make_float3(v[0], v[1], v[2]);
+}
+
 template <bool HEIGHT>
-__device__ __forceinline__ void tap(const float4* __restrict__ disp, uint32_t n, const Base& b,
+__device__ __forceinline__ void tap(gfloat4* __restrict__ disp, uint32_t n, const Base& b,
                                     float px, float pz, TapCache& k, float& dx, float& dz, float& h)
 {
     const float flx = floorf(px), flz = floorf(pz);
@@ -153,8 +169,8 @@ __device__ __forceinline__ void tap(const float4* __restrict__ disp, uint32_t n,
     const uint32_t key = z0 * n + x0;
     if (key != k.key) {
         const uint32_t x1 = (x0 + 1u) & mask, z1 = (z0 + 1u) & mask;
-        const float4 a = disp[key], c = disp[z0 * n + x1];
-        const float4 d = disp[z1 * n + x0], e = disp[z1 * n + x1];
+        const float3 a = texel(disp, key), c = texel(disp, z0 * n + x1);
+        const float3 d = texel(disp, z1 * n + x0), e = texel(disp, z1 * n + x1);
         k.a = make_float3(a.x, a.y, a.z);
         k.ca = make_float3(c.x - a.x, c.y - a.y, c.z - a.z);
         k.d = make_float3(d.x, d.y, d.z);
@@ -183,7 +199,7 @@ __device__ __forceinline__ Solve solve_level(const ocean_hip_level& L, const Bas
                                              float dv0, float tol_cells, int max_iter, bool learned,
                                              TapCache& cache)
 {
-    const float4* __restrict__ disp = reinterpret_cast<const float4*>(L.disp);
+    gfloat4* __restrict__ disp = global_texels(L.disp);
     const float inv_cell = float(L.n) / L.patch_length;
     Solve s{du0, dv0, 0, false, false};
     float prev = 3.0e38f;
@@ -765,8 +781,7 @@ __global__ __launch_bounds__(256) void converged_kernel(
         TapCache cache;
         const Solve s = solve_level(L, b, du0, dv0, tol * inv_cell, kMaxIter, use, cache);
         float dx, dz, h;
-        tap<true>(reinterpret_cast<const float4*>(L.disp), L.n, b, b.fx + s.du, b.fz + s.dv, cache,
-                  dx, dz, h);
+        tap<true>(global_texels(L.disp), L.n, b, b.fx + s.du, b.fz + s.dv, cache, dx, dz, h);
         height += h;
         if (iters) iters[size_t(q) * level_count + lv] = uint8_t(s.iters > 255 ? 255 : s.iters);
         if (!s.converged) fl |= 1u;

@@ -1,6 +1,7 @@
 """Builds liboceanlib_amd.so in-tree: HIP kernels for gfx950 + C++20 host code.
 
     python -m oceanlib_amd.build
+    python -m oceanlib_amd.build --host-sanitize OUT_DIR   (ASan + UBSan on host code)
 
 Every .hip file is compiled by hipcc for gfx950 with the bit-exactness flags
 (no FP contraction, correctly rounded fp32 divide/sqrt, no packed-FP32 ops);
@@ -71,5 +72,40 @@ def build(verbose: bool = True) -> str:
     return LIB
 
 
+# Host-code sanitizers for a checking build (device code is built as usual;
+# each flag reaches only the host compiler).
+HOST_SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                 "-Xarch_host", "-fno-omit-frame-pointer",
+                 "-Xarch_host", "-fno-sanitize-recover=undefined"]
+
+
+def build_host_sanitized(out_dir: str) -> str:
+    """The library with AddressSanitizer and UBSan on its host code, into
+    out_dir (objects and liboceanlib_amd.so), for a program that links the
+    sanitizer runtime itself (e.g. tests/cpp_api built with
+    -fsanitize=address,undefined -shared-libsan); run it with
+    ASAN_OPTIONS=protect_shadow_gap=0:alloc_dealloc_mismatch=0 (the ROCm
+    runtime's code object loader frees some operator-new memory with free)."""
+    os.makedirs(out_dir, exist_ok=True)
+
+    def comp(src):
+        obj = os.path.join(out_dir, os.path.basename(src) + ".o")
+        flags = COMMON + (HIP_FLAGS if src.endswith(".hip") else []) + HOST_SANITIZE
+        res = subprocess.run([hipcc()] + flags + ["-c", src, "-o", obj], capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"compile failed: {src}\n{res.stderr}")
+        return obj
+    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
+        objs = list(ex.map(comp, sources()))
+    lib = os.path.join(out_dir, "liboceanlib_amd.so")
+    subprocess.run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", lib] + objs +
+                   ["-lpthread", "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                    "-shared-libsan"], check=True)
+    return lib
+
+
 if __name__ == "__main__":
-    build()
+    if len(sys.argv) == 3 and sys.argv[1] == "--host-sanitize":
+        print(build_host_sanitized(sys.argv[2]))
+    else:
+        build()

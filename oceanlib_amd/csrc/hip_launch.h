@@ -207,8 +207,12 @@ hipError_t ocean_hip_level_checksums(const ocean_hip_level* levels, uint32_t cou
                                       start offsets, level-major float2 per
                                       (level, query); 6 used at 3 levels */
 #define OCEAN_HIP_MLP_SPACING_PARTIALS 64  /* doubles of scratch per level */
-#define OCEAN_HIP_MLP_COARSE_SIDE 512      /* largest coarse grid side kept in scratch:
-                                              MLP_LEVELS * side^2 float2 */
+#define OCEAN_HIP_MLP_COARSE_SIDE 512      /* largest coarse grid side kept in scratch */
+/* Coarse-grid scratch: per level, side rows of side + 2 16-byte records (a
+ * node column's three rows as bf16 pairs, the row wrapped one record either
+ * side), at a fixed level stride. */
+#define OCEAN_HIP_MLP_COARSE_RECORDS (OCEAN_HIP_MLP_COARSE_SIDE * (OCEAN_HIP_MLP_COARSE_SIDE + 2))
+#define OCEAN_HIP_MLP_COARSE_BYTES (OCEAN_HIP_MLP_LEVELS * OCEAN_HIP_MLP_COARSE_RECORDS * 16u)
 typedef struct ocean_hip_mlp {
     uint16_t* bf16;   /* the weights rounded to bf16, in the kernels' layouts */
     float* master;    /* fp32 parameters (biases are read from here)        */
@@ -231,7 +235,7 @@ hipError_t ocean_hip_mlp_spacing_partial(const ocean_hip_level* levels, uint32_t
  * fixed-point step is <= tol metres (at most 64 taps), summed over levels.
  * mlp != NULL: start from the proposal (needs `spacing_partial` from
  * ocean_hip_mlp_spacing_partial; writes `spacing`, MLP_LEVELS int32; uses
- * `coarse` scratch of MLP_LEVELS * COARSE_SIDE^2 float2 and `prop` scratch of
+ * `coarse` scratch of MLP_COARSE_BYTES and `prop` scratch of
  * count * MLP_PROP floats).
  * iters (count * level_count bytes) and flags (count bytes: 1 = a level did
  * not converge, 2 = a proposal was abandoned) may be NULL. */
@@ -253,12 +257,12 @@ hipError_t ocean_hip_mlp_propose(const ocean_hip_level* levels, uint32_t level_c
 #define OCEAN_HIP_STATIC_MAX_QUERIES 262144u
 #define OCEAN_HIP_STATIC_SCRATCH_BYTES \
     (256u + ((OCEAN_HIP_MLP_LEVELS * OCEAN_HIP_MLP_SPACING_PARTIALS * 8u + 255u) / 256u) * 256u + \
-     OCEAN_HIP_MLP_LEVELS * OCEAN_HIP_MLP_COARSE_SIDE * OCEAN_HIP_MLP_COARSE_SIDE * 8u + \
+     OCEAN_HIP_MLP_COARSE_BYTES + \
      OCEAN_HIP_STATIC_MAX_QUERIES * OCEAN_HIP_MLP_PROP * 4u)
 typedef struct ocean_hip_static_scratch_view {
     int32_t* spacing;          /* OCEAN_HIP_MLP_LEVELS                          */
     double* spacing_partial;   /* OCEAN_HIP_MLP_LEVELS * SPACING_PARTIALS      */
-    float* coarse;             /* OCEAN_HIP_MLP_LEVELS * COARSE_SIDE^2 float2  */
+    float* coarse;             /* OCEAN_HIP_MLP_COARSE_BYTES                   */
     float* prop;               /* max_queries * OCEAN_HIP_MLP_PROP             */
     uint32_t max_queries;
 } ocean_hip_static_scratch_view;

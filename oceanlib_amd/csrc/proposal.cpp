@@ -30,7 +30,7 @@ struct HeightProposal::Impl {
     ocean_hip_mlp mlp{};
     std::int32_t* spacing = nullptr;   // 3 per call (queries)
     double* spacing_partial = nullptr; // 3 * 64
-    float* coarse = nullptr;           // 3 * COARSE_SIDE^2 float2
+    float* coarse = nullptr;           // OCEAN_HIP_MLP_COARSE_BYTES of coarse-grid records
     float* prop = nullptr;             // max_queries * OCEAN_HIP_MLP_PROP
     float* partial = nullptr;          // (max_batch / 128) * params
     float* loss_partial = nullptr;
@@ -143,8 +143,7 @@ HeightProposal::HeightProposal(const float* params, std::uint32_t max_queries)
         up256(P * 4), up256(P * 4), up256(P * 4), up256(P * 4),  // master, m1, m2, grad
         256,                                            // spacing
         up256(ProposalShape::levels * OCEAN_HIP_MLP_SPACING_PARTIALS * 8),
-        up256(std::size_t(ProposalShape::levels) * OCEAN_HIP_MLP_COARSE_SIDE *
-              OCEAN_HIP_MLP_COARSE_SIDE * 8),                   // coarse grids
+        up256(OCEAN_HIP_MLP_COARSE_BYTES),              // coarse grids
         up256(std::size_t(max_queries) * OCEAN_HIP_MLP_PROP * 4),  // prop
         up256(blocks * P * 4),                          // partial
         up256(blocks * 4),                              // loss partial

@@ -233,11 +233,8 @@ __device__ __forceinline__ Solve solve_level(const ocean_hip_level& L, const Bas
 // of the level's coarse grid, every s-th cell in x and z (NC = N / s per
 // side, periodic), nearest to the query and its 8 neighbours, each node's
 // (dx, dz) divided by s * cell; then the query's offset from the nearest node
-// in units of s cells, in [-0.5, 0.5). `coarse`, when not null, holds the
-// level's nodes already scaled (coarse_kernel); otherwise they are read from
-// the displacement itself (the same float values either way).
-__device__ __forceinline__ void level_features(const ocean_hip_level& L, const Base& b, int s,
-                                               const float2* __restrict__ coarse, float* f)
+// in units of s cells, in [-0.5, 0.5).
+__device__ __forceinline__ void level_features(const ocean_hip_level& L, const Base& b, int s, float* f)
 {
     const float4* __restrict__ disp = reinterpret_cast<const float4*>(L.disp);
     const uint32_t n = L.n;
@@ -256,15 +253,9 @@ __device__ __forceinline__ void level_features(const ocean_hip_level& L, const B
 #pragma unroll
         for (int ax = -1; ax <= 1; ++ax) {
             const uint32_t xc = uint32_t(cx + ax) & cmask;
-            if (coarse != nullptr) {
-                const float2 v = coarse[zc * nc + xc];
-                f[k++] = v.x;
-                f[k++] = v.y;
-            } else {
-                const float4 v = disp[(zc * uint32_t(s)) * n + xc * uint32_t(s)];
-                f[k++] = v.x * inv_h;
-                f[k++] = v.z * inv_h;
-            }
+            const float4 v = disp[(zc * uint32_t(s)) * n + xc * uint32_t(s)];
+            f[k++] = v.x * inv_h;
+            f[k++] = v.z * inv_h;
         }
     }
     f[k++] = ux - rx;
@@ -508,17 +499,22 @@ __global__ void spacing_final_kernel(const ocean_hip_level* __restrict__ levels,
 }
 
 // The levels' stencil spacings and coarse grids, for levels whose grid has at
-// most OCEAN_HIP_MLP_COARSE_SIDE nodes a side: node (xc, zc) = the
+// most OCEAN_HIP_MLP_COARSE_SIDE nodes a side. Node (xc, zc) is the
 // displacement's (dx, dz) at cell (s xc, s zc) divided by s * cell, as
-// level_features reads it. Every block adds its level's kSpacingBlocks
-// partial sums (spacing_partial_kernel) in block order, as
-// spacing_final_kernel does, and block 0 writes the spacing. Each block
-// strides over the level's nodes. grid: (kCoarseBlocks, levels).
+// level_features computes it, rounded to a bf16 pair (pack_bf16, as the
+// network's input). The grid is stored as records of 16 bytes, NC rows of
+// NC + 2: record (zc, r) holds the nodes (xc, zc - 1), (xc, zc), (xc, zc + 1)
+// of column xc = r - 1 (mod NC), and a zero. A query's 3 x 3 stencil around
+// node (cx, cz) is then records cx .. cx + 2 of row cz: 48 contiguous bytes.
+// Every block adds its level's kSpacingBlocks partial sums
+// (spacing_partial_kernel) in block order, as spacing_final_kernel does, and
+// block 0 writes the spacing. Each block strides over the level's records.
+// grid: (kCoarseBlocks, levels).
 constexpr uint32_t kCoarseBlocks = 128;
 __global__ __launch_bounds__(256) void coarse_kernel(const ocean_hip_level* __restrict__ levels,
                                                      const double* __restrict__ partial,
                                                      int32_t* __restrict__ spacing,
-                                                     float2* __restrict__ coarse)
+                                                     uint4* __restrict__ coarse)
 {
     __shared__ int32_t s_level;
     const ocean_hip_level L = levels[blockIdx.y];
@@ -530,16 +526,22 @@ __global__ __launch_bounds__(256) void coarse_kernel(const ocean_hip_level* __re
         if (blockIdx.x == 0) spacing[blockIdx.y] = s_level;
     }
     __syncthreads();
-    const int s = s_level;
-    const uint32_t nc = L.n / uint32_t(s);
+    const uint32_t s = uint32_t(s_level);
+    const uint32_t nc = L.n / s, cmask = nc - 1u, pitch = nc + 2u;
     if (nc > OCEAN_HIP_MLP_COARSE_SIDE) return;
     const float inv_h = float(L.n) / (L.patch_length * float(s));
     const float4* __restrict__ disp = reinterpret_cast<const float4*>(L.disp);
-    float2* __restrict__ out = coarse + size_t(blockIdx.y) * OCEAN_HIP_MLP_COARSE_SIDE * OCEAN_HIP_MLP_COARSE_SIDE;
-    for (uint32_t node = blockIdx.x * 256u + threadIdx.x; node < nc * nc; node += kCoarseBlocks * 256u) {
-        const uint32_t xc = node % nc, zc = node / nc;
-        const float4 v = disp[(zc * uint32_t(s)) * L.n + xc * uint32_t(s)];
-        out[node] = make_float2(v.x * inv_h, v.z * inv_h);
+    uint4* __restrict__ out = coarse + size_t(blockIdx.y) * OCEAN_HIP_MLP_COARSE_RECORDS;
+    for (uint32_t r = blockIdx.x * 256u + threadIdx.x; r < nc * pitch; r += kCoarseBlocks * 256u) {
+        const uint32_t zc = r / pitch, xc = (r - zc * pitch + cmask) & cmask;
+        uint32_t w[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t z = (zc + cmask + uint32_t(k)) & cmask;  // zc - 1, zc, zc + 1 (mod NC)
+            const float4 v = disp[(z * s) * L.n + xc * s];
+            w[k] = pack_bf16(v.x * inv_h, v.z * inv_h);
+        }
+        out[r] = make_uint4(w[0], w[1], w[2], 0u);
     }
 }
 
@@ -580,27 +582,33 @@ __device__ __forceinline__ void wave_lds_sync()
 }
 
 // The 20 features of one level (level_features' values and order) as 10
-// bf16 pairs, read from the level's coarse grid: node k's (dx, dz) in pair k,
-// the query's offset from the nearest node in pair 9.
+// bf16 pairs, read from the level's coarse-grid records (coarse_kernel): node
+// k's (dx, dz) in pair k, the query's offset from the nearest node in pair 9.
+// The three records of the stencil's columns are one 48-byte span; record a
+// holds column cx + a - 1, rows cz - 1, cz, cz + 1.
+typedef const __attribute__((address_space(1))) uint4 guint4;
+typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void coarse_features_bf16(const Base& b, float inv_s, uint32_t nc,
-                                                     const float2* __restrict__ coarse, uint32_t* pk)
+                                                     guint4* __restrict__ coarse, uint32_t* pk)
 {
+    typedef const __attribute__((address_space(1))) char gchar;
+    typedef const __attribute__((address_space(1))) u32x4v gu32x4;
     const uint32_t cmask = nc - 1u;
     const float ux = (float(b.ix) + b.fx) * inv_s;
     const float uz = (float(b.iz) + b.fz) * inv_s;
     const float rx = floorf(ux + 0.5f), rz = floorf(uz + 0.5f);
-    const int32_t cx = int32_t(rx), cz = int32_t(rz);
-    int k = 0;
+    const uint32_t cx = uint32_t(int32_t(rx)) & cmask, cz = uint32_t(int32_t(rz)) & cmask;
+    // byte offset of record cx in row cz: < 512 * 514 * 16, fits 32 bits
+    const uint32_t off = (cz * (nc + 2u) + cx) << 4;
+    gu32x4* r = (gu32x4*)((gchar*)coarse + off);
+    const u32x4v c0 = r[0], c1 = r[1], c2 = r[2];
 #pragma unroll
-    for (int bz = -1; bz <= 1; ++bz) {
-        const uint32_t row = (uint32_t(cz + bz) & cmask) * nc;
-#pragma unroll
-        for (int ax = -1; ax <= 1; ++ax) {
-            const float2 v = coarse[row + (uint32_t(cx + ax) & cmask)];
-            pk[k++] = pack_bf16(v.x, v.y);
-        }
+    for (int bz = 0; bz < 3; ++bz) {
+        pk[3 * bz] = c0[bz];
+        pk[3 * bz + 1] = c1[bz];
+        pk[3 * bz + 2] = c2[bz];
     }
-    pk[k] = pack_bf16(ux - rx, uz - rz);
+    pk[9] = pack_bf16(ux - rx, uz - rz);
 }
 
 constexpr int kPropWaves = 4;                 // wavefronts per block
@@ -613,7 +621,7 @@ static_assert(kXP == 60 && kIn == 64, "X holds features 0..59; inputs 60..63 are
 __global__ __launch_bounds__(64 * kPropWaves) __attribute__((amdgpu_waves_per_eu(3))) void proposal_kernel(
     const ocean_hip_level* __restrict__ levels, uint32_t level_count, const float2* __restrict__ xz,
     uint32_t count, const uint16_t* __restrict__ pb, const float* __restrict__ pf,
-    const int32_t* __restrict__ spacing, const float2* __restrict__ coarse, float2* __restrict__ prop)
+    const int32_t* __restrict__ spacing, const uint4* __restrict__ coarse, float2* __restrict__ prop)
 {
     __shared__ InferLds w;
     __shared__ __align__(16) uint32_t tiles[kPropWaves][64 * kXP / 2];
@@ -628,7 +636,7 @@ __global__ __launch_bounds__(64 * kPropWaves) __attribute__((amdgpu_waves_per_eu
     double gl[kLevels];
     float inv_s[kLevels], sp[kLevels];
     uint32_t nl[kLevels], ncl[kLevels];
-    const float2* cgl[kLevels];
+    guint4* cgl[kLevels];
 #pragma unroll
     for (int lv = 0; lv < kLevels; ++lv) {
         gl[lv] = 0.0;
@@ -644,7 +652,7 @@ __global__ __launch_bounds__(64 * kPropWaves) __attribute__((amdgpu_waves_per_eu
             nl[lv] = L.n;
             ncl[lv] = L.n / uint32_t(s);
             if (ncl[lv] <= OCEAN_HIP_MLP_COARSE_SIDE)
-                cgl[lv] = coarse + size_t(lv) * OCEAN_HIP_MLP_COARSE_SIDE * OCEAN_HIP_MLP_COARSE_SIDE;
+                cgl[lv] = (guint4*)(coarse + size_t(lv) * OCEAN_HIP_MLP_COARSE_RECORDS);
         }
     }
     __syncthreads();
@@ -670,7 +678,7 @@ __global__ __launch_bounds__(64 * kPropWaves) __attribute__((amdgpu_waves_per_eu
                         coarse_features_bf16(b, inv_s[lv], ncl[lv], cgl[lv], pk);
                     } else {
                         float f[kFeatPerLevel];
-                        level_features(levels[lv], b, int(sp[lv]), nullptr, f);
+                        level_features(levels[lv], b, int(sp[lv]), f);
 #pragma unroll
                         for (int k = 0; k < kFeatPerLevel / 2; ++k) pk[k] = pack_bf16(f[2 * k], f[2 * k + 1]);
                     }
@@ -812,7 +820,7 @@ __global__ __launch_bounds__(256) void pairs_kernel(const ocean_hip_level* __res
         const ocean_hip_level L = levels[lv];
         const Base b = base_cell(p.x, p.y, L);
         const int s = spacing[lv];
-        level_features(L, b, s, nullptr, f + lv * kFeatPerLevel);
+        level_features(L, b, s, f + lv * kFeatPerLevel);
         const float inv_cell = float(L.n) / L.patch_length;
         TapCache cache;
         const Solve r = solve_level(L, b, 0.0f, 0.0f, tol * inv_cell, 255, false, cache);
@@ -1118,7 +1126,7 @@ extern "C" hipError_t ocean_hip_mlp_propose(const ocean_hip_level* levels, uint3
     if (mlp == nullptr || spacing_partial == nullptr || spacing == nullptr || prop == nullptr ||
         coarse == nullptr)
         return hipErrorInvalidValue;
-    float2* coarse2 = reinterpret_cast<float2*>(coarse);
+    uint4* coarse2 = reinterpret_cast<uint4*>(coarse);
     hipLaunchKernelGGL(coarse_kernel,
                        dim3(kCoarseBlocks, level_count < uint32_t(kLevels) ? level_count : uint32_t(kLevels)),
                        dim3(256), 0, s, levels, spacing_partial, spacing, coarse2);
@@ -1238,8 +1246,7 @@ namespace {
 constexpr size_t kStaticSpacingBytes = 256;
 constexpr size_t kStaticPartialBytes =
     (size_t(OCEAN_HIP_MLP_LEVELS) * OCEAN_HIP_MLP_SPACING_PARTIALS * sizeof(double) + 255) / 256 * 256;
-constexpr size_t kStaticCoarseBytes =
-    size_t(OCEAN_HIP_MLP_LEVELS) * OCEAN_HIP_MLP_COARSE_SIDE * OCEAN_HIP_MLP_COARSE_SIDE * 2 * sizeof(float);
+constexpr size_t kStaticCoarseBytes = OCEAN_HIP_MLP_COARSE_BYTES;
 constexpr size_t kStaticPropBytes = size_t(OCEAN_HIP_STATIC_MAX_QUERIES) * OCEAN_HIP_MLP_PROP * sizeof(float);
 constexpr size_t kStaticBytes = kStaticSpacingBytes + kStaticPartialBytes + kStaticCoarseBytes + kStaticPropBytes;
 static_assert(kStaticBytes == OCEAN_HIP_STATIC_SCRATCH_BYTES, "static scratch layout");

@@ -351,7 +351,7 @@ int  ocean_amd_batch_height_converged(const ocean_amd_batch* batch, uint32_t fir
  * the model's max_queries does not bound it. Limit: count <=
  * OCEAN_AMD_STATIC_MAX_QUERIES (larger counts return OCEAN_ERROR_INVALID_ARG;
  * split them into calls of at most that many). Each GPU holds one copy of the
- * array (about 14.7 MB) until the process ends, shared by every call on that
+ * array (about 21.0 MB) until the process ends, shared by every call on that
  * GPU: calls that overlap on one GPU (other streams, other threads) race on
  * it. Without a model no scratch is used. */
 #define OCEAN_AMD_STATIC_MAX_QUERIES 262144u

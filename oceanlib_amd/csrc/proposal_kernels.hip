@@ -479,6 +479,7 @@ __global__ __launch_bounds__(256) void spacing_partial_kernel(const ocean_hip_le
     const float4* __restrict__ disp = reinterpret_cast<const float4*>(L.disp);
     const uint32_t cells = L.n * L.n;
     double acc = 0.0;
+#pragma unroll 8
     for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < cells; i += kSpacingBlocks * 256u) {
         const float4 v = disp[i];
         acc += double(v.x) * double(v.x) + double(v.z) * double(v.z);
@@ -508,20 +509,24 @@ __global__ void spacing_final_kernel(const ocean_hip_level* __restrict__ levels,
 // node (cx, cz) is then records cx .. cx + 2 of row cz: 48 contiguous bytes.
 // Every block adds its level's kSpacingBlocks partial sums
 // (spacing_partial_kernel) in block order, as spacing_final_kernel does, and
-// block 0 writes the spacing. Each block strides over the level's records.
-// grid: (kCoarseBlocks, levels).
+// block 0 writes the spacing. Each thread then builds a run of kCoarseRun
+// records down one record column (kCoarseRun + 2 node reads), the block
+// striding over the level's runs. grid: (kCoarseBlocks, levels).
 constexpr uint32_t kCoarseBlocks = 128;
+constexpr uint32_t kCoarseRun = 8;
 __global__ __launch_bounds__(256) void coarse_kernel(const ocean_hip_level* __restrict__ levels,
                                                      const double* __restrict__ partial,
                                                      int32_t* __restrict__ spacing,
                                                      uint4* __restrict__ coarse)
 {
+    __shared__ double s_part[kSpacingBlocks];
     __shared__ int32_t s_level;
     const ocean_hip_level L = levels[blockIdx.y];
+    if (threadIdx.x < uint32_t(kSpacingBlocks)) s_part[threadIdx.x] = partial[blockIdx.y * kSpacingBlocks + threadIdx.x];
+    __syncthreads();
     if (threadIdx.x == 0) {
         double sum = 0.0;
-#pragma unroll 8
-        for (int b = 0; b < kSpacingBlocks; ++b) sum += partial[blockIdx.y * kSpacingBlocks + b];
+        for (int b = 0; b < kSpacingBlocks; ++b) sum += s_part[b];
         s_level = spacing_of(sum, L);
         if (blockIdx.x == 0) spacing[blockIdx.y] = s_level;
     }
@@ -532,16 +537,20 @@ __global__ __launch_bounds__(256) void coarse_kernel(const ocean_hip_level* __re
     const float inv_h = float(L.n) / (L.patch_length * float(s));
     const float4* __restrict__ disp = reinterpret_cast<const float4*>(L.disp);
     uint4* __restrict__ out = coarse + size_t(blockIdx.y) * OCEAN_HIP_MLP_COARSE_RECORDS;
-    for (uint32_t r = blockIdx.x * 256u + threadIdx.x; r < nc * pitch; r += kCoarseBlocks * 256u) {
-        const uint32_t zc = r / pitch, xc = (r - zc * pitch + cmask) & cmask;
-        uint32_t w[3];
+    const uint32_t runs = (nc + kCoarseRun - 1u) / kCoarseRun;
+    for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < runs * pitch; t += kCoarseBlocks * 256u) {
+        const uint32_t zr = t / pitch, xr = t - zr * pitch;
+        const uint32_t xc = (xr + cmask) & cmask, z0 = zr * kCoarseRun;  // column r - 1 (mod NC)
+        uint32_t w[kCoarseRun + 2];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const uint32_t z = (zc + cmask + uint32_t(k)) & cmask;  // zc - 1, zc, zc + 1 (mod NC)
+        for (uint32_t k = 0; k < kCoarseRun + 2; ++k) {
+            const uint32_t z = (z0 + cmask + k) & cmask;  // rows z0 - 1 .. z0 + kCoarseRun (mod NC)
             const float4 v = disp[(z * s) * L.n + xc * s];
             w[k] = pack_bf16(v.x * inv_h, v.z * inv_h);
         }
-        out[r] = make_uint4(w[0], w[1], w[2], 0u);
+#pragma unroll
+        for (uint32_t k = 0; k < kCoarseRun; ++k)
+            if (z0 + k < nc) out[(z0 + k) * pitch + xr] = make_uint4(w[k], w[k + 1], w[k + 2], 0u);
     }
 }
 
@@ -585,30 +594,56 @@ __device__ __forceinline__ void wave_lds_sync()
 // bf16 pairs, read from the level's coarse-grid records (coarse_kernel): node
 // k's (dx, dz) in pair k, the query's offset from the nearest node in pair 9.
 // The three records of the stencil's columns are one 48-byte span; record a
-// holds column cx + a - 1, rows cz - 1, cz, cz + 1.
+// holds column cx + a - 1, rows cz - 1, cz, cz + 1. coarse_span gives the
+// span's byte offset and pair 9; the records are then loaded
+// (coarse_record) and unpacked (stencil_pairs).
 typedef const __attribute__((address_space(1))) uint4 guint4;
-typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void coarse_features_bf16(const Base& b, float inv_s, uint32_t nc,
-                                                     guint4* __restrict__ coarse, uint32_t* pk)
+typedef uint32_t u32x3v __attribute__((ext_vector_type(3)));
+__device__ __forceinline__ uint32_t coarse_span(const Base& b, float inv_s, uint32_t nc, uint32_t& offset_pair)
 {
-    typedef const __attribute__((address_space(1))) char gchar;
-    typedef const __attribute__((address_space(1))) u32x4v gu32x4;
     const uint32_t cmask = nc - 1u;
     const float ux = (float(b.ix) + b.fx) * inv_s;
     const float uz = (float(b.iz) + b.fz) * inv_s;
     const float rx = floorf(ux + 0.5f), rz = floorf(uz + 0.5f);
     const uint32_t cx = uint32_t(int32_t(rx)) & cmask, cz = uint32_t(int32_t(rz)) & cmask;
-    // byte offset of record cx in row cz: < 512 * 514 * 16, fits 32 bits
-    const uint32_t off = (cz * (nc + 2u) + cx) << 4;
-    gu32x4* r = (gu32x4*)((gchar*)coarse + off);
-    const u32x4v c0 = r[0], c1 = r[1], c2 = r[2];
+    offset_pair = pack_bf16(ux - rx, uz - rz);
+    // record cx of row cz: < 512 * 514 * 16 bytes, fits 32 bits
+    return (cz * (nc + 2u) + cx) << 4;
+}
+// The record's three node pairs (its fourth dword, the zero, is not read:
+// a 16-byte load's unused register would be reused while the load is in
+// flight, and the compiler then waits for it).
+__device__ __forceinline__ u32x3v coarse_record(guint4* __restrict__ coarse, uint32_t byte_offset)
+{
+    typedef const __attribute__((address_space(1))) char gchar;
+    typedef const __attribute__((address_space(1))) u32x3v gu32x3;
+    return *(gu32x3*)((gchar*)coarse + byte_offset);
+}
+__device__ __forceinline__ void stencil_pairs(u32x3v c0, u32x3v c1, u32x3v c2, uint32_t offset_pair, uint32_t* pk)
+{
 #pragma unroll
     for (int bz = 0; bz < 3; ++bz) {
         pk[3 * bz] = c0[bz];
         pk[3 * bz + 1] = c1[bz];
         pk[3 * bz + 2] = c2[bz];
     }
-    pk[9] = pack_bf16(ux - rx, uz - rz);
+    pk[9] = offset_pair;
+}
+__device__ __forceinline__ void coarse_features_bf16(const Base& b, float inv_s, uint32_t nc,
+                                                     guint4* __restrict__ coarse, uint32_t* pk)
+{
+    uint32_t op;
+    const uint32_t off = coarse_span(b, inv_s, nc, op);
+    stencil_pairs(coarse_record(coarse, off), coarse_record(coarse, off + 16u), coarse_record(coarse, off + 32u),
+                  op, pk);
+}
+
+// Level lv's ten feature pairs into row l of the wavefront's X tile.
+__device__ __forceinline__ void store_level_pairs(uint32_t* X, int l, int lv, const uint32_t* pk)
+{
+    uint2* dst = reinterpret_cast<uint2*>(X + l * (kLevels * kFeatPerLevel / 2) + lv * (kFeatPerLevel / 2));
+#pragma unroll
+    for (int c = 0; c < kFeatPerLevel / 4; ++c) dst[c] = make_uint2(pk[2 * c], pk[2 * c + 1]);
 }
 
 constexpr int kPropWaves = 4;                 // wavefronts per block
@@ -655,6 +690,10 @@ __global__ __launch_bounds__(64 * kPropWaves) __attribute__((amdgpu_waves_per_eu
                 cgl[lv] = (guint4*)(coarse + size_t(lv) * OCEAN_HIP_MLP_COARSE_RECORDS);
         }
     }
+    // The common case, all three levels in use and on coarse grids, gathers
+    // in straight-line code: the nine record loads are all issued before the
+    // first wait.
+    const bool straight = used == uint32_t(kLevels) && cgl[0] != nullptr && cgl[1] != nullptr && cgl[2] != nullptr;
     __syncthreads();
     const uint32_t waves = gridDim.x * kPropWaves;
     const uint32_t first = (blockIdx.x * kPropWaves + (threadIdx.x >> 6)) * 64u;
@@ -668,27 +707,46 @@ __global__ __launch_bounds__(64 * kPropWaves) __attribute__((amdgpu_waves_per_eu
         const uint32_t qn = q + waves * 64u;
         if (qn < count) p_next = xz[qn];
         wave_lds_sync();  // the previous round's reads of X are done
+        if (straight) {
+            // Lanes past the end gather at a stale or zero point: every
+            // offset is wrapped into its grid, the rows are independent
+            // columns of the products, and their outputs are not stored.
+            uint32_t off[kLevels], op[kLevels];
 #pragma unroll
-        for (int lv = 0; lv < kLevels; ++lv) {
-            if (uint32_t(lv) < used) {
+            for (int lv = 0; lv < kLevels; ++lv)
+                off[lv] = coarse_span(base_cell_g(p.x, p.y, gl[lv], nl[lv]), inv_s[lv], ncl[lv], op[lv]);
+            u32x3v c[kLevels][3];
+#pragma unroll
+            for (int lv = 0; lv < kLevels; ++lv)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) c[lv][a] = coarse_record(cgl[lv], off[lv] + 16u * uint32_t(a));
+#pragma unroll
+            for (int lv = 0; lv < kLevels; ++lv) {
                 uint32_t pk[kFeatPerLevel / 2];
-                if (live) {
-                    const Base b = base_cell_g(p.x, p.y, gl[lv], nl[lv]);
-                    if (cgl[lv] != nullptr) {
-                        coarse_features_bf16(b, inv_s[lv], ncl[lv], cgl[lv], pk);
+                stencil_pairs(c[lv][0], c[lv][1], c[lv][2], op[lv], pk);
+                store_level_pairs(X, l, lv, pk);
+            }
+        } else {
+#pragma unroll
+            for (int lv = 0; lv < kLevels; ++lv) {
+                if (uint32_t(lv) < used) {
+                    uint32_t pk[kFeatPerLevel / 2];
+                    if (live) {
+                        const Base b = base_cell_g(p.x, p.y, gl[lv], nl[lv]);
+                        if (cgl[lv] != nullptr) {
+                            coarse_features_bf16(b, inv_s[lv], ncl[lv], cgl[lv], pk);
+                        } else {
+                            float f[kFeatPerLevel];
+                            level_features(levels[lv], b, int(sp[lv]), f);
+#pragma unroll
+                            for (int k = 0; k < kFeatPerLevel / 2; ++k) pk[k] = pack_bf16(f[2 * k], f[2 * k + 1]);
+                        }
                     } else {
-                        float f[kFeatPerLevel];
-                        level_features(levels[lv], b, int(sp[lv]), f);
 #pragma unroll
-                        for (int k = 0; k < kFeatPerLevel / 2; ++k) pk[k] = pack_bf16(f[2 * k], f[2 * k + 1]);
+                        for (int k = 0; k < kFeatPerLevel / 2; ++k) pk[k] = 0u;
                     }
-                } else {
-#pragma unroll
-                    for (int k = 0; k < kFeatPerLevel / 2; ++k) pk[k] = 0u;
+                    store_level_pairs(X, l, lv, pk);
                 }
-                uint2* dst = reinterpret_cast<uint2*>(X + l * (kXP / 2) + lv * (kFeatPerLevel / 2));
-#pragma unroll
-                for (int c = 0; c < kFeatPerLevel / 4; ++c) dst[c] = make_uint2(pk[2 * c], pk[2 * c + 1]);
             }
         }
         wave_lds_sync();

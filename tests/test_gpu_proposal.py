@@ -213,15 +213,12 @@ def test_pairs_match_oracle(lib, sea):
     assert np.all(f[:, 60:] == 0) and np.all(t[:, 6:] == 0)
 
 
-def test_proposal_starts_match_oracle_network(lib, sea):
-    """The proposal kernel (features read from the levels' coarse grids, the
-    network as bf16 MFMA tiles) against the float64 oracle network on the
-    same points' features from the pairs kernel: every level's proposed start
-    offset, in cells, within the bf16 tolerance of the forward test. The
-    point count is not a multiple of a round's 64 queries."""
-    b, _ = sea
-    n = 4096 + 37
-    xz = points(n, 12)
+def check_starts_against_oracle_network(lib, b, n, seed):
+    """Every level's proposed start offset from ocean_amd_proposal_starts, in
+    cells, against the float64 oracle network on the same points' features
+    from the pairs kernel, within the bf16 tolerance of the forward test.
+    Returns the levels' spacings."""
+    xz = points(n, seed)
     params = np.load(PARAMS_FILE, allow_pickle=False) if os.path.exists(PARAMS_FILE) \
         else random_params(5)
     m = model(lib, params, max_queries=n)
@@ -250,6 +247,36 @@ def test_proposal_starts_match_oracle_network(lib, sea):
         scale = np.abs(w).max()
         assert np.median(err) <= 1e-5 * scale, (lv, np.median(err), scale)
         assert err.max() <= 2e-2 * scale, (lv, err.max(), scale)
+    return sp
+
+
+def test_proposal_starts_match_oracle_network(lib, sea):
+    """The proposal kernel with every level on a coarse grid (its
+    straight-line gather of 48-byte record spans) against the oracle network.
+    The point count is not a multiple of a round's 64 queries."""
+    b, _ = sea
+    check_starts_against_oracle_network(lib, b, 4096 + 37, 12)
+
+
+def test_proposal_starts_with_a_level_past_the_coarse_grids(lib):
+    """A 1024^2 level of small waves has stencil spacing 1, so its node grid
+    (1024 a side) is larger than the coarse-grid scratch: the proposal kernel
+    takes its general gather, reading that level's nodes from the
+    displacement itself, and must still match the oracle network."""
+    levels = [dict(size=1024, patch_length=1000.0, wind_speed=3.0, choppiness=1.0, seed=21),
+              dict(size=512, patch_length=150.0, wind_speed=22.0, choppiness=1.3, seed=22),
+              dict(size=512, patch_length=25.0, wind_speed=22.0, choppiness=1.3, seed=23)]
+    arr = (capi.OceanDesc * 3)(*[desc_from(kw) for kw in levels])
+    st = C.c_int(-1)
+    b = lib.ocean_amd_batch_create(arr, 3, C.byref(st))
+    assert b, st.value
+    try:
+        lib.ocean_amd_batch_update(b, T_SEA)
+        assert lib.ocean_amd_batch_synchronize(b) == 0
+        sp = check_starts_against_oracle_network(lib, b, 2048 + 5, 13)
+    finally:
+        lib.ocean_amd_batch_destroy(b)
+    assert 1024 // sp[0] > 512, sp
 
 
 def test_plain_converged_heights_within_0_01_mm(lib, sea):

@@ -320,6 +320,35 @@ __device__ void load_infer(InferLds& w, const uint16_t* __restrict__ pb, const f
     for (int i = threadIdx.x; i < kOut; i += blockDim.x) w.b3[i] = pf[ob3 + i];
 }
 
+// load_infer for a block of NT threads, with every global load issued
+// before the first LDS store: one round trip instead of one per stride.
+template <int NT>
+__device__ __forceinline__ void load_infer_batched(InferLds& w, const uint16_t* __restrict__ pb,
+                                                   const float* __restrict__ pf)
+{
+    constexpr int kParts = (kHid + kHid + kOut) * 16;  // 8-byte parts of the transposed weights
+    static_assert(kParts % NT == 0 && NT >= kHid + kHid + kOut, "block size");
+    const uint2* src = reinterpret_cast<const uint2*>(pb + bT1);
+    const int t = int(threadIdx.x);
+    uint2 v[kParts / NT];
+#pragma unroll
+    for (int k = 0; k < kParts / NT; ++k) v[k] = src[t + k * NT];
+    // thread t < 144 brings bias t of b1, b2, b3 in turn
+    const int bias_at = t < kHid ? ob1 + t : (t < 2 * kHid ? ob2 + t - kHid : ob3 + t - 2 * kHid);
+    const float bias = t < 2 * kHid + kOut ? pf[bias_at] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < kParts / NT; ++k) {
+        const int c = t + k * NT, row = c >> 4, part = c & 15;
+        *reinterpret_cast<uint2*>(w.wt + row * kPW + part * 4) = v[k];
+    }
+    if (t < kHid)
+        w.b1[t] = bias;
+    else if (t < 2 * kHid)
+        w.b2[t - kHid] = bias;
+    else if (t < 2 * kHid + kOut)
+        w.b3[t - 2 * kHid] = bias;
+}
+
 // One 16-row tile: out = act(in * W + b). in: 16 rows at pitch PI; wt: W^T
 // [NO][PWT]; RELU: bf16 out at pitch PO; else fp32 outf at pitch PO.
 template <int NI, int NO, int PI, int PWT, int PO, bool RELU>
@@ -570,12 +599,22 @@ __global__ __launch_bounds__(256) void coarse_kernel(const ocean_hip_level* __re
 // relu(acc + bias) rounded to bf16, as the next layer's B fragment; lane
 // (i, g)'s D fragment of output tile `it` holds units 16 it + 4 g + e.
 // The proposal kernel's accumulators start from the bias (z = b + sum), so
-// here only max(z, 0) remains; max and z > 0 ? z : 0 differ only in the sign
-// of a zero, which leaves every later sum unchanged.
+// here only the rectifier remains. It is applied after the rounding, to each
+// bf16 of a packed pair as a 16-bit integer max with 0 (v_pk_max_i16): a
+// bf16 with its sign bit set is a negative int16, so negative values and -0
+// become +0 and the rest pass unchanged. Rounding to nearest is monotonic
+// and keeps the sign, so this equals rounding z > 0 ? z : 0 for every
+// non-NaN z (a NaN with a clear sign bit passes through, as numpy's maximum
+// lets it).
+typedef short s16x2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t relu_pair(uint32_t v)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2v, v), s16x2v{0, 0}));
+}
 __device__ __forceinline__ bf16x4 relu_bf16(f32x4 z)
 {
-    const uint32_t lo = pack_bf16(fmaxf(z[0], 0.0f), fmaxf(z[1], 0.0f));
-    const uint32_t hi = pack_bf16(fmaxf(z[2], 0.0f), fmaxf(z[3], 0.0f));
+    const uint32_t lo = relu_pair(pack_bf16(z[0], z[1]));
+    const uint32_t hi = relu_pair(pack_bf16(z[2], z[3]));
     return __builtin_bit_cast(bf16x4, u32x2v{lo, hi});
 }
 // Four consecutive fp32 from LDS (16-byte aligned): a bias fragment.
@@ -663,7 +702,7 @@ __global__ __launch_bounds__(64 * kPropWaves) __attribute__((amdgpu_waves_per_eu
     const int l = threadIdx.x & 63;
     const int i = l & 15, g = l >> 4;
     uint32_t* X = tiles[threadIdx.x >> 6];
-    load_infer(w, pb, pf);
+    load_infer_batched<64 * kPropWaves>(w, pb, pf);
     const uint32_t used = level_count < uint32_t(kLevels) ? level_count : uint32_t(kLevels);
     // feature columns of levels not in use stay zero
     for (uint32_t c = used * (kFeatPerLevel / 2); c < uint32_t(kXP / 2); ++c) X[l * (kXP / 2) + c] = 0u;
@@ -1182,7 +1221,7 @@ extern "C" hipError_t ocean_hip_mlp_propose(const ocean_hip_level* levels, uint3
 {
     if (count == 0) return hipSuccess;
     if (mlp == nullptr || spacing_partial == nullptr || spacing == nullptr || prop == nullptr ||
-        coarse == nullptr)
+        coarse == nullptr || level_count == 0)
         return hipErrorInvalidValue;
     uint4* coarse2 = reinterpret_cast<uint4*>(coarse);
     hipLaunchKernelGGL(coarse_kernel,

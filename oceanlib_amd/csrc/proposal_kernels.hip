@@ -22,6 +22,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <mutex>
 #include <stdint.h>
 
@@ -43,6 +44,7 @@ constexpr int ob3 = oW3 + kHid * kOut;
 constexpr int kParams = ob3 + kOut;
 static_assert(kParams == OCEAN_HIP_MLP_PARAMS, "parameter layout");
 constexpr int kMaxIter = 64;
+constexpr int kMaxDevices = 64;  // devices whose per-device launch facts are cached
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -1227,15 +1229,20 @@ extern "C" hipError_t ocean_hip_mlp_propose(const ocean_hip_level* levels, uint3
     hipLaunchKernelGGL(coarse_kernel,
                        dim3(kCoarseBlocks, level_count < uint32_t(kLevels) ? level_count : uint32_t(kLevels)),
                        dim3(256), 0, s, levels, spacing_partial, spacing, coarse2);
-    static int prop_grid = 0;  // resident proposal blocks on this device
+    // resident proposal blocks, looked up once per device (a race between two
+    // first calls only repeats the lookup)
+    static std::atomic<int> resident[kMaxDevices];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    int prop_grid = resident[dev].load(std::memory_order_relaxed);
     if (prop_grid == 0) {
-        int dev = 0, cus = 0, per = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        int cus = 0, per = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, proposal_kernel, 64 * kPropWaves, 0) !=
                 hipSuccess)
             return hipErrorInvalidValue;
         prop_grid = cus * (per > 0 ? per : 1);
+        resident[dev].store(prop_grid, std::memory_order_relaxed);
     }
     const uint32_t need = (count + 64u * kPropWaves - 1u) / (64u * kPropWaves);
     const uint32_t pg = need < uint32_t(prop_grid) ? need : uint32_t(prop_grid);
@@ -1317,15 +1324,19 @@ extern "C" hipError_t ocean_hip_mlp_step(ocean_hip_mlp* mlp, const float* feats,
 
 extern "C" int ocean_hip_mlp_configure()
 {
-    static int done = 0;
-    if (done) return 0;
+    // set once per device (the attributes are idempotent: concurrent first
+    // calls both set them)
+    static std::atomic<bool> done[kMaxDevices];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return -1;
+    if (done[dev].load(std::memory_order_acquire)) return 0;
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(&forward_kernel),
                             hipFuncAttributeMaxDynamicSharedMemorySize, int(kInferSmem)) != hipSuccess)
         return -1;
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(&train_kernel),
                             hipFuncAttributeMaxDynamicSharedMemorySize, int(kTrainSmem)) != hipSuccess)
         return -1;
-    done = 1;
+    done[dev].store(true, std::memory_order_release);
     return 0;
 }
 
@@ -1347,7 +1358,7 @@ constexpr size_t kStaticCoarseBytes = OCEAN_HIP_MLP_COARSE_BYTES;
 constexpr size_t kStaticPropBytes = size_t(OCEAN_HIP_STATIC_MAX_QUERIES) * OCEAN_HIP_MLP_PROP * sizeof(float);
 constexpr size_t kStaticBytes = kStaticSpacingBytes + kStaticPartialBytes + kStaticCoarseBytes + kStaticPropBytes;
 static_assert(kStaticBytes == OCEAN_HIP_STATIC_SCRATCH_BYTES, "static scratch layout");
-constexpr int kStaticMaxDevices = 64;
+constexpr int kStaticMaxDevices = kMaxDevices;
 }  // namespace
 
 static __device__ __align__(256) unsigned char scratch[kStaticBytes];

@@ -322,6 +322,34 @@ def test_learned_zero_and_bad_models_still_converge(lib, sea):
         lib.ocean_amd_proposal_destroy(m)
 
 
+def test_learned_query_on_fewer_levels(lib, sea):
+    """Learned queries over one or two levels (the proposal kernel's general
+    gather, the missing levels' features zero) and over the stack's later
+    levels only: every height within 0.01 mm of the oracle's fixed point over
+    the same levels, every level converged (a poor proposal falls back)."""
+    b, bufs = sea
+    n = 4096 + 11
+    xz = points(n, 14)
+    params = np.load(PARAMS_FILE, allow_pickle=False) if os.path.exists(PARAMS_FILE) \
+        else random_params(5)
+    m = model(lib, params, max_queries=n)
+    try:
+        dxz = DeviceArray.from_numpy(xz)
+        for first, count in ((0, 1), (0, 2), (1, 2)):
+            h = DeviceArray((n,), np.float32)
+            it = DeviceArray((n, count), np.uint8)
+            fl = DeviceArray((n,), np.uint8)
+            assert lib.ocean_amd_batch_height_converged(b, first, count, dxz.ptr, n, TOL, m, h.ptr,
+                                                        it.ptr, fl.ptr) == 0
+            assert lib.ocean_amd_batch_synchronize(b) == 0
+            want, ok = inv.height_converged(bufs[first:first + count], xz[:, 0], xz[:, 1])
+            assert ok.all()
+            assert np.all((fl.numpy() & 1) == 0), (first, count)
+            assert np.abs(h.numpy() - want).max() <= MAX_ERR, (first, count)
+    finally:
+        lib.ocean_amd_proposal_destroy(m)
+
+
 @pytest.mark.skipif(not os.path.exists(PARAMS_FILE), reason="no trained proposal committed")
 def test_trained_proposal_saves_iterations_at_equal_error(lib, sea):
     b, bufs = sea

@@ -16,8 +16,10 @@
 // nodes at spacing s_l cells around the node nearest the query, divided by
 // s_l * cell_l, and the query's offset from that node divided by s_l. Outputs:
 // the start offset of each level, (u - X, v - Z) / (s_l * cell_l). The spacing
-// s_l = clamp(round(2 * rms_l / cell_l), 1, 16) follows each level's current
-// displacement RMS, so the stencil spans the region the solution lies in.
+// s_l, the power of two in 1..16 nearest (in log scale) to 2 * rms_l / cell_l,
+// follows each level's current displacement RMS, so the stencil spans the
+// region the solution lies in; the nodes then sit at multiples of s_l, and
+// each call gathers them once into a coarse grid per level.
 #include "hip_launch.h"
 
 #include <hip/hip_runtime.h>

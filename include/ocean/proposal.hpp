@@ -14,10 +14,11 @@
 // activations on MFMA with fp32 accumulation. Features of level l: the
 // (dx, dz) displacement of the 3x3 grid nodes at spacing s_l cells around the
 // node nearest the query, in units of s_l cells, then the query's offset
-// from that node in the same units; s_l = clamp(round(2 rms_l / cell_l), 1,
-// 16) from the level's displacement RMS. Outputs: each level's start offset
-// (u - x, v - z), in units of s_l cells. It is trained on the device by
-// train_step() on pairs from pairs(), which solves with the plain iteration.
+// from that node in the same units; s_l is the power of two in 1..16 nearest
+// (in log scale) to 2 rms_l / cell_l, from the level's displacement RMS.
+// Outputs: each level's start offset (u - x, v - z), in units of s_l cells.
+// It is trained on the device by train_step() on pairs from pairs(), which
+// solves with the plain iteration.
 #pragma once
 
 #include <cstddef>

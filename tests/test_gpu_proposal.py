@@ -245,7 +245,8 @@ def check_starts_against_oracle_network(lib, b, n, seed):
         w = want[:, 2 * lv:2 * lv + 2] * sp[lv]
         err = np.abs(got[lv] - w)
         scale = np.abs(w).max()
-        assert np.median(err) <= 1e-5 * scale, (lv, np.median(err), scale)
+        if n >= 16:  # the median of a handful of points is one point's error
+            assert np.median(err) <= 1e-5 * scale, (lv, np.median(err), scale)
         assert err.max() <= 2e-2 * scale, (lv, err.max(), scale)
     return sp
 
@@ -256,6 +257,16 @@ def test_proposal_starts_match_oracle_network(lib, sea):
     The point count is not a multiple of a round's 64 queries."""
     b, _ = sea
     check_starts_against_oracle_network(lib, b, 4096 + 37, 12)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 64 * 4 * 3 + 1])
+def test_proposal_starts_at_round_boundaries(lib, sea, n):
+    """Point counts at and around one wavefront's round of 64 queries (and
+    one more than a few blocks' rounds): the lanes past the end gather at a
+    stale point and store nothing; every stored proposal matches the oracle
+    network."""
+    b, _ = sea
+    check_starts_against_oracle_network(lib, b, n, 100 + n)
 
 
 def test_proposal_starts_with_a_level_past_the_coarse_grids(lib):

@@ -28,9 +28,15 @@ __device__ __forceinline__ void sample_point(uint32_t q, const float4* __restric
 {
     if (q >= count) return;
     const float2 p = xz[q];
+    if (out == nullptr) {
+        // Heights only (Ocean::height_at): the same taps, without the normal
+        // and velocity loads.
+        if (heights != nullptr) heights[q] = height_level(disp, n, inv_cell, p.x, p.y);
+        return;
+    }
     const ocean_hip_surface s = sample_level(disp, normal, vel, n, inv_cell, p.x, p.y);
     if (heights != nullptr) heights[q] = s.height;
-    if (out != nullptr) out[q] = s;
+    out[q] = s;
 }
 
 __global__ __launch_bounds__(kBlock) void sample_kernel(

@@ -57,6 +57,19 @@ __global__ __launch_bounds__(kBlock) void cascade_sample_kernel(
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= count) return;
     const float2 p = xz[q];
+    if (out == nullptr) {
+        // Heights only: each level's height taps alone, summed in level order
+        // from 0 as the full query sums them.
+        if (heights == nullptr) return;
+        float height = 0.0f;
+        for (uint32_t l = 0; l < level_count; ++l) {
+            const ocean_hip_level L = levels[l];
+            height += height_level(reinterpret_cast<const float4*>(L.disp), L.n,
+                                   static_cast<float>(L.n) / L.patch_length, p.x, p.y);
+        }
+        heights[q] = height;
+        return;
+    }
     const ocean_hip_surface o = sample_cascade(levels, level_count, p.x, p.y);
     if (heights != nullptr) heights[q] = o.height;
     if (out != nullptr) out[q] = o;

@@ -220,6 +220,43 @@ def test_cascade_rejects_empty_and_bad_levels(lib):
     assert st.value == capi.OCEAN_ERROR_INVALID_ARG
 
 
+@pytest.mark.parametrize("q", [3000, 100_000])
+def test_height_only_queries_equal_the_full_surfaces(lib, q):
+    """Queries that ask for heights alone (no surface output) take the
+    height taps without the normal and velocity loads; their heights equal,
+    bit for bit, the full surface queries' heights at the same points,
+    through ocean_amd_sample_batch_device and through the fused foam step
+    with queries."""
+    from oceanlib_amd.hipmem import DeviceArray
+    desc = desc_from(CONFIG2)
+    st = C.c_int(-1)
+    sim = lib.ocean_create(C.byref(desc), C.byref(st))
+    assert sim, st.value
+    fd = capi.FoamDesc()
+    lib.ocean_amd_foam_desc_init(C.byref(fd))
+    foam = lib.ocean_amd_foam_create(sim, C.byref(fd), C.byref(st))
+    assert foam, st.value
+    try:
+        xz = DeviceArray.from_numpy(query_points(1, q, desc.patch_length, seed=q).reshape(-1))
+        out = DeviceArray((q, 10), np.float32)
+        h_full, h_only, h_fused = (DeviceArray.from_numpy(np.full(q, np.nan, np.float32)) for _ in range(3))
+        lib.ocean_update(sim, 3.25)
+        assert lib.ocean_amd_sample_batch_device(sim, xz.ptr, q, out.ptr, h_full.ptr) == 0
+        assert lib.ocean_amd_sample_batch_device(sim, xz.ptr, q, None, h_only.ptr) == 0
+        assert lib.ocean_amd_foam_update_sample_device(foam, np.float32(1 / 60), xz.ptr, q, None,
+                                                       h_fused.ptr) == 0
+        lib.ocean_amd_synchronize(sim)
+        assert lib.ocean_amd_last_error(sim) == 0
+        full = h_full.numpy()
+        assert not np.isnan(full).any()
+        assert_bitexact(full, out.numpy()[:, 0].copy(), "heights vs surfaces")
+        assert_bitexact(h_only.numpy(), full, "height-only batch query")
+        assert_bitexact(h_fused.numpy(), full, "height-only fused foam step and queries")
+    finally:
+        lib.ocean_amd_foam_destroy(foam)
+        lib.ocean_destroy(sim)
+
+
 def test_fused_foam_step_and_queries_equal_the_two_calls(lib):
     """ocean_amd_foam_update_sample_device (the last foam substep and the
     frame's queries in one launch) gives the same foam field and the same

@@ -16,11 +16,14 @@ struct Tap {
     float w00, w10, w01, w11;
 };
 
+// The reference's periodic wrap of an int index, ((i % n) + n) % n
+// (src/ocean.cpp:322-352). Every grid here has a power-of-two side (16 ..
+// 4096, checked when an ocean is created), and for a two's-complement int
+// that wrap is i & (n - 1): the same index without an integer division (its
+// remainder sequence costs quarter-rate multiplies on every tap).
 __device__ __forceinline__ uint32_t wrap_index(int i, int n)
 {
-    i %= n;
-    if (i < 0) i += n;
-    return static_cast<uint32_t>(i);
+    return static_cast<uint32_t>(i) & static_cast<uint32_t>(n - 1);
 }
 
 __device__ __forceinline__ Tap make_tap(float u, float v, uint32_t n, float inv_cell)
@@ -30,17 +33,18 @@ __device__ __forceinline__ Tap make_tap(float u, float v, uint32_t n, float inv_
     const float fu = floorf(gu);
     const float fv = floorf(gv);
     const int ni = static_cast<int>(n);
+    const uint32_t mask = n - 1u, sh = static_cast<uint32_t>(__builtin_ctz(n));  // n = 2^sh
     const uint32_t x0 = wrap_index(static_cast<int>(fu), ni);
     const uint32_t z0 = wrap_index(static_cast<int>(fv), ni);
-    const uint32_t x1 = (x0 + 1u) % n;
-    const uint32_t z1 = (z0 + 1u) % n;
+    const uint32_t x1 = (x0 + 1u) & mask;
+    const uint32_t z1 = (z0 + 1u) & mask;
     const float tu = gu - fu;
     const float tv = gv - fv;
     Tap t;
-    t.i00 = z0 * n + x0;
-    t.i10 = z0 * n + x1;
-    t.i01 = z1 * n + x0;
-    t.i11 = z1 * n + x1;
+    t.i00 = (z0 << sh) + x0;
+    t.i10 = (z0 << sh) + x1;
+    t.i01 = (z1 << sh) + x0;
+    t.i11 = (z1 << sh) + x1;
     t.w00 = (1.0f - tu) * (1.0f - tv);
     t.w10 = tu * (1.0f - tv);
     t.w01 = (1.0f - tu) * tv;
@@ -164,16 +168,15 @@ __device__ __forceinline__ float sample_wrap(const float* __restrict__ f, float 
     const float fv = floorf(v);
     const float tu = u - fu;
     const float tv = v - fv;
-    int x0 = static_cast<int>(fu) % n;
-    if (x0 < 0) x0 += n;
-    int z0 = static_cast<int>(fv) % n;
-    if (z0 < 0) z0 += n;
-    const int x1 = (x0 + 1) % n;
-    const int z1 = (z0 + 1) % n;
-    const float a = f[z0 * n + x0];
-    const float b = f[z0 * n + x1];
-    const float c = f[z1 * n + x0];
-    const float d = f[z1 * n + x1];
+    const int x0 = static_cast<int>(wrap_index(static_cast<int>(fu), n));
+    const int z0 = static_cast<int>(wrap_index(static_cast<int>(fv), n));
+    const int x1 = (x0 + 1) & (n - 1);
+    const int z1 = (z0 + 1) & (n - 1);
+    const int sh = __builtin_ctz(n);  // n = 2^sh
+    const float a = f[(z0 << sh) + x0];
+    const float b = f[(z0 << sh) + x1];
+    const float c = f[(z1 << sh) + x0];
+    const float d = f[(z1 << sh) + x1];
     return a * (1.0f - tu) * (1.0f - tv) + b * tu * (1.0f - tv) + c * (1.0f - tu) * tv +
            d * tu * tv;
 }

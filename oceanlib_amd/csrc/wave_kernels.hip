@@ -102,7 +102,7 @@ __device__ __forceinline__ void foam_step_cell(uint32_t i, const float4* __restr
                                                const ocean_hip_foam_params& p)
 {
     if (i >= n * n) return;
-    const uint32_t x = i % n, z = i / n;
+    const uint32_t x = i & (n - 1u), z = i >> __builtin_ctz(n);  // n is a power of two
     const float4 w = vel[i];
     const float vx = w.x * p.scale + p.drift_x;
     const float vz = w.z * p.scale + p.drift_z;

@@ -754,20 +754,44 @@ __global__ __launch_bounds__(64 * kPropWaves) __attribute__((amdgpu_waves_per_eu
             // Lanes past the end gather at a stale or zero point: every
             // offset is wrapped into its grid, the rows are independent
             // columns of the products, and their outputs are not stored.
+            //
+            // The wavefront's 64 spans of a level are 192 records; load k
+            // (k = 0, 1, 2) has lane l fetch record 64 k + l of that list,
+            // record a = (64 k + l) % 3 of query row (64 k + l) / 3, so
+            // consecutive lanes read one span's consecutive records and a
+            // load touches about 64 / 3 spans' cache lines instead of 64.
+            // The span's offset comes from its query's lane by a lane
+            // shuffle, and the record's three node pairs (column a, rows
+            // cz - 1, cz, cz + 1) go straight to that row's feature slots
+            // 3 bz + a.
             uint32_t off[kLevels], op[kLevels];
 #pragma unroll
             for (int lv = 0; lv < kLevels; ++lv)
                 off[lv] = coarse_span(base_cell_g(p.x, p.y, gl[lv], nl[lv]), inv_s[lv], ncl[lv], op[lv]);
-            u32x3v c[kLevels][3];
+            // this lane's share of the loads, derived each round (an empty
+            // asm keeps the compiler from holding it across the network)
+            uint32_t lr = uint32_t(l);
+            asm volatile("" : "+v"(lr));
+            uint32_t span_row[3], span_rec[3];
 #pragma unroll
-            for (int lv = 0; lv < kLevels; ++lv)
-#pragma unroll
-                for (int a = 0; a < 3; ++a) c[lv][a] = coarse_record(cgl[lv], off[lv] + 16u * uint32_t(a));
+            for (int k = 0; k < 3; ++k) {
+                span_row[k] = (64u * uint32_t(k) + lr) / 3u;
+                span_rec[k] = (64u * uint32_t(k) + lr) - 3u * span_row[k];
+            }
 #pragma unroll
             for (int lv = 0; lv < kLevels; ++lv) {
-                uint32_t pk[kFeatPerLevel / 2];
-                stencil_pairs(c[lv][0], c[lv][1], c[lv][2], op[lv], pk);
-                store_level_pairs(X, l, lv, pk);
+                u32x3v c[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const uint32_t o = uint32_t(__shfl(int(off[lv]), int(span_row[k]), 64));
+                    c[k] = coarse_record(cgl[lv], o + 16u * span_rec[k]);
+                }
+                X[l * (kXP / 2) + lv * (kFeatPerLevel / 2) + 9] = op[lv];
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+#pragma unroll
+                    for (int bz = 0; bz < 3; ++bz)
+                        X[span_row[k] * (kXP / 2) + lv * (kFeatPerLevel / 2) + 3 * bz + span_rec[k]] = c[k][bz];
             }
         } else {
 #pragma unroll

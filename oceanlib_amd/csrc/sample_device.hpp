@@ -73,6 +73,16 @@ struct CellCache {
     float4 p{}, q{}, r{}, s{};   // zeroed, so no path reads an indeterminate value
 };
 
+__device__ __forceinline__ float4 blend_cached(const Tap& t, const CellCache& c)
+{
+    float4 o;
+    o.x = c.p.x * t.w00 + c.q.x * t.w10 + c.r.x * t.w01 + c.s.x * t.w11;
+    o.y = c.p.y * t.w00 + c.q.y * t.w10 + c.r.y * t.w01 + c.s.y * t.w11;
+    o.z = c.p.z * t.w00 + c.q.z * t.w10 + c.r.z * t.w01 + c.s.z * t.w11;
+    o.w = c.p.w * t.w00 + c.q.w * t.w10 + c.r.w * t.w01 + c.s.w * t.w11;
+    return o;
+}
+
 __device__ __forceinline__ float4 fetch4_cached(const float4* __restrict__ b, const Tap& t,
                                                 CellCache& c)
 {
@@ -160,6 +170,42 @@ __device__ __forceinline__ float height_level(const float4* __restrict__ disp, u
     }
     const Tap t = make_tap(u, v, n, inv_cell);
     return fetch4_cached(disp, t, cache).y;
+}
+
+// height_level with two lanes per point: lanes 2j and 2j + 1 of a
+// wavefront follow the same point. When a tap leaves the cached cell, the
+// even lane loads the cell's two x0 texels and the odd lane its two x1
+// texels, so each row of the cell is one cache line for the pair instead of
+// two lanes' worth, and they exchange them with a lane-pair shuffle (both
+// lanes of a pair always take the same branch). The same texels and the same
+// blend as height_level.
+__device__ __forceinline__ float4 pair_swap(float4 a)
+{
+    return make_float4(__shfl_xor(a.x, 1, 64), __shfl_xor(a.y, 1, 64), __shfl_xor(a.z, 1, 64), a.w);
+}
+__device__ __forceinline__ float height_level_lane_pair(const float4* __restrict__ disp, uint32_t n,
+                                                        float inv_cell, float wx, float wz, bool odd)
+{
+    float u = wx, v = wz;
+    CellCache c;
+#pragma unroll
+    for (int it = 0; it < 5; ++it) {
+        const Tap t = make_tap(u, v, n, inv_cell);
+        if (t.i00 != c.i00) {
+            const float4 m0 = disp[odd ? t.i10 : t.i00], m1 = disp[odd ? t.i11 : t.i01];
+            const float4 o0 = pair_swap(m0), o1 = pair_swap(m1);
+            c.p = odd ? o0 : m0;
+            c.q = odd ? m0 : o0;
+            c.r = odd ? o1 : m1;
+            c.s = odd ? m1 : o1;
+            c.i00 = t.i00;
+        }
+        const float4 d = blend_cached(t, c);
+        if (it == 4) return d.y;
+        u = wx - d.x;
+        v = wz - d.z;
+    }
+    return 0.0f;
 }
 
 __device__ __forceinline__ float sample_wrap(const float* __restrict__ f, float u, float v, int n)

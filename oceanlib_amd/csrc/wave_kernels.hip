@@ -81,17 +81,31 @@ __global__ __launch_bounds__(kBlock) void cascade_sample_kernel(
 // `heights` between the launches, with the reference's float additions in
 // level order (the first level's launch adds to 0.0f, as the reference's sum
 // starts, src/cascade.cpp:94,118-120).
+// Two lanes per point (height_level_lane_pair): kBlock / 2 points per block;
+// a pair past the end follows the last point and stores nothing.
 __global__ __launch_bounds__(kBlock) void cascade_height_level_kernel(
     const ocean_hip_level* __restrict__ level, bool first, const float2* __restrict__ xz,
     uint32_t count, float* __restrict__ heights)
 {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= count) return;
+    const uint32_t q = (blockIdx.x * blockDim.x + threadIdx.x) >> 1;
+    const bool odd = (threadIdx.x & 1u) != 0u;
     const ocean_hip_level L = *level;
-    const float2 p = xz[q];
-    const float h = height_level(reinterpret_cast<const float4*>(L.disp), L.n,
-                                 static_cast<float>(L.n) / L.patch_length, p.x, p.y);
-    heights[q] = (first ? 0.0f : heights[q]) + h;
+    const float2 p = xz[q < count ? q : count - 1u];
+    const float h = height_level_lane_pair(reinterpret_cast<const float4*>(L.disp), L.n,
+                                           static_cast<float>(L.n) / L.patch_length, p.x, p.y, odd);
+    if (q < count && !odd) heights[q] = (first ? 0.0f : heights[q]) + h;
+}
+
+// Heights alone of one ocean for large batches, two lanes per point as above.
+__global__ __launch_bounds__(kBlock) void height_lane_pair_kernel(
+    const float4* __restrict__ disp, uint32_t n, float inv_cell, const float2* __restrict__ xz,
+    uint32_t count, float* __restrict__ heights)
+{
+    const uint32_t q = (blockIdx.x * blockDim.x + threadIdx.x) >> 1;
+    const bool odd = (threadIdx.x & 1u) != 0u;
+    const float2 p = xz[q < count ? q : count - 1u];
+    const float h = height_level_lane_pair(disp, n, inv_cell, p.x, p.y, odd);
+    if (q < count && !odd) heights[q] = h;
 }
 
 // reference src/foam.cpp:89-139
@@ -302,6 +316,13 @@ extern "C" hipError_t ocean_hip_sample(const ocean_hip_level* L, const float* xz
 {
     if (count == 0) return hipSuccess;
     const float inv_cell = static_cast<float>(L->n) / L->patch_length;
+    if (out == nullptr && heights != nullptr && count > 65536u) {
+        // heights alone, many points: two lanes per point
+        hipLaunchKernelGGL(height_lane_pair_kernel, dim3((count + kBlock / 2 - 1) / (kBlock / 2)), dim3(kBlock), 0,
+                           s, reinterpret_cast<const float4*>(L->disp), L->n, inv_cell,
+                           reinterpret_cast<const float2*>(xz), count, heights);
+        return hipGetLastError();
+    }
     const dim3 b = query_block(count), g((count + b.x - 1) / b.x);
     hipLaunchKernelGGL(sample_kernel, g, b, 0, s, reinterpret_cast<const float4*>(L->disp),
                        reinterpret_cast<const float4*>(L->normal),
@@ -317,8 +338,8 @@ extern "C" hipError_t ocean_hip_cascade_sample(const ocean_hip_level* levels,
 {
     if (count == 0) return hipSuccess;
     if (out == nullptr && heights != nullptr && count > 65536u) {
-        // heights alone, many points: level by level
-        const dim3 g((count + kBlock - 1) / kBlock), b(kBlock);
+        // heights alone, many points: level by level, two lanes per point
+        const dim3 g((count + kBlock / 2 - 1) / (kBlock / 2)), b(kBlock);
         for (uint32_t l = 0; l < level_count; ++l)
             hipLaunchKernelGGL(cascade_height_level_kernel, g, b, 0, s, levels + l, l == 0,
                                reinterpret_cast<const float2*>(xz), count, heights);
